@@ -1,0 +1,355 @@
+/*
+ * orc_ops.c -- oracle: 2D lattice operators and the Nc = 1 stencil form.
+ * TEST INFRASTRUCTURE ONLY (see orc.h).
+ *
+ * Vectors are lexicographic, i = y*lx + x; links AoS u[2*i + mu] (mu = 0: x,
+ * mu = 1: y).  Each operator overwrites every entry of lhs and accumulates in
+ * the reference's term order, starting from zero.
+ */
+#include "orc.h"
+
+#include <stdlib.h>
+
+/* tests/multishift/multishift.cpp:634-672: real Laplacian with Nc colours;
+ * `mass` is the m^2 shift, diagonal 4 + mass. */
+void orc_laplace_real(double* lhs, double* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, ly = op->y_fine, nc = op->Nc;
+  const double mass = op->mass;
+  for (int i = 0; i < lx * ly * nc; i++) {
+    const int c = i % nc, s = (i - c) / nc, x = s % lx, y = s / lx;
+    double acc = 0.0;
+    acc = acc - rhs[y * lx * nc + ((x + 1) % lx) * nc + c];
+    acc = acc - rhs[y * lx * nc + ((x + lx - 1) % lx) * nc + c];
+    acc = acc - rhs[((y + 1) % ly) * lx * nc + x * nc + c];
+    acc = acc - rhs[((y + ly - 1) % ly) * lx * nc + x * nc + c];
+    acc = acc + (4 + mass) * rhs[i];
+    lhs[i] = acc;
+  }
+}
+
+/* operators.cpp:28-66: the same stencil on complex vectors */
+void orc_laplace(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, ly = op->y_fine, nc = op->Nc;
+  const double mass = op->mass;
+  for (int i = 0; i < lx * ly * nc; i++) {
+    const int c = i % nc, s = (i - c) / nc, x = s % lx, y = s / lx;
+    orc_c acc = CMPLX(0.0, 0.0);
+    acc = orc_sub(acc, rhs[y * lx * nc + ((x + 1) % lx) * nc + c]);
+    acc = orc_sub(acc, rhs[y * lx * nc + ((x + lx - 1) % lx) * nc + c]);
+    acc = orc_sub(acc, rhs[((y + 1) % ly) * lx * nc + x * nc + c]);
+    acc = orc_sub(acc, rhs[((y + ly - 1) % ly) * lx * nc + x * nc + c]);
+    acc = orc_add(acc, orc_rmul(4 + mass, rhs[i]));
+    lhs[i] = acc;
+  }
+}
+
+/* operators.cpp:73-120: gauged Laplacian, diagonal 4 + mass */
+void orc_laplace_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, ly = op->y_fine;
+  const orc_c* u = op->lattice;
+  const double mass = op->mass;
+  for (int i = 0; i < lx * ly; i++) {
+    const int x = i % lx, y = i / lx;
+    const int xp = (x + 1) % lx, xm = (x + lx - 1) % lx, yp = (y + 1) % ly, ym = (y + ly - 1) % ly;
+    orc_c acc = CMPLX(0.0, 0.0);
+    acc = orc_sub(acc, orc_cmul(u[2 * (y * lx + x)], rhs[y * lx + xp]));
+    acc = orc_sub(acc, orc_cmul(orc_conj(u[2 * (y * lx + xm)]), rhs[y * lx + xm]));
+    acc = orc_sub(acc, orc_cmul(u[2 * (y * lx + x) + 1], rhs[yp * lx + x]));
+    acc = orc_sub(acc, orc_cmul(orc_conj(u[2 * (ym * lx + x) + 1]), rhs[ym * lx + x]));
+    acc = orc_add(acc, orc_rmul(4 + mass, rhs[i]));
+    lhs[i] = acc;
+  }
+}
+
+/* Hopping part of the staggered operator with an overall sign `sg` (+1 for D,
+ * -1 for D^dag) and an optional per-site factor; terms in the order
+ * +x, -x, +y, -y of operators.cpp:215-224 (D) and :403-412 (D^dag).
+ * parity_only: -1 all sites, 0 even sites only, 1 odd sites only (other
+ * sites are written as zero). */
+static void staggered_core(orc_c* lhs, orc_c* rhs, const orc_stag_op* op, double sg, int with_mass,
+                           int parity_only, int gamma5)
+{
+  const int lx = op->x_fine, ly = op->y_fine;
+  const orc_c* u = op->lattice;
+  const double mass = op->mass;
+  for (int i = 0; i < lx * ly; i++) {
+    const int x = i % lx, y = i / lx;
+    const double eta1 = 1 - 2 * (x % 2);
+    if (parity_only >= 0 && (x + y) % 2 != parity_only) {
+      lhs[i] = CMPLX(0.0, 0.0);
+      continue;
+    }
+    const int xp = (x + 1) % lx, xm = (x + lx - 1) % lx, yp = (y + 1) % ly, ym = (y + ly - 1) % ly;
+    orc_c t0, t1, t2, t3;
+    if (gamma5) {
+      /* operators.cpp:339-348: eo_sign*U*rhs, (eo_sign*eta1)*U*rhs */
+      const double eo = ((x + y) % 2 == 0) ? 1.0 : -1.0;
+      t0 = orc_cmul(orc_rmul(eo, u[2 * (y * lx + x)]), rhs[y * lx + xp]);
+      t1 = orc_cmul(orc_rmul(eo, orc_conj(u[2 * (y * lx + xm)])), rhs[y * lx + xm]);
+      t2 = orc_cmul(orc_rmul(eo * eta1, u[2 * (y * lx + x) + 1]), rhs[yp * lx + x]);
+      t3 = orc_cmul(orc_rmul(eo * eta1, orc_conj(u[2 * (ym * lx + x) + 1])), rhs[ym * lx + x]);
+    } else {
+      t0 = orc_cmul(u[2 * (y * lx + x)], rhs[y * lx + xp]);
+      t1 = orc_cmul(orc_conj(u[2 * (y * lx + xm)]), rhs[y * lx + xm]);
+      t2 = orc_cmul(orc_rmul(eta1, u[2 * (y * lx + x) + 1]), rhs[yp * lx + x]);
+      t3 = orc_cmul(orc_rmul(eta1, orc_conj(u[2 * (ym * lx + x) + 1])), rhs[ym * lx + x]);
+    }
+    orc_c acc = CMPLX(0.0, 0.0);
+    if (sg > 0) {
+      acc = orc_sub(acc, t0);
+      acc = orc_add(acc, t1);
+      acc = orc_sub(acc, t2);
+      acc = orc_add(acc, t3);
+    } else {
+      acc = orc_add(acc, t0);
+      acc = orc_sub(acc, t1);
+      acc = orc_add(acc, t2);
+      acc = orc_sub(acc, t3);
+    }
+    acc = orc_rmul(0.5, acc);
+    if (with_mass) {
+      if (gamma5) {
+        const double eo = ((x + y) % 2 == 0) ? 1.0 : -1.0;
+        acc = orc_add(acc, orc_rmul(eo * mass, rhs[i]));
+      } else {
+        acc = orc_add(acc, orc_rmul(mass, rhs[i]));
+      }
+    }
+    lhs[i] = acc;
+  }
+}
+
+void orc_staggered_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  staggered_core(lhs, rhs, (const orc_stag_op*)extra, +1.0, 1, -1, 0);
+}
+void orc_staggered_dagger_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  staggered_core(lhs, rhs, (const orc_stag_op*)extra, -1.0, 1, -1, 0);
+}
+void orc_staggered_gamma5_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  staggered_core(lhs, rhs, (const orc_stag_op*)extra, +1.0, 1, -1, 1);
+}
+void orc_staggered_deo_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  staggered_core(lhs, rhs, (const orc_stag_op*)extra, +1.0, 0, 0, 0);
+}
+void orc_staggered_doe_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  staggered_core(lhs, rhs, (const orc_stag_op*)extra, +1.0, 0, 1, 0);
+}
+
+/* operators.cpp:242-259 */
+void orc_gamma5(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, ly = op->y_fine;
+  for (int i = 0; i < lx * ly; i++) {
+    const int x = i % lx, y = i / lx;
+    lhs[i] = orc_rmul((double)(1 - 2 * ((x + y) % 2)), rhs[i]);
+  }
+}
+
+/* operators.cpp:444-453: D^dag (D rhs) through a temporary */
+void orc_staggered_normal_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  orc_c* tmp = (orc_c*)malloc(sizeof(orc_c) * (size_t)op->x_fine * op->y_fine);
+  orc_staggered_u1(tmp, rhs, extra);
+  orc_staggered_dagger_u1(lhs, tmp, extra);
+  free(tmp);
+}
+
+/* operators.cpp:543-563: rhs_e = m b_e - D_eo b_o on even sites, 0 on odd */
+void orc_staggered_eoprec_prepare(orc_c* rhs_e, orc_c* rhs_orig, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, n = op->x_fine * op->y_fine;
+  orc_staggered_deo_u1(rhs_e, rhs_orig, extra);
+  for (int i = 0; i < n; i++) {
+    const int x = i % lx, y = i / lx;
+    if ((x + y) % 2 == 0) rhs_e[i] = orc_sub(orc_rmul(op->mass, rhs_orig[i]), rhs_e[i]);
+  }
+}
+/* operators.cpp:566-588: (m^2 - D_eo D_oe) on even sites, 0 on odd */
+void orc_staggered_m2mdeodoe_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, n = op->x_fine * op->y_fine;
+  orc_c* tmp = (orc_c*)malloc(sizeof(orc_c) * (size_t)n);
+  orc_staggered_doe_u1(tmp, rhs, extra);
+  orc_staggered_deo_u1(lhs, tmp, extra);
+  for (int i = 0; i < n; i++) {
+    const int x = i % lx, y = i / lx;
+    if ((x + y) % 2 == 0) lhs[i] = orc_sub(orc_rmul(op->mass * op->mass, rhs[i]), lhs[i]);
+  }
+  free(tmp);
+}
+/* operators.cpp:592-616: odd x = (b_o - D_oe x_e)/m, even copied */
+void orc_staggered_eoprec_reconstruct(orc_c* lhs_full, orc_c* lhs_e, orc_c* rhs_o, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, n = op->x_fine * op->y_fine;
+  const double inv_mass = 1.0 / op->mass;
+  orc_staggered_doe_u1(lhs_full, lhs_e, extra);
+  for (int i = 0; i < n; i++) {
+    const int x = i % lx, y = i / lx;
+    if ((x + y) % 2 == 1)
+      lhs_full[i] = orc_rmul(inv_mass, orc_sub(rhs_o[i], lhs_full[i]));
+    else
+      lhs_full[i] = lhs_e[i];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stencil form, Nc = 1.  Coefficient blocks hopping[i + d*V] with
+ * d = 0:+x, 1:+y, 2:-x, 3:-y (coarse_stencil.h:44, operators_stencil.cpp:41-55). */
+static void stencil_build(orc_stencil* s, const orc_stag_op* op, double sg, int gamma5)
+{
+  const int lx = op->x_fine, ly = op->y_fine, V = lx * ly;
+  const int dims[2] = {lx, ly};
+  s->lx = lx;
+  s->ly = ly;
+  for (int i = 0; i < V; i++) {
+    s->clover[i] = CMPLX(0.0, 0.0);
+    int coord[2], color, tmp[2];
+    orc_index_to_coord(dims, 1, i, coord, &color);
+    const int eta1 = 1 - 2 * (coord[0] % 2);
+    const int eo = gamma5 ? (orc_index_is_even(dims, 1, i) ? 1 : -1) : 1;
+    const double h = -0.5 * sg; /* -0.5 for D, +0.5 for D^dag */
+    if (gamma5) {
+      s->hopping[i] = orc_rmul(-0.5 * eo, op->lattice[2 * i]);
+      s->hopping[i + V] = orc_rmul(-0.5 * eo * eta1, op->lattice[2 * i + 1]);
+    } else {
+      s->hopping[i] = orc_rmul(h, op->lattice[2 * i]);
+      s->hopping[i + V] = orc_rmul(h * eta1, op->lattice[2 * i + 1]);
+    }
+    tmp[0] = (coord[0] - 1 + lx) % lx;
+    tmp[1] = coord[1];
+    const orc_c ux = orc_conj(op->lattice[2 * orc_coord_to_index(dims, 1, tmp, 0)]);
+    tmp[0] = coord[0];
+    tmp[1] = (coord[1] - 1 + ly) % ly;
+    const orc_c uy = orc_conj(op->lattice[2 * orc_coord_to_index(dims, 1, tmp, 0) + 1]);
+    if (gamma5) {
+      s->hopping[i + 2 * V] = orc_rmul(0.5 * eo, ux);
+      s->hopping[i + 3 * V] = orc_rmul(0.5 * eo * eta1, uy);
+    } else {
+      s->hopping[i + 2 * V] = orc_rmul(-h, ux);
+      s->hopping[i + 3 * V] = orc_rmul(-h * eta1, uy);
+    }
+  }
+  s->shift = gamma5 ? CMPLX(0.0, 0.0) : CMPLX(op->mass, 0.0);
+  s->eo_shift = gamma5 ? CMPLX(op->mass, 0.0) : CMPLX(0.0, 0.0);
+  s->dof_shift = CMPLX(0.0, 0.0);
+}
+void orc_stencil_staggered(orc_stencil* s, const orc_stag_op* op) { stencil_build(s, op, +1.0, 0); }
+void orc_stencil_staggered_gamma5(orc_stencil* s, const orc_stag_op* op) { stencil_build(s, op, +1.0, 1); }
+void orc_stencil_staggered_dagger(orc_stencil* s, const orc_stag_op* op) { stencil_build(s, op, -1.0, 0); }
+
+/* parity: -1 all sites with clover and shifts (coarse_stencil.cpp:32-171);
+ * 0 even-only hopping (:415-459); 1 odd-only hopping (:580-624). */
+static void stencil_apply(orc_c* lhs, orc_c* rhs, const orc_stencil* s, int parity)
+{
+  const int lx = s->lx, ly = s->ly, V = lx * ly;
+  const int dims[2] = {lx, ly};
+  for (int i = 0; i < V; i++) {
+    orc_c acc = CMPLX(0.0, 0.0);
+    const int even = orc_index_is_even(dims, 1, i);
+    if ((parity == 0 && !even) || (parity == 1 && even)) {
+      lhs[i] = acc;
+      continue;
+    }
+    int coord[2], color, t[2];
+    orc_index_to_coord(dims, 1, i, coord, &color);
+    if (parity < 0) acc = orc_add(acc, orc_cmul(s->clover[i], rhs[orc_coord_to_index(dims, 1, coord, 0)]));
+    t[0] = (coord[0] + 1) % lx; t[1] = coord[1];
+    acc = orc_add(acc, orc_cmul(s->hopping[i], rhs[orc_coord_to_index(dims, 1, t, 0)]));
+    t[0] = coord[0]; t[1] = (coord[1] + 1) % ly;
+    acc = orc_add(acc, orc_cmul(s->hopping[i + V], rhs[orc_coord_to_index(dims, 1, t, 0)]));
+    t[0] = (coord[0] - 1 + lx) % lx; t[1] = coord[1];
+    acc = orc_add(acc, orc_cmul(s->hopping[i + 2 * V], rhs[orc_coord_to_index(dims, 1, t, 0)]));
+    t[0] = coord[0]; t[1] = (coord[1] - 1 + ly) % ly;
+    acc = orc_add(acc, orc_cmul(s->hopping[i + 3 * V], rhs[orc_coord_to_index(dims, 1, t, 0)]));
+    if (parity < 0) {
+      if (cabs(s->shift) != 0.0) acc = orc_add(acc, orc_cmul(s->shift, rhs[i]));
+      if (cabs(s->eo_shift) != 0.0) acc = orc_add(acc, orc_cmul(orc_rmul(even ? 1.0 : -1.0, s->eo_shift), rhs[i]));
+      /* dof_shift acts on colour halves; with nc = 1 every site is in the
+       * lower half (colour 0 < 1/2 is false), coarse_stencil.cpp:165-169 */
+      if (cabs(s->dof_shift) != 0.0) acc = orc_add(acc, orc_cmul(orc_rmul(-1.0, s->dof_shift), rhs[i]));
+    }
+    lhs[i] = acc;
+  }
+}
+void orc_stencil_apply(orc_c* lhs, orc_c* rhs, void* st) { stencil_apply(lhs, rhs, (const orc_stencil*)st, -1); }
+void orc_stencil_apply_eo(orc_c* lhs, orc_c* rhs, void* st) { stencil_apply(lhs, rhs, (const orc_stencil*)st, 0); }
+void orc_stencil_apply_oe(orc_c* lhs, orc_c* rhs, void* st) { stencil_apply(lhs, rhs, (const orc_stencil*)st, 1); }
+
+void orc_stencil_eoprec_prepare(orc_c* rhs_e, orc_c* rhs_orig, void* st)
+{
+  const orc_stencil* s = (const orc_stencil*)st;
+  const int dims[2] = {s->lx, s->ly};
+  orc_stencil_apply_eo(rhs_e, rhs_orig, st);
+  for (int i = 0; i < s->lx * s->ly; i++)
+    if (orc_index_is_even(dims, 1, i)) rhs_e[i] = orc_sub(orc_cmul(s->shift, rhs_orig[i]), rhs_e[i]);
+}
+void orc_stencil_m2mdeodoe(orc_c* lhs, orc_c* rhs, void* st)
+{
+  const orc_stencil* s = (const orc_stencil*)st;
+  const int dims[2] = {s->lx, s->ly}, V = s->lx * s->ly;
+  orc_c* tmp = (orc_c*)malloc(sizeof(orc_c) * (size_t)V);
+  orc_stencil_apply_oe(tmp, rhs, st);
+  orc_stencil_apply_eo(lhs, tmp, st);
+  for (int i = 0; i < V; i++)
+    if (orc_index_is_even(dims, 1, i)) lhs[i] = orc_sub(orc_cmul(orc_cmul(s->shift, s->shift), rhs[i]), lhs[i]);
+  free(tmp);
+}
+void orc_stencil_eoprec_reconstruct(orc_c* lhs_full, orc_c* lhs_e, orc_c* rhs_o, void* st)
+{
+  const orc_stencil* s = (const orc_stencil*)st;
+  const int dims[2] = {s->lx, s->ly}, V = s->lx * s->ly;
+  const double inv_mass = 1.0 / creal(s->shift);
+  orc_stencil_apply_oe(lhs_full, lhs_e, st);
+  for (int i = 0; i < V; i++) {
+    if (!orc_index_is_even(dims, 1, i))
+      lhs_full[i] = orc_rmul(inv_mass, orc_sub(rhs_o[i], lhs_full[i]));
+    else
+      lhs_full[i] = lhs_e[i];
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gradient of S = phi^dag (D^dag D)^-1 phi with D = m - K,
+ * K psi(x) = 1/2 sum_mu eta_mu(x) [U_mu(x) psi(x+mu) - U_mu(x-mu)^* psi(x-mu)].
+ * With chi = (D^dag D)^-1 phi and psi = D chi, dS = -2 Re psi^dag (dD) chi, and
+ * dU/dtheta = iU gives
+ *   dS/dtheta_mu(x) = -eta_mu(x) Im[ psi(x)^* U_mu(x) chi(x+mu) + psi(x+mu)^* U_mu(x)^* chi(x) ],
+ *   dS/dm           = -2 m |chi|^2.
+ * Written in the natural order (no reference counterpart to follow). */
+void orc_staggered_force(double* dS, const orc_c* chi, const orc_c* psi, const orc_stag_op* op)
+{
+  const int lx = op->x_fine, ly = op->y_fine;
+  const orc_c* u = op->lattice;
+  for (int y = 0; y < ly; y++)
+    for (int x = 0; x < lx; x++) {
+      const int i = y * lx + x;
+      const double eta[2] = {1.0, (double)(1 - 2 * (x % 2))};
+      const int nb[2] = {y * lx + (x + 1) % lx, ((y + 1) % ly) * lx + x};
+      for (int mu = 0; mu < 2; mu++) {
+        const orc_c U = u[2 * i + mu];
+        const orc_c a = orc_cmul(orc_cmul(orc_conj(psi[i]), U), chi[nb[mu]]);
+        const orc_c b = orc_cmul(orc_cmul(orc_conj(psi[nb[mu]]), orc_conj(U)), chi[i]);
+        dS[2 * i + mu] = -eta[mu] * (cimag(a) + cimag(b));
+      }
+    }
+}
+double orc_staggered_dS_dm(const orc_c* chi, const orc_stag_op* op)
+{
+  return -2.0 * op->mass * orc_norm2sq_c(chi, op->x_fine * op->y_fine);
+}
