@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X solver path on its headline workload.
+
+Workload (one GPU): CG on the 2D U(1) staggered normal operator D^dag D,
+L = 1024 x 1024, m = 0.02, link angles theta ~ N(0, 0.5^2), complex Gaussian
+rhs, to true relative residual 1e-10 -- plain FP64 CG against mixed-precision
+CG (FP32 Krylov vectors and links, FP64 reliable updates at delta = 0.1).
+Inputs are synthetic (numpy, fixed seed) and resident in HBM before timing.
+
+A "step" is one complete mixed-precision solve from x = 0.  `value` is its
+time to solution in ms (lower is better).  The JSON line also carries the
+FP64 solve's time, the speed-up, iteration counts, the mixed solution's
+relative error against the FP64 one, the stencil roofline measured live
+with HIP events, and the reference CPU path timed on the host on a bounded
+sample of the same solve.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 1024]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "generic-linalg_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_inputs(L: int, sigma: float, seed: int):
+    rng = np.random.default_rng(seed)
+    theta = rng.normal(0.0, sigma, size=2 * L * L)
+    links = np.exp(1j * theta)
+    b = rng.normal(size=L * L) + 1j * rng.normal(size=L * L)
+    return links.astype(np.complex128), b.astype(np.complex128)
+
+
+def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0):
+    """Reference CPU CG (oracle/_ref, built from the reference sources; the
+    oracle restatement where that build is absent), one thread, on a bounded
+    sample: a fixed number of iterations of the same 1024^2 solve, scaled to
+    the iteration count the FP64 solve needs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc  # noqa: E402  (CPU baseline leg only)
+
+    impl = "ref" if orc.ref_available() else "orc"
+    # calibrate: time a few iterations, then size the sample to the budget
+    n_cal = 5
+    t0 = time.perf_counter()
+    orc.solve("cg", "staggered_normal", b, None, links=links, mass=mass, lx=L, ly=L, max_iter=n_cal, eps=1e-10,
+              impl=impl)
+    t_cal = time.perf_counter() - t0
+    per_it = t_cal / (n_cal + 3)  # n iterations + 3 extra operator applications
+    n_it = max(5, min(gpu_fp64_iters, int(budget_s / per_it) - 3))
+    t0 = time.perf_counter()
+    _, info = orc.solve("cg", "staggered_normal", b, None, links=links, mass=mass, lx=L, ly=L, max_iter=n_it,
+                        eps=1e-10, impl=impl)
+    dt = time.perf_counter() - t0
+    per_it = dt / (info.iter + 3)
+    projected_ms = per_it * (gpu_fp64_iters + 3) * 1e3
+    return {
+        "value": round(projected_ms, 1),
+        "unit": "ms (FP64 CG time to 1e-10, projected)",
+        "cores": 1,
+        "kind": "reference" if impl == "ref" else "port",
+        "sample": f"{info.iter} CG iterations of the same {L}x{L} solve on one host core "
+                  f"({dt:.2f} s, {per_it * 1e3:.1f} ms/iteration), scaled to the {gpu_fp64_iters} iterations "
+                  f"the FP64 solve takes",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--L", type=int, default=1024)
+    ap.add_argument("--mass", type=float, default=0.02)
+    ap.add_argument("--sigma", type=float, default=0.5)
+    ap.add_argument("--links", choices=["f32", "f16"], default="f32")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: the solver path has no multi-GPU code; "
+                                   "run with --gpus 1"}))
+        return 2
+
+    import glx
+
+    L, mass = args.L, args.mass
+    V = L * L
+    links, b = make_inputs(L, args.sigma, seed=1337)
+    lat = glx.Lattice(links, L, L, f32=True, f16=True)
+    d_b = glx.DeviceArray(b)
+    d_x = glx.DeviceArray(np.zeros_like(b))
+    d_x64 = glx.DeviceArray(np.zeros_like(b))
+    zero = np.zeros_like(b)
+    link_prec = glx.LINKS_F16 if args.links == "f16" else glx.LINKS_F32
+
+    def solve_fp64():
+        d_x64.upload(zero)
+        glx.synchronize()
+        t0 = time.perf_counter()
+        info = glx.dev_solve("cg", lat, "staggered_normal", mass, d_x64, d_b, max_iter=100000, eps=1e-10)
+        glx.synchronize()
+        return time.perf_counter() - t0, info
+
+    def solve_mixed():
+        d_x.upload(zero)
+        glx.synchronize()
+        t0 = time.perf_counter()
+        info = glx.dev_cg_mixed(lat, "staggered_normal", mass, d_x, d_b, max_iter=100000, eps=1e-10, delta=0.1,
+                                link_prec=link_prec)
+        glx.synchronize()
+        return time.perf_counter() - t0, info
+
+    for _ in range(args.warmup):
+        solve_fp64()
+        solve_mixed()
+    t64 = []
+    for _ in range(max(1, args.steps)):
+        dt, info64 = solve_fp64()
+        t64.append(dt)
+    tmx = []
+    for _ in range(args.steps):
+        dt, infomx = solve_mixed()
+        tmx.append(dt)
+    ms64 = float(np.median(t64)) * 1e3
+    msmx = float(np.median(tmx)) * 1e3
+    x64 = d_x64.download()
+    xmx = d_x.download()
+    err_vs_fp64 = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
+
+    # live roofline of the fused D^dag D stencil (HIP events on its stream)
+    d_in = glx.DeviceArray(b)
+    d_out = glx.DeviceArray(np.zeros_like(b))
+    ms_st64 = glx.time_apply(lat, "staggered_normal", mass, d_out, d_in, reps=200)
+    b32 = b.astype(np.complex64)
+    d_in32 = glx.DeviceArray(b32)
+    d_out32 = glx.DeviceArray(np.zeros_like(b32))
+    ms_st32 = glx.time_apply(lat, "staggered_normal", mass, d_out32, d_in32, reps=200, fp32=True,
+                             link_prec=glx.LINKS_F32)
+    ms_st16 = glx.time_apply(lat, "staggered_normal", mass, d_out32, d_in32, reps=200, fp32=True,
+                             link_prec=glx.LINKS_F16)
+    # algorithmic bytes per application: V * (2 s_v + 2 s_u (+ s_scale))
+    bytes64, bytes32, bytes16 = 64 * V, 32 * V, 28 * V
+    gbs64 = bytes64 / (ms_st64 * 1e-3) / 1e9
+    gbs32 = bytes32 / (ms_st32 * 1e-3) / 1e9
+    gbs16 = bytes16 / (ms_st16 * 1e-3) / 1e9
+    dom_ms, dom_bytes, dom_gbs = (ms_st16, bytes16, gbs16) if args.links == "f16" else (ms_st32, bytes32, gbs32)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(links, b, L, mass, info64.iter)
+        except Exception as e:  # the baseline is reported, never required
+            cpu = {"value": None, "unit": "ms", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
+
+    out = {
+        "metric": "CG time-to-1e-10 & stencil HBM GB/s, 2D U(1) staggered: FP64 vs mixed",
+        "value": round(msmx, 3),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(msmx, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"fp32 bulk ({args.links} links) + fp64 reliable updates",
+        "data": "synthetic (numpy seed 1337): theta ~ N(0, 0.5^2), complex Gaussian rhs",
+        "config": {
+            "workload": f"CG on staggered D^dag D, L={L}x{L}, m={mass}, theta~N(0,{args.sigma}^2), "
+                        f"true relres 1e-10; FP64 CG vs mixed (delta 0.1)",
+            "L": L, "mass": mass, "tol": 1e-10, "delta": 0.1,
+        },
+        "fp64_ms": round(ms64, 3),
+        "speedup_vs_fp64": round(ms64 / msmx, 3),
+        "fp64_iters": info64.iter,
+        "mixed_iters": infomx.iter,
+        "mixed_reliable_updates": infomx.reliable_updates,
+        "mixed_true_relres": float(np.sqrt(infomx.resSq) / np.linalg.norm(b)),
+        "fp64_true_relres": float(np.sqrt(info64.resSq) / np.linalg.norm(b)),
+        "mixed_relerr_vs_fp64": err_vs_fp64,
+        "stencil": {
+            "fp64_ms": ms_st64, "fp64_GBs": round(gbs64, 1),
+            "fp32_ms": ms_st32, "fp32_GBs": round(gbs32, 1),
+            "fp16link_ms": ms_st16, "fp16link_GBs": round(gbs16, 1),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_stencil<OP_STAG_NORMAL, c64, Links{args.links.upper()}> (fused D^dag D)",
+            "achieved": round(dom_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_launch": dom_bytes,
+            "ms_per_launch": dom_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,27 @@
+// generic_cg.h -- CG and restarted CG(n) (generic_cg.h:18-23 of the
+// reference), solved on the GPU by libglx.so.
+#ifndef GLX_COMPAT_CG
+#define GLX_COMPAT_CG
+
+#include <complex>
+#include <string>
+using std::complex;
+
+#include "inverter_struct.h"
+#include "verbosity.h"
+
+inversion_info minv_vector_cg(double* phi, double* phi0, int size, int max_iter, double res,
+                              void (*matrix_vector)(double*, double*, void*), void* extra_info,
+                              inversion_verbose_struct* verbosity = 0);
+inversion_info minv_vector_cg(complex<double>* phi, complex<double>* phi0, int size, int max_iter, double res,
+                              void (*matrix_vector)(complex<double>*, complex<double>*, void*), void* extra_info,
+                              inversion_verbose_struct* verbosity = 0);
+inversion_info minv_vector_cg_restart(double* phi, double* phi0, int size, int max_iter, double res, int restart_freq,
+                                      void (*matrix_vector)(double*, double*, void*), void* extra_info,
+                                      inversion_verbose_struct* verbosity = 0);
+inversion_info minv_vector_cg_restart(complex<double>* phi, complex<double>* phi0, int size, int max_iter, double res,
+                                      int restart_freq,
+                                      void (*matrix_vector)(complex<double>*, complex<double>*, void*),
+                                      void* extra_info, inversion_verbose_struct* verbosity = 0);
+
+#endif

@@ -1,0 +1,811 @@
+// glx_abi.hip -- the C ABI (include/glx.h) over the device solvers.
+//
+// Host-pointer entry points keep the reference's contract
+// (generic_cg.cpp:132-427, generic_cg_m.cpp:23-600, generic_bicgstab.cpp:22-385,
+// generic_gcr.cpp:27-378): the operator callback is recognised by pointer
+// identity and its extra_info (a staggered_u1_op or stencil_2d mirror) is
+// turned into a device operator; vectors and links are copied to HBM, the
+// solve runs on the GPU, and the solution is copied back.  Restarted solvers
+// repeat the device solve on the resident vectors exactly as the reference's
+// restart wrappers do, and every verbosity line is reproduced
+// (verbosity.h:73-129).
+#include "../../include/glx.h"
+#include "glx_solvers.h"
+
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <unordered_map>
+
+using namespace glx;
+
+// ================================================================ errors ==
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return GLX_OK;
+  } catch (const ArgError& e) {
+    g_last_error = e.what();
+    return GLX_ERR_ARG;
+  } catch (const HipError& e) {
+    g_last_error = e.what();
+    return GLX_ERR_HIP;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return GLX_ERR_INTERNAL;
+  }
+}
+
+// ===================================================== verbosity output ==
+namespace {
+
+struct Printer {
+  glx_verbose* v;
+  std::ostringstream line;
+  explicit Printer(glx_verbose* vb) : v(vb) {
+    if (v && !v->out_buf) {
+      line.flags(std::cout.flags());
+      line.precision(std::cout.precision());
+    } else if (v && v->float_format == 1) {
+      line << std::scientific;
+      line.precision(v->float_precision > 0 ? v->float_precision : 6);
+    }
+  }
+  int level() const { return v ? v->verbosity : GLX_VERB_NONE; }
+  std::string prefix() const { return v && v->verb_prefix ? v->verb_prefix : ""; }
+  void flush() {
+    const std::string s = line.str();
+    line.str(std::string());
+    if (!v) return;
+    if (v->out_buf) {
+      if (v->out_cap == 0) return;
+      size_t room = v->out_cap - 1 > v->out_len ? v->out_cap - 1 - v->out_len : 0;
+      size_t n = std::min(room, s.size());
+      std::memcpy(v->out_buf + v->out_len, s.data(), n);
+      v->out_len += n;
+      v->out_buf[v->out_len] = 0;
+    } else {
+      std::cout << s;
+    }
+  }
+  // verbosity.h:73-84
+  void resid(const std::string& alg, int iter, int ops, double relres) {
+    if (level() != GLX_VERB_DETAIL) return;
+    line << prefix() << alg << " Iter " << iter << " Ops " << ops << " RelRes " << relres << "\n";
+    flush();
+  }
+  // verbosity.h:87-98
+  void summary(const std::string& alg, bool success, int iter, int ops, double relres) {
+    const int l = level();
+    if (l != GLX_VERB_SUMMARY && l != GLX_VERB_RESTART_DETAIL && l != GLX_VERB_DETAIL) return;
+    line << prefix() << alg << " Success " << (success ? "Y" : "N") << " Iter " << iter << " Ops " << ops
+         << " RelRes " << relres << "\n";
+    flush();
+  }
+  // verbosity.h:101-117 (the algorithm name is always "CG-M ")
+  void summary_multi(bool success, int iter, int ops, const std::vector<double>& relres) {
+    const int l = level();
+    if (l != GLX_VERB_SUMMARY && l != GLX_VERB_RESTART_DETAIL && l != GLX_VERB_DETAIL) return;
+    line << prefix() << "CG-M " << " Success " << (success ? "Y" : "N") << " Iter " << iter << " Ops " << ops
+         << " RelRes ";
+    for (double r : relres) line << r << " ";
+    line << "\n";
+    flush();
+  }
+  // verbosity.h:120-129
+  void restart(const std::string& alg, int iter, int ops, double relres) {
+    const int l = level();
+    if (l != GLX_VERB_RESTART_DETAIL && l != GLX_VERB_DETAIL) return;
+    line << prefix() << alg << " Iter " << iter << " Ops " << ops << " RelRes " << relres << "\n";
+    flush();
+  }
+};
+
+// verbosity.h:27-43: inner solves of a restarted solver print only at DETAIL
+glx_verbose restart_verbosity(const glx_verbose* v) {
+  glx_verbose r;
+  std::memset(&r, 0, sizeof(r));
+  if (v) {
+    r = *v;
+    if (v->verbosity == GLX_VERB_RESTART_DETAIL || v->verbosity == GLX_VERB_SUMMARY) r.verbosity = GLX_VERB_NONE;
+  }
+  return r;
+}
+
+// ================================================== operator registry ====
+struct RegEntry {
+  int kind;
+  bool has_params;
+  glx_staggered_u1_op params;
+};
+struct Registry {
+  std::mutex mu;
+  std::unordered_map<void*, RegEntry> map;
+  static Registry& get() {
+    static Registry* r = new Registry;
+    return *r;
+  }
+  void add(void* f, int kind, const glx_staggered_u1_op* p) {
+    std::lock_guard<std::mutex> g(mu);
+    RegEntry e{kind, p != nullptr, {}};
+    if (p) e.params = *p;
+    map[f] = e;
+  }
+  bool find(void* f, RegEntry& out) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = map.find(f);
+    if (it == map.end()) return false;
+    out = it->second;
+    return true;
+  }
+  void remove(void* f) {
+    std::lock_guard<std::mutex> g(mu);
+    map.erase(f);
+  }
+};
+
+bool is_stencil_kind(int k) { return op_coeffs(k); }
+bool real_kind(int k) { return k == OP_LAPLACE_REAL; }
+
+struct BuiltinInit {
+  BuiltinInit() {
+    auto& r = Registry::get();
+    r.add(reinterpret_cast<void*>(&glx_square_laplace_real), OP_LAPLACE_REAL, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_laplace), OP_LAPLACE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_laplace_u1), OP_LAPLACE_U1, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_u1), OP_STAG, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_dagger_u1), OP_STAG_DAG, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_normal_u1), OP_STAG_NORMAL, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_gamma5_u1), OP_STAG_G5, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_gamma_5), OP_GAMMA5, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_deo_u1), OP_DEO, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_doe_u1), OP_DOE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_m2mdeodoe_u1), OP_M2MDEODOE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d), OP_STENCIL, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_eo), OP_STENCIL_EO, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_oe), OP_STENCIL_OE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_square_staggered_m2mdeodoe_stencil), OP_STENCIL_M2MDEODOE, nullptr);
+  }
+};
+BuiltinInit g_builtin_init;
+
+// A device operator together with the device storage it points into.
+struct Resolved {
+  DevOperator op;
+  DevBuf links, clover, hop;
+};
+
+size_t elem_bytes(bool cplx) { return cplx ? 16 : 8; }
+
+void upload(DevBuf& buf, const void* host, size_t bytes) {
+  buf = DevBuf(bytes);
+  GLX_HIP(hipMemcpy(buf.get(), host, bytes, hipMemcpyHostToDevice));
+}
+
+// Build the device operator for a solve or a direct application.
+std::unique_ptr<Resolved> resolve(glx_matvec mv, void* extra, int size, bool cplx) {
+  auto R = std::make_unique<Resolved>();
+  RegEntry e;
+  if (!Registry::get().find(reinterpret_cast<void*>(mv), e)) {
+    if (!mv) throw ArgError("matrix_vector is NULL");
+    R->op.host_cb = mv;
+    R->op.host_extra = extra;
+    R->op.host_complex = cplx;
+    R->op.size_ = size;
+    return R;
+  }
+  if (real_kind(e.kind) == cplx) throw ArgError("operator and vector types do not match (real vs complex)");
+  DevOperator& o = R->op;
+  o.kind = e.kind;
+  if (is_stencil_kind(e.kind)) {
+    const glx_stencil_2d* st = static_cast<const glx_stencil_2d*>(extra);
+    if (!st || !st->hopping || !st->clover) throw ArgError("stencil operator needs a glx_stencil_2d with coefficients");
+    o.lx = st->lx;
+    o.ly = st->ly;
+    o.nc = 1;
+    const size_t V = size_t(o.lx) * o.ly;
+    upload(R->clover, st->clover, V * 16);
+    upload(R->hop, st->hopping, 4 * V * 16);
+    o.clover = R->clover.as<c128>();
+    o.hop = R->hop.as<c128>();
+    o.shift = {st->shift[0], st->shift[1]};
+    o.eo_shift = {st->eo_shift[0], st->eo_shift[1]};
+    o.dof_shift = {st->dof_shift[0], st->dof_shift[1]};
+  } else {
+    const glx_staggered_u1_op* p = e.has_params ? &e.params : static_cast<const glx_staggered_u1_op*>(extra);
+    if (!p) throw ArgError("operator needs extra_info (staggered_u1_op) or registered parameters");
+    o.lx = p->x_fine;
+    o.ly = p->y_fine;
+    o.nc = (e.kind == OP_LAPLACE || e.kind == OP_LAPLACE_REAL) ? std::max(p->Nc, 1) : 1;
+    o.mass = p->mass;
+    if (op_links(e.kind)) {
+      if (!p->lattice) throw ArgError("operator needs links (staggered_u1_op.lattice)");
+      upload(R->links, p->lattice, size_t(o.lx) * o.ly * 2 * 16);
+      o.u64 = R->links.as<c128>();
+    }
+  }
+  if (o.lx <= 0 || o.ly <= 0) throw ArgError("lattice dimensions must be positive");
+  if (size >= 0 && size != o.lx * o.ly * o.nc)
+    throw ArgError("size " + std::to_string(size) + " does not match the operator's lattice (" +
+                   std::to_string(o.lx * o.ly * o.nc) + ")");
+  return R;
+}
+
+// Device copies of the caller's vectors.
+struct DevVec {
+  DevBuf buf;
+  size_t bytes;
+  DevVec(const double* host, size_t b) : bytes(b) {
+    buf = DevBuf(b);
+    if (host) GLX_HIP(hipMemcpy(buf.get(), host, b, hipMemcpyHostToDevice));
+  }
+  void to_host(double* host) const { GLX_HIP(hipMemcpy(host, buf.get(), bytes, hipMemcpyDeviceToHost)); }
+  void* get() const { return buf.get(); }
+};
+
+void fill_info(glx_inversion_info* info, const SolveResult& r, const std::string& name) {
+  if (!info) return;
+  info->resSq = r.resSq;
+  info->iter = r.iter;
+  info->success = r.success ? 1 : 0;
+  info->ops_count = r.ops_count;
+  std::snprintf(info->name, sizeof(info->name), "%s", name.c_str());
+  info->reliable_updates = r.reliable_updates;
+}
+
+enum SolverId { S_CG = 0, S_BICGSTAB = 1, S_GCR = 2 };
+const char* solver_name(int s) { return s == S_CG ? "CG" : (s == S_BICGSTAB ? "BiCGStab" : "GCR"); }
+
+// ops_count at the per-iteration print of iteration k (0-based), following
+// where each reference solver increments it.
+int ops_at_iter(int s, int k) {
+  if (s == S_BICGSTAB) return 3 + 2 * k;
+  return 2 + k;
+}
+
+SolveResult run_solver(int s, const DevOperator& op, void* x, const void* b, int max_iter, double eps, bool hist) {
+  SolveOpts o;
+  o.max_iter = max_iter;
+  o.eps = eps;
+  o.want_hist = hist;
+  if (s == S_CG) return dev_cg(op, x, b, o);
+  if (s == S_BICGSTAB) return dev_bicgstab(op, x, b, o);
+  if (s == S_GCR) return dev_gcr(op, x, b, o);
+  throw ArgError("unknown solver");
+}
+
+// One non-restarted solve with its verbosity lines.
+SolveResult solve_and_print(int s, const DevOperator& op, void* x, const void* b, int max_iter, double eps,
+                            glx_verbose* verb) {
+  Printer pr(verb);
+  const bool detail = pr.level() == GLX_VERB_DETAIL;
+  SolveResult r = run_solver(s, op, x, b, max_iter, eps, detail);
+  const std::string alg = solver_name(s);
+  if (detail)
+    for (int k = 0; k < int(r.relres_hist.size()); k++) pr.resid(alg, k + 1, ops_at_iter(s, k), r.relres_hist[k]);
+  pr.summary(alg, r.success, r.iter, r.ops_count, std::sqrt(r.resSq) / r.bsqrt);
+  return r;
+}
+
+// Restart driver: generic_cg.cpp:235-275 (and the BiCGStab / GCR copies).
+SolveResult solve_restarted(int s, const DevOperator& op, void* x, const void* b, int max_iter, double eps,
+                            int restart_freq, glx_verbose* verb, std::string& name) {
+  std::ostringstream nm;
+  nm << solver_name(s) << "(" << restart_freq << ")";
+  name = nm.str();
+  glx_verbose inner = restart_verbosity(verb);
+  Printer pr(verb);
+  int iter = 0, ops = 0;
+  SolveResult r;
+  do {
+    if (verb) inner.out_len = verb->out_len;
+    r = solve_and_print(s, op, x, b, restart_freq, eps, verb ? &inner : nullptr);
+    if (verb) verb->out_len = inner.out_len;
+    iter += r.iter;
+    ops += r.ops_count;
+    pr.restart(name, iter, ops, std::sqrt(r.resSq) / r.bsqrt);
+  } while (iter < max_iter && !r.success && std::sqrt(r.resSq) / r.bsqrt > eps);
+  r.iter = iter;
+  r.ops_count = ops;
+  pr.summary(name, r.success, iter, ops, std::sqrt(r.resSq) / r.bsqrt);
+  r.success = !(std::sqrt(r.resSq) / r.bsqrt > eps);
+  return r;
+}
+
+int host_solve(int s, bool cplx, double* phi, double* phi0, int size, int max_iter, double eps, int restart_freq,
+               glx_matvec mv, void* extra, glx_verbose* verb, glx_inversion_info* info) {
+  return guarded([&] {
+    if (!phi || !phi0) throw ArgError("phi and phi0 must not be NULL");
+    if (size <= 0) throw ArgError("size must be positive");
+    auto R = resolve(mv, extra, size, cplx);
+    const size_t bytes = size_t(size) * elem_bytes(cplx);
+    DevVec x(phi, bytes), b(phi0, bytes);
+    SolveResult r;
+    std::string name = solver_name(s);
+    if (restart_freq > 0) {
+      r = solve_restarted(s, R->op, x.get(), b.get(), max_iter, eps, restart_freq, verb, name);
+    } else {
+      r = solve_and_print(s, R->op, x.get(), b.get(), max_iter, eps, verb);
+    }
+    x.to_host(phi);
+    fill_info(info, r, name);
+  });
+}
+
+int host_cg_m(bool cplx, double** phi, double* phi0, int n_shift, int size, int resid_freq_check, int max_iter,
+              double eps, double* shifts, glx_matvec mv, void* extra, int worst_first, glx_verbose* verb,
+              glx_inversion_info* info) {
+  return guarded([&] {
+    if (!phi || !phi0 || !shifts) throw ArgError("phi, phi0 and shifts must not be NULL");
+    if (size <= 0 || n_shift <= 0) throw ArgError("size and n_shift must be positive");
+    auto R = resolve(mv, extra, size, cplx);
+    const size_t bytes = size_t(size) * elem_bytes(cplx);
+    DevVec b(phi0, bytes);
+    std::vector<std::unique_ptr<DevVec>> xs;
+    std::vector<void*> xp;
+    for (int k = 0; k < n_shift; k++) {
+      xs.emplace_back(new DevVec(nullptr, bytes));
+      xp.push_back(xs.back()->get());
+    }
+    Printer pr(verb);
+    const bool detail = pr.level() == GLX_VERB_DETAIL;
+    SolveOpts o;
+    o.max_iter = max_iter;
+    o.eps = eps;
+    o.want_hist = detail;
+    SolveResult r = dev_cg_m(R->op, xp.data(), b.get(), n_shift, shifts, resid_freq_check, worst_first != 0, o);
+    for (int k = 0; k < n_shift; k++) xs[k]->to_host(phi[k]);
+    if (detail)
+      for (int k = 0; k < int(r.relres_hist.size()); k++) pr.resid("CG-M", k + 1, 1 + k, r.relres_hist[k]);
+    std::vector<double> rel(n_shift);
+    for (int k = 0; k < n_shift; k++) rel[k] = std::sqrt(r.resSqmrhs[k]) / r.bsqrt;
+    pr.summary_multi(r.success, r.iter, r.ops_count, rel);
+    fill_info(info, r, "CG-M");
+    if (info && info->resSqmrhs)
+      for (int k = 0; k < n_shift && (info->n_rhs <= 0 || k < info->n_rhs); k++) info->resSqmrhs[k] = r.resSqmrhs[k];
+  });
+}
+
+// Direct application of a registered operator to host vectors.
+void host_apply(glx_matvec self, double* lhs, double* rhs, void* extra, bool cplx) {
+  int rc = guarded([&] {
+    auto R = resolve(self, extra, -1, cplx);
+    const size_t bytes = size_t(R->op.size()) * elem_bytes(cplx);
+    DevVec in(rhs, bytes), out(nullptr, bytes);
+    dev_apply(R->op, out.get(), in.get(), LINKS_F64, 0, nullptr);
+    out.to_host(lhs);
+  });
+  if (rc != GLX_OK) {
+    std::fprintf(stderr, "glx: operator application failed: %s\n", g_last_error.c_str());
+    // the callback contract has no error channel: make the failure visible
+    const glx_staggered_u1_op* p = static_cast<const glx_staggered_u1_op*>(extra);
+    if (p && p->x_fine > 0 && p->y_fine > 0) {
+      size_t n = size_t(p->x_fine) * p->y_fine * std::max(p->Nc, 1) * (cplx ? 2 : 1);
+      for (size_t i = 0; i < n; i++) lhs[i] = std::nan("");
+    }
+  }
+}
+
+// ----------------------------------------------- stencil coefficient build
+// operators_stencil.cpp:14-169: hopping blocks {+x, +y, -x, -y}; `sg` = +1
+// for D, -1 for D^dag; g5 multiplies by eps(x) = (-1)^(x+y).
+__global__ void k_build_stencil(int lx, int ly, const c128* __restrict__ u, c128* __restrict__ clover,
+                                c128* __restrict__ hop, double sg, int g5) {
+  const int V = lx * ly;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+    const int x = i % lx, y = i / lx;
+    const double eta = 1 - 2 * (x % 2);
+    const double eo = g5 ? (((x + y) % 2 == 0) ? 1.0 : -1.0) : 1.0;
+    const int xm = (x - 1 + lx) % lx, ym = (y - 1 + ly) % ly;
+    const double h = -0.5 * sg * eo;
+    clover[i] = {0.0, 0.0};
+    hop[i] = h * u[2 * i];
+    hop[V + i] = (h * eta) * u[2 * i + 1];
+    hop[2 * V + i] = (-h) * conj(u[2 * (y * lx + xm)]);
+    hop[3 * V + i] = (-h * eta) * conj(u[2 * (ym * lx + x) + 1]);
+  }
+}
+
+int build_stencil(glx_stencil_2d* st, const glx_staggered_u1_op* op, double sg, int g5) {
+  return guarded([&] {
+    if (!st || !op || !op->lattice || !st->clover || !st->hopping) throw ArgError("NULL stencil or operator");
+    const int lx = op->x_fine, ly = op->y_fine;
+    if (lx <= 0 || ly <= 0) throw ArgError("lattice dimensions must be positive");
+    const size_t V = size_t(lx) * ly;
+    DevBuf u, cl(V * 16), hp(4 * V * 16);
+    upload(u, op->lattice, 2 * V * 16);
+    hipLaunchKernelGGL(k_build_stencil, dim3(point_blocks(int(V))), dim3(kBlock), 0, nullptr, lx, ly, u.as<c128>(),
+                       cl.as<c128>(), hp.as<c128>(), sg, g5);
+    GLX_CHECK_LAUNCH();
+    GLX_HIP(hipMemcpy(st->clover, cl.get(), V * 16, hipMemcpyDeviceToHost));
+    GLX_HIP(hipMemcpy(st->hopping, hp.get(), 4 * V * 16, hipMemcpyDeviceToHost));
+    st->lx = lx;
+    st->ly = ly;
+    st->shift[0] = g5 ? 0.0 : op->mass;
+    st->shift[1] = 0.0;
+    st->eo_shift[0] = g5 ? op->mass : 0.0;
+    st->eo_shift[1] = 0.0;
+    st->dof_shift[0] = st->dof_shift[1] = 0.0;
+  });
+}
+
+// even/odd prepare and reconstruct on the device (operators.cpp:543-616,
+// operators_stencil.cpp:179-236): one operator application plus a pointwise
+// combination.
+__global__ void k_eo_prepare(int lx, int ly, c128 m, const c128* __restrict__ b, c128* __restrict__ out) {
+  const int V = lx * ly;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+    const int x = i % lx, y = i / lx;
+    if (((x + y) & 1) == 0) out[i] = m * b[i] - out[i];
+  }
+}
+__global__ void k_eo_reconstruct(int lx, int ly, double inv_mass, const c128* __restrict__ xe,
+                                 const c128* __restrict__ bo, c128* __restrict__ out) {
+  const int V = lx * ly;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+    const int x = i % lx, y = i / lx;
+    if (((x + y) & 1) == 1) out[i] = inv_mass * (bo[i] - out[i]);
+    else out[i] = xe[i];
+  }
+}
+
+void eo_prepare(glx_matvec deo_fn, double* rhs_e, double* rhs, void* extra, bool stencil) {
+  int rc = guarded([&] {
+    auto R = resolve(deo_fn, extra, -1, true);
+    const size_t V = size_t(R->op.lx) * R->op.ly;
+    DevVec b(rhs, V * 16), out(nullptr, V * 16);
+    dev_apply(R->op, out.get(), b.get(), LINKS_F64, 0, nullptr);
+    c128 m = stencil ? R->op.shift : c128{R->op.mass, 0.0};
+    hipLaunchKernelGGL(k_eo_prepare, dim3(point_blocks(int(V))), dim3(kBlock), 0, nullptr, R->op.lx, R->op.ly, m,
+                       b.buf.as<c128>(), out.buf.as<c128>());
+    GLX_CHECK_LAUNCH();
+    out.to_host(rhs_e);
+  });
+  if (rc != GLX_OK) std::fprintf(stderr, "glx: eoprec_prepare failed: %s\n", g_last_error.c_str());
+}
+void eo_reconstruct(glx_matvec doe_fn, double* full, double* lhs_e, double* rhs_o, void* extra, bool stencil) {
+  int rc = guarded([&] {
+    auto R = resolve(doe_fn, extra, -1, true);
+    const size_t V = size_t(R->op.lx) * R->op.ly;
+    DevVec xe(lhs_e, V * 16), bo(rhs_o, V * 16), out(nullptr, V * 16);
+    dev_apply(R->op, out.get(), xe.get(), LINKS_F64, 0, nullptr);
+    const double inv_mass = 1.0 / (stencil ? R->op.shift.re : R->op.mass);
+    hipLaunchKernelGGL(k_eo_reconstruct, dim3(point_blocks(int(V))), dim3(kBlock), 0, nullptr, R->op.lx, R->op.ly,
+                       inv_mass, xe.buf.as<c128>(), bo.buf.as<c128>(), out.buf.as<c128>());
+    GLX_CHECK_LAUNCH();
+    out.to_host(full);
+  });
+  if (rc != GLX_OK) std::fprintf(stderr, "glx: eoprec_reconstruct failed: %s\n", g_last_error.c_str());
+}
+
+}  // namespace
+
+// =========================================================== lattices ====
+struct glx_lattice_s {
+  int lx = 0, ly = 0, flags = 0;
+  DevBuf u64, u32, u16, sc16;
+};
+
+namespace {
+// FP32 / FP16 link copies from the FP64 links (on the device).
+__global__ void k_links_lowp(int n, const c128* __restrict__ u, c64* __restrict__ u32, HalfLink* __restrict__ u16,
+                             float* __restrict__ sc) {
+  for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const c128 a = u[2 * s], b = u[2 * s + 1];
+    if (u32) {
+      u32[2 * s] = {float(a.re), float(a.im)};
+      u32[2 * s + 1] = {float(b.re), float(b.im)};
+    }
+    if (u16) {
+      double m = fmax(fmax(fabs(a.re), fabs(a.im)), fmax(fabs(b.re), fabs(b.im)));
+      if (m == 0.0) m = 1.0;
+      const float scale = float(m);
+      const double inv = 1.0 / double(scale);
+      HalfLink h;
+      h.x = __floats2half2_rn(float(a.re * inv), float(a.im * inv));
+      h.y = __floats2half2_rn(float(b.re * inv), float(b.im * inv));
+      u16[s] = h;
+      sc[s] = scale;
+    }
+  }
+}
+
+void lattice_fill(glx_lattice_s* L, const double* links) {
+  const size_t V = size_t(L->lx) * L->ly;
+  GLX_HIP(hipMemcpy(L->u64.get(), links, 2 * V * 16, hipMemcpyHostToDevice));
+  if (L->flags & (GLX_LINKS_F32 | GLX_LINKS_F16)) {
+    hipLaunchKernelGGL(k_links_lowp, dim3(point_blocks(int(V))), dim3(kBlock), 0, nullptr, int(V),
+                       L->u64.as<c128>(), (L->flags & GLX_LINKS_F32) ? L->u32.as<c64>() : nullptr,
+                       (L->flags & GLX_LINKS_F16) ? L->u16.as<HalfLink>() : nullptr,
+                       (L->flags & GLX_LINKS_F16) ? L->sc16.as<float>() : nullptr);
+    GLX_CHECK_LAUNCH();
+  }
+  GLX_HIP(hipDeviceSynchronize());
+}
+
+DevOperator lattice_op(glx_lattice lat, int kind, double mass, int nc) {
+  DevOperator o;
+  o.kind = kind;
+  o.mass = mass;
+  o.nc = (kind == OP_LAPLACE || kind == OP_LAPLACE_REAL) ? std::max(nc, 1) : 1;
+  if (is_stencil_kind(kind)) throw ArgError("stencil_2d operators take coefficients: use the host entry points");
+  if (kind < 0 || kind > OP_M2MDEODOE) throw ArgError("unknown operator kind");
+  if (op_links(kind) && !lat) throw ArgError("this operator needs a lattice with links");
+  if (lat) {
+    o.lx = lat->lx;
+    o.ly = lat->ly;
+    o.u64 = lat->u64.as<c128>();
+    o.u32 = (lat->flags & GLX_LINKS_F32) ? lat->u32.as<c64>() : nullptr;
+    o.u16 = (lat->flags & GLX_LINKS_F16) ? lat->u16.as<HalfLink>() : nullptr;
+    o.sc16 = (lat->flags & GLX_LINKS_F16) ? lat->sc16.as<float>() : nullptr;
+  }
+  return o;
+}
+}  // namespace
+
+// ====================================================== exported C ABI ====
+extern "C" {
+
+void glx_square_laplace_real(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_laplace_real, lhs, rhs, extra, false); }
+void glx_square_laplace(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_laplace, lhs, rhs, extra, true); }
+void glx_square_laplace_u1(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_laplace_u1, lhs, rhs, extra, true); }
+void glx_square_staggered_u1(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_staggered_u1, lhs, rhs, extra, true); }
+void glx_gamma_5(double* lhs, double* rhs, void* extra) { host_apply(&glx_gamma_5, lhs, rhs, extra, true); }
+void glx_square_staggered_gamma5_u1(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_staggered_gamma5_u1, lhs, rhs, extra, true); }
+void glx_square_staggered_dagger_u1(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_staggered_dagger_u1, lhs, rhs, extra, true); }
+void glx_square_staggered_normal_u1(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_staggered_normal_u1, lhs, rhs, extra, true); }
+void glx_square_staggered_deo_u1(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_staggered_deo_u1, lhs, rhs, extra, true); }
+void glx_square_staggered_doe_u1(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_staggered_doe_u1, lhs, rhs, extra, true); }
+void glx_square_staggered_m2mdeodoe_u1(double* lhs, double* rhs, void* extra) { host_apply(&glx_square_staggered_m2mdeodoe_u1, lhs, rhs, extra, true); }
+void glx_square_staggered_eoprec_prepare(double* rhs_e, double* rhs_orig, void* extra) {
+  eo_prepare(&glx_square_staggered_deo_u1, rhs_e, rhs_orig, extra, false);
+}
+void glx_square_staggered_eoprec_reconstruct(double* lhs_full, double* lhs_e, double* rhs_o, void* extra) {
+  eo_reconstruct(&glx_square_staggered_doe_u1, lhs_full, lhs_e, rhs_o, extra, false);
+}
+
+int glx_get_square_staggered_u1_stencil(glx_stencil_2d* st, const glx_staggered_u1_op* op) { return build_stencil(st, op, +1.0, 0); }
+int glx_get_square_staggered_gamma5_u1_stencil(glx_stencil_2d* st, const glx_staggered_u1_op* op) { return build_stencil(st, op, +1.0, 1); }
+int glx_get_square_staggered_dagger_u1_stencil(glx_stencil_2d* st, const glx_staggered_u1_op* op) { return build_stencil(st, op, -1.0, 0); }
+void glx_apply_stencil_2d(double* lhs, double* rhs, void* st) { host_apply(&glx_apply_stencil_2d, lhs, rhs, st, true); }
+void glx_apply_stencil_2d_eo(double* lhs, double* rhs, void* st) { host_apply(&glx_apply_stencil_2d_eo, lhs, rhs, st, true); }
+void glx_apply_stencil_2d_oe(double* lhs, double* rhs, void* st) { host_apply(&glx_apply_stencil_2d_oe, lhs, rhs, st, true); }
+void glx_apply_square_staggered_m2mdeodoe_stencil(double* lhs, double* rhs, void* st) {
+  host_apply(&glx_apply_square_staggered_m2mdeodoe_stencil, lhs, rhs, st, true);
+}
+void glx_apply_square_staggered_eoprec_prepare_stencil(double* rhs_e, double* rhs_orig, void* st) {
+  eo_prepare(&glx_apply_stencil_2d_eo, rhs_e, rhs_orig, st, true);
+}
+void glx_apply_square_staggered_eoprec_reconstruct_stencil(double* lhs_full, double* lhs_e, double* rhs_o, void* st) {
+  eo_reconstruct(&glx_apply_stencil_2d_oe, lhs_full, lhs_e, rhs_o, st, true);
+}
+
+int glx_register_operator(glx_matvec callback, int kind, const glx_staggered_u1_op* params) {
+  return guarded([&] {
+    if (!callback) throw ArgError("callback is NULL");
+    if (kind < 0 || kind > OP_STENCIL_M2MDEODOE) throw ArgError("unknown operator kind");
+    Registry::get().add(reinterpret_cast<void*>(callback), kind, params);
+  });
+}
+int glx_unregister_operator(glx_matvec callback) {
+  return guarded([&] { Registry::get().remove(reinterpret_cast<void*>(callback)); });
+}
+
+#define GLX_SOLVE_ENTRY(NAME, S, CPLX)                                                                         \
+  int NAME(double* phi, double* phi0, int size, int max_iter, double eps, glx_matvec mv, void* extra,            \
+           glx_verbose* verb, glx_inversion_info* info) {                                                      \
+    return host_solve(S, CPLX, phi, phi0, size, max_iter, eps, 0, mv, extra, verb, info);                      \
+  }
+#define GLX_RESTART_ENTRY(NAME, S, CPLX)                                                                       \
+  int NAME(double* phi, double* phi0, int size, int max_iter, double eps, int restart_freq, glx_matvec mv,       \
+           void* extra, glx_verbose* verb, glx_inversion_info* info) {                                         \
+    if (restart_freq <= 0) {                                                                                   \
+      g_last_error = "restart_freq must be positive";                                                          \
+      return GLX_ERR_ARG;                                                                                      \
+    }                                                                                                          \
+    return host_solve(S, CPLX, phi, phi0, size, max_iter, eps, restart_freq, mv, extra, verb, info);           \
+  }
+GLX_SOLVE_ENTRY(glx_minv_vector_cg_r, S_CG, false)
+GLX_SOLVE_ENTRY(glx_minv_vector_cg_c, S_CG, true)
+GLX_SOLVE_ENTRY(glx_minv_vector_bicgstab_r, S_BICGSTAB, false)
+GLX_SOLVE_ENTRY(glx_minv_vector_bicgstab_c, S_BICGSTAB, true)
+GLX_SOLVE_ENTRY(glx_minv_vector_gcr_r, S_GCR, false)
+GLX_SOLVE_ENTRY(glx_minv_vector_gcr_c, S_GCR, true)
+GLX_RESTART_ENTRY(glx_minv_vector_cg_restart_r, S_CG, false)
+GLX_RESTART_ENTRY(glx_minv_vector_cg_restart_c, S_CG, true)
+GLX_RESTART_ENTRY(glx_minv_vector_bicgstab_restart_r, S_BICGSTAB, false)
+GLX_RESTART_ENTRY(glx_minv_vector_bicgstab_restart_c, S_BICGSTAB, true)
+GLX_RESTART_ENTRY(glx_minv_vector_gcr_restart_r, S_GCR, false)
+GLX_RESTART_ENTRY(glx_minv_vector_gcr_restart_c, S_GCR, true)
+#undef GLX_SOLVE_ENTRY
+#undef GLX_RESTART_ENTRY
+
+int glx_minv_vector_cg_m_r(double** phi, double* phi0, int n_shift, int size, int resid_freq_check, int max_iter,
+                           double eps, double* shifts, glx_matvec mv, void* extra, int worst_first,
+                           glx_verbose* verb, glx_inversion_info* info) {
+  return host_cg_m(false, phi, phi0, n_shift, size, resid_freq_check, max_iter, eps, shifts, mv, extra, worst_first,
+                   verb, info);
+}
+int glx_minv_vector_cg_m_c(double** phi, double* phi0, int n_shift, int size, int resid_freq_check, int max_iter,
+                           double eps, double* shifts, glx_matvec mv, void* extra, int worst_first,
+                           glx_verbose* verb, glx_inversion_info* info) {
+  return host_cg_m(true, phi, phi0, n_shift, size, resid_freq_check, max_iter, eps, shifts, mv, extra, worst_first,
+                   verb, info);
+}
+
+// ---------------------------------------------------------- device API --
+int glx_lattice_create(glx_lattice* lat, const double* links, int lx, int ly, int flags) {
+  return guarded([&] {
+    if (!lat || !links) throw ArgError("lat and links must not be NULL");
+    if (lx <= 0 || ly <= 0) throw ArgError("lattice dimensions must be positive");
+    auto L = std::make_unique<glx_lattice_s>();
+    L->lx = lx;
+    L->ly = ly;
+    L->flags = flags;
+    const size_t V = size_t(lx) * ly;
+    L->u64 = DevBuf(2 * V * 16);
+    if (flags & GLX_LINKS_F32) L->u32 = DevBuf(2 * V * 8);
+    if (flags & GLX_LINKS_F16) {
+      L->u16 = DevBuf(V * sizeof(HalfLink));
+      L->sc16 = DevBuf(V * sizeof(float));
+    }
+    lattice_fill(L.get(), links);
+    *lat = L.release();
+  });
+}
+int glx_lattice_update_links(glx_lattice lat, const double* links) {
+  return guarded([&] {
+    if (!lat || !links) throw ArgError("lat and links must not be NULL");
+    lattice_fill(lat, links);
+  });
+}
+int glx_lattice_destroy(glx_lattice lat) {
+  return guarded([&] {
+    GLX_HIP(hipDeviceSynchronize());
+    delete lat;
+  });
+}
+
+int glx_dev_solve(int solver, glx_lattice lat, int kind, double mass, int nc, double* x_dev, const double* b_dev,
+                  int max_iter, double eps, glx_verbose* verb, glx_inversion_info* info) {
+  return guarded([&] {
+    if (!x_dev || !b_dev) throw ArgError("x_dev and b_dev must not be NULL");
+    DevOperator o = lattice_op(lat, kind, mass, nc);
+    if (o.lx <= 0) throw ArgError("operator without links needs a lattice for its dimensions");
+    SolveResult r = solve_and_print(solver, o, x_dev, b_dev, max_iter, eps, verb);
+    fill_info(info, r, solver_name(solver));
+  });
+}
+
+int glx_dev_cg_m(glx_lattice lat, int kind, double mass, int nc, double** x_dev, const double* b_dev, int n_shift,
+                 const double* shifts, int resid_freq_check, int max_iter, double eps, int worst_first,
+                 glx_verbose* verb, glx_inversion_info* info) {
+  return guarded([&] {
+    if (!x_dev || !b_dev || !shifts) throw ArgError("x_dev, b_dev and shifts must not be NULL");
+    DevOperator o = lattice_op(lat, kind, mass, nc);
+    if (o.lx <= 0) throw ArgError("operator without links needs a lattice for its dimensions");
+    Printer pr(verb);
+    SolveOpts so;
+    so.max_iter = max_iter;
+    so.eps = eps;
+    so.want_hist = pr.level() == GLX_VERB_DETAIL;
+    std::vector<void*> xp(x_dev, x_dev + n_shift);
+    SolveResult r = dev_cg_m(o, xp.data(), b_dev, n_shift, shifts, resid_freq_check, worst_first != 0, so);
+    for (int k = 0; k < int(r.relres_hist.size()); k++) pr.resid("CG-M", k + 1, 1 + k, r.relres_hist[k]);
+    std::vector<double> rel(n_shift);
+    for (int k = 0; k < n_shift; k++) rel[k] = std::sqrt(r.resSqmrhs[k]) / r.bsqrt;
+    pr.summary_multi(r.success, r.iter, r.ops_count, rel);
+    fill_info(info, r, "CG-M");
+    if (info && info->resSqmrhs)
+      for (int k = 0; k < n_shift && (info->n_rhs <= 0 || k < info->n_rhs); k++) info->resSqmrhs[k] = r.resSqmrhs[k];
+  });
+}
+
+int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
+                     double eps, double delta, int link_prec, glx_verbose* verb, glx_inversion_info* info) {
+  return guarded([&] {
+    if (!x_dev || !b_dev) throw ArgError("x_dev and b_dev must not be NULL");
+    if (!(delta > 0.0 && delta < 1.0)) throw ArgError("delta must be in (0, 1)");
+    DevOperator o = lattice_op(lat, kind, mass, 1);
+    if (o.lx <= 0) throw ArgError("operator without links needs a lattice for its dimensions");
+    if (op_links(kind)) {
+      if (link_prec == LINKS_F16 && !o.u16) throw ArgError("lattice was created without GLX_LINKS_F16");
+      if (link_prec != LINKS_F16 && !o.u32) throw ArgError("lattice was created without GLX_LINKS_F32");
+    }
+    Printer pr(verb);
+    SolveOpts so;
+    so.max_iter = max_iter;
+    so.eps = eps;
+    so.want_hist = pr.level() == GLX_VERB_DETAIL;
+    MixedOpts mo;
+    mo.link_prec = link_prec == LINKS_F16 ? LINKS_F16 : LINKS_F32;
+    mo.delta = delta;
+    SolveResult r = dev_cg_mixed(o, x_dev, b_dev, so, mo);
+    for (int k = 0; k < int(r.relres_hist.size()); k++) pr.resid("CG-mixed", k + 1, 2 + k, r.relres_hist[k]);
+    pr.summary("CG-mixed", r.success, r.iter, r.ops_count, std::sqrt(r.resSq) / r.bsqrt);
+    fill_info(info, r, "CG-mixed");
+  });
+}
+
+int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int link_prec, void* out_dev,
+                  const void* in_dev) {
+  return guarded([&] {
+    if (!out_dev || !in_dev) throw ArgError("out_dev and in_dev must not be NULL");
+    DevOperator o = lattice_op(lat, kind, mass, nc);
+    if (o.lx <= 0) throw ArgError("operator without links needs a lattice for its dimensions");
+    dev_apply(o, out_dev, in_dev, link_prec, fp32, nullptr);
+  });
+}
+
+int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int link_prec, void* out_dev,
+                       const void* in_dev, int reps, double* ms_per_apply) {
+  return guarded([&] {
+    if (!out_dev || !in_dev || !ms_per_apply || reps <= 0) throw ArgError("bad arguments");
+    DevOperator o = lattice_op(lat, kind, mass, 1);
+    if (o.lx <= 0) throw ArgError("operator without links needs a lattice for its dimensions");
+    dev_apply(o, out_dev, in_dev, link_prec, fp32, nullptr);  // warm-up
+    hipEvent_t a, b;
+    GLX_HIP(hipEventCreate(&a));
+    GLX_HIP(hipEventCreate(&b));
+    RedSpace red;
+    red.init(kMaxPointBlocks, nullptr);
+    GLX_HIP(hipDeviceSynchronize());
+    GLX_HIP(hipEventRecord(a, nullptr));
+    for (int k = 0; k < reps; k++) {
+      if (fp32)
+        run_stencil<c64>(nullptr, o, link_prec, InPlain<c64>{static_cast<const c64*>(in_dev)},
+                         static_cast<c64*>(out_dev), EpiNone<c64>{}, FinNone{}, red.ws(), nullptr);
+      else if (kind == OP_LAPLACE_REAL)
+        run_stencil<double>(nullptr, o, LINKS_F64, InPlain<double>{static_cast<const double*>(in_dev)},
+                            static_cast<double*>(out_dev), EpiNone<double>{}, FinNone{}, red.ws(), nullptr);
+      else
+        run_stencil<c128>(nullptr, o, LINKS_F64, InPlain<c128>{static_cast<const c128*>(in_dev)},
+                          static_cast<c128*>(out_dev), EpiNone<c128>{}, FinNone{}, red.ws(), nullptr);
+    }
+    GLX_HIP(hipEventRecord(b, nullptr));
+    GLX_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    GLX_HIP(hipEventElapsedTime(&ms, a, b));
+    *ms_per_apply = double(ms) / reps;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  });
+}
+
+int glx_dev_alloc(void** ptr, size_t bytes) {
+  return guarded([&] {
+    if (!ptr) throw ArgError("ptr is NULL");
+    GLX_HIP(hipMalloc(ptr, bytes));
+  });
+}
+int glx_dev_free(void* ptr) {
+  return guarded([&] { GLX_HIP(hipFree(ptr)); });
+}
+int glx_memcpy_htod(void* dst, const void* src, size_t bytes) {
+  return guarded([&] { GLX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int glx_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+  return guarded([&] { GLX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int glx_device_synchronize(void) {
+  return guarded([&] { GLX_HIP(hipDeviceSynchronize()); });
+}
+int glx_device_count(int* count) {
+  return guarded([&] {
+    if (!count) throw ArgError("count is NULL");
+    *count = 0;
+    GLX_HIP(hipGetDeviceCount(count));
+  });
+}
+int glx_release_cache(void) {
+  return guarded([&] { DevicePool::get().trim(); });
+}
+const char* glx_last_error(void) { return g_last_error.c_str(); }
+int glx_abi_version(void) { return GLX_ABI_VERSION; }
+
+}  // extern "C"

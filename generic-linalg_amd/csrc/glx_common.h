@@ -1,0 +1,251 @@
+// glx_common.h -- device-side building blocks shared by every kernel of the
+// library: complex arithmetic for the two precisions, wave64 + LDS
+// reductions, the deterministic last-block finalisation that turns per-block
+// partial sums into solver scalars on the device, and HIP error handling.
+//
+// Target: gfx950 (MI355X), wave64.  All reductions accumulate in FP64 and are
+// bit-reproducible for a given launch shape: partial sums are combined in a
+// fixed order by the last block to finish, never with floating-point atomics.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace glx {
+
+// ---------------------------------------------------------------- errors --
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+struct ArgError : std::runtime_error {
+  explicit ArgError(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define GLX_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      throw ::glx::HipError(std::string(#call) + ": " + hipGetErrorString(e_) + " (" __FILE__ \
+                            ":" + std::to_string(__LINE__) + ")");                             \
+  } while (0)
+
+#define GLX_CHECK_LAUNCH() GLX_HIP(hipGetLastError())
+
+// --------------------------------------------------------------- complex --
+template <class R>
+struct alignas(2 * sizeof(R)) cx {
+  R re, im;
+};
+using c128 = cx<double>;
+using c64 = cx<float>;
+
+template <class R>
+__host__ __device__ __forceinline__ cx<R> mk(R a, R b) { return cx<R>{a, b}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator+(cx<R> a, cx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator-(cx<R> a, cx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator-(cx<R> a) { return {-a.re, -a.im}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator*(cx<R> a, cx<R> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <class R>
+__host__ __device__ __forceinline__ cx<R> operator*(R t, cx<R> z) { return {t * z.re, t * z.im}; }
+template <class R>
+__host__ __device__ __forceinline__ cx<R> conj(cx<R> z) { return {z.re, -z.im}; }
+// conj(a) * b
+template <class R>
+__host__ __device__ __forceinline__ cx<R> cmulc(cx<R> a, cx<R> b) {
+  return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+// complex division by Smith's method (the algorithm libgcc's __divdc3 uses for
+// finite operands)
+template <class R>
+__host__ __device__ __forceinline__ cx<R> cdiv(cx<R> a, cx<R> b) {
+  R c = b.re, d = b.im;
+  if (fabs(c) < fabs(d)) {
+    R ratio = c / d, denom = c * ratio + d;
+    return {(a.re * ratio + a.im) / denom, (a.im * ratio - a.re) / denom};
+  }
+  R ratio = d / c, denom = d * ratio + c;
+  return {(a.im * ratio + a.re) / denom, (a.im - a.re * ratio) / denom};
+}
+
+// Uniform helpers so that solver kernels are written once for real and
+// complex element types.  `Acc<T>` is the FP64 accumulation type of a dot.
+template <class T> struct Acc;
+template <> struct Acc<double> { using type = double; };
+template <> struct Acc<float> { using type = double; };
+template <> struct Acc<c128> { using type = c128; };
+template <> struct Acc<c64> { using type = c128; };
+
+template <class T> struct Real;
+template <> struct Real<double> { using type = double; };
+template <> struct Real<float> { using type = float; };
+template <> struct Real<c128> { using type = double; };
+template <> struct Real<c64> { using type = float; };
+
+template <class T> struct is_cx { static constexpr bool value = false; };
+template <class R> struct is_cx<cx<R>> { static constexpr bool value = true; };
+
+__host__ __device__ __forceinline__ double norm2(double a) { return a * a; }
+__host__ __device__ __forceinline__ double norm2(float a) { return double(a) * double(a); }
+__host__ __device__ __forceinline__ double norm2(c128 a) { return a.re * a.re + a.im * a.im; }
+__host__ __device__ __forceinline__ double norm2(c64 a) {
+  double r = a.re, i = a.im;
+  return r * r + i * i;
+}
+// conj(a)*b accumulated in FP64
+__host__ __device__ __forceinline__ double dotc(double a, double b) { return a * b; }
+__host__ __device__ __forceinline__ double dotc(float a, float b) { return double(a) * double(b); }
+__host__ __device__ __forceinline__ c128 dotc(c128 a, c128 b) { return cmulc(a, b); }
+__host__ __device__ __forceinline__ c128 dotc(c64 a, c64 b) {
+  c128 x{a.re, a.im}, y{b.re, b.im};
+  return cmulc(x, y);
+}
+
+// Scalars carried on the device are FP64 (complex or real); `S<T>` is the
+// scalar type that multiplies a T element, `scal(s, v)` applies it in T's
+// precision.
+template <class T> struct Scal;
+template <> struct Scal<double> { using type = double; };
+template <> struct Scal<float> { using type = double; };
+template <> struct Scal<c128> { using type = c128; };
+template <> struct Scal<c64> { using type = c128; };
+
+__host__ __device__ __forceinline__ double scal(double s, double v) { return s * v; }
+__host__ __device__ __forceinline__ float scal(double s, float v) { return float(s) * v; }
+__host__ __device__ __forceinline__ c128 scal(c128 s, c128 v) { return s * v; }
+__host__ __device__ __forceinline__ c64 scal(c128 s, c64 v) {
+  c64 t{float(s.re), float(s.im)};
+  return t * v;
+}
+__host__ __device__ __forceinline__ double rscal(double s, double v) { return s * v; }
+__host__ __device__ __forceinline__ float rscal(double s, float v) { return float(s) * v; }
+__host__ __device__ __forceinline__ c128 rscal(double s, c128 v) { return s * v; }
+__host__ __device__ __forceinline__ c64 rscal(double s, c64 v) { return float(s) * v; }
+
+template <class T> __host__ __device__ __forceinline__ T zero_of();
+template <> __host__ __device__ __forceinline__ double zero_of<double>() { return 0.0; }
+template <> __host__ __device__ __forceinline__ float zero_of<float>() { return 0.0f; }
+template <> __host__ __device__ __forceinline__ c128 zero_of<c128>() { return {0.0, 0.0}; }
+template <> __host__ __device__ __forceinline__ c64 zero_of<c64>() { return {0.0f, 0.0f}; }
+
+// Conversion between precisions of the same kind.
+__host__ __device__ __forceinline__ void cvt(double& d, double s) { d = s; }
+__host__ __device__ __forceinline__ void cvt(float& d, double s) { d = float(s); }
+__host__ __device__ __forceinline__ void cvt(double& d, float s) { d = s; }
+__host__ __device__ __forceinline__ void cvt(float& d, float s) { d = s; }
+__host__ __device__ __forceinline__ void cvt(c128& d, c128 s) { d = s; }
+__host__ __device__ __forceinline__ void cvt(c64& d, c128 s) { d = {float(s.re), float(s.im)}; }
+__host__ __device__ __forceinline__ void cvt(c128& d, c64 s) { d = {s.re, s.im}; }
+__host__ __device__ __forceinline__ void cvt(c64& d, c64 s) { d = s; }
+
+// ------------------------------------------------------------ reductions --
+constexpr int kBlock = 256;         // threads per block everywhere
+constexpr int kWaves = kBlock / 64; // wave64
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum NR values over the block; result valid in thread 0.  Fixed order.
+template <int NR>
+__device__ __forceinline__ void block_sum(double (&v)[NR], double (*sm)[kWaves]) {
+#pragma unroll
+  for (int k = 0; k < NR; k++) v[k] = wave_sum(v[k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NR; k++) sm[k][wave] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NR; k++) {
+      double s = 0.0;
+#pragma unroll
+      for (int w = 0; w < kWaves; w++) s += sm[k][w];
+      v[k] = s;
+    }
+  }
+}
+
+// Reduction workspace: partial sums [nblocks * NR] and one arrival counter.
+struct RedWs {
+  double* partials;
+  unsigned* counter;
+};
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block-level part of a grid reduction.  Every block of the grid must call it
+// exactly once with its per-thread partials.  The last block to arrive sums
+// the per-block partials in block order (deterministic), then thread 0 calls
+// fin(total) and re-arms the counter.  `bid`/`nblk` are the linear block id
+// and count of the grid.
+template <int NR, class Fin>
+__device__ __forceinline__ void grid_reduce(double (&v)[NR], RedWs ws, int bid, int nblk, Fin fin) {
+  __shared__ double sm[NR > 0 ? NR : 1][kWaves];
+  __shared__ int last;
+  block_sum<NR>(v, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NR; k++)
+      __hip_atomic_store(&ws.partials[bid * NR + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned t = atomicAdd(ws.counter, 1u);
+    last = (t == unsigned(nblk - 1));
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double acc[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int k = 0; k < NR; k++) acc[k] = 0.0;
+  // each thread sums a fixed strided subset in increasing block order
+  for (int b = threadIdx.x; b < nblk; b += kBlock) {
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] += load_coherent(&ws.partials[b * NR + k]);
+  }
+  __syncthreads();
+  block_sum<NR>(acc, sm);
+  if (threadIdx.x == 0) {
+    fin(acc);
+    __threadfence();
+    atomicExch(ws.counter, 0u);
+  }
+}
+
+// -------------------------------------------------------------- geometry --
+// Lattice tile of one block for the stencil kernels.
+constexpr int kTX = 64;
+constexpr int kTY = 8;
+
+// Periodic wrap of a coordinate that lies in [-2, n + kTX + 2).
+__device__ __forceinline__ int wrap_coord(int v, int n) {
+  if (v < 0) v += n;
+  if (v >= n) v -= n;
+  if (v >= n || v < 0) v = ((v % n) + n) % n;
+  return v;
+}
+
+// Linear block id remapped so that each of the 8 XCDs works on a contiguous
+// range of tiles (the hardware deals consecutive block ids round-robin to the
+// XCDs); neighbouring tiles then share an L2.
+__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
+  const int per = (nblk + 7) >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+
+}  // namespace glx

@@ -1,0 +1,221 @@
+// glx_compat.cpp -- the reference's C++ interface (generic_cg.h,
+// generic_cg_m.h, generic_bicgstab.h, generic_gcr.h, operators.h) implemented
+// over the C ABI, so that existing drivers recompile against
+// generic-linalg_amd/compat/ and link libglx.so in place of the reference's
+// object files.  The operator functions defined here are registered as device
+// operators: a solver given one of them runs entirely on the GPU.
+#include <complex>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/glx.h"
+#include "../compat/generic_inverters.h"
+#include "../compat/operators.h"
+
+namespace {
+
+typedef void (*rmv)(double*, double*, void*);
+typedef void (*cmv)(complex<double>*, complex<double>*, void*);
+
+glx_verbose* to_c(inversion_verbose_struct* v, glx_verbose& out) {
+  std::memset(&out, 0, sizeof(out));
+  if (!v) return nullptr;
+  out.verbosity = v->verbosity;
+  out.verb_prefix = v->verb_prefix.c_str();
+  out.precond_verbosity = v->precond_verbosity;
+  out.precond_verb_prefix = v->precond_verb_prefix.c_str();
+  return &out;
+}
+
+void check(int rc, const char* what) {
+  if (rc != GLX_OK) throw std::runtime_error(std::string(what) + ": " + glx_last_error());
+}
+
+inversion_info to_info(const glx_inversion_info& c) {
+  inversion_info r;
+  r.resSq = c.resSq;
+  r.iter = c.iter;
+  r.success = c.success != 0;
+  r.name = c.name;
+  r.ops_count = c.ops_count;
+  return r;
+}
+
+double* D(complex<double>* p) { return reinterpret_cast<double*>(p); }
+glx_matvec M(rmv f) { return reinterpret_cast<glx_matvec>(f); }
+glx_matvec M(cmv f) { return reinterpret_cast<glx_matvec>(f); }
+
+typedef int (*csolve)(double*, double*, int, int, double, glx_matvec, void*, glx_verbose*, glx_inversion_info*);
+typedef int (*crestart)(double*, double*, int, int, double, int, glx_matvec, void*, glx_verbose*,
+                        glx_inversion_info*);
+
+inversion_info call(csolve f, const char* what, double* phi, double* phi0, int size, int max_iter, double res,
+                    glx_matvec mv, void* extra, inversion_verbose_struct* verb) {
+  glx_verbose cv;
+  glx_inversion_info ci;
+  std::memset(&ci, 0, sizeof(ci));
+  check(f(phi, phi0, size, max_iter, res, mv, extra, to_c(verb, cv), &ci), what);
+  return to_info(ci);
+}
+inversion_info call_r(crestart f, const char* what, double* phi, double* phi0, int size, int max_iter, double res,
+                      int restart_freq, glx_matvec mv, void* extra, inversion_verbose_struct* verb) {
+  glx_verbose cv;
+  glx_inversion_info ci;
+  std::memset(&ci, 0, sizeof(ci));
+  check(f(phi, phi0, size, max_iter, res, restart_freq, mv, extra, to_c(verb, cv), &ci), what);
+  return to_info(ci);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- solvers ----
+inversion_info minv_vector_cg(double* phi, double* phi0, int size, int max_iter, double res, rmv mv, void* extra,
+                              inversion_verbose_struct* verb) {
+  return call(glx_minv_vector_cg_r, "minv_vector_cg", phi, phi0, size, max_iter, res, M(mv), extra, verb);
+}
+inversion_info minv_vector_cg(complex<double>* phi, complex<double>* phi0, int size, int max_iter, double res, cmv mv,
+                              void* extra, inversion_verbose_struct* verb) {
+  return call(glx_minv_vector_cg_c, "minv_vector_cg", D(phi), D(phi0), size, max_iter, res, M(mv), extra, verb);
+}
+inversion_info minv_vector_cg_restart(double* phi, double* phi0, int size, int max_iter, double res,
+                                      int restart_freq, rmv mv, void* extra, inversion_verbose_struct* verb) {
+  return call_r(glx_minv_vector_cg_restart_r, "minv_vector_cg_restart", phi, phi0, size, max_iter, res, restart_freq,
+                M(mv), extra, verb);
+}
+inversion_info minv_vector_cg_restart(complex<double>* phi, complex<double>* phi0, int size, int max_iter,
+                                      double res, int restart_freq, cmv mv, void* extra,
+                                      inversion_verbose_struct* verb) {
+  return call_r(glx_minv_vector_cg_restart_c, "minv_vector_cg_restart", D(phi), D(phi0), size, max_iter, res,
+                restart_freq, M(mv), extra, verb);
+}
+inversion_info minv_vector_bicgstab(double* phi, double* phi0, int size, int max_iter, double res, rmv mv,
+                                    void* extra, inversion_verbose_struct* verb) {
+  return call(glx_minv_vector_bicgstab_r, "minv_vector_bicgstab", phi, phi0, size, max_iter, res, M(mv), extra, verb);
+}
+inversion_info minv_vector_bicgstab(complex<double>* phi, complex<double>* phi0, int size, int max_iter, double res,
+                                    cmv mv, void* extra, inversion_verbose_struct* verb) {
+  return call(glx_minv_vector_bicgstab_c, "minv_vector_bicgstab", D(phi), D(phi0), size, max_iter, res, M(mv), extra,
+              verb);
+}
+inversion_info minv_vector_bicgstab_restart(double* phi, double* phi0, int size, int max_iter, double res,
+                                            int restart_freq, rmv mv, void* extra, inversion_verbose_struct* verb) {
+  return call_r(glx_minv_vector_bicgstab_restart_r, "minv_vector_bicgstab_restart", phi, phi0, size, max_iter, res,
+                restart_freq, M(mv), extra, verb);
+}
+inversion_info minv_vector_bicgstab_restart(complex<double>* phi, complex<double>* phi0, int size, int max_iter,
+                                            double res, int restart_freq, cmv mv, void* extra,
+                                            inversion_verbose_struct* verb) {
+  return call_r(glx_minv_vector_bicgstab_restart_c, "minv_vector_bicgstab_restart", D(phi), D(phi0), size, max_iter,
+                res, restart_freq, M(mv), extra, verb);
+}
+inversion_info minv_vector_gcr(double* phi, double* phi0, int size, int max_iter, double res, rmv mv, void* extra,
+                               inversion_verbose_struct* verb) {
+  return call(glx_minv_vector_gcr_r, "minv_vector_gcr", phi, phi0, size, max_iter, res, M(mv), extra, verb);
+}
+inversion_info minv_vector_gcr(complex<double>* phi, complex<double>* phi0, int size, int max_iter, double res,
+                               cmv mv, void* extra, inversion_verbose_struct* verb) {
+  return call(glx_minv_vector_gcr_c, "minv_vector_gcr", D(phi), D(phi0), size, max_iter, res, M(mv), extra, verb);
+}
+inversion_info minv_vector_gcr_restart(double* phi, double* phi0, int size, int max_iter, double res,
+                                       int restart_freq, rmv mv, void* extra, inversion_verbose_struct* verb) {
+  return call_r(glx_minv_vector_gcr_restart_r, "minv_vector_gcr_restart", phi, phi0, size, max_iter, res,
+                restart_freq, M(mv), extra, verb);
+}
+inversion_info minv_vector_gcr_restart(complex<double>* phi, complex<double>* phi0, int size, int max_iter,
+                                       double res, int restart_freq, cmv mv, void* extra,
+                                       inversion_verbose_struct* verb) {
+  return call_r(glx_minv_vector_gcr_restart_c, "minv_vector_gcr_restart", D(phi), D(phi0), size, max_iter, res,
+                restart_freq, M(mv), extra, verb);
+}
+
+inversion_info minv_vector_cg_m(double** phi, double* phi0, int n_shift, int size, int resid_freq_check,
+                                int max_iter, double eps, double* shifts, rmv mv, void* extra, bool worst_first,
+                                inversion_verbose_struct* verb) {
+  inversion_info r(n_shift);
+  glx_verbose cv;
+  glx_inversion_info ci;
+  std::memset(&ci, 0, sizeof(ci));
+  ci.resSqmrhs = r.resSqmrhs;
+  ci.n_rhs = n_shift;
+  check(glx_minv_vector_cg_m_r(phi, phi0, n_shift, size, resid_freq_check, max_iter, eps, shifts, M(mv), extra,
+                               worst_first ? 1 : 0, to_c(verb, cv), &ci),
+        "minv_vector_cg_m");
+  r.resSq = ci.resSq;
+  r.iter = ci.iter;
+  r.success = ci.success != 0;
+  r.name = ci.name;
+  r.ops_count = ci.ops_count;
+  return r;
+}
+inversion_info minv_vector_cg_m(complex<double>** phi, complex<double>* phi0, int n_shift, int size,
+                                int resid_freq_check, int max_iter, double eps, double* shifts, cmv mv, void* extra,
+                                bool worst_first, inversion_verbose_struct* verb) {
+  inversion_info r(n_shift);
+  glx_verbose cv;
+  glx_inversion_info ci;
+  std::memset(&ci, 0, sizeof(ci));
+  ci.resSqmrhs = r.resSqmrhs;
+  ci.n_rhs = n_shift;
+  check(glx_minv_vector_cg_m_c(reinterpret_cast<double**>(phi), D(phi0), n_shift, size, resid_freq_check, max_iter,
+                               eps, shifts, M(mv), extra, worst_first ? 1 : 0, to_c(verb, cv),
+                               &ci),
+        "minv_vector_cg_m");
+  r.resSq = ci.resSq;
+  r.iter = ci.iter;
+  r.success = ci.success != 0;
+  r.name = ci.name;
+  r.ops_count = ci.ops_count;
+  return r;
+}
+
+// ----------------------------------------------------------- operators ----
+void square_laplace(complex<double>* l, complex<double>* r, void* e) { glx_square_laplace(D(l), D(r), e); }
+void square_laplace_u1(complex<double>* l, complex<double>* r, void* e) { glx_square_laplace_u1(D(l), D(r), e); }
+void square_staggered_u1(complex<double>* l, complex<double>* r, void* e) { glx_square_staggered_u1(D(l), D(r), e); }
+void gamma_5(complex<double>* l, complex<double>* r, void* e) { glx_gamma_5(D(l), D(r), e); }
+void square_staggered_gamma5_u1(complex<double>* l, complex<double>* r, void* e) {
+  glx_square_staggered_gamma5_u1(D(l), D(r), e);
+}
+void square_staggered_dagger_u1(complex<double>* l, complex<double>* r, void* e) {
+  glx_square_staggered_dagger_u1(D(l), D(r), e);
+}
+void square_staggered_normal_u1(complex<double>* l, complex<double>* r, void* e) {
+  glx_square_staggered_normal_u1(D(l), D(r), e);
+}
+void square_staggered_deo_u1(complex<double>* l, complex<double>* r, void* e) { glx_square_staggered_deo_u1(D(l), D(r), e); }
+void square_staggered_doe_u1(complex<double>* l, complex<double>* r, void* e) { glx_square_staggered_doe_u1(D(l), D(r), e); }
+void square_staggered_m2mdeodoe_u1(complex<double>* l, complex<double>* r, void* e) {
+  glx_square_staggered_m2mdeodoe_u1(D(l), D(r), e);
+}
+void square_staggered_eoprec_prepare(complex<double>* rhs_e, complex<double>* rhs_orig, void* e) {
+  glx_square_staggered_eoprec_prepare(D(rhs_e), D(rhs_orig), e);
+}
+void square_staggered_eoprec_reconstruct(complex<double>* full, complex<double>* lhs_e, complex<double>* rhs_o,
+                                         void* e) {
+  glx_square_staggered_eoprec_reconstruct(D(full), D(lhs_e), D(rhs_o), e);
+}
+void square_laplace(double* l, double* r, void* e) { glx_square_laplace_real(l, r, e); }
+
+namespace {
+// Register the C++ operator functions so that solvers recognise them.
+struct RegisterCompat {
+  RegisterCompat() {
+    cmv c;
+    rmv rl = square_laplace;
+    glx_register_operator(M(rl), GLX_OP_LAPLACE_REAL, nullptr);
+    c = square_laplace;
+    glx_register_operator(M(c), GLX_OP_LAPLACE, nullptr);
+    glx_register_operator(M(square_laplace_u1), GLX_OP_LAPLACE_U1, nullptr);
+    glx_register_operator(M(square_staggered_u1), GLX_OP_STAGGERED, nullptr);
+    glx_register_operator(M(gamma_5), GLX_OP_GAMMA5, nullptr);
+    glx_register_operator(M(square_staggered_gamma5_u1), GLX_OP_STAGGERED_GAMMA5, nullptr);
+    glx_register_operator(M(square_staggered_dagger_u1), GLX_OP_STAGGERED_DAGGER, nullptr);
+    glx_register_operator(M(square_staggered_normal_u1), GLX_OP_STAGGERED_NORMAL, nullptr);
+    glx_register_operator(M(square_staggered_deo_u1), GLX_OP_DEO, nullptr);
+    glx_register_operator(M(square_staggered_doe_u1), GLX_OP_DOE, nullptr);
+    glx_register_operator(M(square_staggered_m2mdeodoe_u1), GLX_OP_M2MDEODOE, nullptr);
+  }
+};
+RegisterCompat g_register_compat;
+}  // namespace

@@ -1,0 +1,232 @@
+// glx_runtime.h -- host-side runtime of the device solvers: a caching device
+// allocator, the per-solve workspace (reduction partials, scalar state,
+// pinned mirror), the operator descriptor and its dispatch onto the stencil
+// kernel template, and the generic pointwise kernel with an optional grid
+// reduction.
+#pragma once
+
+#include "glx_stencil.h"
+
+#include <functional>
+#include <map>
+#include <mutex>
+#include <vector>
+
+namespace glx {
+
+// ------------------------------------------------------------- allocator --
+// Grow-only cache of device allocations keyed by size; blocks are reused by
+// later solves instead of going back to hipMalloc.
+class DevicePool {
+ public:
+  static DevicePool& get();
+  void* alloc(size_t bytes);
+  void release(void* p);
+  void trim();  // frees every cached block
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> live_;
+};
+
+// RAII device buffer from the pool.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) : bytes_(bytes) { p_ = bytes ? DevicePool::get().alloc(bytes) : nullptr; }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    reset();
+    p_ = o.p_;
+    bytes_ = o.bytes_;
+    o.p_ = nullptr;
+    return *this;
+  }
+  ~DevBuf() { reset(); }
+  void reset() {
+    if (p_) DevicePool::get().release(p_);
+    p_ = nullptr;
+  }
+  template <class T>
+  T* as() const { return static_cast<T*>(p_); }
+  void* get() const { return p_; }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Upper bound on blocks of any reducing launch (pointwise kernels use at most
+// kMaxPointBlocks; stencil launches use one block per 64x8 tile).
+constexpr int kMaxPointBlocks = 2048;
+constexpr int kMaxNR = 16;
+
+// Reduction workspace for one solve.  Counters are zeroed once and re-armed
+// by the kernels themselves.
+struct RedSpace {
+  DevBuf partials, counter;
+  void init(int max_blocks, hipStream_t st) {
+    partials = DevBuf(size_t(max_blocks) * kMaxNR * sizeof(double));
+    counter = DevBuf(64);
+    GLX_HIP(hipMemsetAsync(counter.get(), 0, 64, st));
+  }
+  RedWs ws() const { return RedWs{partials.as<double>(), counter.as<unsigned>()}; }
+};
+
+// -------------------------------------------------------------- operator --
+// A device operator: kind + lattice + device-resident links (FP64, plus the
+// FP32 / FP16 copies used by the mixed-precision solver) or stencil
+// coefficients.  `host_cb` instead marks an unrecognised host callback.
+struct DevOperator {
+  int kind = -1;
+  int lx = 0, ly = 0, nc = 1;
+  double mass = 0.0;
+  const c128* u64 = nullptr;
+  const c64* u32 = nullptr;
+  const HalfLink* u16 = nullptr;
+  const float* sc16 = nullptr;
+  const c128* clover = nullptr;
+  const c128* hop = nullptr;
+  c128 shift{0, 0}, eo_shift{0, 0}, dof_shift{0, 0};
+  // host callback fallback
+  void (*host_cb)(double*, double*, void*) = nullptr;
+  void* host_extra = nullptr;
+  bool host_complex = true;
+  int size_ = 0;
+
+  int size() const { return host_cb ? size_ : lx * ly * nc; }
+  bool is_host() const { return host_cb != nullptr; }
+  bool tiled() const { return !host_cb && nc == 1; }
+};
+
+enum LinkPrec : int { LINKS_F64 = 0, LINKS_F32 = 1, LINKS_F16 = 2 };
+
+template <class LK>
+inline OpDev<LK> make_opdev(const DevOperator& o, LK lk) {
+  OpDev<LK> d;
+  d.lx = o.lx;
+  d.ly = o.ly;
+  d.mass = o.mass;
+  d.links = lk;
+  d.clover = o.clover;
+  d.hop = o.hop;
+  d.shift = o.shift;
+  d.eo_shift = o.eo_shift;
+  d.dof_shift = o.dof_shift;
+  return d;
+}
+
+template <class T>
+constexpr bool kind_valid_for(int k) {
+  if constexpr (is_cx<T>::value) return k != OP_LAPLACE_REAL;
+  else return k == OP_LAPLACE_REAL;
+}
+
+// Launch the tiled stencil kernel for a runtime operator kind.
+template <class T, class LK, class In, class Epi, class Fin>
+void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T* out, Epi epi, Fin fin, RedWs ws,
+                   const int* done) {
+  const Geom g = make_geom(o.lx, o.ly);
+  const OpDev<LK> od = make_opdev(o, lk);
+#define GLX_CASE(K)                                                                     \
+  case K:                                                                               \
+    if constexpr (kind_valid_for<T>(K)) {                                               \
+      launch_stencil<K, T, LK, In, Epi, Fin>(st, g, od, in, out, epi, fin, ws, done);   \
+      return;                                                                           \
+    }                                                                                   \
+    break;
+  switch (o.kind) {
+    GLX_CASE(OP_LAPLACE_REAL)
+    GLX_CASE(OP_LAPLACE)
+    GLX_CASE(OP_LAPLACE_U1)
+    GLX_CASE(OP_STAG)
+    GLX_CASE(OP_STAG_DAG)
+    GLX_CASE(OP_STAG_NORMAL)
+    GLX_CASE(OP_STAG_G5)
+    GLX_CASE(OP_GAMMA5)
+    GLX_CASE(OP_DEO)
+    GLX_CASE(OP_DOE)
+    GLX_CASE(OP_M2MDEODOE)
+    GLX_CASE(OP_STENCIL)
+    GLX_CASE(OP_STENCIL_EO)
+    GLX_CASE(OP_STENCIL_OE)
+    GLX_CASE(OP_STENCIL_M2MDEODOE)
+    default:
+      break;
+  }
+#undef GLX_CASE
+  throw ArgError("operator kind " + std::to_string(o.kind) + " is not available for this element type");
+}
+
+// Stencil launch with the link precision chosen for T: FP64 links for FP64
+// vectors; FP32 or FP16 links for FP32 vectors.
+template <class T, class In, class Epi, class Fin>
+void run_stencil(hipStream_t st, const DevOperator& o, int link_prec, In in, T* out, Epi epi, Fin fin, RedWs ws,
+                 const int* done) {
+  if (op_links(o.kind)) {
+    if constexpr (std::is_same<typename Real<T>::type, double>::value) {
+      if (!o.u64) throw ArgError("operator has no FP64 links on the device");
+      dispatch_kind<T>(st, o, LinksF64{o.u64}, in, out, epi, fin, ws, done);
+    } else {
+      if (link_prec == LINKS_F16) {
+        if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");
+        dispatch_kind<T>(st, o, LinksF16{o.u16, o.sc16}, in, out, epi, fin, ws, done);
+      } else {
+        if (!o.u32) throw ArgError("operator has no FP32 links on the device");
+        dispatch_kind<T>(st, o, LinksF32{o.u32}, in, out, epi, fin, ws, done);
+      }
+    }
+  } else {
+    if constexpr (std::is_same<typename Real<T>::type, double>::value) {
+      dispatch_kind<T>(st, o, LinksF64{nullptr}, in, out, epi, fin, ws, done);
+    } else {
+      dispatch_kind<T>(st, o, LinksF32{nullptr}, in, out, epi, fin, ws, done);
+    }
+  }
+}
+
+// ------------------------------------------------------ pointwise kernel --
+// f(i, part) updates element i and adds its share to the F::NR partial sums
+// (part has exactly that many entries).
+template <int NR, class F, class Fin>
+__global__ __launch_bounds__(kBlock) void k_point(int n, F f, Fin fin, RedWs ws, const int* __restrict__ done) {
+  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  f.init();
+  double part[NR > 0 ? NR : 1];
+#pragma unroll
+  for (int k = 0; k < NR; k++) part[k] = 0.0;
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) f(i, part);
+  if constexpr (NR > 0) grid_reduce<NR>(part, ws, blockIdx.x, gridDim.x, fin);
+}
+
+inline int point_blocks(int n) {
+  int b = (n + kBlock - 1) / kBlock;
+  if (b > kMaxPointBlocks) b = kMaxPointBlocks;
+  if (b < 1) b = 1;
+  return b;
+}
+
+// The number of partial sums is the functor's own F::NR, so a launch can
+// never disagree with what the functor accumulates.
+template <class F, class Fin>
+void launch_point(hipStream_t st, int n, F f, Fin fin, RedWs ws, const int* done) {
+  static_assert(F::NR >= 0 && F::NR <= kMaxNR, "too many partial sums");
+  hipLaunchKernelGGL((k_point<F::NR, F, Fin>), dim3(point_blocks(n)), dim3(kBlock), 0, st, n, f, fin, ws, done);
+  GLX_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------- plain operator apply --
+// out = A in on the device (no fusion).  For a host callback the vector goes
+// through host memory; the callback is skipped when `done` is set on device.
+void apply_operator(hipStream_t st, const DevOperator& o, int link_prec, void* out, const void* in, int elem_bytes,
+                    RedSpace& red, const int* done);
+
+// Minimum number of workgroups a stencil launch can use on this lattice.
+inline int stencil_blocks(int lx, int ly) { return make_geom(lx, ly).nblk; }
+
+}  // namespace glx

@@ -1,0 +1,66 @@
+// glx_solvers.h -- device-resident Krylov solvers (host orchestration API).
+//
+// Every solver keeps its vectors in HBM and its scalars in a small device
+// state block that the reduction finalisers update; the host only enqueues
+// kernels and, once per chunk of iterations, reads back the convergence word.
+// Kernels check that word on entry and return at once after convergence, so
+// the iteration sequence (and hence the iteration count) is exactly the
+// reference's.
+#pragma once
+
+#include "glx_runtime.h"
+
+#include <string>
+#include <vector>
+
+namespace glx {
+
+enum Verb : int { VERB_PASS_THROUGH = -1, VERB_NONE = 0, VERB_SUMMARY = 1, VERB_RESTART_DETAIL = 2, VERB_DETAIL = 3 };
+
+// Result of one solve, mirroring inversion_info (inverter_struct.h:14-89).
+struct SolveResult {
+  double resSq = 0.0;
+  int iter = 0;
+  bool success = false;
+  int ops_count = 0;
+  std::string name;
+  std::vector<double> relres_hist;  // iterated relative residual per iteration
+  std::vector<double> resSqmrhs;    // multishift: per-shift true residual^2
+  double bsqrt = 0.0;
+  // mixed precision extras
+  int reliable_updates = 0;
+};
+
+struct SolveOpts {
+  int max_iter = 10000;
+  double eps = 1e-10;
+  bool want_hist = false;   // copy the per-iteration residuals back
+  hipStream_t stream = nullptr;
+  int chunk = 16;           // iterations enqueued between convergence checks
+};
+
+// x (in: initial guess, out: solution) and b are device pointers of the
+// operator's element type (double for the real Laplacian, interleaved
+// complex otherwise).
+SolveResult dev_cg(const DevOperator& op, void* x, const void* b, const SolveOpts& o);
+SolveResult dev_bicgstab(const DevOperator& op, void* x, const void* b, const SolveOpts& o);
+SolveResult dev_gcr(const DevOperator& op, void* x, const void* b, const SolveOpts& o);
+// Multishift CG: xs[n_shift] device solution vectors; shifts are read only
+// (the reference's in-place permutation is kept inside the device state and
+// undone before return).
+SolveResult dev_cg_m(const DevOperator& op, void* const* xs, const void* b, int n_shift, const double* shifts,
+                     int resid_freq_check, bool worst_first, const SolveOpts& o);
+
+// Mixed-precision CG with reliable updates (no reference counterpart):
+// FP32 Krylov vectors and FP32 or FP16 links, FP64 true residual and
+// solution accumulation whenever the iterated residual drops by `delta`.
+struct MixedOpts {
+  int link_prec = LINKS_F32;
+  double delta = 0.1;
+};
+SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m);
+
+// Plain operator application on device vectors.
+void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st);
+
+}  // namespace glx

@@ -1,0 +1,1311 @@
+// glx_solvers.hip -- device-resident Krylov solvers.
+//
+// CG              generic_cg.cpp:132-231 (f64), :278-382 (c128)
+// BiCGStab        generic_bicgstab.cpp:22-158, :205-341
+// GCR             generic_gcr.cpp:27-156, :202-333
+// CG-M            generic_cg_m.cpp:23-310, :312-600
+// Mixed CG        (no reference counterpart) FP32 Krylov + FP64 reliable updates
+//
+// Each iteration is a handful of kernels: operator applications are the tiled
+// stencil kernel with the solver's vector update fused into its prologue and
+// the next inner product fused into its epilogue; the remaining vector
+// updates are pointwise kernels with fused norms.  All scalars (alpha, beta,
+// zeta, ...) are computed on the device by the reduction finalisers and read
+// by the next kernel; the host reads the 16-byte control word once per chunk.
+#include "glx_solvers.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace glx {
+
+// =========================================================== allocator ====
+DevicePool& DevicePool::get() {
+  static DevicePool* pool = new DevicePool;  // outlives every static user
+  return *pool;
+}
+void* DevicePool::alloc(size_t bytes) {
+  int dev = 0;
+  GLX_HIP(hipGetDevice(&dev));
+  bytes = (bytes + 255) & ~size_t(255);
+  const size_t key = bytes * 64 + size_t(dev);
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = free_.find(key);
+    if (it != free_.end()) {
+      void* p = it->second;
+      free_.erase(it);
+      live_[p] = key;
+      return p;
+    }
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    trim();
+    GLX_HIP(hipMalloc(&p, bytes));
+  }
+  std::lock_guard<std::mutex> g(mu_);
+  live_[p] = key;
+  return p;
+}
+void DevicePool::release(void* p) {
+  std::lock_guard<std::mutex> g(mu_);
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  free_.emplace(it->second, p);
+  live_.erase(it);
+}
+void DevicePool::trim() {
+  std::lock_guard<std::mutex> g(mu_);
+  if (!free_.empty()) (void)hipDeviceSynchronize();
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+}
+
+// ========================================================= solver state ===
+constexpr int kMaxShift = 64;
+
+struct SolveState {
+  // control word (first 16 bytes, mirrored to the host)
+  int k, done, kfinal, n_rem;
+  int max_iter, resid_freq, worst_first, n_shift;
+  double eps, bsqrt, rsq, rsqNew;
+  double beta_r, resSq, maxrr, delta;
+  int reliable, pad1, pad2, pad3;
+  c128 alpha, alpha_prev, beta, beta_prev, rho, omega, pap;
+  double* hist;
+  double* resvec;
+  // multishift
+  c128 zeta[kMaxShift], zeta_p[kMaxShift], alpha_s[kMaxShift], beta_s[kMaxShift];
+  double shifts[kMaxShift];
+  int mapping[kMaxShift];
+  int vec[kMaxShift];
+};
+
+struct Mirror {
+  int k, done, kfinal, n_rem;
+};
+
+template <class T>
+__device__ __forceinline__ typename Scal<T>::type as_scal(c128 a) {
+  if constexpr (is_cx<T>::value) return a;
+  else return a.re;
+}
+
+// Pinned slots for the control-word read-back (per host thread).
+struct PinnedSlots {
+  Mirror* m = nullptr;
+  hipEvent_t ev[2];
+  PinnedSlots() {
+    GLX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m), 2 * sizeof(Mirror), hipHostMallocDefault));
+    GLX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    GLX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  }
+};
+static PinnedSlots& pinned() {
+  static thread_local PinnedSlots* s = new PinnedSlots;
+  return *s;
+}
+
+// Per-solve workspace: device state + reduction space.
+struct Work {
+  hipStream_t st;
+  DevBuf state_buf;
+  SolveState* ds;
+  RedSpace red;
+  DevBuf hist;
+  Work(hipStream_t s, int max_blocks, int max_iter, bool want_hist) : st(s) {
+    state_buf = DevBuf(sizeof(SolveState));
+    ds = state_buf.as<SolveState>();
+    red.init(std::max(max_blocks, kMaxPointBlocks), st);
+    SolveState h;
+    std::memset(&h, 0, sizeof(h));
+    h.max_iter = max_iter;
+    if (max_iter > 0) {
+      hist = DevBuf(size_t(max_iter) * sizeof(double));
+      h.hist = hist.as<double>();
+    }
+    (void)want_hist;
+    upload(h);
+  }
+  void upload(const SolveState& h) {
+    GLX_HIP(hipMemcpyAsync(ds, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    GLX_HIP(hipStreamSynchronize(st));
+  }
+  SolveState download() {
+    SolveState h;
+    GLX_HIP(hipMemcpyAsync(&h, ds, sizeof(h), hipMemcpyDeviceToHost, st));
+    GLX_HIP(hipStreamSynchronize(st));
+    return h;
+  }
+  const int* done() const { return &ds->done; }
+  RedWs ws() const { return red.ws(); }
+  std::vector<double> hist_host(int n) {
+    std::vector<double> v(std::max(n, 0));
+    if (n > 0) {
+      GLX_HIP(hipMemcpyAsync(v.data(), hist.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+      GLX_HIP(hipStreamSynchronize(st));
+    }
+    return v;
+  }
+};
+
+// Enqueue iterations chunk by chunk, keeping one chunk queued while the
+// control word of the previous one is inspected.  `enq(k)` enqueues the
+// kernels of iteration k; `pause(m)` handles a device-requested pause
+// (done == 2) and returns the next iteration index.
+template <class Enq, class Pause>
+Mirror run_loop(Work& w, int max_iter, int chunk, Enq enq, Pause pause) {
+  PinnedSlots& pin = pinned();
+  int next = 0;
+  int cur = 0;
+  bool inflight = false;
+  Mirror m{0, 0, -1, 0};
+  auto enqueue_chunk = [&](int slot) {
+    const int end = std::min(max_iter, next + chunk);
+    for (; next < end; next++) enq(next);
+    GLX_HIP(hipMemcpyAsync(&pin.m[slot], w.ds, sizeof(Mirror), hipMemcpyDeviceToHost, w.st));
+    GLX_HIP(hipEventRecord(pin.ev[slot], w.st));
+  };
+  if (max_iter <= 0) return m;
+  enqueue_chunk(cur);
+  inflight = true;
+  for (;;) {
+    const bool more = next < max_iter;
+    if (more) enqueue_chunk(cur ^ 1);
+    GLX_HIP(hipEventSynchronize(pin.ev[cur]));
+    m = pin.m[cur];
+    if (m.done == 1) break;
+    if (m.done == 2) {
+      // drain the queued chunk (its kernels returned at once), then resume
+      if (more) GLX_HIP(hipEventSynchronize(pin.ev[cur ^ 1]));
+      next = pause(m);
+      if (next < 0) {
+        m = pin.m[cur];
+        break;
+      }
+      if (next <= m.kfinal) throw HipError("solver state did not advance after a pause");
+      enqueue_chunk(cur);
+      continue;
+    }
+    if (!more) {
+      // every iteration is queued: the device stops at max_iter - 1 at the latest
+      GLX_HIP(hipStreamSynchronize(w.st));
+      GLX_HIP(hipMemcpy(&m, w.ds, sizeof(Mirror), hipMemcpyDeviceToHost));
+      if (m.done == 2) {
+        next = pause(m);
+        if (next >= 0 && next <= m.kfinal) throw HipError("solver state did not advance after a pause");
+        if (next >= 0 && next < max_iter) {
+          enqueue_chunk(cur);
+          continue;
+        }
+      }
+      break;
+    }
+    cur ^= 1;
+  }
+  (void)inflight;
+  GLX_HIP(hipStreamSynchronize(w.st));
+  GLX_HIP(hipMemcpy(&m, w.ds, sizeof(Mirror), hipMemcpyDeviceToHost));
+  return m;
+}
+
+// ================================================== generic functors ======
+// Pointwise kernel that materialises a prologue over the whole vector (used
+// when the operator cannot take a fused prologue: host callbacks, Nc > 1).
+template <class In>
+struct PtFromIn {
+  static constexpr int NR = 0;
+  In in;
+  __device__ void init() { in.init(); }
+  __device__ void operator()(int i, double*) { (void)in.load(i, true); }
+};
+template <class T, class Epi>
+struct PtFromEpi {
+  static constexpr int NR = Epi::NR;
+  Epi epi;
+  const T* in;
+  const T* out;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) { epi.acc(part, i, in[i], out[i]); }
+};
+
+// Nc > 1 Laplacian (colour index fastest), one thread per entry.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_laplace_nc(int lx, int ly, int nc, double m2, const T* __restrict__ in,
+                                                       T* __restrict__ out, const int* done) {
+  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const int n = lx * ly * nc;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int c = i % nc, s = i / nc, x = s % lx, y = s / lx;
+    const int xp = x + 1 == lx ? 0 : x + 1, xm = x == 0 ? lx - 1 : x - 1;
+    const int yp = y + 1 == ly ? 0 : y + 1, ym = y == 0 ? ly - 1 : y - 1;
+    T acc = zero_of<T>();
+    acc = acc - in[(y * lx + xp) * nc + c];
+    acc = acc - in[(y * lx + xm) * nc + c];
+    acc = acc - in[(yp * lx + x) * nc + c];
+    acc = acc - in[(ym * lx + x) * nc + c];
+    out[i] = acc + rscal(4.0 + m2, in[i]);
+  }
+}
+
+template <class T>
+void apply_typed(hipStream_t st, const DevOperator& o, int link_prec, T* out, const T* in, RedWs ws, const int* done) {
+  if (o.is_host()) {
+    const size_t bytes = size_t(o.size()) * sizeof(T);
+    std::vector<char> hin(bytes), hout(bytes);
+    int hdone = 0;
+    GLX_HIP(hipMemcpyAsync(hin.data(), in, bytes, hipMemcpyDeviceToHost, st));
+    if (done) GLX_HIP(hipMemcpyAsync(&hdone, done, sizeof(int), hipMemcpyDeviceToHost, st));
+    GLX_HIP(hipStreamSynchronize(st));
+    if (hdone) return;
+    o.host_cb(reinterpret_cast<double*>(hout.data()), reinterpret_cast<double*>(hin.data()), o.host_extra);
+    GLX_HIP(hipMemcpyAsync(out, hout.data(), bytes, hipMemcpyHostToDevice, st));
+    GLX_HIP(hipStreamSynchronize(st));
+    return;
+  }
+  if (o.nc != 1) {
+    if (o.kind != OP_LAPLACE && o.kind != OP_LAPLACE_REAL)
+      throw ArgError("Nc > 1 is supported for the Laplacians only");
+    hipLaunchKernelGGL(k_laplace_nc<T>, dim3(point_blocks(o.size())), dim3(kBlock), 0, st, o.lx, o.ly, o.nc, o.mass,
+                       in, out, done);
+    GLX_CHECK_LAUNCH();
+    return;
+  }
+  run_stencil<T>(st, o, link_prec, InPlain<T>{in}, out, EpiNone<T>{}, FinNone{}, ws, done);
+}
+
+// Operator application with a prologue and an epilogue reduction: one fused
+// stencil launch when the operator is tiled, else prologue kernel + apply +
+// reduction kernel.  `mat` is the vector the prologue materialises.
+template <class T, class In, class Epi, class Fin>
+void run_fused(Work& w, const DevOperator& o, int link_prec, In in, const T* mat, T* out, Epi epi, Fin fin) {
+  if (o.tiled()) {
+    run_stencil<T>(w.st, o, link_prec, in, out, epi, fin, w.ws(), w.done());
+    return;
+  }
+  launch_point(w.st, o.size(), PtFromIn<In>{in}, FinNone{}, w.ws(), w.done());
+  apply_typed<T>(w.st, o, link_prec, out, mat, w.ws(), w.done());
+  if constexpr (Epi::NR > 0) {
+    launch_point(w.st, o.size(), PtFromEpi<T, Epi>{epi, mat, out}, fin, w.ws(), w.done());
+  }
+}
+
+// ------------------------------------------------------- shared pieces ----
+// ||a - b||^2 into s->resSq (or resvec[idx] when idx >= 0)
+template <class T>
+struct PtDiffNorm {
+  static constexpr int NR = 1;
+  const T* a;
+  const T* b;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const T d = a[i] - b[i];
+    part[0] += norm2(d);
+  }
+};
+struct FinResSq {
+  SolveState* s;
+  int idx;
+  __device__ void operator()(const double* v) const {
+    if (idx < 0) s->resSq = v[0];
+    else s->resvec[idx] = v[0];
+  }
+};
+// y += a x with a = state alpha (complex or real)
+template <class T>
+struct PtAxpyAlpha {
+  static constexpr int NR = 0;
+  T* y;
+  const T* x;
+  const SolveState* s;
+  typename Scal<T>::type a;
+  __device__ void init() { a = as_scal<T>(s->alpha); }
+  __device__ void operator()(int i, double*) { y[i] = y[i] + scal(a, x[i]); }
+};
+
+template <class T>
+double true_residual(Work& w, const DevOperator& o, T* x, const T* b, T* tmp) {
+  apply_typed<T>(w.st, o, LINKS_F64, tmp, x, w.ws(), nullptr);
+  launch_point(w.st, o.size(), PtDiffNorm<T>{tmp, b}, FinResSq{w.ds, -1}, w.ws(), nullptr);
+  return 0.0;
+}
+
+// ================================================================== CG ====
+template <class T>
+struct PtCgInit {
+  static constexpr int NR = 2;
+  T* r;
+  const T* b;
+  const T* ax;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const T bi = b[i];
+    const T ri = bi - ax[i];
+    r[i] = ri;
+    part[0] += norm2(bi);
+    part[1] += norm2(ri);
+  }
+};
+struct FinCgInit {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    s->bsqrt = sqrt(v[0]);
+    s->rsq = v[1];
+    s->maxrr = sqrt(v[1]);
+  }
+};
+// p_new = r + beta p_old (all visited sites); owned: store p_new and
+// x += alpha_prev p_old (the previous iteration's solution update).
+template <class T, class TX>
+struct InCG {
+  const T* r;
+  const T* pold;
+  T* pnew;
+  TX* x;
+  const SolveState* s;
+  typename Scal<T>::type ap;
+  double beta;
+  __device__ void init() {
+    beta = s->beta_r;
+    ap = as_scal<T>(s->alpha_prev);
+  }
+  __device__ T load(int j, bool own) {
+    const T po = pold[j];
+    const T pn = r[j] + rscal(beta, po);
+    if (own) {
+      pnew[j] = pn;
+      if (x) {
+        if constexpr (std::is_same<T, TX>::value) {
+          x[j] = x[j] + scal(ap, po);
+        } else {
+          TX a;
+          cvt(a, scal(ap, po));
+          x[j] = x[j] + a;
+        }
+      }
+    }
+    return pn;
+  }
+};
+template <class T>
+struct FinCgAlpha {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    if constexpr (is_cx<T>::value) {
+      s->pap = {v[0], v[1]};
+      s->alpha = cdiv(c128{s->rsq, 0.0}, s->pap);
+    } else {
+      s->pap = {v[0], 0.0};
+      s->alpha = {s->rsq / v[0], 0.0};
+    }
+  }
+};
+template <class T>
+struct PtCgR {
+  static constexpr int NR = 1;
+  T* r;
+  const T* Ap;
+  const SolveState* s;
+  typename Scal<T>::type a;
+  __device__ void init() { a = as_scal<T>(s->alpha); }
+  __device__ void operator()(int i, double* part) {
+    const T v = r[i] - scal(a, Ap[i]);
+    r[i] = v;
+    part[0] += norm2(v);
+  }
+};
+struct FinCgNorm {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    const double rn = v[0];
+    const int k = s->k;
+    if (s->hist) s->hist[k] = sqrt(rn) / s->bsqrt;
+    s->rsqNew = rn;
+    if (sqrt(rn) < s->eps * s->bsqrt || k == s->max_iter - 1) {
+      s->kfinal = k;
+      s->done = 1;
+    } else {
+      s->beta_r = rn / s->rsq;
+      s->rsq = rn;
+      s->alpha_prev = s->alpha;
+      s->k = k + 1;
+    }
+  }
+};
+
+template <class T>
+SolveResult cg_impl(const DevOperator& o, T* x, const T* b, const SolveOpts& opt) {
+  const int n = o.size();
+  Work w(opt.stream, stencil_blocks(std::max(o.lx, 1), std::max(o.ly, 1)), opt.max_iter, opt.want_hist);
+  SolveState h;
+  std::memset(&h, 0, sizeof(h));
+  h.max_iter = opt.max_iter;
+  h.eps = opt.eps;
+  h.hist = w.hist.as<double>();
+  w.upload(h);
+  DevBuf rb(n * sizeof(T)), pb0(n * sizeof(T)), pb1(n * sizeof(T)), apb(n * sizeof(T));
+  T* r = rb.as<T>();
+  T* p[2] = {pb0.as<T>(), pb1.as<T>()};
+  T* Ap = apb.as<T>();
+  GLX_HIP(hipMemsetAsync(p[0], 0, n * sizeof(T), w.st));
+  // r = b - A x; ||b||, ||r||^2
+  apply_typed<T>(w.st, o, LINKS_F64, Ap, x, w.ws(), nullptr);
+  launch_point(w.st, n, PtCgInit<T>{r, b, Ap}, FinCgInit{w.ds}, w.ws(), nullptr);
+
+  auto enq = [&](int k) {
+    T* pold = p[k & 1];
+    T* pnew = p[(k + 1) & 1];
+    run_fused<T>(w, o, LINKS_F64, InCG<T, T>{r, pold, pnew, x, w.ds}, pnew, Ap, EpiDotInOut<T>{}, FinCgAlpha<T>{w.ds});
+    launch_point(w.st, n, PtCgR<T>{r, Ap, w.ds}, FinCgNorm{w.ds}, w.ws(), w.done());
+  };
+  Mirror m = run_loop(w, opt.max_iter, opt.chunk, enq, [](const Mirror&) { return -1; });
+
+  SolveResult res;
+  res.name = "CG";
+  int kf = opt.max_iter > 0 ? m.kfinal : 0;
+  if (opt.max_iter > 0) {
+    // the last iteration's x += alpha p (deferred into the next prologue)
+    launch_point(w.st, n, PtAxpyAlpha<T>{x, p[(kf + 1) & 1], w.ds}, FinNone{}, w.ws(), nullptr);
+  }
+  true_residual<T>(w, o, x, b, Ap);
+  SolveState hs = w.download();
+  res.success = opt.max_iter > 0 ? (kf != opt.max_iter - 1) : true;
+  res.iter = kf + 1;
+  res.ops_count = res.iter + 2;
+  res.resSq = hs.resSq;
+  res.bsqrt = hs.bsqrt;
+  if (opt.want_hist && opt.max_iter > 0) res.relres_hist = w.hist_host(res.iter);
+  return res;
+}
+
+// ============================================================ BiCGStab ====
+template <class T>
+struct PtBicgInit {
+  static constexpr int NR = (is_cx<T>::value ? 3 : 2);
+  T* r;
+  T* r0;
+  T* p;
+  const T* b;
+  const T* ax;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const T bi = b[i];
+    const T ri = bi - ax[i];
+    r[i] = ri;
+    r0[i] = ri;
+    p[i] = ri;
+    part[0] += norm2(bi);
+    if constexpr (is_cx<T>::value) {
+      const c128 d = dotc(ri, ri);
+      part[1] += d.re;
+      part[2] += d.im;
+    } else {
+      part[1] += dotc(ri, ri);
+    }
+  }
+};
+template <class T>
+struct FinBicgInit {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    s->bsqrt = sqrt(v[0]);
+    if constexpr (is_cx<T>::value) s->rho = {v[1], v[2]};
+    else s->rho = {v[1], 0.0};
+  }
+};
+template <class T>
+struct FinBicgAlpha {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    if constexpr (is_cx<T>::value) s->alpha = cdiv(s->rho, c128{v[0], v[1]});
+    else s->alpha = {s->rho.re / v[0], 0.0};
+  }
+};
+// s = r - alpha Ap
+template <class T>
+struct InBicgS {
+  const T* r;
+  const T* Ap;
+  T* sv;
+  const SolveState* st;
+  typename Scal<T>::type a;
+  __device__ void init() { a = as_scal<T>(st->alpha); }
+  __device__ T load(int j, bool own) {
+    const T v = r[j] - scal(a, Ap[j]);
+    if (own) sv[j] = v;
+    return v;
+  }
+};
+template <class T>
+struct FinBicgOmega {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    if constexpr (is_cx<T>::value) s->omega = cdiv(c128{v[0], v[1]}, c128{v[2], 0.0});
+    else s->omega = {v[0] / v[1], 0.0};
+  }
+};
+// x += alpha p + omega s; r = s - omega As; ||r||^2, <r0, r>
+template <class T>
+struct PtBicgUpd {
+  static constexpr int NR = (is_cx<T>::value ? 3 : 2);
+  T* x;
+  T* r;
+  const T* p;
+  const T* sv;
+  const T* As;
+  const T* r0;
+  const SolveState* st;
+  typename Scal<T>::type a, om;
+  __device__ void init() {
+    a = as_scal<T>(st->alpha);
+    om = as_scal<T>(st->omega);
+  }
+  __device__ void operator()(int i, double* part) {
+    const T si = sv[i];
+    x[i] = (x[i] + scal(a, p[i])) + scal(om, si);
+    const T ri = si - scal(om, As[i]);
+    r[i] = ri;
+    part[0] += norm2(ri);
+    if constexpr (is_cx<T>::value) {
+      const c128 d = dotc(r0[i], ri);
+      part[1] += d.re;
+      part[2] += d.im;
+    } else {
+      part[1] += dotc(r0[i], ri);
+    }
+  }
+};
+template <class T>
+struct FinBicgUpd {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    const double rn = v[0];
+    const int k = s->k;
+    if (s->hist) s->hist[k] = sqrt(rn) / s->bsqrt;
+    if (sqrt(rn) < s->eps * s->bsqrt || k == s->max_iter - 1) {
+      s->kfinal = k;
+      s->done = 1;
+      return;
+    }
+    if constexpr (is_cx<T>::value) {
+      const c128 rho_new{v[1], v[2]};
+      s->beta = cdiv(rho_new, s->rho) * cdiv(s->alpha, s->omega);
+      s->rho = rho_new;
+    } else {
+      s->beta = {v[1] / s->rho.re * (s->alpha.re / s->omega.re), 0.0};
+      s->rho = {v[1], 0.0};
+    }
+    s->k = k + 1;
+  }
+};
+// p_new = r + beta (p_old - omega Ap_old)
+template <class T>
+struct InBicgP {
+  const T* r;
+  const T* pold;
+  const T* Apold;
+  T* pnew;
+  const SolveState* st;
+  typename Scal<T>::type be, om;
+  __device__ void init() {
+    be = as_scal<T>(st->beta);
+    om = as_scal<T>(st->omega);
+  }
+  __device__ T load(int j, bool own) {
+    const T v = r[j] + scal(be, pold[j] - scal(om, Apold[j]));
+    if (own) pnew[j] = v;
+    return v;
+  }
+};
+
+template <class T>
+SolveResult bicgstab_impl(const DevOperator& o, T* x, const T* b, const SolveOpts& opt) {
+  const int n = o.size();
+  Work w(opt.stream, stencil_blocks(std::max(o.lx, 1), std::max(o.ly, 1)), opt.max_iter, opt.want_hist);
+  SolveState h;
+  std::memset(&h, 0, sizeof(h));
+  h.max_iter = opt.max_iter;
+  h.eps = opt.eps;
+  h.hist = w.hist.as<double>();
+  w.upload(h);
+  const size_t vb = n * sizeof(T);
+  DevBuf rB(vb), r0B(vb), pB0(vb), pB1(vb), aB0(vb), aB1(vb), sB(vb), asB(vb);
+  T* r = rB.as<T>();
+  T* r0 = r0B.as<T>();
+  T* p[2] = {pB0.as<T>(), pB1.as<T>()};
+  T* Ap[2] = {aB0.as<T>(), aB1.as<T>()};
+  T* sv = sB.as<T>();
+  T* As = asB.as<T>();
+  apply_typed<T>(w.st, o, LINKS_F64, Ap[0], x, w.ws(), nullptr);
+  launch_point(w.st, n, PtBicgInit<T>{r, r0, p[0], b, Ap[0]}, FinBicgInit<T>{w.ds}, w.ws(),
+                                        nullptr);
+  run_fused<T>(w, o, LINKS_F64, InPlain<T>{p[0]}, p[0], Ap[0], EpiDotAuxOut<T>{r0}, FinBicgAlpha<T>{w.ds});
+
+  auto enq = [&](int k) {
+    const int c = k & 1, nx = (k + 1) & 1;
+    run_fused<T>(w, o, LINKS_F64, InBicgS<T>{r, Ap[c], sv, w.ds}, sv, As, EpiOmega<T>{}, FinBicgOmega<T>{w.ds});
+    launch_point(w.st, n, PtBicgUpd<T>{x, r, p[c], sv, As, r0, w.ds}, FinBicgUpd<T>{w.ds},
+                                          w.ws(), w.done());
+    run_fused<T>(w, o, LINKS_F64, InBicgP<T>{r, p[c], Ap[c], p[nx], w.ds}, p[nx], Ap[nx], EpiDotAuxOut<T>{r0},
+                 FinBicgAlpha<T>{w.ds});
+  };
+  Mirror m = run_loop(w, opt.max_iter, opt.chunk, enq, [](const Mirror&) { return -1; });
+  SolveResult res;
+  res.name = "BiCGStab";
+  const int kf = opt.max_iter > 0 ? m.kfinal : 0;
+  true_residual<T>(w, o, x, b, As);
+  SolveState hs = w.download();
+  res.success = opt.max_iter > 0 ? (kf != opt.max_iter - 1) : true;
+  res.iter = kf + 1;
+  res.ops_count = 2 * res.iter + 2;
+  if (opt.max_iter <= 0) res.ops_count = 3;
+  res.resSq = hs.resSq;
+  res.bsqrt = hs.bsqrt;
+  if (opt.want_hist && opt.max_iter > 0) res.relres_hist = w.hist_host(res.iter);
+  return res;
+}
+
+// ================================================================= GCR ====
+template <class T>
+struct PtGcrInit {
+  static constexpr int NR = 1;
+  T* r;
+  T* p0;
+  const T* b;
+  const T* ax;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const T bi = b[i];
+    const T ri = bi - ax[i];
+    r[i] = ri;
+    p0[i] = ri;
+    part[0] += norm2(bi);
+  }
+};
+struct FinBsqrt {
+  SolveState* s;
+  __device__ void operator()(const double* v) const { s->bsqrt = sqrt(v[0]); }
+};
+// <Ap_k, r> and ||Ap_k||^2 -> alpha = <Ap_k, r> / ||Ap_k||^2
+template <class T>
+struct PtGcrAlpha {
+  static constexpr int NR = (is_cx<T>::value ? 3 : 2);
+  const T* apk;
+  const T* r;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const T a = apk[i];
+    if constexpr (is_cx<T>::value) {
+      const c128 d = dotc(a, r[i]);
+      part[0] += d.re;
+      part[1] += d.im;
+      part[2] += norm2(a);
+    } else {
+      part[0] += dotc(a, r[i]);
+      part[1] += norm2(a);
+    }
+  }
+};
+template <class T>
+struct FinGcrAlpha {
+  SolveState* s;
+  double* norms;
+  int k;
+  __device__ void operator()(const double* v) const {
+    if constexpr (is_cx<T>::value) {
+      norms[k] = v[2];
+      s->alpha = {v[0] / v[2], v[1] / v[2]};
+    } else {
+      norms[k] = v[1];
+      s->alpha = {v[0] / v[1], 0.0};
+    }
+  }
+};
+// x += alpha p_k; r -= alpha Ap_k; ||r||^2
+template <class T>
+struct PtGcrXR {
+  static constexpr int NR = 1;
+  T* x;
+  T* r;
+  const T* pk;
+  const T* apk;
+  const SolveState* s;
+  typename Scal<T>::type a;
+  __device__ void init() { a = as_scal<T>(s->alpha); }
+  __device__ void operator()(int i, double* part) {
+    x[i] = x[i] + scal(a, pk[i]);
+    const T ri = r[i] - scal(a, apk[i]);
+    r[i] = ri;
+    part[0] += norm2(ri);
+  }
+};
+struct FinGcrNorm {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    const double rn = v[0];
+    const int k = s->k;
+    if (s->hist) s->hist[k] = sqrt(rn) / s->bsqrt;
+    if (sqrt(rn) < s->eps * s->bsqrt || k == s->max_iter - 1) {
+      s->kfinal = k;
+      s->done = 1;
+    } else {
+      s->k = k + 1;
+    }
+  }
+};
+// beta_j = -<Ap_j, Ar> / ||Ap_j||^2 for a group of up to 8 stored directions
+constexpr int kGcrGroup = 8;
+template <class T>
+struct PtGcrDots {
+  static constexpr int NR = ((is_cx<T>::value ? 2 : 1) * kGcrGroup);
+  T* const* aptab;
+  const T* ar;
+  int j0, cnt;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const T a = ar[i];
+    for (int g = 0; g < cnt; g++) {
+      const T v = aptab[j0 + g][i];
+      if constexpr (is_cx<T>::value) {
+        const c128 d = dotc(v, a);
+        part[2 * g] += d.re;
+        part[2 * g + 1] += d.im;
+      } else {
+        part[g] += dotc(v, a);
+      }
+    }
+  }
+};
+template <class T>
+struct FinGcrBeta {
+  c128* betas;
+  const double* norms;
+  int j0, cnt;
+  __device__ void operator()(const double* v) const {
+    for (int g = 0; g < cnt; g++) {
+      const double nrm = norms[j0 + g];
+      if constexpr (is_cx<T>::value) betas[j0 + g] = {-v[2 * g] / nrm, -v[2 * g + 1] / nrm};
+      else betas[j0 + g] = {-v[g] / nrm, 0.0};
+    }
+  }
+};
+// p_{k+1} = r + sum_j beta_j p_j; Ap_{k+1} = Ar + sum_j beta_j Ap_j
+template <class T>
+struct PtGcrComb {
+  static constexpr int NR = 0;
+  T* const* ptab;
+  T* const* aptab;
+  const c128* betas;
+  const T* r;
+  const T* ar;
+  int k;
+  __device__ void init() {}
+  __device__ void operator()(int i, double*) {
+    T pv = r[i], av = ar[i];
+    for (int j = 0; j <= k; j++) {
+      const typename Scal<T>::type bj = as_scal<T>(betas[j]);
+      pv = pv + scal(bj, ptab[j][i]);
+      av = av + scal(bj, aptab[j][i]);
+    }
+    ptab[k + 1][i] = pv;
+    aptab[k + 1][i] = av;
+  }
+};
+
+template <class T>
+SolveResult gcr_impl(const DevOperator& o, T* x, const T* b, const SolveOpts& opt) {
+  const int n = o.size();
+  Work w(opt.stream, stencil_blocks(std::max(o.lx, 1), std::max(o.ly, 1)), opt.max_iter, opt.want_hist);
+  SolveState h;
+  std::memset(&h, 0, sizeof(h));
+  h.max_iter = opt.max_iter;
+  h.eps = opt.eps;
+  h.hist = w.hist.as<double>();
+  w.upload(h);
+  const size_t vb = n * sizeof(T);
+  const int cap = std::max(opt.max_iter, 0) + 1;
+  DevBuf rB(vb), arB(vb), tmpB(vb);
+  DevBuf ptabB(sizeof(T*) * cap), aptabB(sizeof(T*) * cap), betasB(sizeof(c128) * cap), normsB(sizeof(double) * cap);
+  T* r = rB.as<T>();
+  T* ar = arB.as<T>();
+  T** ptab = ptabB.as<T*>();
+  T** aptab = aptabB.as<T*>();
+  std::vector<DevBuf> store;
+  std::vector<T*> hp, hap;
+  auto add_dir = [&]() {
+    store.emplace_back(vb);
+    hp.push_back(store.back().as<T>());
+    store.emplace_back(vb);
+    hap.push_back(store.back().as<T>());
+    const int j = int(hp.size()) - 1;
+    GLX_HIP(hipMemcpyAsync(ptab + j, &hp[j], sizeof(T*), hipMemcpyHostToDevice, w.st));
+    GLX_HIP(hipMemcpyAsync(aptab + j, &hap[j], sizeof(T*), hipMemcpyHostToDevice, w.st));
+    // hp / hap are reserved up front, so the sources stay valid until the end
+  };
+  hp.reserve(cap);
+  hap.reserve(cap);
+  add_dir();
+  apply_typed<T>(w.st, o, LINKS_F64, tmpB.as<T>(), x, w.ws(), nullptr);
+  launch_point(w.st, n, PtGcrInit<T>{r, hp[0], b, tmpB.as<T>()}, FinBsqrt{w.ds}, w.ws(), nullptr);
+  apply_typed<T>(w.st, o, LINKS_F64, hap[0], hp[0], w.ws(), nullptr);
+  c128* betas = betasB.as<c128>();
+  double* norms = normsB.as<double>();
+  auto enq = [&](int k) {
+    launch_point(w.st, n, PtGcrAlpha<T>{hap[k], r}, FinGcrAlpha<T>{w.ds, norms, k}, w.ws(), w.done());
+    launch_point(w.st, n, PtGcrXR<T>{x, r, hp[k], hap[k], w.ds}, FinGcrNorm{w.ds}, w.ws(), w.done());
+    if (k == opt.max_iter - 1) return;  // the device stops here at the latest
+    apply_typed<T>(w.st, o, LINKS_F64, ar, r, w.ws(), w.done());
+    for (int j0 = 0; j0 <= k; j0 += kGcrGroup) {
+      const int cnt = std::min(kGcrGroup, k + 1 - j0);
+      launch_point(w.st, n, PtGcrDots<T>{aptab, ar, j0, cnt},
+                                                            FinGcrBeta<T>{betas, norms, j0, cnt}, w.ws(), w.done());
+    }
+    add_dir();
+    launch_point(w.st, n, PtGcrComb<T>{ptab, aptab, betas, r, ar, k}, FinNone{}, w.ws(), w.done());
+  };
+  Mirror m = run_loop(w, opt.max_iter, opt.chunk, enq, [](const Mirror&) { return -1; });
+  SolveResult res;
+  res.name = "GCR";
+  const int kf = opt.max_iter > 0 ? m.kfinal : 0;
+  true_residual<T>(w, o, x, b, tmpB.as<T>());
+  SolveState hs = w.download();
+  res.success = opt.max_iter > 0 ? (kf != opt.max_iter - 1) : true;
+  res.iter = kf + 1;
+  res.ops_count = res.iter + 2;
+  res.resSq = hs.resSq;
+  res.bsqrt = hs.bsqrt;
+  if (opt.want_hist && opt.max_iter > 0) res.relres_hist = w.hist_host(res.iter);
+  return res;
+}
+
+// =========================================================== multishift ===
+template <class T>
+struct PtNormB {
+  static constexpr int NR = 1;
+  const T* b;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) { part[0] += norm2(b[i]); }
+};
+struct FinCgmInit {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    s->bsqrt = sqrt(v[0]);
+    s->rsq = v[0];
+  }
+};
+// beta = -rsq / <p, Ap>, then the zeta / beta_s recurrences for active shifts
+// (generic_cg_m.cpp:106-122, :396-412).
+template <class T>
+struct FinCgmBeta {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    s->beta_prev = s->beta;
+    if constexpr (is_cx<T>::value) {
+      s->beta = cdiv(c128{-s->rsq, 0.0}, c128{v[0], v[1]});
+      const c128 b = s->beta, bp = s->beta_prev, a = s->alpha;
+      for (int n = 0; n < s->n_rem; n++) {
+        const c128 z = s->zeta[n], zp = s->zeta_p[n];
+        const c128 num = (z * zp) * bp;
+        const c128 sb = s->shifts[n] * b;
+        const c128 one_m{1.0 - sb.re, -sb.im};
+        const c128 den = (b * a) * (zp - z) + (zp * bp) * one_m;
+        s->zeta[n] = cdiv(num, den);
+        s->zeta_p[n] = z;
+        s->beta_s[n] = cdiv(b * s->zeta[n], s->zeta_p[n]);
+      }
+    } else {
+      s->beta = {-s->rsq / v[0], 0.0};
+      const double b = s->beta.re, bp = s->beta_prev.re, a = s->alpha.re;
+      for (int n = 0; n < s->n_rem; n++) {
+        const double z = s->zeta[n].re, zp = s->zeta_p[n].re;
+        const double zn = (z * zp * bp) / (b * a * (zp - z) + zp * bp * (1.0 - s->shifts[n] * b));
+        s->zeta[n] = {zn, 0.0};
+        s->zeta_p[n] = {z, 0.0};
+        s->beta_s[n] = {b * zn / z, 0.0};
+      }
+    }
+  }
+};
+// x_s -= beta_s p_s for active shifts; r += beta Ap; ||r||^2
+template <class T>
+struct PtCgmX {
+  static constexpr int NR = 1;
+  T* const* xtab;
+  T* const* ptab;
+  T* r;
+  const T* Ap;
+  const SolveState* s;
+  int n_rem;
+  typename Scal<T>::type be;
+  __device__ void init() {
+    n_rem = s->n_rem;
+    be = as_scal<T>(s->beta);
+  }
+  __device__ void operator()(int i, double* part) {
+    for (int n = 0; n < n_rem; n++) {
+      const int v = s->vec[n];
+      T* xv = xtab[v];
+      xv[i] = xv[i] - scal(as_scal<T>(s->beta_s[n]), ptab[v][i]);
+    }
+    const T ri = r[i] + scal(be, Ap[i]);
+    r[i] = ri;
+    part[0] += norm2(ri);
+  }
+};
+// convergence bookkeeping, permutation and the alpha recurrences
+// (generic_cg_m.cpp:139-215, :429-505)
+template <class T>
+struct FinCgmR {
+  SolveState* s;
+  __device__ static void swp(c128& a, c128& b) {
+    const c128 t = a;
+    a = b;
+    b = t;
+  }
+  __device__ void operator()(const double* v) const {
+    const double rn = v[0];
+    const int k = s->k;
+    if (s->hist) s->hist[k] = sqrt(rn) / s->bsqrt;
+    const double tol = s->eps * s->bsqrt;
+    if (k % s->resid_freq == 0) {
+      for (int n = 0; n < s->n_rem; n++) {
+        double zabs;
+        if constexpr (is_cx<T>::value) zabs = hypot(s->zeta[n].re, s->zeta[n].im);
+        else zabs = s->zeta[n].re;  // the real version compares zeta itself
+        if (zabs * sqrt(rn) < tol) {
+          s->n_rem--;
+          const int e = s->n_rem;
+          if (e != n) {
+            int t = s->mapping[e]; s->mapping[e] = s->mapping[n]; s->mapping[n] = t;
+            t = s->vec[e]; s->vec[e] = s->vec[n]; s->vec[n] = t;
+            swp(s->alpha_s[e], s->alpha_s[n]);
+            swp(s->beta_s[e], s->beta_s[n]);
+            swp(s->zeta[e], s->zeta[n]);
+            swp(s->zeta_p[e], s->zeta_p[n]);
+            const double d = s->shifts[e]; s->shifts[e] = s->shifts[n]; s->shifts[n] = d;
+            n--;
+          }
+        }
+      }
+    }
+    const double z0 = hypot(s->zeta[0].re, s->zeta[0].im);
+    if ((s->worst_first && z0 * sqrt(rn) < tol) || s->n_rem == 0 || k == s->max_iter - 1) {
+      s->kfinal = k;
+      s->done = 1;
+      return;
+    }
+    const double al = rn / s->rsq;
+    s->alpha = {al, 0.0};
+    s->rsq = rn;
+    for (int n = 0; n < s->n_rem; n++) {
+      if constexpr (is_cx<T>::value) {
+        s->alpha_s[n] = cdiv((s->alpha * s->zeta[n]) * s->beta_s[n], s->zeta_p[n] * s->beta);
+      } else {
+        s->alpha_s[n] = {al * s->zeta[n].re * s->beta_s[n].re / (s->zeta_p[n].re * s->beta.re), 0.0};
+      }
+    }
+    s->k = k + 1;
+  }
+};
+// p_s = zeta r + alpha_s p_s for active shifts; p = r + alpha p
+template <class T>
+struct PtCgmP {
+  static constexpr int NR = 0;
+  T* const* ptab;
+  T* p;
+  const T* r;
+  const SolveState* s;
+  int n_rem;
+  typename Scal<T>::type al;
+  __device__ void init() {
+    n_rem = s->n_rem;
+    al = as_scal<T>(s->alpha);
+  }
+  __device__ void operator()(int i, double*) {
+    const T ri = r[i];
+    for (int n = 0; n < n_rem; n++) {
+      T* pv = ptab[s->vec[n]];
+      pv[i] = scal(as_scal<T>(s->zeta[n]), ri) + scal(as_scal<T>(s->alpha_s[n]), pv[i]);
+    }
+    p[i] = ri + scal(al, p[i]);
+  }
+};
+// ||(A + sigma) x - b||^2 for one shift
+template <class T>
+struct PtShiftRes {
+  static constexpr int NR = 1;
+  const T* ax;
+  const T* x;
+  const T* b;
+  double sigma;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const T v = ax[i] + rscal(sigma, x[i]);
+    part[0] += norm2(v - b[i]);
+  }
+};
+
+template <class T>
+SolveResult cg_m_impl(const DevOperator& o, T* const* xs, const T* b, int n_shift, const double* shifts,
+                      int resid_freq, bool worst_first, const SolveOpts& opt) {
+  if (n_shift < 1 || n_shift > kMaxShift) throw ArgError("n_shift must be in [1, 64]");
+  if (resid_freq < 1) throw ArgError("resid_freq_check must be >= 1");
+  const int n = o.size();
+  Work w(opt.stream, stencil_blocks(std::max(o.lx, 1), std::max(o.ly, 1)), opt.max_iter, opt.want_hist);
+  SolveState h;
+  std::memset(&h, 0, sizeof(h));
+  h.max_iter = opt.max_iter;
+  h.eps = opt.eps;
+  h.hist = w.hist.as<double>();
+  h.n_rem = n_shift;
+  h.n_shift = n_shift;
+  h.resid_freq = resid_freq;
+  h.worst_first = worst_first ? 1 : 0;
+  h.beta = {1.0, 0.0};
+  h.alpha = {0.0, 0.0};
+  for (int k = 0; k < n_shift; k++) {
+    h.zeta[k] = h.zeta_p[k] = h.beta_s[k] = {1.0, 0.0};
+    h.alpha_s[k] = {0.0, 0.0};
+    h.shifts[k] = shifts[k];
+    h.mapping[k] = k;
+    h.vec[k] = k;
+  }
+  DevBuf resv(sizeof(double) * n_shift);
+  h.resvec = resv.as<double>();
+  w.upload(h);
+  const size_t vb = n * sizeof(T);
+  DevBuf rB(vb), pB(vb), apB(vb), xtabB(sizeof(T*) * n_shift), ptabB(sizeof(T*) * n_shift);
+  std::vector<DevBuf> ps;
+  std::vector<T*> hps(n_shift), hxs(n_shift);
+  for (int k = 0; k < n_shift; k++) {
+    ps.emplace_back(vb);
+    hps[k] = ps.back().as<T>();
+    hxs[k] = xs[k];
+    GLX_HIP(hipMemcpyAsync(hps[k], b, vb, hipMemcpyDeviceToDevice, w.st));
+    GLX_HIP(hipMemsetAsync(xs[k], 0, vb, w.st));
+  }
+  T** xtab = xtabB.as<T*>();
+  T** ptab = ptabB.as<T*>();
+  GLX_HIP(hipMemcpyAsync(xtab, hxs.data(), sizeof(T*) * n_shift, hipMemcpyHostToDevice, w.st));
+  GLX_HIP(hipMemcpyAsync(ptab, hps.data(), sizeof(T*) * n_shift, hipMemcpyHostToDevice, w.st));
+  T* r = rB.as<T>();
+  T* p = pB.as<T>();
+  T* Ap = apB.as<T>();
+  GLX_HIP(hipMemcpyAsync(r, b, vb, hipMemcpyDeviceToDevice, w.st));
+  GLX_HIP(hipMemcpyAsync(p, b, vb, hipMemcpyDeviceToDevice, w.st));
+  launch_point(w.st, n, PtNormB<T>{b}, FinCgmInit{w.ds}, w.ws(), nullptr);
+  run_fused<T>(w, o, LINKS_F64, InPlain<T>{p}, p, Ap, EpiDotInOut<T>{}, FinCgmBeta<T>{w.ds});
+  auto enq = [&](int) {
+    launch_point(w.st, n, PtCgmX<T>{xtab, ptab, r, Ap, w.ds}, FinCgmR<T>{w.ds}, w.ws(), w.done());
+    launch_point(w.st, n, PtCgmP<T>{ptab, p, r, w.ds}, FinNone{}, w.ws(), w.done());
+    run_fused<T>(w, o, LINKS_F64, InPlain<T>{p}, p, Ap, EpiDotInOut<T>{}, FinCgmBeta<T>{w.ds});
+  };
+  // hipMemcpyAsync of pageable host vectors above must finish before they die
+  GLX_HIP(hipStreamSynchronize(w.st));
+  Mirror m = run_loop(w, opt.max_iter, opt.chunk, enq, [](const Mirror&) { return -1; });
+  const int kf = opt.max_iter > 0 ? m.kfinal : 0;
+  for (int k = 0; k < n_shift; k++) {
+    apply_typed<T>(w.st, o, LINKS_F64, Ap, xs[k], w.ws(), nullptr);
+    launch_point(w.st, n, PtShiftRes<T>{Ap, xs[k], b, shifts[k]}, FinResSq{w.ds, k}, w.ws(), nullptr);
+  }
+  SolveState hs = w.download();
+  SolveResult res;
+  res.name = "CG-M";
+  res.success = opt.max_iter > 0 ? (kf != opt.max_iter - 1) : true;
+  res.iter = kf + 1;
+  res.ops_count = 1 + (opt.max_iter > 0 ? kf : 0) + n_shift;
+  res.resSq = 0.0;
+  res.bsqrt = hs.bsqrt;
+  res.resSqmrhs.resize(n_shift);
+  GLX_HIP(hipMemcpy(res.resSqmrhs.data(), resv.get(), sizeof(double) * n_shift, hipMemcpyDeviceToHost));
+  if (opt.want_hist && opt.max_iter > 0) res.relres_hist = w.hist_host(res.iter);
+  return res;
+}
+
+// ===================================================== mixed precision ====
+// FP64 r = b - A x (into r64), r32 = r64, ||r||^2 -> reliable update decision
+template <class TD, class TS>
+struct PtMixInit {
+  static constexpr int NR = 2;
+  TD* r;
+  TS* rs;
+  const TD* b;
+  const TD* ax;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const TD bi = b[i];
+    const TD ri = bi - ax[i];
+    r[i] = ri;
+    TS v;
+    cvt(v, ri);
+    rs[i] = v;
+    part[0] += norm2(bi);
+    part[1] += norm2(ri);
+  }
+};
+// iterated residual in FP32, decision for a reliable update
+struct FinMixNorm {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    const double rn = v[0];
+    const int k = s->k;
+    const double rnorm = sqrt(rn);
+    if (s->hist) s->hist[k] = rnorm / s->bsqrt;
+    if (rnorm > s->maxrr) s->maxrr = rnorm;
+    s->rsqNew = rn;
+    if (rnorm < s->delta * s->maxrr || rnorm < s->eps * s->bsqrt || k == s->max_iter - 1) {
+      s->done = 2;  // host runs the FP64 reliable update
+      s->kfinal = k;
+      return;
+    }
+    s->beta_r = rn / s->rsq;
+    s->rsq = rn;
+    s->alpha_prev = s->alpha;
+    s->k = k + 1;
+  }
+};
+// x64 += x32 + alpha p32; x32 = 0
+template <class TD, class TS>
+struct PtMixAccum {
+  static constexpr int NR = 0;
+  TD* x;
+  TS* xs;
+  const TS* p;
+  const SolveState* s;
+  typename Scal<TD>::type a;
+  __device__ void init() { a = as_scal<TD>(s->alpha); }
+  __device__ void operator()(int i, double*) {
+    TD xl, pl;
+    cvt(xl, xs[i]);
+    cvt(pl, p[i]);
+    x[i] = x[i] + (xl + scal(a, pl));
+    xs[i] = zero_of<TS>();
+  }
+};
+struct FinMixReliable {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    const double rn = v[1];  // PtMixInit: v[0] = ||b||^2, v[1] = ||r||^2
+    const int k = s->k;
+    s->reliable += 1;
+    s->resSq = rn;
+    if (sqrt(rn) < s->eps * s->bsqrt || k >= s->max_iter - 1) {
+      s->kfinal = k;
+      s->done = 1;
+      return;
+    }
+    s->beta_r = rn / s->rsq;
+    s->rsq = rn;
+    s->maxrr = sqrt(rn);
+    s->alpha_prev = {0.0, 0.0};
+    s->k = k + 1;
+    s->done = 0;
+  }
+};
+
+template <class TD, class TS>
+SolveResult cg_mixed_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpts& opt, const MixedOpts& mo) {
+  if (!o.tiled()) throw ArgError("mixed-precision CG needs a device operator");
+  const int n = o.size();
+  Work w(opt.stream, stencil_blocks(o.lx, o.ly), opt.max_iter, opt.want_hist);
+  SolveState h;
+  std::memset(&h, 0, sizeof(h));
+  h.max_iter = opt.max_iter;
+  h.eps = opt.eps;
+  h.delta = mo.delta;
+  h.hist = w.hist.as<double>();
+  w.upload(h);
+  DevBuf r64B(n * sizeof(TD)), t64B(n * sizeof(TD));
+  DevBuf rsB(n * sizeof(TS)), xsB(n * sizeof(TS)), p0B(n * sizeof(TS)), p1B(n * sizeof(TS)), apB(n * sizeof(TS));
+  TD* r64 = r64B.as<TD>();
+  TD* t64 = t64B.as<TD>();
+  TS* rs = rsB.as<TS>();
+  TS* xs = xsB.as<TS>();
+  TS* p[2] = {p0B.as<TS>(), p1B.as<TS>()};
+  TS* Ap = apB.as<TS>();
+  GLX_HIP(hipMemsetAsync(p[0], 0, n * sizeof(TS), w.st));
+  GLX_HIP(hipMemsetAsync(xs, 0, n * sizeof(TS), w.st));
+  apply_typed<TD>(w.st, o, LINKS_F64, t64, x, w.ws(), nullptr);
+  launch_point(w.st, n, PtMixInit<TD, TS>{r64, rs, b, t64}, FinCgInit{w.ds}, w.ws(), nullptr);
+  auto enq = [&](int k) {
+    TS* pold = p[k & 1];
+    TS* pnew = p[(k + 1) & 1];
+    run_stencil<TS>(w.st, o, mo.link_prec, InCG<TS, TS>{rs, pold, pnew, xs, w.ds}, Ap, EpiDotInOut<TS>{},
+                    FinCgAlpha<TS>{w.ds}, w.ws(), w.done());
+    launch_point(w.st, n, PtCgR<TS>{rs, Ap, w.ds}, FinMixNorm{w.ds}, w.ws(), w.done());
+  };
+  auto pause = [&](const Mirror& m) -> int {
+    // reliable update after iteration m.kfinal: fold x32 and the pending
+    // alpha p into x64, recompute r = b - A x in FP64, restart from r.
+    const int k = m.kfinal;
+    launch_point(w.st, n, PtMixAccum<TD, TS>{x, xs, p[(k + 1) & 1], w.ds}, FinNone{}, w.ws(), nullptr);
+    apply_typed<TD>(w.st, o, LINKS_F64, t64, x, w.ws(), nullptr);
+    launch_point(w.st, n, PtMixInit<TD, TS>{r64, rs, b, t64}, FinMixReliable{w.ds}, w.ws(), nullptr);
+    GLX_HIP(hipStreamSynchronize(w.st));
+    Mirror mm;
+    GLX_HIP(hipMemcpy(&mm, w.ds, sizeof(Mirror), hipMemcpyDeviceToHost));
+    if (mm.done == 1) return -1;
+    return mm.k;
+  };
+  Mirror m = run_loop(w, opt.max_iter, opt.chunk, enq, pause);
+  SolveState hs = w.download();
+  SolveResult res;
+  res.name = "CG-mixed";
+  res.iter = (m.done == 1 ? hs.kfinal : hs.k) + 1;
+  res.success = sqrt(hs.resSq) < opt.eps * hs.bsqrt;
+  res.resSq = hs.resSq;
+  res.bsqrt = hs.bsqrt;
+  res.reliable_updates = hs.reliable;
+  res.ops_count = res.iter + 1 + hs.reliable;
+  if (opt.want_hist && opt.max_iter > 0) res.relres_hist = w.hist_host(std::min(res.iter, opt.max_iter));
+  return res;
+}
+
+// ============================================================ dispatch ====
+static bool real_op(const DevOperator& o) {
+  return o.is_host() ? !o.host_complex : o.kind == OP_LAPLACE_REAL;
+}
+
+SolveResult dev_cg(const DevOperator& op, void* x, const void* b, const SolveOpts& o) {
+  if (real_op(op)) return cg_impl<double>(op, static_cast<double*>(x), static_cast<const double*>(b), o);
+  return cg_impl<c128>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o);
+}
+SolveResult dev_bicgstab(const DevOperator& op, void* x, const void* b, const SolveOpts& o) {
+  if (real_op(op)) return bicgstab_impl<double>(op, static_cast<double*>(x), static_cast<const double*>(b), o);
+  return bicgstab_impl<c128>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o);
+}
+SolveResult dev_gcr(const DevOperator& op, void* x, const void* b, const SolveOpts& o) {
+  if (real_op(op)) return gcr_impl<double>(op, static_cast<double*>(x), static_cast<const double*>(b), o);
+  return gcr_impl<c128>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o);
+}
+SolveResult dev_cg_m(const DevOperator& op, void* const* xs, const void* b, int n_shift, const double* shifts,
+                     int resid_freq_check, bool worst_first, const SolveOpts& o) {
+  if (real_op(op))
+    return cg_m_impl<double>(op, reinterpret_cast<double* const*>(xs), static_cast<const double*>(b), n_shift, shifts,
+                             resid_freq_check, worst_first, o);
+  return cg_m_impl<c128>(op, reinterpret_cast<c128* const*>(xs), static_cast<const c128*>(b), n_shift, shifts,
+                         resid_freq_check, worst_first, o);
+}
+SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m) {
+  if (real_op(op))
+    return cg_mixed_impl<double, float>(op, static_cast<double*>(x), static_cast<const double*>(b), o, m);
+  return cg_mixed_impl<c128, c64>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
+}
+
+void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st) {
+  RedSpace red;
+  red.init(kMaxPointBlocks, st);
+  if (real_op(op)) {
+    if (fp32) apply_typed<float>(st, op, link_prec, static_cast<float*>(out), static_cast<const float*>(in), red.ws(), nullptr);
+    else apply_typed<double>(st, op, link_prec, static_cast<double*>(out), static_cast<const double*>(in), red.ws(), nullptr);
+  } else {
+    if (fp32) apply_typed<c64>(st, op, link_prec, static_cast<c64*>(out), static_cast<const c64*>(in), red.ws(), nullptr);
+    else apply_typed<c128>(st, op, link_prec, static_cast<c128*>(out), static_cast<const c128*>(in), red.ws(), nullptr);
+  }
+  GLX_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace glx

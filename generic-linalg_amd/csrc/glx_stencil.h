@@ -1,0 +1,447 @@
+// glx_stencil.h -- the lattice operators of the solver path as one tiled HIP
+// kernel template.
+//
+// Layout in HBM (identical to the reference's host layout, so no conversion
+// is ever needed at the boundary):
+//   vectors  lexicographic i = y*lx + x, element double / complex (re,im)
+//   links    AoS u[2*i + mu], mu = 0 (x), 1 (y)
+//
+// One block of 256 threads owns a kTX x kTY = 64 x 8 tile.  Phase 0 stages
+// the input on the tile plus an H-site halo (H = 1 for D, D^dag, Laplace;
+// H = 2 for the fused normal operator) and the links on the tile plus an
+// (H-1)-site halo into LDS with coalesced row loads; the input goes through
+// a caller-supplied prologue functor, which is where solvers fuse their
+// vector updates (e.g. CG's p = r + beta p).  Phase 1 (H = 2 only) forms
+// q = K p on the tile plus a one-site halo in LDS.  Phase 2 writes the owned
+// outputs and feeds them to an epilogue functor that accumulates per-thread
+// FP64 partial sums; a deterministic grid reduction then hands the totals to
+// a finaliser that runs on one device thread (the solver's scalar update).
+//
+// Staggered operator (operators.cpp:184-239): D = m - K,
+//   K psi(x) = 1/2 sum_mu eta_mu(x) [U_mu(x) psi(x+mu) - U_mu(x-mu)^* psi(x-mu)],
+//   eta_x = 1, eta_y(x) = (-1)^x;  D^dag = m + K (operators.cpp:372-441);
+//   D^dag D = m^2 - K^2 (operators.cpp:444-453), computed here in one pass.
+#pragma once
+
+#include "glx_common.h"
+
+#include <hip/hip_fp16.h>
+
+namespace glx {
+
+// Operator kinds; numbering shared with the C ABI (include/glx.h).
+enum OpKind : int {
+  OP_LAPLACE_REAL = 0,   // tests/multishift/multishift.cpp:634-672 (real)
+  OP_LAPLACE = 1,        // operators.cpp:28-66
+  OP_LAPLACE_U1 = 2,     // operators.cpp:73-120
+  OP_STAG = 3,           // operators.cpp:184-239
+  OP_STAG_DAG = 4,       // operators.cpp:372-441
+  OP_STAG_NORMAL = 5,    // operators.cpp:444-453
+  OP_STAG_G5 = 6,        // operators.cpp:316-368
+  OP_GAMMA5 = 7,         // operators.cpp:242-259
+  OP_DEO = 8,            // operators.cpp:456-496
+  OP_DOE = 9,            // operators.cpp:499-539
+  OP_M2MDEODOE = 10,     // operators.cpp:566-588
+  OP_STENCIL = 11,       // coarse_stencil.cpp:12-172 (Nc = 1)
+  OP_STENCIL_EO = 14,    // coarse_stencil.cpp:395-459
+  OP_STENCIL_OE = 15,    // coarse_stencil.cpp:560-624
+  OP_STENCIL_M2MDEODOE = 16,  // operators_stencil.cpp:195-214
+};
+
+__host__ __device__ constexpr int op_halo(int k) {
+  return (k == OP_STAG_NORMAL || k == OP_M2MDEODOE || k == OP_STENCIL_M2MDEODOE) ? 2 : 1;
+}
+__host__ __device__ constexpr bool op_links(int k) {
+  return k == OP_LAPLACE_U1 || k == OP_STAG || k == OP_STAG_DAG || k == OP_STAG_NORMAL || k == OP_STAG_G5 ||
+         k == OP_DEO || k == OP_DOE || k == OP_M2MDEODOE;
+}
+__host__ __device__ constexpr bool op_coeffs(int k) {
+  return k == OP_STENCIL || k == OP_STENCIL_EO || k == OP_STENCIL_OE || k == OP_STENCIL_M2MDEODOE;
+}
+
+// ---------------------------------------------------------- link storage --
+// FP64 and FP32 links are stored as complex pairs per site; FP16 links as a
+// half2 pair per site plus one FP32 scale per site (the largest component
+// magnitude of the site's two links), U = scale * half(U / scale).
+struct LinksF64 {
+  const c128* u;
+  using C = c128;
+  __device__ __forceinline__ void load(int s, C& ux, C& uy) const {
+    ux = u[2 * s];
+    uy = u[2 * s + 1];
+  }
+};
+struct LinksF32 {
+  const c64* u;
+  using C = c64;
+  __device__ __forceinline__ void load(int s, C& ux, C& uy) const {
+    ux = u[2 * s];
+    uy = u[2 * s + 1];
+  }
+};
+struct alignas(8) HalfLink {
+  __half2 x, y;
+};
+struct LinksF16 {
+  const HalfLink* u;
+  const float* scale;
+  using C = c64;
+  __device__ __forceinline__ void load(int s, C& ux, C& uy) const {
+    const HalfLink h = u[s];
+    const float sc = scale[s];
+    const float2 a = __half22float2(h.x), b = __half22float2(h.y);
+    ux = {sc * a.x, sc * a.y};
+    uy = {sc * b.x, sc * b.y};
+  }
+};
+struct LinksNone {
+  using C = c64;
+  __device__ __forceinline__ void load(int, C&, C&) const {}
+};
+
+// Operator parameters, passed by value to kernels.
+template <class LK>
+struct OpDev {
+  int lx, ly;
+  double mass;        // staggered m, or the Laplacian's m^2 shift
+  LK links;
+  // stencil form (FP64)
+  const c128* clover;
+  const c128* hop;    // hop[d*V + i], d = 0:+x 1:+y 2:-x 3:-y
+  c128 shift, eo_shift, dof_shift;
+};
+
+// ------------------------------------------------------------- prologue --
+// A prologue produces the operator input at global site j; `own` is true for
+// exactly one visit of each site of the lattice (the owning tile).
+// init() runs once per thread before the first load (it may cache scalars
+// read from the solver state).
+template <class T>
+struct InPlain {
+  const T* p;
+  __device__ __forceinline__ void init() {}
+  __device__ __forceinline__ T load(int j, bool) const { return p[j]; }
+};
+
+// ------------------------------------------------------------- epilogue --
+// NR = number of FP64 partial sums; acc(v, i, in_i, out_i) adds a site's share.
+template <class T>
+struct EpiNone {
+  static constexpr int NR = 0;
+  __device__ __forceinline__ void acc(double*, int, T, T) const {}
+};
+// <in, out> = sum conj(in) out
+template <class T>
+struct EpiDotInOut {
+  static constexpr int NR = is_cx<T>::value ? 2 : 1;
+  __device__ __forceinline__ void acc(double* v, int, T in, T out) const {
+    if constexpr (is_cx<T>::value) {
+      c128 d = dotc(in, out);
+      v[0] += d.re;
+      v[1] += d.im;
+    } else {
+      v[0] += dotc(in, out);
+    }
+  }
+};
+// <aux, out>
+template <class T>
+struct EpiDotAuxOut {
+  static constexpr int NR = is_cx<T>::value ? 2 : 1;
+  const T* aux;
+  __device__ __forceinline__ void acc(double* v, int i, T, T out) const {
+    if constexpr (is_cx<T>::value) {
+      c128 d = dotc(aux[i], out);
+      v[0] += d.re;
+      v[1] += d.im;
+    } else {
+      v[0] += dotc(aux[i], out);
+    }
+  }
+};
+// <out, in> and <out, out> (BiCGStab's omega numerator and denominator)
+template <class T>
+struct EpiOmega {
+  static constexpr int NR = is_cx<T>::value ? 3 : 2;
+  __device__ __forceinline__ void acc(double* v, int, T in, T out) const {
+    if constexpr (is_cx<T>::value) {
+      c128 d = dotc(out, in);
+      v[0] += d.re;
+      v[1] += d.im;
+      v[2] += norm2(out);
+    } else {
+      v[0] += dotc(out, in);
+      v[1] += norm2(out);
+    }
+  }
+};
+
+struct FinNone {
+  __device__ void operator()(const double*) const {}
+};
+
+// ---------------------------------------------------------- site algebra --
+// Apply U (or U^*) of compute type C to a vector element of type T.
+template <class T, class C>
+__device__ __forceinline__ T umul(C u, T v) {
+  if constexpr (is_cx<T>::value) {
+    using R = typename Real<T>::type;
+    cx<R> uu{R(u.re), R(u.im)};
+    return uu * v;
+  } else {
+    return v;
+  }
+}
+template <class T, class C>
+__device__ __forceinline__ T ucmul(C u, T v) {
+  if constexpr (is_cx<T>::value) {
+    using R = typename Real<T>::type;
+    cx<R> uu{R(u.re), R(u.im)};
+    return cmulc(uu, v);
+  } else {
+    return v;
+  }
+}
+
+// K F at one point: F has row pitch fw, centre index fc; links region has row
+// pitch lw, centre index lc (links stored as pairs ux, uy).
+template <class T, class C>
+__device__ __forceinline__ T hop_k(const T* F, int fw, int fc, const C* L, int lw, int lc, bool eta_neg) {
+  using R = typename Real<T>::type;
+  const T fxp = F[fc + 1], fxm = F[fc - 1], fyp = F[fc + fw], fym = F[fc - fw];
+  const C ux = L[2 * lc], uy = L[2 * lc + 1], uxm = L[2 * (lc - 1)], uym = L[2 * (lc - lw) + 1];
+  T ax = umul(ux, fxp) - ucmul(uxm, fxm);
+  T ay = umul(uy, fyp) - ucmul(uym, fym);
+  if (eta_neg) ay = -ay;
+  return R(0.5) * (ax + ay);
+}
+
+// ---------------------------------------------------------------- kernel --
+struct Geom {
+  int lx, ly;
+  int ntx, nty;   // tiles per direction
+  int ntiles;
+  int nblk;       // launched blocks (multiple of 8, >= ntiles)
+};
+
+template <int KIND, class T, class LK, class In, class Epi, class Fin>
+__global__ __launch_bounds__(kBlock) void k_stencil(Geom g, OpDev<LK> op, In in, T* __restrict__ out, Epi epi,
+                                                    Fin fin, RedWs ws, const int* __restrict__ done) {
+  constexpr int H = op_halo(KIND);
+  constexpr bool LINKS = op_links(KIND);
+  constexpr int PW = kTX + 2 * H, PH = kTY + 2 * H;  // staged input region
+  constexpr int LW = PW - 1, LH = PH - 1;              // staged links region
+  constexpr int QW = kTX + 2, QH = kTY + 2;            // q region (H = 2)
+  using C = typename LK::C;
+  using R = typename Real<T>::type;
+
+  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  in.init();
+
+  __shared__ T sp[PH * PW];
+  __shared__ C su[LINKS ? 2 * LW * LH : 2];
+  __shared__ T sq[H == 2 ? QW * QH : 1];
+
+  const int bid = blockIdx.x;
+  const int tile = xcd_tile(bid, g.nblk);
+  const bool active = tile < g.ntiles;
+  const int lx = g.lx, ly = g.ly;
+  double part[Epi::NR > 0 ? Epi::NR : 1];
+#pragma unroll
+  for (int k = 0; k < Epi::NR; k++) part[k] = 0.0;
+
+  if (active) {
+    const int x0 = (tile % g.ntx) * kTX, y0 = (tile / g.ntx) * kTY;
+    // phase 0: stage input and links
+    for (int e = threadIdx.x; e < PW * PH; e += kBlock) {
+      const int ry = e / PW, rx = e - ry * PW;
+      const int ux = x0 + rx - H, uy = y0 + ry - H;
+      const int gx = wrap_coord(ux, lx), gy = wrap_coord(uy, ly);
+      const bool own = rx >= H && rx < H + kTX && ry >= H && ry < H + kTY && ux < lx && uy < ly;
+      sp[e] = in.load(gy * lx + gx, own);
+    }
+    if constexpr (LINKS) {
+      for (int e = threadIdx.x; e < LW * LH; e += kBlock) {
+        const int ry = e / LW, rx = e - ry * LW;
+        const int gx = wrap_coord(x0 + rx - H, lx), gy = wrap_coord(y0 + ry - H, ly);
+        C a, b;
+        op.links.load(gy * lx + gx, a, b);
+        su[2 * e] = a;
+        su[2 * e + 1] = b;
+      }
+    }
+    __syncthreads();
+
+    // phase 1: q = K p on the tile plus a one-site halo
+    if constexpr (H == 2) {
+      for (int e = threadIdx.x; e < QW * QH; e += kBlock) {
+        const int qy = e / QW, qx = e - qy * QW;
+        const int gx = wrap_coord(x0 + qx - 1, lx), gy = wrap_coord(y0 + qy - 1, ly);
+        const int pc = (qy + 1) * PW + (qx + 1);
+        T q;
+        if constexpr (KIND == OP_STENCIL_M2MDEODOE) {
+          // odd sites: oe(p) = hopping terms; even: 0
+          if (((gx + gy) & 1) == 1) {
+            const int s = gy * lx + gx, V = lx * ly;
+            const T pxp = sp[pc + 1], pyp = sp[pc + PW], pxm = sp[pc - 1], pym = sp[pc - PW];
+            if constexpr (is_cx<T>::value) {
+              T acc = zero_of<T>();
+              auto cf = [&](int d) { c128 h = op.hop[d * V + s]; return cx<R>{R(h.re), R(h.im)}; };
+              acc = acc + cf(0) * pxp;
+              acc = acc + cf(1) * pyp;
+              acc = acc + cf(2) * pxm;
+              acc = acc + cf(3) * pym;
+              q = acc;
+            } else {
+              q = zero_of<T>();
+            }
+          } else {
+            q = zero_of<T>();
+          }
+        } else {
+          const int lc = (qy + 1) * LW + (qx + 1);
+          q = hop_k<T, C>(sp, PW, pc, su, LW, lc, (gx & 1) != 0);
+        }
+        sq[e] = q;
+      }
+      __syncthreads();
+    }
+
+    // phase 2: owned outputs
+    for (int s = threadIdx.x; s < kTX * kTY; s += kBlock) {
+      const int ty = s / kTX, tx = s - ty * kTX;
+      const int gx = x0 + tx, gy = y0 + ty;
+      if (gx >= lx || gy >= ly) continue;
+      const int i = gy * lx + gx;
+      const int pc = (ty + H) * PW + (tx + H);
+      const T pin = sp[pc];
+      const bool odd = ((gx + gy) & 1) != 0;
+      const bool eta_neg = (gx & 1) != 0;
+      T v;
+      if constexpr (KIND == OP_LAPLACE_REAL || KIND == OP_LAPLACE) {
+        // (4 + m2) p - sum of the four neighbours, reference term order
+        T acc = zero_of<T>();
+        acc = acc - sp[pc + 1];
+        acc = acc - sp[pc - 1];
+        acc = acc - sp[pc + PW];
+        acc = acc - sp[pc - PW];
+        v = acc + rscal(4.0 + op.mass, pin);
+      } else if constexpr (KIND == OP_LAPLACE_U1) {
+        const int lc = (ty + H) * LW + (tx + H);
+        const C ux = su[2 * lc], uy = su[2 * lc + 1], uxm = su[2 * (lc - 1)], uym = su[2 * (lc - LW) + 1];
+        T acc = zero_of<T>();
+        acc = acc - umul(ux, sp[pc + 1]);
+        acc = acc - ucmul(uxm, sp[pc - 1]);
+        acc = acc - umul(uy, sp[pc + PW]);
+        acc = acc - ucmul(uym, sp[pc - PW]);
+        v = acc + rscal(4.0 + op.mass, pin);
+      } else if constexpr (KIND == OP_STAG || KIND == OP_STAG_DAG || KIND == OP_STAG_G5 || KIND == OP_DEO ||
+                           KIND == OP_DOE) {
+        // links region shares the input region's origin (x0-H, y0-H); its
+        // pitch is LW, so the link centre index differs from pc
+        const int lc = (ty + H) * LW + (tx + H);
+        T kp;
+        {
+          const T fxp = sp[pc + 1], fxm = sp[pc - 1], fyp = sp[pc + PW], fym = sp[pc - PW];
+          const C ux = su[2 * lc], uy = su[2 * lc + 1], uxm = su[2 * (lc - 1)], uym = su[2 * (lc - LW) + 1];
+          T ax = umul(ux, fxp) - ucmul(uxm, fxm);
+          T ay = umul(uy, fyp) - ucmul(uym, fym);
+          if (eta_neg) ay = -ay;
+          kp = R(0.5) * (ax + ay);
+        }
+        if constexpr (KIND == OP_STAG) {
+          v = rscal(op.mass, pin) - kp;
+        } else if constexpr (KIND == OP_STAG_DAG) {
+          v = rscal(op.mass, pin) + kp;
+        } else if constexpr (KIND == OP_STAG_G5) {
+          v = rscal(op.mass, pin) - kp;
+          if (odd) v = -v;
+        } else if constexpr (KIND == OP_DEO) {
+          v = odd ? zero_of<T>() : -kp;
+        } else {
+          v = odd ? -kp : zero_of<T>();
+        }
+      } else if constexpr (KIND == OP_STAG_NORMAL || KIND == OP_M2MDEODOE) {
+        const int lc = (ty + H) * LW + (tx + H);
+        const int qc = (ty + 1) * QW + (tx + 1);
+        const T fxp = sq[qc + 1], fxm = sq[qc - 1], fyp = sq[qc + QW], fym = sq[qc - QW];
+        const C ux = su[2 * lc], uy = su[2 * lc + 1], uxm = su[2 * (lc - 1)], uym = su[2 * (lc - LW) + 1];
+        T ax = umul(ux, fxp) - ucmul(uxm, fxm);
+        T ay = umul(uy, fyp) - ucmul(uym, fym);
+        if (eta_neg) ay = -ay;
+        const T kq = R(0.5) * (ax + ay);
+        v = rscal(op.mass * op.mass, pin) - kq;
+        if constexpr (KIND == OP_M2MDEODOE) {
+          if (odd) v = zero_of<T>();
+        }
+      } else if constexpr (KIND == OP_GAMMA5) {
+        v = odd ? -pin : pin;
+      } else if constexpr (KIND == OP_STENCIL || KIND == OP_STENCIL_EO || KIND == OP_STENCIL_OE) {
+        if constexpr (is_cx<T>::value) {
+          const int V = lx * ly;
+          auto cf = [&](c128 h) { return cx<R>{R(h.re), R(h.im)}; };
+          const bool run = (KIND == OP_STENCIL) || (KIND == OP_STENCIL_EO && !odd) || (KIND == OP_STENCIL_OE && odd);
+          T acc = zero_of<T>();
+          if (run) {
+            if constexpr (KIND == OP_STENCIL) acc = acc + cf(op.clover[i]) * pin;
+            acc = acc + cf(op.hop[i]) * sp[pc + 1];
+            acc = acc + cf(op.hop[V + i]) * sp[pc + PW];
+            acc = acc + cf(op.hop[2 * V + i]) * sp[pc - 1];
+            acc = acc + cf(op.hop[3 * V + i]) * sp[pc - PW];
+            if constexpr (KIND == OP_STENCIL) {
+              if (op.shift.re != 0.0 || op.shift.im != 0.0) acc = acc + cf(op.shift) * pin;
+              if (op.eo_shift.re != 0.0 || op.eo_shift.im != 0.0)
+                acc = acc + (odd ? R(-1) : R(1)) * cf(op.eo_shift) * pin;
+              if (op.dof_shift.re != 0.0 || op.dof_shift.im != 0.0) acc = acc - cf(op.dof_shift) * pin;
+            }
+          }
+          v = acc;
+        } else {
+          v = zero_of<T>();
+        }
+      } else if constexpr (KIND == OP_STENCIL_M2MDEODOE) {
+        if constexpr (is_cx<T>::value) {
+          const int V = lx * ly;
+          auto cf = [&](c128 h) { return cx<R>{R(h.re), R(h.im)}; };
+          if (odd) {
+            v = zero_of<T>();
+          } else {
+            const int qc = (ty + 1) * QW + (tx + 1);
+            T acc = zero_of<T>();
+            acc = acc + cf(op.hop[i]) * sq[qc + 1];
+            acc = acc + cf(op.hop[V + i]) * sq[qc + QW];
+            acc = acc + cf(op.hop[2 * V + i]) * sq[qc - 1];
+            acc = acc + cf(op.hop[3 * V + i]) * sq[qc - QW];
+            v = cf(op.shift * op.shift) * pin - acc;
+          }
+        } else {
+          v = zero_of<T>();
+        }
+      }
+      out[i] = v;
+      epi.acc(part, i, pin, v);
+    }
+  }
+  if constexpr (Epi::NR > 0) grid_reduce<Epi::NR>(part, ws, bid, g.nblk, fin);
+}
+
+inline Geom make_geom(int lx, int ly) {
+  Geom g;
+  g.lx = lx;
+  g.ly = ly;
+  g.ntx = (lx + kTX - 1) / kTX;
+  g.nty = (ly + kTY - 1) / kTY;
+  g.ntiles = g.ntx * g.nty;
+  g.nblk = ((g.ntiles + 7) / 8) * 8;
+  return g;
+}
+
+template <int KIND, class T, class LK, class In, class Epi, class Fin>
+void launch_stencil(hipStream_t st, const Geom& g, const OpDev<LK>& op, In in, T* out, Epi epi, Fin fin, RedWs ws,
+                    const int* done) {
+  hipLaunchKernelGGL((k_stencil<KIND, T, LK, In, Epi, Fin>), dim3(g.nblk), dim3(kBlock), 0, st, g, op, in, out, epi,
+                     fin, ws, done);
+  GLX_CHECK_LAUNCH();
+}
+
+}  // namespace glx

@@ -1,0 +1,461 @@
+"""Python binding of libglx.so, the MI355X solver library (ctypes, no torch).
+
+Mirrors the reference's interface for the solver path: ``minv_vector_cg``,
+``minv_vector_cg_restart``, ``minv_vector_cg_m``, ``minv_vector_bicgstab``
+(+ ``_restart``), ``minv_vector_gcr`` (+ ``_restart``) with the operator
+callbacks of ``operators.h`` named as in the reference (``square_staggered_u1``
+...).  Vectors are numpy arrays (float64 for the real Laplacian, complex128
+otherwise); links are a complex128 array of 2*lx*ly values,
+``links[2*(y*lx+x)+mu]``.  Every call runs on the GPU through the C ABI of
+include/glx.h; there is no CPU path, and a missing library or GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "libglx.so")
+
+OP_KINDS = {
+    "laplace_real": 0,
+    "laplace": 1,
+    "laplace_u1": 2,
+    "staggered": 3,
+    "staggered_dagger": 4,
+    "staggered_normal": 5,
+    "staggered_gamma5": 6,
+    "gamma5": 7,
+    "deo": 8,
+    "doe": 9,
+    "m2mdeodoe": 10,
+    "stencil": 11,
+    "stencil_eo": 14,
+    "stencil_oe": 15,
+    "stencil_m2mdeodoe": 16,
+}
+# C function implementing each operator callback (include/glx.h)
+OP_SYMBOLS = {
+    "laplace_real": "glx_square_laplace_real",
+    "laplace": "glx_square_laplace",
+    "laplace_u1": "glx_square_laplace_u1",
+    "staggered": "glx_square_staggered_u1",
+    "staggered_dagger": "glx_square_staggered_dagger_u1",
+    "staggered_normal": "glx_square_staggered_normal_u1",
+    "staggered_gamma5": "glx_square_staggered_gamma5_u1",
+    "gamma5": "glx_gamma_5",
+    "deo": "glx_square_staggered_deo_u1",
+    "doe": "glx_square_staggered_doe_u1",
+    "m2mdeodoe": "glx_square_staggered_m2mdeodoe_u1",
+    "stencil": "glx_apply_stencil_2d",
+    "stencil_dagger": "glx_apply_stencil_2d",
+    "stencil_gamma5": "glx_apply_stencil_2d",
+    "stencil_eo": "glx_apply_stencil_2d_eo",
+    "stencil_oe": "glx_apply_stencil_2d_oe",
+    "stencil_m2mdeodoe": "glx_apply_square_staggered_m2mdeodoe_stencil",
+}
+SOLVER_IDS = {"cg": 0, "bicgstab": 1, "gcr": 2}
+VERB_NONE, VERB_SUMMARY, VERB_RESTART_DETAIL, VERB_DETAIL = 0, 1, 2, 3
+LINKS_F32, LINKS_F16 = 1, 2
+
+
+class GlxError(RuntimeError):
+    pass
+
+
+class Verbose(C.Structure):
+    _fields_ = [("verbosity", C.c_int), ("verb_prefix", C.c_char_p), ("precond_verbosity", C.c_int),
+                ("precond_verb_prefix", C.c_char_p), ("out_buf", C.c_void_p), ("out_cap", C.c_size_t),
+                ("out_len", C.c_size_t), ("float_format", C.c_int), ("float_precision", C.c_int)]
+
+
+class InversionInfo(C.Structure):
+    _fields_ = [("resSq", C.c_double), ("iter", C.c_int), ("success", C.c_int), ("ops_count", C.c_int),
+                ("name", C.c_char * 32), ("resSqmrhs", C.c_void_p), ("n_rhs", C.c_int),
+                ("reliable_updates", C.c_int)]
+
+
+class StaggeredU1Op(C.Structure):
+    _fields_ = [("lattice", C.c_void_p), ("mass", C.c_double), ("x_fine", C.c_int), ("y_fine", C.c_int),
+                ("Nc", C.c_int), ("wilson_coeff", C.c_double)]
+
+
+class Stencil2D(C.Structure):
+    _fields_ = [("lx", C.c_int), ("ly", C.c_int), ("clover", C.c_void_p), ("hopping", C.c_void_p),
+                ("shift", C.c_double * 2), ("eo_shift", C.c_double * 2), ("dof_shift", C.c_double * 2)]
+
+
+@dataclass
+class Info:
+    """inversion_info (inverter_struct.h:14-89)."""
+    resSq: float
+    iter: int
+    success: bool
+    ops_count: int
+    name: str
+    resSqmrhs: list = field(default_factory=list)
+    reliable_updates: int = 0
+    text: str = ""
+
+
+_lib = None
+
+
+def lib():
+    """Load libglx.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GlxError(f"{LIB_PATH} is missing: build it with `make -C {PKG}`")
+        L = C.CDLL(LIB_PATH)
+        vp, i, d = C.c_void_p, C.c_int, C.c_double
+        VP = C.POINTER(Verbose)
+        IP = C.POINTER(InversionInfo)
+        L.glx_last_error.restype = C.c_char_p
+        for nm in ("cg", "bicgstab", "gcr"):
+            for t in ("r", "c"):
+                f = getattr(L, f"glx_minv_vector_{nm}_{t}")
+                f.restype = i
+                f.argtypes = [vp, vp, i, i, d, vp, vp, VP, IP]
+                f = getattr(L, f"glx_minv_vector_{nm}_restart_{t}")
+                f.restype = i
+                f.argtypes = [vp, vp, i, i, d, i, vp, vp, VP, IP]
+        for t in ("r", "c"):
+            f = getattr(L, f"glx_minv_vector_cg_m_{t}")
+            f.restype = i
+            f.argtypes = [vp, vp, i, i, i, i, d, vp, vp, vp, i, VP, IP]
+        for sym in set(OP_SYMBOLS.values()):
+            getattr(L, sym).argtypes = [vp, vp, vp]
+        L.glx_square_staggered_eoprec_prepare.argtypes = [vp, vp, vp]
+        L.glx_square_staggered_eoprec_reconstruct.argtypes = [vp, vp, vp, vp]
+        L.glx_apply_square_staggered_eoprec_prepare_stencil.argtypes = [vp, vp, vp]
+        L.glx_apply_square_staggered_eoprec_reconstruct_stencil.argtypes = [vp, vp, vp, vp]
+        for nm in ("glx_get_square_staggered_u1_stencil", "glx_get_square_staggered_gamma5_u1_stencil",
+                   "glx_get_square_staggered_dagger_u1_stencil"):
+            getattr(L, nm).restype = i
+            getattr(L, nm).argtypes = [C.POINTER(Stencil2D), C.POINTER(StaggeredU1Op)]
+        L.glx_register_operator.restype = i
+        L.glx_register_operator.argtypes = [vp, i, C.POINTER(StaggeredU1Op)]
+        L.glx_unregister_operator.restype = i
+        L.glx_unregister_operator.argtypes = [vp]
+        L.glx_lattice_create.restype = i
+        L.glx_lattice_create.argtypes = [C.POINTER(vp), vp, i, i, i]
+        L.glx_lattice_update_links.restype = i
+        L.glx_lattice_update_links.argtypes = [vp, vp]
+        L.glx_lattice_destroy.restype = i
+        L.glx_lattice_destroy.argtypes = [vp]
+        L.glx_dev_solve.restype = i
+        L.glx_dev_solve.argtypes = [i, vp, i, d, i, vp, vp, i, d, VP, IP]
+        L.glx_dev_cg_m.restype = i
+        L.glx_dev_cg_m.argtypes = [vp, i, d, i, vp, vp, i, vp, i, i, d, i, VP, IP]
+        L.glx_dev_cg_mixed.restype = i
+        L.glx_dev_cg_mixed.argtypes = [vp, i, d, vp, vp, i, d, d, i, VP, IP]
+        L.glx_dev_apply.restype = i
+        L.glx_dev_apply.argtypes = [vp, i, d, i, i, i, vp, vp]
+        L.glx_dev_time_apply.restype = i
+        L.glx_dev_time_apply.argtypes = [vp, i, d, i, i, vp, vp, i, C.POINTER(C.c_double)]
+        L.glx_dev_alloc.restype = i
+        L.glx_dev_alloc.argtypes = [C.POINTER(vp), C.c_size_t]
+        L.glx_dev_free.restype = i
+        L.glx_dev_free.argtypes = [vp]
+        L.glx_memcpy_htod.restype = i
+        L.glx_memcpy_htod.argtypes = [vp, vp, C.c_size_t]
+        L.glx_memcpy_dtoh.restype = i
+        L.glx_memcpy_dtoh.argtypes = [vp, vp, C.c_size_t]
+        L.glx_device_synchronize.restype = i
+        L.glx_device_count.restype = i
+        L.glx_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.glx_release_cache.restype = i
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise GlxError(f"{what} failed ({rc}): {lib().glx_last_error().decode()}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib().glx_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def op_callback(op: str) -> int:
+    """Address of the C callback for an operator name."""
+    return C.cast(getattr(lib(), OP_SYMBOLS[op]), C.c_void_p).value
+
+
+class _Extra:
+    """Keeps the extra_info struct (and the arrays it points at) alive."""
+
+    def __init__(self, op: str, links, mass: float, lx: int, ly: int, nc: int = 1):
+        self.links = None if links is None else np.ascontiguousarray(links, dtype=np.complex128)
+        self.st = StaggeredU1Op(None if self.links is None else self.links.ctypes.data, mass, lx, ly, nc, 0.0)
+        self.ptr = C.cast(C.pointer(self.st), C.c_void_p)
+        if op.startswith("stencil"):
+            V = lx * ly
+            self.clover = np.zeros(V, np.complex128)
+            self.hopping = np.zeros(4 * V, np.complex128)
+            self.sten = Stencil2D(lx, ly, self.clover.ctypes.data, self.hopping.ctypes.data)
+            builder = {"stencil_dagger": "glx_get_square_staggered_dagger_u1_stencil",
+                       "stencil_gamma5": "glx_get_square_staggered_gamma5_u1_stencil"}.get(
+                op, "glx_get_square_staggered_u1_stencil")
+            _check(getattr(lib(), builder)(C.byref(self.sten), C.byref(self.st)), builder)
+            self.ptr = C.cast(C.pointer(self.sten), C.c_void_p)
+
+
+class _Text:
+    def __init__(self, level: int, prefix: str, fmt: str):
+        self.cap = 1 << 22
+        self.buf = C.create_string_buffer(self.cap)
+        self.prefix = prefix.encode()
+        sci = "e" in fmt
+        prec = int(fmt.split(".")[1].rstrip("eg")) if "." in fmt else 6
+        self.v = Verbose(level, self.prefix, 0, b"", C.cast(self.buf, C.c_void_p), self.cap, 0, 1 if sci else 0, prec)
+
+    def text(self) -> str:
+        return C.string_at(C.cast(self.buf, C.c_void_p), self.v.out_len).decode()
+
+
+def _info(ci: InversionInfo, text: str = "", res=None) -> Info:
+    return Info(ci.resSq, ci.iter, bool(ci.success), ci.ops_count, ci.name.decode(),
+                [] if res is None else list(res), ci.reliable_updates, text)
+
+
+# --------------------------------------------------------------------------
+def apply(op: str, rhs, links=None, mass: float = 0.0, lx: int = 0, ly: int = 0, nc: int = 1):
+    """lhs = A rhs on the GPU through the registered operator callback."""
+    real = op == "laplace_real"
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64 if real else np.complex128)
+    lhs = np.empty_like(rhs)
+    ex = _Extra(op, links, mass, lx, ly, nc)
+    getattr(lib(), OP_SYMBOLS[op])(lhs.ctypes.data, rhs.ctypes.data, ex.ptr)
+    if np.isnan(lhs).any():
+        raise GlxError(f"{op} failed: {lib().glx_last_error().decode()}")
+    return lhs
+
+
+def eoprec_prepare(rhs, links, mass, lx, ly, stencil=False):
+    rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+    out = np.empty_like(rhs)
+    ex = _Extra("stencil" if stencil else "staggered", links, mass, lx, ly)
+    fn = lib().glx_apply_square_staggered_eoprec_prepare_stencil if stencil else lib().glx_square_staggered_eoprec_prepare
+    fn(out.ctypes.data, rhs.ctypes.data, ex.ptr)
+    return out
+
+
+def eoprec_reconstruct(lhs_e, rhs_o, links, mass, lx, ly, stencil=False):
+    lhs_e = np.ascontiguousarray(lhs_e, dtype=np.complex128)
+    rhs_o = np.ascontiguousarray(rhs_o, dtype=np.complex128)
+    out = np.empty_like(lhs_e)
+    ex = _Extra("stencil" if stencil else "staggered", links, mass, lx, ly)
+    fn = (lib().glx_apply_square_staggered_eoprec_reconstruct_stencil if stencil
+          else lib().glx_square_staggered_eoprec_reconstruct)
+    fn(out.ctypes.data, lhs_e.ctypes.data, rhs_o.ctypes.data, ex.ptr)
+    return out
+
+
+def stencil_coeffs(op: str, links, mass: float, lx: int, ly: int):
+    """(clover, hopping, shifts6) of get_square_staggered_*_u1_stencil."""
+    ex = _Extra(op, links, mass, lx, ly)
+    s = ex.sten
+    sh = np.array([s.shift[0], s.shift[1], s.eo_shift[0], s.eo_shift[1], s.dof_shift[0], s.dof_shift[1]])
+    return ex.clover.copy(), ex.hopping.copy(), sh
+
+
+def solve(solver: str, op: str, b, x0=None, *, links=None, mass=0.0, lx=0, ly=0, nc=1, max_iter=10000, eps=1e-10,
+          restart_freq=0, verb_level=0, prefix="", fmt="%g", callback=None, extra=None):
+    """minv_vector_{cg,bicgstab,gcr}[_restart] with host vectors.
+
+    `callback` / `extra` override the operator with a raw callback address and
+    extra_info pointer (e.g. a ctypes host callback)."""
+    real = op == "laplace_real"
+    dt = np.float64 if real else np.complex128
+    b = np.ascontiguousarray(b, dtype=dt)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=dt, copy=True)
+    ex = None if callback is not None else _Extra(op, links, mass, lx, ly, nc)
+    cb = callback if callback is not None else op_callback(op)
+    ep = extra if callback is not None else ex.ptr
+    tx = _Text(verb_level, prefix, fmt)
+    ci = InversionInfo()
+    t = "r" if real else "c"
+    L = lib()
+    if restart_freq > 0:
+        rc = getattr(L, f"glx_minv_vector_{solver}_restart_{t}")(
+            x.ctypes.data, b.ctypes.data, b.size, max_iter, eps, restart_freq, cb, ep, C.byref(tx.v), C.byref(ci))
+    else:
+        rc = getattr(L, f"glx_minv_vector_{solver}_{t}")(
+            x.ctypes.data, b.ctypes.data, b.size, max_iter, eps, cb, ep, C.byref(tx.v), C.byref(ci))
+    _check(rc, f"minv_vector_{solver}")
+    return x, _info(ci, tx.text())
+
+
+def minv_vector_cg(*a, **k):
+    return solve("cg", *a, **k)
+
+
+def minv_vector_bicgstab(*a, **k):
+    return solve("bicgstab", *a, **k)
+
+
+def minv_vector_gcr(*a, **k):
+    return solve("gcr", *a, **k)
+
+
+def minv_vector_cg_m(op: str, b, shifts, *, links=None, mass=0.0, lx=0, ly=0, nc=1, resid_freq_check=10,
+                     max_iter=10000, eps=1e-10, worst_first=False, verb_level=0, prefix="", fmt="%g"):
+    """Multishift CG.  Returns (X[n_shift, size], Info with resSqmrhs, shifts)."""
+    real = op == "laplace_real"
+    dt = np.float64 if real else np.complex128
+    b = np.ascontiguousarray(b, dtype=dt)
+    sh = np.array(shifts, dtype=np.float64, copy=True)
+    n = sh.size
+    X = np.zeros((n, b.size), dtype=dt)
+    res = np.zeros(n)
+    ex = _Extra(op, links, mass, lx, ly, nc)
+    tx = _Text(verb_level, prefix, fmt)
+    ci = InversionInfo()
+    ci.resSqmrhs = res.ctypes.data
+    ci.n_rhs = n
+    ptrs = (C.c_void_p * n)(*[X[k].ctypes.data for k in range(n)])
+    f = lib().glx_minv_vector_cg_m_r if real else lib().glx_minv_vector_cg_m_c
+    _check(f(ptrs, b.ctypes.data, n, b.size, resid_freq_check, max_iter, eps, sh.ctypes.data, op_callback(op), ex.ptr,
+             int(worst_first), C.byref(tx.v), C.byref(ci)), "minv_vector_cg_m")
+    return X, _info(ci, tx.text(), res), sh
+
+
+def register_operator(callback_addr: int, op: str, lx=0, ly=0, mass=0.0, nc=1, links=None):
+    """Bind a host callback to a device operator (glx_register_operator)."""
+    st = None
+    keep = None
+    if lx > 0:
+        keep = None if links is None else np.ascontiguousarray(links, dtype=np.complex128)
+        st = StaggeredU1Op(None if keep is None else keep.ctypes.data, mass, lx, ly, nc, 0.0)
+    _check(lib().glx_register_operator(callback_addr, OP_KINDS[op], None if st is None else C.byref(st)),
+           "glx_register_operator")
+    return keep
+
+
+def unregister_operator(callback_addr: int):
+    _check(lib().glx_unregister_operator(callback_addr), "glx_unregister_operator")
+
+
+# ---------------------------------------------------------------- device --
+class DeviceArray:
+    """A device buffer holding a numpy array's bytes."""
+
+    def __init__(self, arr=None, *, nbytes: int = 0, dtype=np.complex128, shape=None):
+        self.dtype = np.dtype(dtype if arr is None else np.asarray(arr).dtype)
+        self.shape = shape if arr is None else np.asarray(arr).shape
+        self.nbytes = nbytes if arr is None else np.asarray(arr).nbytes
+        p = C.c_void_p()
+        _check(lib().glx_dev_alloc(C.byref(p), max(self.nbytes, 1)), "glx_dev_alloc")
+        self.ptr = p.value
+        if arr is not None:
+            self.upload(arr)
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes == self.nbytes
+        _check(lib().glx_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "glx_memcpy_htod")
+
+    def download(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        _check(lib().glx_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes), "glx_memcpy_dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().glx_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Lattice:
+    """A gauge field resident in HBM (FP64 + optional FP32 / FP16 copies)."""
+
+    def __init__(self, links, lx: int, ly: int, f32: bool = True, f16: bool = True):
+        self.lx, self.ly = lx, ly
+        self.links = np.ascontiguousarray(links, dtype=np.complex128)
+        h = C.c_void_p()
+        flags = (LINKS_F32 if f32 else 0) | (LINKS_F16 if f16 else 0)
+        _check(lib().glx_lattice_create(C.byref(h), self.links.ctypes.data, lx, ly, flags), "glx_lattice_create")
+        self.h = h.value
+
+    def update(self, links):
+        self.links = np.ascontiguousarray(links, dtype=np.complex128)
+        _check(lib().glx_lattice_update_links(self.h, self.links.ctypes.data), "glx_lattice_update_links")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().glx_lattice_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dev_solve(solver: str, lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_iter=10000, eps=1e-10,
+              nc=1, verb_level=0, prefix="", fmt="%g") -> Info:
+    tx = _Text(verb_level, prefix, fmt)
+    ci = InversionInfo()
+    _check(lib().glx_dev_solve(SOLVER_IDS[solver], None if lat is None else lat.h, OP_KINDS[op], mass, nc, x.ptr,
+                               b.ptr, max_iter, eps, C.byref(tx.v), C.byref(ci)), "glx_dev_solve")
+    return _info(ci, tx.text())
+
+
+def dev_cg_mixed(lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_iter=100000, eps=1e-10, delta=0.1,
+                 link_prec=LINKS_F32, verb_level=0) -> Info:
+    tx = _Text(verb_level, "", "%g")
+    ci = InversionInfo()
+    _check(lib().glx_dev_cg_mixed(lat.h, OP_KINDS[op], mass, x.ptr, b.ptr, max_iter, eps, delta, link_prec,
+                                  C.byref(tx.v), C.byref(ci)), "glx_dev_cg_mixed")
+    return _info(ci, tx.text())
+
+
+def dev_cg_m(lat, op: str, mass: float, xs, b: DeviceArray, shifts, resid_freq_check=10, max_iter=10000, eps=1e-10,
+             worst_first=False, nc=1) -> Info:
+    sh = np.ascontiguousarray(shifts, dtype=np.float64)
+    n = sh.size
+    res = np.zeros(n)
+    ci = InversionInfo()
+    ci.resSqmrhs = res.ctypes.data
+    ci.n_rhs = n
+    ptrs = (C.c_void_p * n)(*[x.ptr for x in xs])
+    _check(lib().glx_dev_cg_m(None if lat is None else lat.h, OP_KINDS[op], mass, nc, ptrs, b.ptr, n, sh.ctypes.data,
+                              resid_freq_check, max_iter, eps, int(worst_first), None, C.byref(ci)), "glx_dev_cg_m")
+    return _info(ci, "", res)
+
+
+def dev_apply(lat, op: str, mass: float, out: DeviceArray, inp: DeviceArray, fp32=False, link_prec=LINKS_F32, nc=1):
+    _check(lib().glx_dev_apply(None if lat is None else lat.h, OP_KINDS[op], mass, nc, int(fp32), link_prec, out.ptr,
+                               inp.ptr), "glx_dev_apply")
+
+
+def time_apply(lat, op: str, mass: float, out: DeviceArray, inp: DeviceArray, reps=100, fp32=False,
+               link_prec=LINKS_F32) -> float:
+    ms = C.c_double(0.0)
+    _check(lib().glx_dev_time_apply(lat.h, OP_KINDS[op], mass, int(fp32), link_prec, out.ptr, inp.ptr, reps,
+                                    C.byref(ms)), "glx_dev_time_apply")
+    return ms.value
+
+
+def synchronize():
+    _check(lib().glx_device_synchronize(), "glx_device_synchronize")
+
+
+def release_cache():
+    _check(lib().glx_release_cache(), "glx_release_cache")

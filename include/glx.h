@@ -1,0 +1,273 @@
+/*
+ * glx.h -- C ABI of the MI355X-native (gfx950, HIP) solver path of
+ * generic-linalg: Krylov solvers (CG, CG(n), multishift CG, BiCGStab,
+ * BiCGStab(n), GCR, GCR(n)), the 2D Laplace and U(1) staggered operators
+ * they call, and a mixed-precision CG with reliable updates.
+ *
+ * Plain C types only: complex<double> vectors are passed as double* pointing
+ * at interleaved (re, im) pairs, the layout of std::complex<double>.  Every
+ * function returns a glx_status (0 = success); glx_last_error() describes the
+ * last failure of the calling thread.
+ *
+ * Two families of entry points:
+ *   glx_minv_vector_*   host pointers, the reference's semantics one for one
+ *                       (iteration count, success flag, ops_count, resSq,
+ *                       verbosity lines).  When `matrix_vector` is one of the
+ *                       registered device operators below, the whole solve
+ *                       runs on the GPU; any other callback is called on the
+ *                       host for each operator application, with the Krylov
+ *                       vectors and reductions still on the device.
+ *   glx_dev_*           device pointers (data already resident in HBM), for
+ *                       callers that keep fields on the GPU across solves.
+ *
+ * Nothing here falls back to a CPU implementation: without a usable GPU the
+ * functions return GLX_ERR_HIP.
+ */
+#ifndef GLX_H
+#define GLX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLX_ABI_VERSION 1
+
+typedef enum {
+  GLX_OK = 0,
+  GLX_ERR_ARG = 1,      /* invalid argument */
+  GLX_ERR_HIP = 2,      /* HIP runtime failure (no device, out of memory, ...) */
+  GLX_ERR_INTERNAL = 3
+} glx_status;
+
+/* Levels of inversion_verbose_level (verbosity.h:9-16). */
+enum {
+  GLX_VERB_PASS_THROUGH = -1,
+  GLX_VERB_NONE = 0,
+  GLX_VERB_SUMMARY = 1,
+  GLX_VERB_RESTART_DETAIL = 2,
+  GLX_VERB_DETAIL = 3
+};
+
+/* inversion_verbose_struct (verbosity.h:18-24) as plain data.  The solver
+ * prints the reference's lines (verbosity.h:73-129) to std::cout with its
+ * current formatting, or, when out_buf is set, appends them to out_buf
+ * (NUL-terminated, out_len updated) using float_format: 0 = "%g",
+ * 1 = "%.<float_precision>e". */
+typedef struct {
+  int verbosity;
+  const char* verb_prefix;
+  int precond_verbosity;
+  const char* precond_verb_prefix;
+  char* out_buf;
+  size_t out_cap;
+  size_t out_len;
+  int float_format;
+  int float_precision;
+} glx_verbose;
+
+/* inversion_info (inverter_struct.h:14-89) as plain data.  resSqmrhs, when
+ * non-null, is a caller buffer of n_rhs doubles that multishift solves fill. */
+typedef struct {
+  double resSq;
+  int iter;
+  int success;
+  int ops_count;
+  char name[32];
+  double* resSqmrhs;
+  int n_rhs;
+  int reliable_updates; /* mixed-precision CG only */
+} glx_inversion_info;
+
+/* staggered_u1_op (operators.h:7-15), same field order.  lattice points at
+ * 2*x_fine*y_fine complex links, AoS lattice[2*(y*x_fine + x) + mu]. */
+typedef struct {
+  double* lattice;
+  double mass;
+  int x_fine;
+  int y_fine;
+  int Nc;
+  double wilson_coeff;
+} glx_staggered_u1_op;
+
+/* The operator callback contract of the reference (generic_cg.h:18-19):
+ * lhs = A rhs, overwriting all `size` entries of lhs. */
+typedef void (*glx_matvec)(double* lhs, double* rhs, void* extra_info);
+
+/* ------------------------------------------------------------------------
+ * Registered device operators.  Passing one of these as `matrix_vector`
+ * runs the solve on the GPU; calling one directly applies the operator on
+ * the GPU to host vectors.  extra_info is a glx_staggered_u1_op*.
+ * ---------------------------------------------------------------------- */
+/* real (double) Laplacian, diag 4 + mass, Nc colours
+ * (tests/multishift/multishift.cpp:634-672) */
+void glx_square_laplace_real(double* lhs, double* rhs, void* extra_info);
+void glx_square_laplace(double* lhs, double* rhs, void* extra_info);              /* operators.cpp:28-66 */
+void glx_square_laplace_u1(double* lhs, double* rhs, void* extra_info);           /* operators.cpp:73-120 */
+void glx_square_staggered_u1(double* lhs, double* rhs, void* extra_info);         /* operators.cpp:184-239 */
+void glx_gamma_5(double* lhs, double* rhs, void* extra_info);                     /* operators.cpp:242-259 */
+void glx_square_staggered_gamma5_u1(double* lhs, double* rhs, void* extra_info);  /* operators.cpp:316-368 */
+void glx_square_staggered_dagger_u1(double* lhs, double* rhs, void* extra_info);  /* operators.cpp:372-441 */
+void glx_square_staggered_normal_u1(double* lhs, double* rhs, void* extra_info);  /* operators.cpp:444-453 */
+void glx_square_staggered_deo_u1(double* lhs, double* rhs, void* extra_info);     /* operators.cpp:456-496 */
+void glx_square_staggered_doe_u1(double* lhs, double* rhs, void* extra_info);     /* operators.cpp:499-539 */
+void glx_square_staggered_m2mdeodoe_u1(double* lhs, double* rhs, void* extra_info); /* operators.cpp:566-588 */
+/* operators.cpp:543-563 and :592-616 */
+void glx_square_staggered_eoprec_prepare(double* rhs_e, double* rhs_orig, void* extra_info);
+void glx_square_staggered_eoprec_reconstruct(double* lhs_full, double* lhs_e, double* rhs_o, void* extra_info);
+
+/* stencil_2d (coarse_stencil.h:33-154) for Nc = 1 as plain data: clover[V],
+ * hopping[4V] with blocks {+x, +y, -x, -y}, complex shifts as (re, im). */
+typedef struct {
+  int lx, ly;
+  double* clover;
+  double* hopping;
+  double shift[2];
+  double eo_shift[2];
+  double dof_shift[2];
+} glx_stencil_2d;
+/* builders (operators_stencil.cpp:14-169); clover and hopping must point at
+ * caller storage of V and 4V complex values */
+int glx_get_square_staggered_u1_stencil(glx_stencil_2d* st, const glx_staggered_u1_op* op);
+int glx_get_square_staggered_gamma5_u1_stencil(glx_stencil_2d* st, const glx_staggered_u1_op* op);
+int glx_get_square_staggered_dagger_u1_stencil(glx_stencil_2d* st, const glx_staggered_u1_op* op);
+/* operators with extra_info = glx_stencil_2d* (coarse_stencil.cpp:12-722,
+ * operators_stencil.cpp:179-236) */
+void glx_apply_stencil_2d(double* lhs, double* rhs, void* stencil);
+void glx_apply_stencil_2d_eo(double* lhs, double* rhs, void* stencil);
+void glx_apply_stencil_2d_oe(double* lhs, double* rhs, void* stencil);
+void glx_apply_square_staggered_m2mdeodoe_stencil(double* lhs, double* rhs, void* stencil);
+void glx_apply_square_staggered_eoprec_prepare_stencil(double* rhs_e, double* rhs_orig, void* stencil);
+void glx_apply_square_staggered_eoprec_reconstruct_stencil(double* lhs_full, double* lhs_e, double* rhs_o, void* stencil);
+
+/* Operator kinds (numbering of the device kernels). */
+enum {
+  GLX_OP_LAPLACE_REAL = 0,
+  GLX_OP_LAPLACE = 1,
+  GLX_OP_LAPLACE_U1 = 2,
+  GLX_OP_STAGGERED = 3,
+  GLX_OP_STAGGERED_DAGGER = 4,
+  GLX_OP_STAGGERED_NORMAL = 5,
+  GLX_OP_STAGGERED_GAMMA5 = 6,
+  GLX_OP_GAMMA5 = 7,
+  GLX_OP_DEO = 8,
+  GLX_OP_DOE = 9,
+  GLX_OP_M2MDEODOE = 10,
+  GLX_OP_STENCIL = 11,
+  GLX_OP_STENCIL_EO = 14,
+  GLX_OP_STENCIL_OE = 15,
+  GLX_OP_STENCIL_M2MDEODOE = 16
+};
+
+/* Bind a caller's own host callback to a device operator, so that solves
+ * given that callback run on the GPU (e.g. square_laplace.cpp:182-222, whose
+ * extra_info is NULL: pass its lattice size and diagonal shift here).  With
+ * params == NULL, extra_info of each solve is read as glx_staggered_u1_op*. */
+int glx_register_operator(glx_matvec callback, int kind, const glx_staggered_u1_op* params);
+int glx_unregister_operator(glx_matvec callback);
+
+/* ------------------------------------------------------------------------
+ * Solvers, host pointers (generic_cg.h:18-23, generic_cg_m.h:16-17,
+ * generic_bicgstab.h:17-21, generic_gcr.h:39-44).  _r: real double vectors;
+ * _c: complex vectors as interleaved doubles.  verbosity may be NULL.
+ * ---------------------------------------------------------------------- */
+int glx_minv_vector_cg_r(double* phi, double* phi0, int size, int max_iter, double eps, glx_matvec matrix_vector,
+                         void* extra_info, glx_verbose* verbosity, glx_inversion_info* info);
+int glx_minv_vector_cg_c(double* phi, double* phi0, int size, int max_iter, double eps, glx_matvec matrix_vector,
+                         void* extra_info, glx_verbose* verbosity, glx_inversion_info* info);
+int glx_minv_vector_cg_restart_r(double* phi, double* phi0, int size, int max_iter, double eps, int restart_freq,
+                                 glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
+                                 glx_inversion_info* info);
+int glx_minv_vector_cg_restart_c(double* phi, double* phi0, int size, int max_iter, double eps, int restart_freq,
+                                 glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
+                                 glx_inversion_info* info);
+int glx_minv_vector_bicgstab_r(double* phi, double* phi0, int size, int max_iter, double eps,
+                               glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
+                               glx_inversion_info* info);
+int glx_minv_vector_bicgstab_c(double* phi, double* phi0, int size, int max_iter, double eps,
+                               glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
+                               glx_inversion_info* info);
+int glx_minv_vector_bicgstab_restart_r(double* phi, double* phi0, int size, int max_iter, double eps,
+                                       int restart_freq, glx_matvec matrix_vector, void* extra_info,
+                                       glx_verbose* verbosity, glx_inversion_info* info);
+int glx_minv_vector_bicgstab_restart_c(double* phi, double* phi0, int size, int max_iter, double eps,
+                                       int restart_freq, glx_matvec matrix_vector, void* extra_info,
+                                       glx_verbose* verbosity, glx_inversion_info* info);
+int glx_minv_vector_gcr_r(double* phi, double* phi0, int size, int max_iter, double eps, glx_matvec matrix_vector,
+                          void* extra_info, glx_verbose* verbosity, glx_inversion_info* info);
+int glx_minv_vector_gcr_c(double* phi, double* phi0, int size, int max_iter, double eps, glx_matvec matrix_vector,
+                          void* extra_info, glx_verbose* verbosity, glx_inversion_info* info);
+int glx_minv_vector_gcr_restart_r(double* phi, double* phi0, int size, int max_iter, double eps, int restart_freq,
+                                  glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
+                                  glx_inversion_info* info);
+int glx_minv_vector_gcr_restart_c(double* phi, double* phi0, int size, int max_iter, double eps, int restart_freq,
+                                  glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
+                                  glx_inversion_info* info);
+/* Multishift CG: phi[n_shift] host vectors (zeroed, then solutions);
+ * shifts[n_shift] is read (left unchanged on return, as the reference
+ * restores it); info->resSqmrhs (n_shift doubles) receives the per-shift
+ * squared true residuals; info->resSq is 0 as in the reference. */
+int glx_minv_vector_cg_m_r(double** phi, double* phi0, int n_shift, int size, int resid_freq_check, int max_iter,
+                           double eps, double* shifts, glx_matvec matrix_vector, void* extra_info, int worst_first,
+                           glx_verbose* verbosity, glx_inversion_info* info);
+int glx_minv_vector_cg_m_c(double** phi, double* phi0, int n_shift, int size, int resid_freq_check, int max_iter,
+                           double eps, double* shifts, glx_matvec matrix_vector, void* extra_info, int worst_first,
+                           glx_verbose* verbosity, glx_inversion_info* info);
+
+/* ------------------------------------------------------------------------
+ * Device-resident API.
+ * ---------------------------------------------------------------------- */
+/* A gauge field resident in HBM: FP64 links, plus FP32 and FP16 (+ per-site
+ * FP32 scale) copies when flags include GLX_LINKS_F32 / GLX_LINKS_F16.
+ * links is a host array of 2*lx*ly complex values (AoS as above). */
+enum { GLX_LINKS_F32 = 1, GLX_LINKS_F16 = 2 };
+typedef struct glx_lattice_s* glx_lattice;
+int glx_lattice_create(glx_lattice* lat, const double* links, int lx, int ly, int flags);
+int glx_lattice_update_links(glx_lattice lat, const double* links);
+int glx_lattice_destroy(glx_lattice lat);
+
+/* Solver selection for glx_dev_solve. */
+enum { GLX_SOLVER_CG = 0, GLX_SOLVER_BICGSTAB = 1, GLX_SOLVER_GCR = 2 };
+/* x_dev (in: guess, out: solution) and b_dev are device vectors of
+ * lx*ly*nc elements (double for GLX_OP_LAPLACE_REAL, complex otherwise).
+ * lat may be NULL for operators without links. */
+int glx_dev_solve(int solver, glx_lattice lat, int kind, double mass, int nc, double* x_dev, const double* b_dev,
+                  int max_iter, double eps, glx_verbose* verbosity, glx_inversion_info* info);
+int glx_dev_cg_m(glx_lattice lat, int kind, double mass, int nc, double** x_dev, const double* b_dev, int n_shift,
+                 const double* shifts, int resid_freq_check, int max_iter, double eps, int worst_first,
+                 glx_verbose* verbosity, glx_inversion_info* info);
+/* Mixed-precision CG (no reference counterpart): FP32 Krylov vectors with
+ * FP32 (link_prec = 1) or FP16 + scale (link_prec = 2) links; FP64 true
+ * residual r = b - A x and FP64 solution accumulation at every reliable
+ * update, triggered when the iterated residual falls below delta times its
+ * maximum since the last update.  Stops when the FP64 true relative
+ * residual is below eps. */
+int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
+                     double eps, double delta, int link_prec, glx_verbose* verbosity, glx_inversion_info* info);
+/* out = A in on device vectors; fp32 selects complex<float> vectors with the
+ * FP32 (link_prec 1) or FP16 (link_prec 2) links. */
+int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int link_prec, void* out_dev,
+                  const void* in_dev);
+/* Average duration in ms of `reps` back-to-back applications, timed with HIP
+ * events on the stream the kernels run on. */
+int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int link_prec, void* out_dev,
+                       const void* in_dev, int reps, double* ms_per_apply);
+
+/* Device memory helpers (so that callers need no HIP headers). */
+int glx_dev_alloc(void** ptr, size_t bytes);
+int glx_dev_free(void* ptr);
+int glx_memcpy_htod(void* dst_dev, const void* src_host, size_t bytes);
+int glx_memcpy_dtoh(void* dst_host, const void* src_dev, size_t bytes);
+int glx_device_synchronize(void);
+int glx_device_count(int* count);
+/* Frees the library's cached device buffers. */
+int glx_release_cache(void);
+const char* glx_last_error(void);
+int glx_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLX_H */

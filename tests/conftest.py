@@ -1,0 +1,35 @@
+"""pytest configuration: the `gpu` marker and import paths.
+
+GPU tests call the HIP library through its C ABI and compare with the
+oracle (oracle/orc.py) on the same seeded inputs.  Run them with
+``pytest -m gpu`` on a machine with an MI355X; ``-m "not gpu"`` runs the CPU
+suite (oracle pinning, host logic, library load/exports).
+"""
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "generic-linalg_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
+
+
+@pytest.fixture(scope="session")
+def glx():
+    import glx as g
+    g.lib()
+    assert g.device_count() > 0, "no GPU visible: GPU tests need an MI355X"
+    return g
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import orc as o
+    o.orc_lib()
+    return o

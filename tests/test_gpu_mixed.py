@@ -1,0 +1,69 @@
+"""Mixed-precision CG with reliable updates (no reference counterpart).
+
+FP32 Krylov vectors with FP32 or FP16 (+ per-site scale) links; the FP64
+true residual r = b - A x is recomputed at every reliable update and the
+solve stops on it.  Checks: the returned FP64 true relative residual is
+<= tol; the solution agrees with the FP64 oracle's to the accuracy the
+tolerance allows (relative error <= 100 * tol * cond-ish factor, stated per
+case); the device's reported residual equals the oracle's recomputation.
+"""
+import numpy as np
+import pytest
+
+from helpers import lattice_inputs, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("link_prec", [1, 2])
+@pytest.mark.parametrize("L,mass,beta", [(64, 0.05, 4.0), (48, 0.1, 6.25), (130, 0.05, 4.0)])
+def test_mixed_cg_staggered(glx, orc, L, mass, beta, link_prec):
+    links, bc, _ = lattice_inputs(orc, L, L, beta=beta, seed=31 + L)
+    lat = glx.Lattice(links, L, L)
+    db = glx.DeviceArray(bc)
+    dx = glx.DeviceArray(np.zeros_like(bc))
+    info = glx.dev_cg_mixed(lat, "staggered_normal", mass, dx, db, max_iter=20000, eps=1e-10, delta=0.1,
+                            link_prec=link_prec)
+    x = dx.download()
+    assert info.success
+    assert info.reliable_updates >= 1
+    # independent check of the true residual with the oracle's FP64 operator
+    r = bc - orc.apply("staggered_normal", x, links, mass, L, L)
+    relres = np.linalg.norm(r) / np.linalg.norm(bc)
+    assert relres <= 1e-10 * 1.001, relres
+    assert abs(np.sqrt(info.resSq) / np.linalg.norm(bc) - relres) <= 1e-3 * relres
+    xo, io = orc.solve("cg", "staggered_normal", bc, None, links=links, mass=mass, lx=L, ly=L, max_iter=20000,
+                       eps=1e-10)
+    # both solutions satisfy |A x - b| <= 1e-10 |b|; their difference is
+    # bounded by cond(A) * 2e-10 with cond(A) <= (m^2 + 2) / m^2
+    cond = (mass * mass + 2.0) / (mass * mass)
+    assert relerr(x, xo) <= 2e-10 * cond, (relerr(x, xo), cond)
+    # iteration inflation of the FP32 Krylov space stays modest
+    assert info.iter <= 1.5 * io.iter + 20, (info.iter, io.iter)
+
+
+def test_mixed_cg_real_laplace(glx, orc):
+    L = 64
+    g = orc.Rng(4)
+    b = g.gaussian_real(L * L)
+    dx = glx.DeviceArray(np.zeros_like(b))
+    db = glx.DeviceArray(b)
+    lat = glx.Lattice(np.ones(2 * L * L, np.complex128), L, L)
+    info = glx.dev_cg_mixed(lat, "laplace_real", 0.01, dx, db, max_iter=20000, eps=1e-10)
+    x = dx.download()
+    r = b - orc.apply("laplace_real", x, None, 0.01, L, L)
+    assert info.success and np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10 * 1.001
+
+
+def test_mixed_cg_max_iter(glx, orc):
+    """An unconverged mixed solve reports success = false and its true residual."""
+    L = 32
+    links, bc, _ = lattice_inputs(orc, L, L, beta=4.0, seed=2)
+    lat = glx.Lattice(links, L, L)
+    db = glx.DeviceArray(bc)
+    dx = glx.DeviceArray(np.zeros_like(bc))
+    info = glx.dev_cg_mixed(lat, "staggered_normal", 0.02, dx, db, max_iter=10, eps=1e-10)
+    x = dx.download()
+    assert not info.success
+    r = bc - orc.apply("staggered_normal", x, links, 0.02, L, L)
+    assert abs(np.sqrt(info.resSq) - np.linalg.norm(r)) <= 1e-6 * np.linalg.norm(r)
