@@ -1,11 +1,12 @@
 // glx_common.h -- device-side building blocks shared by every kernel of the
 // library: complex arithmetic for the two precisions, wave64 + LDS
-// reductions, the deterministic last-block finalisation that turns per-block
+// reductions, the deterministic finalisation kernel that turns per-block
 // partial sums into solver scalars on the device, and HIP error handling.
 //
 // Target: gfx950 (MI355X), wave64.  All reductions accumulate in FP64 and are
-// bit-reproducible for a given launch shape: partial sums are combined in a
-// fixed order by the last block to finish, never with floating-point atomics.
+// bit-reproducible for a given launch shape: per-block partial sums are
+// combined in block order by a one-block kernel, never with floating-point
+// atomics.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -179,52 +180,41 @@ __device__ __forceinline__ void block_sum(double (&v)[NR], double (*sm)[kWaves])
   }
 }
 
-// Reduction workspace: partial sums [nblocks * NR] and one arrival counter.
+// Reduction workspace: partial sums [nblocks * NR].
 struct RedWs {
   double* partials;
-  unsigned* counter;
 };
 
-__device__ __forceinline__ double load_coherent(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Block-level part of a grid reduction.  Every block of the grid must call it
-// exactly once with its per-thread partials.  The last block to arrive sums
-// the per-block partials in block order (deterministic), then thread 0 calls
-// fin(total) and re-arms the counter.  `bid`/`nblk` are the linear block id
-// and count of the grid.
-template <int NR, class Fin>
-__device__ __forceinline__ void grid_reduce(double (&v)[NR], RedWs ws, int bid, int nblk, Fin fin) {
+// Block-level part of a grid reduction: every block of the grid stores its
+// NR partial sums (plain stores, no device-scope fence).  A one-block
+// finalisation kernel launched right after (k_finalize) sums them in block
+// order -- deterministic for a given launch shape -- and hands the totals to
+// the solver's finaliser.  Kernel boundaries make the partials visible
+// across XCDs, which avoids an L2 write-back per block.
+template <int NR>
+__device__ __forceinline__ void block_partials(double (&v)[NR], RedWs ws, int bid) {
   __shared__ double sm[NR > 0 ? NR : 1][kWaves];
-  __shared__ int last;
   block_sum<NR>(v, sm);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NR; k++)
-      __hip_atomic_store(&ws.partials[bid * NR + k], v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned t = atomicAdd(ws.counter, 1u);
-    last = (t == unsigned(nblk - 1));
+    for (int k = 0; k < NR; k++) ws.partials[bid * NR + k] = v[k];
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double acc[NR > 0 ? NR : 1];
+}
+
+template <int NR, class Fin>
+__global__ __launch_bounds__(kBlock) void k_finalize(int nblk, Fin fin, RedWs ws, const int* __restrict__ done) {
+  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  __shared__ double sm[NR][kWaves];
+  double acc[NR];
 #pragma unroll
   for (int k = 0; k < NR; k++) acc[k] = 0.0;
   // each thread sums a fixed strided subset in increasing block order
   for (int b = threadIdx.x; b < nblk; b += kBlock) {
 #pragma unroll
-    for (int k = 0; k < NR; k++) acc[k] += load_coherent(&ws.partials[b * NR + k]);
+    for (int k = 0; k < NR; k++) acc[k] += ws.partials[b * NR + k];
   }
-  __syncthreads();
   block_sum<NR>(acc, sm);
-  if (threadIdx.x == 0) {
-    fin(acc);
-    __threadfence();
-    atomicExch(ws.counter, 0u);
-  }
+  if (threadIdx.x == 0) fin(acc);
 }
 
 // -------------------------------------------------------------- geometry --

@@ -65,16 +65,11 @@ class DevBuf {
 constexpr int kMaxPointBlocks = 2048;
 constexpr int kMaxNR = 16;
 
-// Reduction workspace for one solve.  Counters are zeroed once and re-armed
-// by the kernels themselves.
+// Reduction workspace for one solve: per-block partial sums.
 struct RedSpace {
-  DevBuf partials, counter;
-  void init(int max_blocks, hipStream_t st) {
-    partials = DevBuf(size_t(max_blocks) * kMaxNR * sizeof(double));
-    counter = DevBuf(64);
-    GLX_HIP(hipMemsetAsync(counter.get(), 0, 64, st));
-  }
-  RedWs ws() const { return RedWs{partials.as<double>(), counter.as<unsigned>()}; }
+  DevBuf partials;
+  void init(int max_blocks, hipStream_t) { partials = DevBuf(size_t(max_blocks) * kMaxNR * sizeof(double)); }
+  RedWs ws() const { return RedWs{partials.as<double>()}; }
 };
 
 // -------------------------------------------------------------- operator --
@@ -201,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_point(int n, F f, Fin fin, RedWs ws,
   for (int k = 0; k < NR; k++) part[k] = 0.0;
   const int stride = gridDim.x * kBlock;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) f(i, part);
-  if constexpr (NR > 0) grid_reduce<NR>(part, ws, blockIdx.x, gridDim.x, fin);
+  if constexpr (NR > 0) block_partials<NR>(part, ws, blockIdx.x);
 }
 
 inline int point_blocks(int n) {
@@ -216,8 +211,13 @@ inline int point_blocks(int n) {
 template <class F, class Fin>
 void launch_point(hipStream_t st, int n, F f, Fin fin, RedWs ws, const int* done) {
   static_assert(F::NR >= 0 && F::NR <= kMaxNR, "too many partial sums");
-  hipLaunchKernelGGL((k_point<F::NR, F, Fin>), dim3(point_blocks(n)), dim3(kBlock), 0, st, n, f, fin, ws, done);
+  const int nb = point_blocks(n);
+  hipLaunchKernelGGL((k_point<F::NR, F, Fin>), dim3(nb), dim3(kBlock), 0, st, n, f, fin, ws, done);
   GLX_CHECK_LAUNCH();
+  if constexpr (F::NR > 0) {
+    hipLaunchKernelGGL((k_finalize<F::NR, Fin>), dim3(1), dim3(kBlock), 0, st, nb, fin, ws, done);
+    GLX_CHECK_LAUNCH();
+  }
 }
 
 // ---------------------------------------------------- plain operator apply --

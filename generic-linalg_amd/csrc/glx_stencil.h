@@ -14,8 +14,9 @@
 // vector updates (e.g. CG's p = r + beta p).  Phase 1 (H = 2 only) forms
 // q = K p on the tile plus a one-site halo in LDS.  Phase 2 writes the owned
 // outputs and feeds them to an epilogue functor that accumulates per-thread
-// FP64 partial sums; a deterministic grid reduction then hands the totals to
-// a finaliser that runs on one device thread (the solver's scalar update).
+// FP64 partial sums; per-block sums are stored and a one-block kernel adds
+// them in block order and hands the totals to a finaliser that runs on one
+// device thread (the solver's scalar update).
 //
 // Staggered operator (operators.cpp:184-239): D = m - K,
 //   K psi(x) = 1/2 sum_mu eta_mu(x) [U_mu(x) psi(x+mu) - U_mu(x-mu)^* psi(x-mu)],
@@ -422,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Geom g, OpDev<LK> op, In in,
       epi.acc(part, i, pin, v);
     }
   }
-  if constexpr (Epi::NR > 0) grid_reduce<Epi::NR>(part, ws, bid, g.nblk, fin);
+  if constexpr (Epi::NR > 0) block_partials<Epi::NR>(part, ws, bid);
 }
 
 inline Geom make_geom(int lx, int ly) {
@@ -442,6 +443,10 @@ void launch_stencil(hipStream_t st, const Geom& g, const OpDev<LK>& op, In in, T
   hipLaunchKernelGGL((k_stencil<KIND, T, LK, In, Epi, Fin>), dim3(g.nblk), dim3(kBlock), 0, st, g, op, in, out, epi,
                      fin, ws, done);
   GLX_CHECK_LAUNCH();
+  if constexpr (Epi::NR > 0) {
+    hipLaunchKernelGGL((k_finalize<Epi::NR, Fin>), dim3(1), dim3(kBlock), 0, st, g.nblk, fin, ws, done);
+    GLX_CHECK_LAUNCH();
+  }
 }
 
 }  // namespace glx

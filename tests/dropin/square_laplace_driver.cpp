@@ -1,0 +1,113 @@
+// Drop-in check: a driver written against the reference's C++ interface
+// (the pattern of square_laplace.cpp and tests/multishift/multishift.cpp),
+// compiled against generic-linalg_amd/compat/ and linked with libglx.so
+// instead of the reference's object files.  It solves with the library's
+// registered operators (whole solve on the GPU) and with its own host
+// callback (Krylov vectors on the GPU, operator on the host), prints the
+// solvers' own verbosity lines and the explicit residuals.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <iomanip>
+#include <iostream>
+#include <random>
+
+#include "generic_inverters.h"
+#include "operators.h"
+
+// the driver's own operator: the real Laplacian with a compile-time size
+// (square_laplace.cpp:182-222 with N = 32, MASS = 0.01)
+#define N 32
+#define MASS 0.01
+void square_laplacian(double* lhs, double* rhs, void*) {
+  for (int i = 0; i < N * N; i++) {
+    int x = i % N, y = i / N;
+    lhs[i] = 0.0;
+    lhs[i] = lhs[i] - rhs[y * N + ((x + 1) % N)];
+    lhs[i] = lhs[i] - rhs[y * N + ((x + N - 1) % N)];
+    lhs[i] = lhs[i] - rhs[((y + 1) % N) * N + x];
+    lhs[i] = lhs[i] - rhs[((y + N - 1) % N) * N + x];
+    lhs[i] = lhs[i] + (4 + MASS) * rhs[i];
+  }
+}
+
+int main() {
+  std::cout << std::setiosflags(std::ios::scientific) << std::setprecision(6);
+  const int L = 32, V = L * L;
+  std::mt19937 gen(1337u);
+  std::normal_distribution<> dist(0.0, 0.4);
+  complex<double>* lattice = new complex<double>[2 * V];
+  for (int i = 0; i < 2 * V; i++) lattice[i] = std::polar(1.0, dist(gen));
+  staggered_u1_op stagif;
+  stagif.lattice = lattice;
+  stagif.mass = 0.05;
+  stagif.x_fine = L;
+  stagif.y_fine = L;
+  stagif.Nc = 1;
+  stagif.wilson_coeff = 0.0;
+
+  complex<double>* b = new complex<double>[V];
+  complex<double>* x = new complex<double>[V];
+  complex<double>* chk = new complex<double>[V];
+  std::normal_distribution<> g1(0.0, 1.0);
+  for (int i = 0; i < V; i++) {
+    b[i] = complex<double>(g1(gen), g1(gen));
+    x[i] = 0.0;
+  }
+  inversion_verbose_struct verb;
+  verb.verbosity = VERB_SUMMARY;
+  verb.verb_prefix = "[CG]: ";
+  verb.precond_verbosity = VERB_NONE;
+  verb.precond_verb_prefix = "";
+
+  // 1. CG on the staggered normal operator: registered device operator
+  inversion_info invif = minv_vector_cg(x, b, V, 10000, 1e-10, square_staggered_normal_u1, &stagif, &verb);
+  square_staggered_normal_u1(chk, x, &stagif);
+  double r = 0.0, bn = 0.0;
+  for (int i = 0; i < V; i++) {
+    r += std::norm(chk[i] - b[i]);
+    bn += std::norm(b[i]);
+  }
+  std::cout << "[TEST]: cg " << invif.name << " iter " << invif.iter << " ops " << invif.ops_count << " success "
+            << invif.success << " relres " << std::sqrt(r / bn) << "\n";
+
+  // 2. BiCGStab on D, restarted GCR on D
+  for (int i = 0; i < V; i++) x[i] = 0.0;
+  verb.verb_prefix = "[BICGSTAB]: ";
+  invif = minv_vector_bicgstab(x, b, V, 10000, 1e-10, square_staggered_u1, &stagif, &verb);
+  std::cout << "[TEST]: bicgstab iter " << invif.iter << " success " << invif.success << "\n";
+  for (int i = 0; i < V; i++) x[i] = 0.0;
+  verb.verb_prefix = "[GCR]: ";
+  invif = minv_vector_gcr_restart(x, b, V, 10000, 1e-10, 32, square_staggered_u1, &stagif, &verb);
+  std::cout << "[TEST]: gcr_restart " << invif.name << " success " << invif.success << "\n";
+
+  // 3. multishift CG on the gauged Laplacian; the shifts array is restored
+  double shifts[4] = {0.0, 0.01, 0.005, 0.1};
+  complex<double>* xs[4];
+  for (int k = 0; k < 4; k++) xs[k] = new complex<double>[V];
+  stagif.mass = 0.001;
+  verb.verb_prefix = "[CG-M]: ";
+  invif = minv_vector_cg_m(xs, b, 4, V, 10, 10000, 1e-10, shifts, square_laplace_u1, &stagif, true, &verb);
+  std::cout << "[TEST]: cg_m iter " << invif.iter << " resSq " << invif.resSq << " shifts " << shifts[1] << " "
+            << shifts[2] << " resSqmrhs0 " << invif.resSqmrhs[0] << "\n";
+
+  // 4. the driver's own host callback (real Laplacian)
+  double* br = new double[N * N];
+  double* xr = new double[N * N];
+  for (int i = 0; i < N * N; i++) {
+    br[i] = g1(gen);
+    xr[i] = 0.0;
+  }
+  verb.verb_prefix = "[HOST-CB]: ";
+  invif = minv_vector_cg(xr, br, N * N, 4000, 1e-10, square_laplacian, nullptr, &verb);
+  double* cr = new double[N * N];
+  square_laplacian(cr, xr, nullptr);
+  double rr = 0.0, bb = 0.0;
+  for (int i = 0; i < N * N; i++) {
+    rr += (cr[i] - br[i]) * (cr[i] - br[i]);
+    bb += br[i] * br[i];
+  }
+  std::cout << "[TEST]: host_callback iter " << invif.iter << " success " << invif.success << " relres "
+            << std::sqrt(rr / bb) << "\n";
+  return 0;
+}
