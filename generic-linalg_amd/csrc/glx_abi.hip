@@ -23,6 +23,9 @@ using namespace glx;
 
 // ================================================================ errors ==
 static thread_local std::string g_last_error;
+// launch tuning of the single-pass and mixed-precision CG (glx_set_tuning)
+static int g_group = 8;
+static int g_use_graph = 1;
 
 template <class F>
 static int guarded(F&& f) {
@@ -722,6 +725,8 @@ int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, cons
     so.max_iter = max_iter;
     so.eps = eps;
     so.want_hist = pr.level() == GLX_VERB_DETAIL;
+    so.group = g_group;
+    so.use_graph = g_use_graph != 0;
     MixedOpts mo;
     mo.link_prec = link_prec == LINKS_F16 ? LINKS_F16 : LINKS_F32;
     mo.delta = delta;
@@ -729,6 +734,34 @@ int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, cons
     for (int k = 0; k < int(r.relres_hist.size()); k++) pr.resid("CG-mixed", k + 1, 2 + k, r.relres_hist[k]);
     pr.summary("CG-mixed", r.success, r.iter, r.ops_count, std::sqrt(r.resSq) / r.bsqrt);
     fill_info(info, r, "CG-mixed");
+  });
+}
+
+int glx_set_tuning(int group, int use_graph) {
+  return guarded([&] {
+    if (group < 1 || group > 1024) throw ArgError("group must be in [1, 1024]");
+    g_group = group;
+    g_use_graph = use_graph ? 1 : 0;
+  });
+}
+
+int glx_dev_cg_fused(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
+                     double eps, glx_verbose* verb, glx_inversion_info* info) {
+  return guarded([&] {
+    if (!x_dev || !b_dev) throw ArgError("x_dev and b_dev must not be NULL");
+    DevOperator o = lattice_op(lat, kind, mass, 1);
+    if (o.lx <= 0) throw ArgError("operator without links needs a lattice for its dimensions");
+    Printer pr(verb);
+    SolveOpts so;
+    so.max_iter = max_iter;
+    so.eps = eps;
+    so.want_hist = pr.level() == GLX_VERB_DETAIL;
+    so.group = g_group;
+    so.use_graph = g_use_graph != 0;
+    SolveResult r = dev_cg_fused(o, x_dev, b_dev, so);
+    for (int k = 0; k < int(r.relres_hist.size()); k++) pr.resid("CG-fused", k + 1, 2 + k, r.relres_hist[k]);
+    pr.summary("CG-fused", r.success, r.iter, r.ops_count, std::sqrt(r.resSq) / r.bsqrt);
+    fill_info(info, r, "CG-fused");
   });
 }
 
@@ -759,13 +792,13 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
     for (int k = 0; k < reps; k++) {
       if (fp32)
         run_stencil<c64>(nullptr, o, link_prec, InPlain<c64>{static_cast<const c64*>(in_dev)},
-                         static_cast<c64*>(out_dev), EpiNone<c64>{}, FinNone{}, red.ws(), nullptr);
+                         static_cast<c64*>(out_dev), EpiNone<c64>{}, FinNone{}, red.ws(), kNoGate);
       else if (kind == OP_LAPLACE_REAL)
         run_stencil<double>(nullptr, o, LINKS_F64, InPlain<double>{static_cast<const double*>(in_dev)},
-                            static_cast<double*>(out_dev), EpiNone<double>{}, FinNone{}, red.ws(), nullptr);
+                            static_cast<double*>(out_dev), EpiNone<double>{}, FinNone{}, red.ws(), kNoGate);
       else
         run_stencil<c128>(nullptr, o, LINKS_F64, InPlain<c128>{static_cast<const c128*>(in_dev)},
-                          static_cast<c128*>(out_dev), EpiNone<c128>{}, FinNone{}, red.ws(), nullptr);
+                          static_cast<c128*>(out_dev), EpiNone<c128>{}, FinNone{}, red.ws(), kNoGate);
     }
     GLX_HIP(hipEventRecord(b, nullptr));
     GLX_HIP(hipEventSynchronize(b));

@@ -201,9 +201,20 @@ __device__ __forceinline__ void block_partials(double (&v)[NR], RedWs ws, int bi
   }
 }
 
+// A kernel's gate: it runs only while the solver's control word equals
+// `want` (0: iterating; 2: a reliable update is pending); a null flag always
+// runs.  Every block reads the same word, written by an earlier kernel.
+struct Gate {
+  const int* flag;
+  int want;
+};
+__device__ __forceinline__ bool gate_closed(Gate g) {
+  return g.flag && __hip_atomic_load(g.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g.want;
+}
+
 template <int NR, class Fin>
-__global__ __launch_bounds__(kBlock) void k_finalize(int nblk, Fin fin, RedWs ws, const int* __restrict__ done) {
-  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+__global__ __launch_bounds__(kBlock) void k_finalize(int nblk, Fin fin, RedWs ws, Gate gate) {
+  if (gate_closed(gate)) return;
   __shared__ double sm[NR][kWaves];
   double acc[NR];
 #pragma unroll

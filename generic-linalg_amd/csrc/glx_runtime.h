@@ -65,6 +65,8 @@ class DevBuf {
 constexpr int kMaxPointBlocks = 2048;
 constexpr int kMaxNR = 16;
 
+constexpr Gate kNoGate{nullptr, 0};
+
 // Reduction workspace for one solve: per-block partial sums.
 struct RedSpace {
   DevBuf partials;
@@ -124,7 +126,7 @@ constexpr bool kind_valid_for(int k) {
 // Launch the tiled stencil kernel for a runtime operator kind.
 template <class T, class LK, class In, class Epi, class Fin>
 void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T* out, Epi epi, Fin fin, RedWs ws,
-                   const int* done) {
+                   Gate done) {
   const Geom g = make_geom(o.lx, o.ly);
   const OpDev<LK> od = make_opdev(o, lk);
 #define GLX_CASE(K)                                                                     \
@@ -161,7 +163,7 @@ void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T*
 // vectors; FP32 or FP16 links for FP32 vectors.
 template <class T, class In, class Epi, class Fin>
 void run_stencil(hipStream_t st, const DevOperator& o, int link_prec, In in, T* out, Epi epi, Fin fin, RedWs ws,
-                 const int* done) {
+                 Gate done) {
   if (op_links(o.kind)) {
     if constexpr (std::is_same<typename Real<T>::type, double>::value) {
       if (!o.u64) throw ArgError("operator has no FP64 links on the device");
@@ -188,8 +190,8 @@ void run_stencil(hipStream_t st, const DevOperator& o, int link_prec, In in, T* 
 // f(i, part) updates element i and adds its share to the F::NR partial sums
 // (part has exactly that many entries).
 template <int NR, class F, class Fin>
-__global__ __launch_bounds__(kBlock) void k_point(int n, F f, Fin fin, RedWs ws, const int* __restrict__ done) {
-  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+__global__ __launch_bounds__(kBlock) void k_point(int n, F f, Fin fin, RedWs ws, Gate gate) {
+  if (gate_closed(gate)) return;
   f.init();
   double part[NR > 0 ? NR : 1];
 #pragma unroll
@@ -209,7 +211,7 @@ inline int point_blocks(int n) {
 // The number of partial sums is the functor's own F::NR, so a launch can
 // never disagree with what the functor accumulates.
 template <class F, class Fin>
-void launch_point(hipStream_t st, int n, F f, Fin fin, RedWs ws, const int* done) {
+void launch_point(hipStream_t st, int n, F f, Fin fin, RedWs ws, Gate done) {
   static_assert(F::NR >= 0 && F::NR <= kMaxNR, "too many partial sums");
   const int nb = point_blocks(n);
   hipLaunchKernelGGL((k_point<F::NR, F, Fin>), dim3(nb), dim3(kBlock), 0, st, n, f, fin, ws, done);
@@ -224,7 +226,7 @@ void launch_point(hipStream_t st, int n, F f, Fin fin, RedWs ws, const int* done
 // out = A in on the device (no fusion).  For a host callback the vector goes
 // through host memory; the callback is skipped when `done` is set on device.
 void apply_operator(hipStream_t st, const DevOperator& o, int link_prec, void* out, const void* in, int elem_bytes,
-                    RedSpace& red, const int* done);
+                    RedSpace& red, Gate done);
 
 // Minimum number of workgroups a stencil launch can use on this lattice.
 inline int stencil_blocks(int lx, int ly) { return make_geom(lx, ly).nblk; }

@@ -37,6 +37,8 @@ struct SolveOpts {
   bool want_hist = false;   // copy the per-iteration residuals back
   hipStream_t stream = nullptr;
   int chunk = 16;           // iterations enqueued between convergence checks
+  int group = 8;            // single-pass CG: iterations per enqueued group
+  bool use_graph = true;    // single-pass CG: replay one captured hipGraph per group
 };
 
 // x (in: initial guess, out: solution) and b are device pointers of the
@@ -59,6 +61,12 @@ struct MixedOpts {
   double delta = 0.1;
 };
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m);
+
+// FP64 CG in the single-pass form the mixed solver uses (one fused stencil
+// per iteration, |r_{k+1}|^2 from the recurrence): the same algorithm as
+// dev_cg with a different rounding, for callers that want the fastest FP64
+// solve rather than the reference's exact sequence of operations.
+SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const SolveOpts& o);
 
 // Plain operator application on device vectors.
 void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st);

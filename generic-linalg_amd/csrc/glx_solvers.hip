@@ -140,7 +140,8 @@ struct Work {
     GLX_HIP(hipStreamSynchronize(st));
     return h;
   }
-  const int* done() const { return &ds->done; }
+  Gate done() const { return Gate{&ds->done, 0}; }
+  Gate paused() const { return Gate{&ds->done, 2}; }
   RedWs ws() const { return red.ws(); }
   std::vector<double> hist_host(int n) {
     std::vector<double> v(std::max(n, 0));
@@ -215,12 +216,15 @@ Mirror run_loop(Work& w, int max_iter, int chunk, Enq enq, Pause pause) {
 // ================================================== generic functors ======
 // Pointwise kernel that materialises a prologue over the whole vector (used
 // when the operator cannot take a fused prologue: host callbacks, Nc > 1).
-template <class In>
+template <class T, class In>
 struct PtFromIn {
   static constexpr int NR = 0;
   In in;
   __device__ void init() { in.init(); }
-  __device__ void operator()(int i, double*) { (void)in.load(i, true); }
+  __device__ void operator()(int i, double*) {
+    T aux;
+    (void)in.load(i, true, aux);
+  }
 };
 template <class T, class Epi>
 struct PtFromEpi {
@@ -229,14 +233,14 @@ struct PtFromEpi {
   const T* in;
   const T* out;
   __device__ void init() {}
-  __device__ void operator()(int i, double* part) { epi.acc(part, i, in[i], out[i]); }
+  __device__ void operator()(int i, double* part) { epi.acc(part, i, in[i], out[i], in[i]); }
 };
 
 // Nc > 1 Laplacian (colour index fastest), one thread per entry.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_laplace_nc(int lx, int ly, int nc, double m2, const T* __restrict__ in,
-                                                       T* __restrict__ out, const int* done) {
-  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+                                                       T* __restrict__ out, Gate gate) {
+  if (gate_closed(gate)) return;
   const int n = lx * ly * nc;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const int c = i % nc, s = i / nc, x = s % lx, y = s / lx;
@@ -252,15 +256,15 @@ __global__ __launch_bounds__(kBlock) void k_laplace_nc(int lx, int ly, int nc, d
 }
 
 template <class T>
-void apply_typed(hipStream_t st, const DevOperator& o, int link_prec, T* out, const T* in, RedWs ws, const int* done) {
+void apply_typed(hipStream_t st, const DevOperator& o, int link_prec, T* out, const T* in, RedWs ws, Gate done) {
   if (o.is_host()) {
     const size_t bytes = size_t(o.size()) * sizeof(T);
     std::vector<char> hin(bytes), hout(bytes);
-    int hdone = 0;
+    int hdone = done.want;
     GLX_HIP(hipMemcpyAsync(hin.data(), in, bytes, hipMemcpyDeviceToHost, st));
-    if (done) GLX_HIP(hipMemcpyAsync(&hdone, done, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (done.flag) GLX_HIP(hipMemcpyAsync(&hdone, done.flag, sizeof(int), hipMemcpyDeviceToHost, st));
     GLX_HIP(hipStreamSynchronize(st));
-    if (hdone) return;
+    if (hdone != done.want) return;
     o.host_cb(reinterpret_cast<double*>(hout.data()), reinterpret_cast<double*>(hin.data()), o.host_extra);
     GLX_HIP(hipMemcpyAsync(out, hout.data(), bytes, hipMemcpyHostToDevice, st));
     GLX_HIP(hipStreamSynchronize(st));
@@ -286,7 +290,7 @@ void run_fused(Work& w, const DevOperator& o, int link_prec, In in, const T* mat
     run_stencil<T>(w.st, o, link_prec, in, out, epi, fin, w.ws(), w.done());
     return;
   }
-  launch_point(w.st, o.size(), PtFromIn<In>{in}, FinNone{}, w.ws(), w.done());
+  launch_point(w.st, o.size(), PtFromIn<T, In>{in}, FinNone{}, w.ws(), w.done());
   apply_typed<T>(w.st, o, link_prec, out, mat, w.ws(), w.done());
   if constexpr (Epi::NR > 0) {
     launch_point(w.st, o.size(), PtFromEpi<T, Epi>{epi, mat, out}, fin, w.ws(), w.done());
@@ -328,8 +332,8 @@ struct PtAxpyAlpha {
 
 template <class T>
 double true_residual(Work& w, const DevOperator& o, T* x, const T* b, T* tmp) {
-  apply_typed<T>(w.st, o, LINKS_F64, tmp, x, w.ws(), nullptr);
-  launch_point(w.st, o.size(), PtDiffNorm<T>{tmp, b}, FinResSq{w.ds, -1}, w.ws(), nullptr);
+  apply_typed<T>(w.st, o, LINKS_F64, tmp, x, w.ws(), kNoGate);
+  launch_point(w.st, o.size(), PtDiffNorm<T>{tmp, b}, FinResSq{w.ds, -1}, w.ws(), kNoGate);
   return 0.0;
 }
 
@@ -361,6 +365,7 @@ struct FinCgInit {
 // x += alpha_prev p_old (the previous iteration's solution update).
 template <class T, class TX>
 struct InCG {
+  static constexpr bool kAux = false;
   const T* r;
   const T* pold;
   T* pnew;
@@ -372,7 +377,8 @@ struct InCG {
     beta = s->beta_r;
     ap = as_scal<T>(s->alpha_prev);
   }
-  __device__ T load(int j, bool own) {
+  __device__ T* target(T* out) const { return out; }
+  __device__ T load(int j, bool own, T&) {
     const T po = pold[j];
     const T pn = r[j] + rscal(beta, po);
     if (own) {
@@ -452,8 +458,8 @@ SolveResult cg_impl(const DevOperator& o, T* x, const T* b, const SolveOpts& opt
   T* Ap = apb.as<T>();
   GLX_HIP(hipMemsetAsync(p[0], 0, n * sizeof(T), w.st));
   // r = b - A x; ||b||, ||r||^2
-  apply_typed<T>(w.st, o, LINKS_F64, Ap, x, w.ws(), nullptr);
-  launch_point(w.st, n, PtCgInit<T>{r, b, Ap}, FinCgInit{w.ds}, w.ws(), nullptr);
+  apply_typed<T>(w.st, o, LINKS_F64, Ap, x, w.ws(), kNoGate);
+  launch_point(w.st, n, PtCgInit<T>{r, b, Ap}, FinCgInit{w.ds}, w.ws(), kNoGate);
 
   auto enq = [&](int k) {
     T* pold = p[k & 1];
@@ -468,7 +474,7 @@ SolveResult cg_impl(const DevOperator& o, T* x, const T* b, const SolveOpts& opt
   int kf = opt.max_iter > 0 ? m.kfinal : 0;
   if (opt.max_iter > 0) {
     // the last iteration's x += alpha p (deferred into the next prologue)
-    launch_point(w.st, n, PtAxpyAlpha<T>{x, p[(kf + 1) & 1], w.ds}, FinNone{}, w.ws(), nullptr);
+    launch_point(w.st, n, PtAxpyAlpha<T>{x, p[(kf + 1) & 1], w.ds}, FinNone{}, w.ws(), kNoGate);
   }
   true_residual<T>(w, o, x, b, Ap);
   SolveState hs = w.download();
@@ -527,13 +533,15 @@ struct FinBicgAlpha {
 // s = r - alpha Ap
 template <class T>
 struct InBicgS {
+  static constexpr bool kAux = false;
   const T* r;
   const T* Ap;
   T* sv;
   const SolveState* st;
   typename Scal<T>::type a;
   __device__ void init() { a = as_scal<T>(st->alpha); }
-  __device__ T load(int j, bool own) {
+  __device__ T* target(T* out) const { return out; }
+  __device__ T load(int j, bool own, T&) {
     const T v = r[j] - scal(a, Ap[j]);
     if (own) sv[j] = v;
     return v;
@@ -604,6 +612,7 @@ struct FinBicgUpd {
 // p_new = r + beta (p_old - omega Ap_old)
 template <class T>
 struct InBicgP {
+  static constexpr bool kAux = false;
   const T* r;
   const T* pold;
   const T* Apold;
@@ -614,7 +623,8 @@ struct InBicgP {
     be = as_scal<T>(st->beta);
     om = as_scal<T>(st->omega);
   }
-  __device__ T load(int j, bool own) {
+  __device__ T* target(T* out) const { return out; }
+  __device__ T load(int j, bool own, T&) {
     const T v = r[j] + scal(be, pold[j] - scal(om, Apold[j]));
     if (own) pnew[j] = v;
     return v;
@@ -639,9 +649,8 @@ SolveResult bicgstab_impl(const DevOperator& o, T* x, const T* b, const SolveOpt
   T* Ap[2] = {aB0.as<T>(), aB1.as<T>()};
   T* sv = sB.as<T>();
   T* As = asB.as<T>();
-  apply_typed<T>(w.st, o, LINKS_F64, Ap[0], x, w.ws(), nullptr);
-  launch_point(w.st, n, PtBicgInit<T>{r, r0, p[0], b, Ap[0]}, FinBicgInit<T>{w.ds}, w.ws(),
-                                        nullptr);
+  apply_typed<T>(w.st, o, LINKS_F64, Ap[0], x, w.ws(), kNoGate);
+  launch_point(w.st, n, PtBicgInit<T>{r, r0, p[0], b, Ap[0]}, FinBicgInit<T>{w.ds}, w.ws(), kNoGate);
   run_fused<T>(w, o, LINKS_F64, InPlain<T>{p[0]}, p[0], Ap[0], EpiDotAuxOut<T>{r0}, FinBicgAlpha<T>{w.ds});
 
   auto enq = [&](int k) {
@@ -848,9 +857,9 @@ SolveResult gcr_impl(const DevOperator& o, T* x, const T* b, const SolveOpts& op
   hp.reserve(cap);
   hap.reserve(cap);
   add_dir();
-  apply_typed<T>(w.st, o, LINKS_F64, tmpB.as<T>(), x, w.ws(), nullptr);
-  launch_point(w.st, n, PtGcrInit<T>{r, hp[0], b, tmpB.as<T>()}, FinBsqrt{w.ds}, w.ws(), nullptr);
-  apply_typed<T>(w.st, o, LINKS_F64, hap[0], hp[0], w.ws(), nullptr);
+  apply_typed<T>(w.st, o, LINKS_F64, tmpB.as<T>(), x, w.ws(), kNoGate);
+  launch_point(w.st, n, PtGcrInit<T>{r, hp[0], b, tmpB.as<T>()}, FinBsqrt{w.ds}, w.ws(), kNoGate);
+  apply_typed<T>(w.st, o, LINKS_F64, hap[0], hp[0], w.ws(), kNoGate);
   c128* betas = betasB.as<c128>();
   double* norms = normsB.as<double>();
   auto enq = [&](int k) {
@@ -1096,7 +1105,7 @@ SolveResult cg_m_impl(const DevOperator& o, T* const* xs, const T* b, int n_shif
   T* Ap = apB.as<T>();
   GLX_HIP(hipMemcpyAsync(r, b, vb, hipMemcpyDeviceToDevice, w.st));
   GLX_HIP(hipMemcpyAsync(p, b, vb, hipMemcpyDeviceToDevice, w.st));
-  launch_point(w.st, n, PtNormB<T>{b}, FinCgmInit{w.ds}, w.ws(), nullptr);
+  launch_point(w.st, n, PtNormB<T>{b}, FinCgmInit{w.ds}, w.ws(), kNoGate);
   run_fused<T>(w, o, LINKS_F64, InPlain<T>{p}, p, Ap, EpiDotInOut<T>{}, FinCgmBeta<T>{w.ds});
   auto enq = [&](int) {
     launch_point(w.st, n, PtCgmX<T>{xtab, ptab, r, Ap, w.ds}, FinCgmR<T>{w.ds}, w.ws(), w.done());
@@ -1108,8 +1117,8 @@ SolveResult cg_m_impl(const DevOperator& o, T* const* xs, const T* b, int n_shif
   Mirror m = run_loop(w, opt.max_iter, opt.chunk, enq, [](const Mirror&) { return -1; });
   const int kf = opt.max_iter > 0 ? m.kfinal : 0;
   for (int k = 0; k < n_shift; k++) {
-    apply_typed<T>(w.st, o, LINKS_F64, Ap, xs[k], w.ws(), nullptr);
-    launch_point(w.st, n, PtShiftRes<T>{Ap, xs[k], b, shifts[k]}, FinResSq{w.ds, k}, w.ws(), nullptr);
+    apply_typed<T>(w.st, o, LINKS_F64, Ap, xs[k], w.ws(), kNoGate);
+    launch_point(w.st, n, PtShiftRes<T>{Ap, xs[k], b, shifts[k]}, FinResSq{w.ds, k}, w.ws(), kNoGate);
   }
   SolveState hs = w.download();
   SolveResult res;
@@ -1125,58 +1134,205 @@ SolveResult cg_m_impl(const DevOperator& o, T* const* xs, const T* b, int n_shif
   return res;
 }
 
-// ===================================================== mixed precision ====
-// FP64 r = b - A x (into r64), r32 = r64, ||r||^2 -> reliable update decision
+// ======================================== single-pass CG (FP64 and mixed) ===
+// One fused stencil launch per iteration.  At iteration k (buffers chosen by
+// the device's own k parity: old o = (k+1)&1, new n = k&1):
+//   prologue  r_k = r_{k-1} - alpha_{k-1} Ap_{k-1};  p_k = r_k + beta_{k-1} p_{k-1}
+//             owned sites: store r_k, p_k and x += alpha_{k-1} p_{k-1}
+//   stencil   Ap_k = A p_k
+//   epilogue  <p,Ap>, |Ap|^2, <Ap,r>, |r|^2 (direct) in one set of partials
+//   finaliser alpha_k = |r_k|^2 / <p,Ap>;
+//             |r_{k+1}|^2 = |r_k|^2 - 2 Re(conj(alpha) <Ap,r>) + |alpha|^2 |Ap|^2
+//             beta_k = |r_{k+1}|^2 / |r_k|^2; convergence on |r_{k+1}|.
+// The mixed solver runs this in FP32 (links FP32 or FP16) and, when the
+// iterated residual has dropped by delta since the last update (or reached
+// the tolerance), pauses (control word 2); the gated reliable-update kernels
+// then fold the FP32 solution into the FP64 one, recompute r = b - A x in
+// FP64 with FP64 links, restart the recurrence from it and resume -- all on
+// the device, so the host only enqueues groups of iterations (or replays one
+// captured hipGraph of a group) and reads the control word per group.
+
+// Two slots per vector in one allocation of 2n elements; slot c starts at
+// element (c & 1) * n.  Offset arithmetic rather than a pointer array: a
+// dynamically indexed member array of a kernel argument is copied to per-lane
+// scratch memory.
+template <class T>
+struct FastBufs {
+  T* r;
+  T* p;
+  T* ap;
+  long n;
+  __host__ __device__ __forceinline__ T* rs(int c) const { return r + (c & 1) * n; }
+  __host__ __device__ __forceinline__ T* ps(int c) const { return p + (c & 1) * n; }
+  __host__ __device__ __forceinline__ T* aps(int c) const { return ap + (c & 1) * n; }
+};
+
+template <class T, class TX>
+struct InFast {
+  static constexpr bool kAux = true;
+  FastBufs<T> b;
+  TX* x;
+  const SolveState* s;
+  const T* ro;
+  const T* apo;
+  const T* po;
+  T* rn;
+  T* pn;
+  T* apn;
+  typename Scal<T>::type a;
+  double beta;
+  __device__ void init() {
+    const int k = s->k, o = (k + 1) & 1, n = k & 1;
+    ro = b.rs(o);
+    apo = b.aps(o);
+    po = b.ps(o);
+    rn = b.rs(n);
+    pn = b.ps(n);
+    apn = b.aps(n);
+    a = as_scal<T>(s->alpha);
+    beta = s->beta_r;
+  }
+  __device__ T* target(T*) const { return apn; }
+  __device__ T load(int j, bool own, T& aux) {
+    const T pold = po[j];
+    const T r = ro[j] - scal(a, apo[j]);
+    const T p = r + rscal(beta, pold);
+    if (own) {
+      rn[j] = r;
+      pn[j] = p;
+      aux = r;
+      if constexpr (std::is_same<T, TX>::value) {
+        x[j] = x[j] + scal(a, pold);
+      } else {
+        TX d;
+        cvt(d, scal(a, pold));
+        x[j] = x[j] + d;
+      }
+    }
+    return p;
+  }
+};
+
+// <p,Ap>, |Ap|^2, <Ap,r>, |r|^2
+template <class T>
+struct EpiFast {
+  static constexpr int NR = is_cx<T>::value ? 6 : 4;
+  __device__ __forceinline__ void acc(double* v, int, T p, T ap, T r) const {
+    if constexpr (is_cx<T>::value) {
+      const c128 d1 = dotc(p, ap), d2 = dotc(ap, r);
+      v[0] += d1.re;
+      v[1] += d1.im;
+      v[2] += norm2(ap);
+      v[3] += d2.re;
+      v[4] += d2.im;
+      v[5] += norm2(r);
+    } else {
+      v[0] += dotc(p, ap);
+      v[1] += norm2(ap);
+      v[2] += dotc(ap, r);
+      v[3] += norm2(r);
+    }
+  }
+};
+
+template <class T, bool MIXED>
+struct FinFast {
+  SolveState* s;
+  __device__ void operator()(const double* v) const {
+    c128 pap, apr;
+    double apap, rr;
+    if constexpr (is_cx<T>::value) {
+      pap = {v[0], v[1]};
+      apap = v[2];
+      apr = {v[3], v[4]};
+      rr = v[5];
+    } else {
+      pap = {v[0], 0.0};
+      apap = v[1];
+      apr = {v[2], 0.0};
+      rr = v[3];
+    }
+    const int k = s->k;
+    c128 alpha;
+    if constexpr (is_cx<T>::value) alpha = cdiv(c128{rr, 0.0}, pap);
+    else alpha = {rr / pap.re, 0.0};
+    // Re(conj(alpha) <Ap, r>)
+    const double re = alpha.re * apr.re + alpha.im * apr.im;
+    double rn = rr - 2.0 * re + (alpha.re * alpha.re + alpha.im * alpha.im) * apap;
+    if (!(rn > 0.0)) rn = 0.0;
+    s->alpha = alpha;
+    s->rsq = rr;
+    s->rsqNew = rn;
+    const double rnorm = sqrt(rn);
+    if (s->hist) s->hist[k] = rnorm / s->bsqrt;
+    if constexpr (MIXED) {
+      if (rnorm > s->maxrr) s->maxrr = rnorm;
+      if (rnorm < s->delta * s->maxrr || rnorm < s->eps * s->bsqrt || k == s->max_iter - 1) {
+        s->kfinal = k;
+        s->done = 2;
+        return;
+      }
+    } else {
+      if (rnorm < s->eps * s->bsqrt || k == s->max_iter - 1) {
+        s->kfinal = k;
+        s->done = 1;
+        return;
+      }
+    }
+    s->beta_r = rn / rr;
+    s->k = k + 1;
+  }
+};
+
+// setup: r_old = b - A x, Ap_old = 0, p_old = 0, with |b| and |r|
 template <class TD, class TS>
-struct PtMixInit {
+struct PtFastInit {
   static constexpr int NR = 2;
-  TD* r;
-  TS* rs;
+  TS* r;
+  TS* ap;
+  TS* p;
   const TD* b;
   const TD* ax;
   __device__ void init() {}
   __device__ void operator()(int i, double* part) {
     const TD bi = b[i];
     const TD ri = bi - ax[i];
-    r[i] = ri;
     TS v;
     cvt(v, ri);
-    rs[i] = v;
+    r[i] = v;
+    ap[i] = zero_of<TS>();
+    p[i] = zero_of<TS>();
     part[0] += norm2(bi);
     part[1] += norm2(ri);
   }
 };
-// iterated residual in FP32, decision for a reliable update
-struct FinMixNorm {
+struct FinFastInit {
   SolveState* s;
   __device__ void operator()(const double* v) const {
-    const double rn = v[0];
-    const int k = s->k;
-    const double rnorm = sqrt(rn);
-    if (s->hist) s->hist[k] = rnorm / s->bsqrt;
-    if (rnorm > s->maxrr) s->maxrr = rnorm;
-    s->rsqNew = rn;
-    if (rnorm < s->delta * s->maxrr || rnorm < s->eps * s->bsqrt || k == s->max_iter - 1) {
-      s->done = 2;  // host runs the FP64 reliable update
-      s->kfinal = k;
-      return;
-    }
-    s->beta_r = rn / s->rsq;
-    s->rsq = rn;
-    s->alpha_prev = s->alpha;
-    s->k = k + 1;
+    s->bsqrt = sqrt(v[0]);
+    s->rsq = v[1];
+    s->maxrr = sqrt(v[1]);
+    s->alpha = {0.0, 0.0};
+    s->beta_r = 0.0;
+    s->k = 0;
+    s->done = 0;
   }
 };
-// x64 += x32 + alpha p32; x32 = 0
+
+// reliable update, step 1 (gated on a pause): x64 += x32 + alpha_k p_k; x32 = 0
 template <class TD, class TS>
-struct PtMixAccum {
+struct PtFastAccum {
   static constexpr int NR = 0;
   TD* x;
   TS* xs;
-  const TS* p;
+  FastBufs<TS> bufs;
   const SolveState* s;
+  const TS* p;
   typename Scal<TD>::type a;
-  __device__ void init() { a = as_scal<TD>(s->alpha); }
+  __device__ void init() {
+    p = bufs.ps(s->k);
+    a = as_scal<TD>(s->alpha);
+  }
   __device__ void operator()(int i, double*) {
     TD xl, pl;
     cvt(xl, xs[i]);
@@ -1185,32 +1341,81 @@ struct PtMixAccum {
     xs[i] = zero_of<TS>();
   }
 };
-struct FinMixReliable {
+// step 3: r_true = b - A x in FP64, written (rounded) as the recurrence's
+// old residual r[k&1]; |r_true|^2 decides convergence
+template <class TD, class TS>
+struct PtFastTrue {
+  static constexpr int NR = 1;
+  FastBufs<TS> bufs;
+  const TD* b;
+  const TD* ax;
+  const SolveState* s;
+  TS* r;
+  __device__ void init() { r = bufs.rs(s->k); }
+  __device__ void operator()(int i, double* part) {
+    const TD d = b[i] - ax[i];
+    TS v;
+    cvt(v, d);
+    r[i] = v;
+    part[0] += norm2(d);
+  }
+};
+struct FinFastReliable {
   SolveState* s;
   __device__ void operator()(const double* v) const {
-    const double rn = v[1];  // PtMixInit: v[0] = ||b||^2, v[1] = ||r||^2
+    const double rt = v[0];
     const int k = s->k;
     s->reliable += 1;
-    s->resSq = rn;
-    if (sqrt(rn) < s->eps * s->bsqrt || k >= s->max_iter - 1) {
+    s->resSq = rt;
+    if (sqrt(rt) < s->eps * s->bsqrt || k >= s->max_iter - 1) {
       s->kfinal = k;
       s->done = 1;
       return;
     }
-    s->beta_r = rn / s->rsq;
-    s->rsq = rn;
-    s->maxrr = sqrt(rn);
-    s->alpha_prev = {0.0, 0.0};
+    // the next prologue forms r_{k+1} = r_true - 0 * Ap and p_{k+1} = r_true + beta p_k
+    s->beta_r = rt / s->rsq;
+    s->alpha = {0.0, 0.0};
+    s->maxrr = sqrt(rt);
     s->k = k + 1;
     s->done = 0;
   }
 };
+// FP64 single-pass CG: the pending alpha p after convergence
+template <class T>
+struct PtFastFinal {
+  static constexpr int NR = 0;
+  T* x;
+  FastBufs<T> bufs;
+  const SolveState* s;
+  const T* p;
+  typename Scal<T>::type a;
+  __device__ void init() {
+    p = bufs.ps(s->kfinal);
+    a = as_scal<T>(s->alpha);
+  }
+  __device__ void operator()(int i, double*) { x[i] = x[i] + scal(a, p[i]); }
+};
 
-template <class TD, class TS>
-SolveResult cg_mixed_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpts& opt, const MixedOpts& mo) {
-  if (!o.tiled()) throw ArgError("mixed-precision CG needs a device operator");
+// A non-default (blocking) stream owned by one solve: graph capture needs it,
+// and it still orders after work the caller queued on the null stream.
+struct OwnedStream {
+  hipStream_t st = nullptr;
+  OwnedStream() { GLX_HIP(hipStreamCreate(&st)); }
+  ~OwnedStream() {
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
+  }
+};
+
+template <class TD, class TS, bool MIXED>
+SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpts& opt, const MixedOpts& mo) {
+  if (!o.tiled()) throw ArgError("single-pass CG needs a device stencil operator");
   const int n = o.size();
-  Work w(opt.stream, stencil_blocks(o.lx, o.ly), opt.max_iter, opt.want_hist);
+  OwnedStream own;
+  hipStream_t st = opt.stream ? opt.stream : own.st;
+  Work w(st, stencil_blocks(o.lx, o.ly), opt.max_iter, opt.want_hist);
   SolveState h;
   std::memset(&h, 0, sizeof(h));
   h.max_iter = opt.max_iter;
@@ -1218,49 +1423,112 @@ SolveResult cg_mixed_impl(const DevOperator& o, TD* x, const TD* b, const SolveO
   h.delta = mo.delta;
   h.hist = w.hist.as<double>();
   w.upload(h);
-  DevBuf r64B(n * sizeof(TD)), t64B(n * sizeof(TD));
-  DevBuf rsB(n * sizeof(TS)), xsB(n * sizeof(TS)), p0B(n * sizeof(TS)), p1B(n * sizeof(TS)), apB(n * sizeof(TS));
-  TD* r64 = r64B.as<TD>();
-  TD* t64 = t64B.as<TD>();
-  TS* rs = rsB.as<TS>();
-  TS* xs = xsB.as<TS>();
-  TS* p[2] = {p0B.as<TS>(), p1B.as<TS>()};
-  TS* Ap = apB.as<TS>();
-  GLX_HIP(hipMemsetAsync(p[0], 0, n * sizeof(TS), w.st));
-  GLX_HIP(hipMemsetAsync(xs, 0, n * sizeof(TS), w.st));
-  apply_typed<TD>(w.st, o, LINKS_F64, t64, x, w.ws(), nullptr);
-  launch_point(w.st, n, PtMixInit<TD, TS>{r64, rs, b, t64}, FinCgInit{w.ds}, w.ws(), nullptr);
-  auto enq = [&](int k) {
-    TS* pold = p[k & 1];
-    TS* pnew = p[(k + 1) & 1];
-    run_stencil<TS>(w.st, o, mo.link_prec, InCG<TS, TS>{rs, pold, pnew, xs, w.ds}, Ap, EpiDotInOut<TS>{},
-                    FinCgAlpha<TS>{w.ds}, w.ws(), w.done());
-    launch_point(w.st, n, PtCgR<TS>{rs, Ap, w.ds}, FinMixNorm{w.ds}, w.ws(), w.done());
+  std::vector<DevBuf> keep;
+  auto vec = [&](size_t bytes) {
+    keep.emplace_back(bytes);
+    return keep.back().get();
   };
-  auto pause = [&](const Mirror& m) -> int {
-    // reliable update after iteration m.kfinal: fold x32 and the pending
-    // alpha p into x64, recompute r = b - A x in FP64, restart from r.
-    const int k = m.kfinal;
-    launch_point(w.st, n, PtMixAccum<TD, TS>{x, xs, p[(k + 1) & 1], w.ds}, FinNone{}, w.ws(), nullptr);
-    apply_typed<TD>(w.st, o, LINKS_F64, t64, x, w.ws(), nullptr);
-    launch_point(w.st, n, PtMixInit<TD, TS>{r64, rs, b, t64}, FinMixReliable{w.ds}, w.ws(), nullptr);
-    GLX_HIP(hipStreamSynchronize(w.st));
-    Mirror mm;
-    GLX_HIP(hipMemcpy(&mm, w.ds, sizeof(Mirror), hipMemcpyDeviceToHost));
-    if (mm.done == 1) return -1;
-    return mm.k;
+  FastBufs<TS> fb;
+  fb.n = n;
+  fb.r = static_cast<TS*>(vec(2 * n * sizeof(TS)));
+  fb.p = static_cast<TS*>(vec(2 * n * sizeof(TS)));
+  fb.ap = static_cast<TS*>(vec(2 * n * sizeof(TS)));
+  TD* t64 = static_cast<TD*>(vec(n * sizeof(TD)));
+  TS* xs = nullptr;
+  if constexpr (MIXED) {
+    xs = static_cast<TS*>(vec(n * sizeof(TS)));
+    GLX_HIP(hipMemsetAsync(xs, 0, n * sizeof(TS), st));
+  }
+  // r_{-1} = b - A x in slot 1 (k = 0 reads slot 1 as "old"), Ap_{-1} = p_{-1} = 0
+  apply_typed<TD>(st, o, LINKS_F64, t64, x, w.ws(), kNoGate);
+  launch_point(st, n, PtFastInit<TD, TS>{fb.rs(1), fb.aps(1), fb.ps(1), b, t64}, FinFastInit{w.ds}, w.ws(), kNoGate);
+  const int link_prec = MIXED ? mo.link_prec : LINKS_F64;
+  auto enq_iter = [&]() {
+    if constexpr (MIXED) {
+      run_stencil<TS>(st, o, link_prec, InFast<TS, TS>{fb, xs, w.ds}, fb.aps(0), EpiFast<TS>{}, FinFast<TS, true>{w.ds},
+                      w.ws(), w.done());
+    } else {
+      run_stencil<TS>(st, o, link_prec, InFast<TS, TD>{fb, x, w.ds}, fb.aps(0), EpiFast<TS>{}, FinFast<TS, false>{w.ds},
+                      w.ws(), w.done());
+    }
   };
-  Mirror m = run_loop(w, opt.max_iter, opt.chunk, enq, pause);
+  auto enq_reliable = [&]() {
+    if constexpr (MIXED) {
+      launch_point(st, n, PtFastAccum<TD, TS>{x, xs, fb, w.ds}, FinNone{}, w.ws(), w.paused());
+      apply_typed<TD>(st, o, LINKS_F64, t64, x, w.ws(), w.paused());
+      launch_point(st, n, PtFastTrue<TD, TS>{fb, b, t64, w.ds}, FinFastReliable{w.ds}, w.ws(), w.paused());
+    }
+  };
+  const int group = std::max(1, opt.group);
+  auto enq_group = [&]() {
+    for (int g = 0; g < group; g++) enq_iter();
+    enq_reliable();
+  };
+  // optional: one captured graph of a group, replayed
+  hipGraphExec_t exec = nullptr;
+  if (opt.use_graph) {
+    hipGraph_t graph = nullptr;
+    GLX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enq_group();
+    GLX_HIP(hipStreamEndCapture(st, &graph));
+    GLX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    GLX_HIP(hipGraphDestroy(graph));
+  }
+  auto launch = [&]() {
+    if (exec) GLX_HIP(hipGraphLaunch(exec, st));
+    else enq_group();
+  };
+  // pipelined: one group queued ahead of the one whose control word is read
+  PinnedSlots& pin = pinned();
+  const long max_groups = 4L * (opt.max_iter / group + 2) + 16;
+  long groups = 0;
+  Mirror m{0, 0, -1, 0};
+  int cur = 0;
+  auto submit = [&](int slot) {
+    launch();
+    groups++;
+    GLX_HIP(hipMemcpyAsync(&pin.m[slot], w.ds, sizeof(Mirror), hipMemcpyDeviceToHost, st));
+    GLX_HIP(hipEventRecord(pin.ev[slot], st));
+  };
+  if (opt.max_iter > 0) {
+    submit(cur);
+    for (;;) {
+      submit(cur ^ 1);
+      GLX_HIP(hipEventSynchronize(pin.ev[cur]));
+      m = pin.m[cur];
+      if (m.done == 1) break;
+      if (groups > max_groups) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        throw HipError("single-pass CG made no progress");
+      }
+      cur ^= 1;
+    }
+  }
+  GLX_HIP(hipStreamSynchronize(st));
+  if (exec) GLX_HIP(hipGraphExecDestroy(exec));
   SolveState hs = w.download();
   SolveResult res;
-  res.name = "CG-mixed";
-  res.iter = (m.done == 1 ? hs.kfinal : hs.k) + 1;
-  res.success = sqrt(hs.resSq) < opt.eps * hs.bsqrt;
+  res.name = MIXED ? "CG-mixed" : "CG-fused";
+  if (opt.max_iter <= 0) {
+    res.iter = 1;
+    res.success = true;
+    return res;
+  }
+  if constexpr (!MIXED) {
+    launch_point(st, n, PtFastFinal<TD>{x, fb, w.ds}, FinNone{}, w.ws(), kNoGate);
+    true_residual<TD>(w, o, x, b, t64);
+    hs = w.download();
+    res.success = hs.kfinal != opt.max_iter - 1;
+    res.ops_count = hs.kfinal + 3;
+  } else {
+    res.success = sqrt(hs.resSq) < opt.eps * hs.bsqrt;
+    res.ops_count = hs.kfinal + 2 + hs.reliable;
+  }
+  res.iter = hs.kfinal + 1;
   res.resSq = hs.resSq;
   res.bsqrt = hs.bsqrt;
   res.reliable_updates = hs.reliable;
-  res.ops_count = res.iter + 1 + hs.reliable;
-  if (opt.want_hist && opt.max_iter > 0) res.relres_hist = w.hist_host(std::min(res.iter, opt.max_iter));
+  if (opt.want_hist) res.relres_hist = w.hist_host(res.iter);
   return res;
 }
 
@@ -1291,19 +1559,25 @@ SolveResult dev_cg_m(const DevOperator& op, void* const* xs, const void* b, int 
 }
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m) {
   if (real_op(op))
-    return cg_mixed_impl<double, float>(op, static_cast<double*>(x), static_cast<const double*>(b), o, m);
-  return cg_mixed_impl<c128, c64>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
+    return fastcg_impl<double, float, true>(op, static_cast<double*>(x), static_cast<const double*>(b), o, m);
+  return fastcg_impl<c128, c64, true>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
+}
+SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const SolveOpts& o) {
+  MixedOpts m;
+  if (real_op(op))
+    return fastcg_impl<double, double, false>(op, static_cast<double*>(x), static_cast<const double*>(b), o, m);
+  return fastcg_impl<c128, c128, false>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
 }
 
 void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st) {
   RedSpace red;
   red.init(kMaxPointBlocks, st);
   if (real_op(op)) {
-    if (fp32) apply_typed<float>(st, op, link_prec, static_cast<float*>(out), static_cast<const float*>(in), red.ws(), nullptr);
-    else apply_typed<double>(st, op, link_prec, static_cast<double*>(out), static_cast<const double*>(in), red.ws(), nullptr);
+    if (fp32) apply_typed<float>(st, op, link_prec, static_cast<float*>(out), static_cast<const float*>(in), red.ws(), kNoGate);
+    else apply_typed<double>(st, op, link_prec, static_cast<double*>(out), static_cast<const double*>(in), red.ws(), kNoGate);
   } else {
-    if (fp32) apply_typed<c64>(st, op, link_prec, static_cast<c64*>(out), static_cast<const c64*>(in), red.ws(), nullptr);
-    else apply_typed<c128>(st, op, link_prec, static_cast<c128*>(out), static_cast<const c128*>(in), red.ws(), nullptr);
+    if (fp32) apply_typed<c64>(st, op, link_prec, static_cast<c64*>(out), static_cast<const c64*>(in), red.ws(), kNoGate);
+    else apply_typed<c128>(st, op, link_prec, static_cast<c128*>(out), static_cast<const c128*>(in), red.ws(), kNoGate);
   }
   GLX_HIP(hipStreamSynchronize(st));
 }

@@ -117,11 +117,17 @@ struct OpDev {
 // exactly one visit of each site of the lattice (the owning tile).
 // init() runs once per thread before the first load (it may cache scalars
 // read from the solver state).
+//
+// A prologue may also hand an auxiliary value of each owned site to the
+// epilogue (kAux, written through `aux`), and choose the output vector
+// (target), e.g. by the solver's iteration parity read in init().
 template <class T>
 struct InPlain {
+  static constexpr bool kAux = false;
   const T* p;
   __device__ __forceinline__ void init() {}
-  __device__ __forceinline__ T load(int j, bool) const { return p[j]; }
+  __device__ __forceinline__ T load(int j, bool, T&) const { return p[j]; }
+  __device__ __forceinline__ T* target(T* out) const { return out; }
 };
 
 // ------------------------------------------------------------- epilogue --
@@ -129,13 +135,13 @@ struct InPlain {
 template <class T>
 struct EpiNone {
   static constexpr int NR = 0;
-  __device__ __forceinline__ void acc(double*, int, T, T) const {}
+  __device__ __forceinline__ void acc(double*, int, T, T, T) const {}
 };
 // <in, out> = sum conj(in) out
 template <class T>
 struct EpiDotInOut {
   static constexpr int NR = is_cx<T>::value ? 2 : 1;
-  __device__ __forceinline__ void acc(double* v, int, T in, T out) const {
+  __device__ __forceinline__ void acc(double* v, int, T in, T out, T) const {
     if constexpr (is_cx<T>::value) {
       c128 d = dotc(in, out);
       v[0] += d.re;
@@ -150,7 +156,7 @@ template <class T>
 struct EpiDotAuxOut {
   static constexpr int NR = is_cx<T>::value ? 2 : 1;
   const T* aux;
-  __device__ __forceinline__ void acc(double* v, int i, T, T out) const {
+  __device__ __forceinline__ void acc(double* v, int i, T, T out, T) const {
     if constexpr (is_cx<T>::value) {
       c128 d = dotc(aux[i], out);
       v[0] += d.re;
@@ -164,7 +170,7 @@ struct EpiDotAuxOut {
 template <class T>
 struct EpiOmega {
   static constexpr int NR = is_cx<T>::value ? 3 : 2;
-  __device__ __forceinline__ void acc(double* v, int, T in, T out) const {
+  __device__ __forceinline__ void acc(double* v, int, T in, T out, T) const {
     if constexpr (is_cx<T>::value) {
       c128 d = dotc(out, in);
       v[0] += d.re;
@@ -226,8 +232,8 @@ struct Geom {
 };
 
 template <int KIND, class T, class LK, class In, class Epi, class Fin>
-__global__ __launch_bounds__(kBlock) void k_stencil(Geom g, OpDev<LK> op, In in, T* __restrict__ out, Epi epi,
-                                                    Fin fin, RedWs ws, const int* __restrict__ done) {
+__global__ __launch_bounds__(kBlock) void k_stencil(Geom g, OpDev<LK> op, In in, T* __restrict__ out_arg, Epi epi,
+                                                    Fin fin, RedWs ws, Gate gate) {
   constexpr int H = op_halo(KIND);
   constexpr bool LINKS = op_links(KIND);
   constexpr int PW = kTX + 2 * H, PH = kTY + 2 * H;  // staged input region
@@ -236,10 +242,12 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Geom g, OpDev<LK> op, In in,
   using C = typename LK::C;
   using R = typename Real<T>::type;
 
-  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  if (gate_closed(gate)) return;
   in.init();
+  T* __restrict__ out = in.target(out_arg);
 
   __shared__ T sp[PH * PW];
+  __shared__ T sa[In::kAux ? kTX * kTY : 1];
   __shared__ C su[LINKS ? 2 * LW * LH : 2];
   __shared__ T sq[H == 2 ? QW * QH : 1];
 
@@ -259,7 +267,11 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Geom g, OpDev<LK> op, In in,
       const int ux = x0 + rx - H, uy = y0 + ry - H;
       const int gx = wrap_coord(ux, lx), gy = wrap_coord(uy, ly);
       const bool own = rx >= H && rx < H + kTX && ry >= H && ry < H + kTY && ux < lx && uy < ly;
-      sp[e] = in.load(gy * lx + gx, own);
+      T aux;
+      sp[e] = in.load(gy * lx + gx, own, aux);
+      if constexpr (In::kAux) {
+        if (own) sa[(ry - H) * kTX + (rx - H)] = aux;
+      }
     }
     if constexpr (LINKS) {
       for (int e = threadIdx.x; e < LW * LH; e += kBlock) {
@@ -420,7 +432,8 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Geom g, OpDev<LK> op, In in,
         }
       }
       out[i] = v;
-      epi.acc(part, i, pin, v);
+      if constexpr (In::kAux) epi.acc(part, i, pin, v, sa[s]);
+      else epi.acc(part, i, pin, v, pin);
     }
   }
   if constexpr (Epi::NR > 0) block_partials<Epi::NR>(part, ws, bid);
@@ -439,12 +452,12 @@ inline Geom make_geom(int lx, int ly) {
 
 template <int KIND, class T, class LK, class In, class Epi, class Fin>
 void launch_stencil(hipStream_t st, const Geom& g, const OpDev<LK>& op, In in, T* out, Epi epi, Fin fin, RedWs ws,
-                    const int* done) {
+                    Gate gate) {
   hipLaunchKernelGGL((k_stencil<KIND, T, LK, In, Epi, Fin>), dim3(g.nblk), dim3(kBlock), 0, st, g, op, in, out, epi,
-                     fin, ws, done);
+                     fin, ws, gate);
   GLX_CHECK_LAUNCH();
   if constexpr (Epi::NR > 0) {
-    hipLaunchKernelGGL((k_finalize<Epi::NR, Fin>), dim3(1), dim3(kBlock), 0, st, g.nblk, fin, ws, done);
+    hipLaunchKernelGGL((k_finalize<Epi::NR, Fin>), dim3(1), dim3(kBlock), 0, st, g.nblk, fin, ws, gate);
     GLX_CHECK_LAUNCH();
   }
 }

@@ -154,6 +154,10 @@ def lib():
         L.glx_dev_cg_m.argtypes = [vp, i, d, i, vp, vp, i, vp, i, i, d, i, VP, IP]
         L.glx_dev_cg_mixed.restype = i
         L.glx_dev_cg_mixed.argtypes = [vp, i, d, vp, vp, i, d, d, i, VP, IP]
+        L.glx_dev_cg_fused.restype = i
+        L.glx_dev_cg_fused.argtypes = [vp, i, d, vp, vp, i, d, VP, IP]
+        L.glx_set_tuning.restype = i
+        L.glx_set_tuning.argtypes = [i, i]
         L.glx_dev_apply.restype = i
         L.glx_dev_apply.argtypes = [vp, i, d, i, i, i, vp, vp]
         L.glx_dev_time_apply.restype = i
@@ -424,6 +428,19 @@ def dev_cg_mixed(lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_
     _check(lib().glx_dev_cg_mixed(lat.h, OP_KINDS[op], mass, x.ptr, b.ptr, max_iter, eps, delta, link_prec,
                                   C.byref(tx.v), C.byref(ci)), "glx_dev_cg_mixed")
     return _info(ci, tx.text())
+
+
+def dev_cg_fused(lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_iter=100000, eps=1e-10,
+                 verb_level=0) -> Info:
+    tx = _Text(verb_level, "", "%g")
+    ci = InversionInfo()
+    _check(lib().glx_dev_cg_fused(lat.h, OP_KINDS[op], mass, x.ptr, b.ptr, max_iter, eps, C.byref(tx.v),
+                                  C.byref(ci)), "glx_dev_cg_fused")
+    return _info(ci, tx.text())
+
+
+def set_tuning(group: int = 8, use_graph: bool = True):
+    _check(lib().glx_set_tuning(group, int(use_graph)), "glx_set_tuning")
 
 
 def dev_cg_m(lat, op: str, mass: float, xs, b: DeviceArray, shifts, resid_freq_check=10, max_iter=10000, eps=1e-10,

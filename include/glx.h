@@ -245,6 +245,16 @@ int glx_dev_cg_m(glx_lattice lat, int kind, double mass, int nc, double** x_dev,
  * residual is below eps. */
 int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
                      double eps, double delta, int link_prec, glx_verbose* verbosity, glx_inversion_info* info);
+/* FP64 CG in single-pass form (no reference counterpart as such): the same
+ * algorithm as GLX_SOLVER_CG with one fused stencil launch per iteration
+ * (|r_{k+1}|^2 from the recurrence), so the rounding -- not the method --
+ * differs from the reference's sequence of operations. */
+int glx_dev_cg_fused(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
+                     double eps, glx_verbose* verbosity, glx_inversion_info* info);
+/* Launch tuning of the single-pass CG and the mixed-precision CG: iterations
+ * per enqueued group (default 8) and whether a group is captured once as a
+ * hipGraph and replayed (default 1). */
+int glx_set_tuning(int group, int use_graph);
 /* out = A in on device vectors; fp32 selects complex<float> vectors with the
  * FP32 (link_prec 1) or FP16 (link_prec 2) links. */
 int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int link_prec, void* out_dev,

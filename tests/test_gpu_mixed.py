@@ -67,3 +67,28 @@ def test_mixed_cg_max_iter(glx, orc):
     assert not info.success
     r = bc - orc.apply("staggered_normal", x, links, 0.02, L, L)
     assert abs(np.sqrt(info.resSq) - np.linalg.norm(r)) <= 1e-6 * np.linalg.norm(r)
+
+
+@pytest.mark.parametrize("L,mass,op", [(64, 0.05, "staggered_normal"), (130, 0.1, "staggered_normal"),
+                                        (64, 0.01, "laplace_real")])
+@pytest.mark.parametrize("graph", [True, False])
+def test_fused_fp64_cg(glx, orc, L, mass, op, graph):
+    """FP64 single-pass CG: same algorithm as the reference CG with the
+    residual norm from the recurrence; iteration count within 2 of the
+    oracle's, true residual below tolerance, solution within the accuracy the
+    tolerance allows."""
+    links, bc, br = lattice_inputs(orc, L, L, beta=4.0, seed=51 + L)
+    b = br if op == "laplace_real" else bc
+    lat = glx.Lattice(links, L, L)
+    glx.set_tuning(group=8, use_graph=graph)
+    db = glx.DeviceArray(b)
+    dx = glx.DeviceArray(np.zeros_like(b))
+    info = glx.dev_cg_fused(lat, op, mass, dx, db, max_iter=20000, eps=1e-10)
+    glx.set_tuning()
+    x = dx.download()
+    xo, io = orc.solve("cg", op, b, None, links=links, mass=mass, lx=L, ly=L, max_iter=20000, eps=1e-10)
+    assert info.success and abs(info.iter - io.iter) <= 2, (info.iter, io.iter)
+    r = b - orc.apply(op, x, links, mass, L, L)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.05e-10
+    cond = (8 + mass) / mass if op == "laplace_real" else (mass * mass + 2) / (mass * mass)
+    assert relerr(x, xo) <= 2e-10 * cond
