@@ -810,6 +810,80 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
   });
 }
 
+int glx_set_normal_rows(int lx, int ly, int esize, int rows) {
+  return guarded([&] {
+    if (rows < 0 || (rows > 0 && (rows < 8 || rows % 4 != 0))) throw ArgError("rows must be 0 or a multiple of 4 >= 8");
+    if (lx < 0 || ly < 0 || (lx == 0) != (ly == 0)) throw ArgError("bad lattice size");
+    if (lx > 0 && esize != 8 && esize != 16) throw ArgError("esize must be 8 (complex FP32) or 16 (complex FP64)");
+    set_normal_rows(lx, ly, esize, rows);
+  });
+}
+
+int glx_get_normal_rows(int lx, int ly, int esize, int* rows) {
+  return guarded([&] {
+    if (!rows) throw ArgError("rows is NULL");
+    *rows = normal_rows_for(lx, ly, esize);
+  });
+}
+
+int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, double* ms64, double* ms32) {
+  return guarded([&] {
+    if (!lat || reps <= 0) throw ArgError("bad arguments");
+    DevOperator o = lattice_op(lat, OP_STAG_NORMAL, 0.1, 1);
+    const int n = o.size();
+    DevBuf in(size_t(n) * sizeof(c128)), out(size_t(n) * sizeof(c128));
+    GLX_HIP(hipMemset(in.get(), 0, size_t(n) * sizeof(c128)));
+    const int cands[] = {0, 8, 12, 16, 24, 32, 48, 64};
+    hipEvent_t a, b;
+    GLX_HIP(hipEventCreate(&a));
+    GLX_HIP(hipEventCreate(&b));
+    RedSpace red;
+    red.init(kMaxPointBlocks, nullptr);
+    auto time_one = [&](int fp32) {
+      float ms = 0.f;
+      GLX_HIP(hipEventRecord(a, nullptr));
+      for (int k = 0; k < reps; k++) {
+        if (fp32)
+          run_stencil<c64>(nullptr, o, LINKS_F32, InPlain<c64>{in.as<c64>()}, out.as<c64>(), EpiNone<c64>{},
+                           FinNone{}, red.ws(), kNoGate);
+        else
+          run_stencil<c128>(nullptr, o, LINKS_F64, InPlain<c128>{in.as<c128>()}, out.as<c128>(), EpiNone<c128>{},
+                            FinNone{}, red.ws(), kNoGate);
+      }
+      GLX_HIP(hipEventRecord(b, nullptr));
+      GLX_HIP(hipEventSynchronize(b));
+      GLX_HIP(hipEventElapsedTime(&ms, a, b));
+      return double(ms) / reps;
+    };
+    for (int fp32 = 0; fp32 < 2; fp32++) {
+      if (fp32 && !o.u32) continue;
+      const int esize = fp32 ? 8 : 16;
+      int best = 0;
+      double best_ms = 1e300;
+      for (int c : cands) {
+        if (c > 0 && !normal_fits(o.lx, o.ly, c)) continue;
+        set_normal_rows(o.lx, o.ly, esize, c);
+        time_one(fp32);  // warm-up
+        const double t = time_one(fp32);
+        if (t < best_ms) {
+          best_ms = t;
+          best = c;
+        }
+      }
+      set_normal_rows(o.lx, o.ly, esize, best);
+      if (fp32) {
+        if (rows32) *rows32 = best;
+        if (ms32) *ms32 = best_ms;
+      } else {
+        if (rows64) *rows64 = best;
+        if (ms64) *ms64 = best_ms;
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  });
+}
+
 int glx_dev_alloc(void** ptr, size_t bytes) {
   return guarded([&] {
     if (!ptr) throw ArgError("ptr is NULL");

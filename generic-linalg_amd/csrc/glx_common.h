@@ -186,18 +186,20 @@ struct RedWs {
 };
 
 // Block-level part of a grid reduction: every block of the grid stores its
-// NR partial sums (plain stores, no device-scope fence).  A one-block
-// finalisation kernel launched right after (k_finalize) sums them in block
-// order -- deterministic for a given launch shape -- and hands the totals to
-// the solver's finaliser.  Kernel boundaries make the partials visible
-// across XCDs, which avoids an L2 write-back per block.
+// NR partial sums (plain stores, no device-scope fence) component-major,
+// partials[k * gridDim.x + block], so the finaliser's loads coalesce.  A
+// one-block finalisation kernel launched right after (k_finalize) sums them
+// in block order -- deterministic for a given launch shape -- and hands the
+// totals to the solver's finaliser.  Kernel boundaries make the partials
+// visible across XCDs, which avoids an L2 write-back per block.
 template <int NR>
 __device__ __forceinline__ void block_partials(double (&v)[NR], RedWs ws, int bid) {
   __shared__ double sm[NR > 0 ? NR : 1][kWaves];
   block_sum<NR>(v, sm);
   if (threadIdx.x == 0) {
+    const int nb = gridDim.x;
 #pragma unroll
-    for (int k = 0; k < NR; k++) ws.partials[bid * NR + k] = v[k];
+    for (int k = 0; k < NR; k++) ws.partials[k * nb + bid] = v[k];
   }
 }
 
@@ -212,6 +214,11 @@ __device__ __forceinline__ bool gate_closed(Gate g) {
   return g.flag && __hip_atomic_load(g.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != g.want;
 }
 
+// nblk = grid size of the kernel that wrote the partials.  Each thread sums
+// blocks t, t + kBlock, t + 2 kBlock, ... in increasing order; the loads of
+// kFinUnroll consecutive terms are issued together, so one sweep costs one
+// memory latency instead of kFinUnroll.
+constexpr int kFinUnroll = 8;
 template <int NR, class Fin>
 __global__ __launch_bounds__(kBlock) void k_finalize(int nblk, Fin fin, RedWs ws, Gate gate) {
   if (gate_closed(gate)) return;
@@ -219,10 +226,19 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int nblk, Fin fin, RedWs ws
   double acc[NR];
 #pragma unroll
   for (int k = 0; k < NR; k++) acc[k] = 0.0;
-  // each thread sums a fixed strided subset in increasing block order
-  for (int b = threadIdx.x; b < nblk; b += kBlock) {
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += kBlock * kFinUnroll) {
+    double v[kFinUnroll][NR];
 #pragma unroll
-    for (int k = 0; k < NR; k++) acc[k] += ws.partials[b * NR + k];
+    for (int u = 0; u < kFinUnroll; u++) {
+      const int b = b0 + u * kBlock;
+#pragma unroll
+      for (int k = 0; k < NR; k++) v[u][k] = b < nblk ? ws.partials[k * nblk + b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kFinUnroll; u++) {
+#pragma unroll
+      for (int k = 0; k < NR; k++) acc[k] += v[u][k];
+    }
   }
   block_sum<NR>(acc, sm);
   if (threadIdx.x == 0) fin(acc);

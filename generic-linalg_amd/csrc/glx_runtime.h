@@ -5,7 +5,7 @@
 // reduction.
 #pragma once
 
-#include "glx_stencil.h"
+#include "glx_normal.h"
 
 #include <functional>
 #include <map>
@@ -123,12 +123,28 @@ constexpr bool kind_valid_for(int k) {
   else return k == OP_LAPLACE_REAL;
 }
 
+// Launch shape of the row-marching normal-operator kernel: rows per block
+// for a lattice size and vector element size in bytes (set by the autotuner
+// or by hand; 0 = use the tiled kernel), with a default for sizes not in the
+// table.
+int normal_rows_for(int lx, int ly, int esize);
+void set_normal_rows(int lx, int ly, int esize, int rows);  // lx = 0: set the default
+
 // Launch the tiled stencil kernel for a runtime operator kind.
 template <class T, class LK, class In, class Epi, class Fin>
 void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T* out, Epi epi, Fin fin, RedWs ws,
                    Gate done) {
   const Geom g = make_geom(o.lx, o.ly);
   const OpDev<LK> od = make_opdev(o, lk);
+  if constexpr (is_cx<T>::value) {
+    if (o.kind == OP_STAG_NORMAL) {
+      const int sr = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
+      if (sr > 0 && normal_fits(o.lx, o.ly, sr)) {
+        launch_normal<T, LK, In, Epi, Fin>(st, make_ngeom(o.lx, o.ly, sr), od, in, out, epi, fin, ws, done);
+        return;
+      }
+    }
+  }
 #define GLX_CASE(K)                                                                     \
   case K:                                                                               \
     if constexpr (kind_valid_for<T>(K)) {                                               \

@@ -14,6 +14,8 @@
 // by the next kernel; the host reads the 16-byte control word once per chunk.
 #include "glx_solvers.h"
 
+#include <tuple>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -21,6 +23,31 @@
 namespace glx {
 
 // =========================================================== allocator ====
+namespace {
+struct NormalTable {
+  std::mutex mu;
+  int def = 16;
+  std::map<std::tuple<int, int, int>, int> rows;
+};
+NormalTable& normal_table() {
+  static NormalTable t;
+  return t;
+}
+}  // namespace
+
+int normal_rows_for(int lx, int ly, int esize) {
+  NormalTable& t = normal_table();
+  std::lock_guard<std::mutex> g(t.mu);
+  auto it = t.rows.find(std::make_tuple(lx, ly, esize));
+  return it == t.rows.end() ? t.def : it->second;
+}
+void set_normal_rows(int lx, int ly, int esize, int rows) {
+  NormalTable& t = normal_table();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (lx == 0) t.def = rows;
+  else t.rows[std::make_tuple(lx, ly, esize)] = rows;
+}
+
 DevicePool& DevicePool::get() {
   static DevicePool* pool = new DevicePool;  // outlives every static user
   return *pool;
@@ -378,23 +405,34 @@ struct InCG {
     ap = as_scal<T>(s->alpha_prev);
   }
   __device__ T* target(T* out) const { return out; }
-  __device__ T load(int j, bool own, T&) {
-    const T po = pold[j];
-    const T pn = r[j] + rscal(beta, po);
+  struct Raw {
+    T r, po;
+    TX xv;
+  };
+  __device__ Raw fetch(int j, bool own) const {
+    Raw w;
+    w.po = pold[j];
+    w.r = r[j];
+    w.xv = (own && x) ? x[j] : zero_of<TX>();
+    return w;
+  }
+  __device__ T finish(const Raw& w, int j, bool own, T&) const {
+    const T pn = w.r + rscal(beta, w.po);
     if (own) {
       pnew[j] = pn;
       if (x) {
         if constexpr (std::is_same<T, TX>::value) {
-          x[j] = x[j] + scal(ap, po);
+          x[j] = w.xv + scal(ap, w.po);
         } else {
           TX a;
-          cvt(a, scal(ap, po));
-          x[j] = x[j] + a;
+          cvt(a, scal(ap, w.po));
+          x[j] = w.xv + a;
         }
       }
     }
     return pn;
   }
+  __device__ T load(int j, bool own, T& aux) const { return finish(fetch(j, own), j, own, aux); }
 };
 template <class T>
 struct FinCgAlpha {
@@ -541,11 +579,16 @@ struct InBicgS {
   typename Scal<T>::type a;
   __device__ void init() { a = as_scal<T>(st->alpha); }
   __device__ T* target(T* out) const { return out; }
-  __device__ T load(int j, bool own, T&) {
-    const T v = r[j] - scal(a, Ap[j]);
+  struct Raw {
+    T r, ap;
+  };
+  __device__ Raw fetch(int j, bool) const { return Raw{r[j], Ap[j]}; }
+  __device__ T finish(const Raw& w, int j, bool own, T&) const {
+    const T v = w.r - scal(a, w.ap);
     if (own) sv[j] = v;
     return v;
   }
+  __device__ T load(int j, bool own, T& aux) const { return finish(fetch(j, own), j, own, aux); }
 };
 template <class T>
 struct FinBicgOmega {
@@ -624,11 +667,16 @@ struct InBicgP {
     om = as_scal<T>(st->omega);
   }
   __device__ T* target(T* out) const { return out; }
-  __device__ T load(int j, bool own, T&) {
-    const T v = r[j] + scal(be, pold[j] - scal(om, Apold[j]));
+  struct Raw {
+    T r, po, apo;
+  };
+  __device__ Raw fetch(int j, bool) const { return Raw{r[j], pold[j], Apold[j]}; }
+  __device__ T finish(const Raw& w, int j, bool own, T&) const {
+    const T v = w.r + scal(be, w.po - scal(om, w.apo));
     if (own) pnew[j] = v;
     return v;
   }
+  __device__ T load(int j, bool own, T& aux) const { return finish(fetch(j, own), j, own, aux); }
 };
 
 template <class T>
@@ -1193,24 +1241,36 @@ struct InFast {
     beta = s->beta_r;
   }
   __device__ T* target(T*) const { return apn; }
-  __device__ T load(int j, bool own, T& aux) {
-    const T pold = po[j];
-    const T r = ro[j] - scal(a, apo[j]);
-    const T p = r + rscal(beta, pold);
+  struct Raw {
+    T po, ro, apo;
+    TX xv;
+  };
+  __device__ Raw fetch(int j, bool own) const {
+    Raw w;
+    w.po = po[j];
+    w.ro = ro[j];
+    w.apo = apo[j];
+    w.xv = own ? x[j] : zero_of<TX>();
+    return w;
+  }
+  __device__ T finish(const Raw& w, int j, bool own, T& aux) const {
+    const T r = w.ro - scal(a, w.apo);
+    const T p = r + rscal(beta, w.po);
     if (own) {
       rn[j] = r;
       pn[j] = p;
       aux = r;
       if constexpr (std::is_same<T, TX>::value) {
-        x[j] = x[j] + scal(a, pold);
+        x[j] = w.xv + scal(a, w.po);
       } else {
         TX d;
-        cvt(d, scal(a, pold));
-        x[j] = x[j] + d;
+        cvt(d, scal(a, w.po));
+        x[j] = w.xv + d;
       }
     }
     return p;
   }
+  __device__ T load(int j, bool own, T& aux) const { return finish(fetch(j, own), j, own, aux); }
 };
 
 // <p,Ap>, |Ap|^2, <Ap,r>, |r|^2

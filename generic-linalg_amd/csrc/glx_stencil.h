@@ -64,21 +64,34 @@ __host__ __device__ constexpr bool op_coeffs(int k) {
 // FP64 and FP32 links are stored as complex pairs per site; FP16 links as a
 // half2 pair per site plus one FP32 scale per site (the largest component
 // magnitude of the site's two links), U = scale * half(U / scale).
+// Each storage type offers load(s, ux, uy) and the same split in two, so a
+// kernel can issue the memory reads early and convert later: Raw, fetch(s)
+// (reads only) and finish(raw, ux, uy) (arithmetic only).
 struct LinksF64 {
   const c128* u;
   using C = c128;
-  __device__ __forceinline__ void load(int s, C& ux, C& uy) const {
-    ux = u[2 * s];
-    uy = u[2 * s + 1];
+  struct Raw {
+    c128 a, b;
+  };
+  __device__ __forceinline__ Raw fetch(int s) const { return Raw{u[2 * s], u[2 * s + 1]}; }
+  __device__ __forceinline__ static void finish(const Raw& r, C& ux, C& uy) {
+    ux = r.a;
+    uy = r.b;
   }
+  __device__ __forceinline__ void load(int s, C& ux, C& uy) const { finish(fetch(s), ux, uy); }
 };
 struct LinksF32 {
   const c64* u;
   using C = c64;
-  __device__ __forceinline__ void load(int s, C& ux, C& uy) const {
-    ux = u[2 * s];
-    uy = u[2 * s + 1];
+  struct alignas(16) Raw {
+    c64 a, b;
+  };
+  __device__ __forceinline__ Raw fetch(int s) const { return reinterpret_cast<const Raw*>(u)[s]; }
+  __device__ __forceinline__ static void finish(const Raw& r, C& ux, C& uy) {
+    ux = r.a;
+    uy = r.b;
   }
+  __device__ __forceinline__ void load(int s, C& ux, C& uy) const { finish(fetch(s), ux, uy); }
 };
 struct alignas(8) HalfLink {
   __half2 x, y;
@@ -87,16 +100,23 @@ struct LinksF16 {
   const HalfLink* u;
   const float* scale;
   using C = c64;
-  __device__ __forceinline__ void load(int s, C& ux, C& uy) const {
-    const HalfLink h = u[s];
-    const float sc = scale[s];
-    const float2 a = __half22float2(h.x), b = __half22float2(h.y);
-    ux = {sc * a.x, sc * a.y};
-    uy = {sc * b.x, sc * b.y};
+  struct Raw {
+    HalfLink h;
+    float sc;
+  };
+  __device__ __forceinline__ Raw fetch(int s) const { return Raw{u[s], scale[s]}; }
+  __device__ __forceinline__ static void finish(const Raw& r, C& ux, C& uy) {
+    const float2 a = __half22float2(r.h.x), b = __half22float2(r.h.y);
+    ux = {r.sc * a.x, r.sc * a.y};
+    uy = {r.sc * b.x, r.sc * b.y};
   }
+  __device__ __forceinline__ void load(int s, C& ux, C& uy) const { finish(fetch(s), ux, uy); }
 };
 struct LinksNone {
   using C = c64;
+  struct Raw {};
+  __device__ __forceinline__ Raw fetch(int) const { return Raw{}; }
+  __device__ __forceinline__ static void finish(const Raw&, C&, C&) {}
   __device__ __forceinline__ void load(int, C&, C&) const {}
 };
 
@@ -121,11 +141,21 @@ struct OpDev {
 // A prologue may also hand an auxiliary value of each owned site to the
 // epilogue (kAux, written through `aux`), and choose the output vector
 // (target), e.g. by the solver's iteration parity read in init().
+//
+// load(j, own, aux) is also offered split in two, so a kernel can issue the
+// reads of a later row before it computes the current one: fetch(j, own)
+// performs only the reads into a Raw record, finish(raw, j, own, aux) the
+// arithmetic and the owned-site stores; load = finish(fetch).
 template <class T>
 struct InPlain {
   static constexpr bool kAux = false;
   const T* p;
+  struct Raw {
+    T v;
+  };
   __device__ __forceinline__ void init() {}
+  __device__ __forceinline__ Raw fetch(int j, bool) const { return Raw{p[j]}; }
+  __device__ __forceinline__ T finish(const Raw& r, int, bool, T&) const { return r.v; }
   __device__ __forceinline__ T load(int j, bool, T&) const { return p[j]; }
   __device__ __forceinline__ T* target(T* out) const { return out; }
 };
@@ -210,17 +240,24 @@ __device__ __forceinline__ T ucmul(C u, T v) {
   }
 }
 
-// K F at one point: F has row pitch fw, centre index fc; links region has row
-// pitch lw, centre index lc (links stored as pairs ux, uy).
+// K at one point from its four neighbours F(x+x), F(x-x), F(x+y), F(x-y) and
+// the links U_x(x), U_x(x-x), U_y(x), U_y(x-y):
+//   1/2 [U_x F(x+x) - U_x(x-x)^* F(x-x) + eta_y (U_y F(x+y) - U_y(x-y)^* F(x-y))]
 template <class T, class C>
-__device__ __forceinline__ T hop_k(const T* F, int fw, int fc, const C* L, int lw, int lc, bool eta_neg) {
+__device__ __forceinline__ T khop(T fxp, T fxm, T fyp, T fym, C ux, C uxm, C uy, C uym, bool eta_neg) {
   using R = typename Real<T>::type;
-  const T fxp = F[fc + 1], fxm = F[fc - 1], fyp = F[fc + fw], fym = F[fc - fw];
-  const C ux = L[2 * lc], uy = L[2 * lc + 1], uxm = L[2 * (lc - 1)], uym = L[2 * (lc - lw) + 1];
   T ax = umul(ux, fxp) - ucmul(uxm, fxm);
   T ay = umul(uy, fyp) - ucmul(uym, fym);
   if (eta_neg) ay = -ay;
   return R(0.5) * (ax + ay);
+}
+
+// K F at one point: F has row pitch fw, centre index fc; links region has row
+// pitch lw, centre index lc (links stored as pairs ux, uy).
+template <class T, class C>
+__device__ __forceinline__ T hop_k(const T* F, int fw, int fc, const C* L, int lw, int lc, bool eta_neg) {
+  return khop<T, C>(F[fc + 1], F[fc - 1], F[fc + fw], F[fc - fw], L[2 * lc], L[2 * (lc - 1)], L[2 * lc + 1],
+                    L[2 * (lc - lw) + 1], eta_neg);
 }
 
 // ---------------------------------------------------------------- kernel --

@@ -158,6 +158,12 @@ def lib():
         L.glx_dev_cg_fused.argtypes = [vp, i, d, vp, vp, i, d, VP, IP]
         L.glx_set_tuning.restype = i
         L.glx_set_tuning.argtypes = [i, i]
+        L.glx_set_normal_rows.restype = i
+        L.glx_set_normal_rows.argtypes = [i, i, i, i]
+        L.glx_get_normal_rows.restype = i
+        L.glx_get_normal_rows.argtypes = [i, i, i, C.POINTER(i)]
+        L.glx_autotune_normal.restype = i
+        L.glx_autotune_normal.argtypes = [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(d), C.POINTER(d)]
         L.glx_dev_apply.restype = i
         L.glx_dev_apply.argtypes = [vp, i, d, i, i, i, vp, vp]
         L.glx_dev_time_apply.restype = i
@@ -441,6 +447,50 @@ def dev_cg_fused(lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_
 
 def set_tuning(group: int = 8, use_graph: bool = True):
     _check(lib().glx_set_tuning(group, int(use_graph)), "glx_set_tuning")
+
+
+def set_normal_rows(lx: int, ly: int, esize: int, rows: int):
+    """Rows per block of the row-marching D^dag D kernel for an lx x ly
+    lattice and element size (16 complex FP64, 8 complex FP32); 0 = tiled
+    kernel; lx = ly = 0 sets the default."""
+    _check(lib().glx_set_normal_rows(lx, ly, esize, rows), "glx_set_normal_rows")
+
+
+def get_normal_rows(lx: int, ly: int, esize: int) -> int:
+    r = C.c_int(0)
+    _check(lib().glx_get_normal_rows(lx, ly, esize, C.byref(r)), "glx_get_normal_rows")
+    return r.value
+
+
+def autotune_normal(lat, reps: int = 20, cache: str | None = None) -> dict:
+    """Pick the fastest D^dag D launch shape for this lattice (FP64 and FP32)
+    and install it.  With `cache`, a JSON file of earlier results is read
+    first (an entry for this size is installed without timing) and updated."""
+    import json
+    key = f"{lat.lx}x{lat.ly}"
+    table = {}
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            table = json.load(f)
+    if key in table:
+        e = table[key]
+        set_normal_rows(lat.lx, lat.ly, 16, int(e["rows64"]))
+        if e.get("rows32") is not None:
+            set_normal_rows(lat.lx, lat.ly, 8, int(e["rows32"]))
+        return dict(e, cached=True)
+    r64, r32 = C.c_int(-1), C.c_int(-1)
+    m64, m32 = C.c_double(0.0), C.c_double(0.0)
+    _check(lib().glx_autotune_normal(lat.h, reps, C.byref(r64), C.byref(r32), C.byref(m64), C.byref(m32)),
+           "glx_autotune_normal")
+    e = {"rows64": r64.value, "ms64": m64.value,
+         "rows32": r32.value if r32.value >= 0 else None, "ms32": m32.value if r32.value >= 0 else None}
+    if cache:
+        table[key] = e
+        tmp = cache + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        os.replace(tmp, cache)
+    return dict(e, cached=False)
 
 
 def dev_cg_m(lat, op: str, mass: float, xs, b: DeviceArray, shifts, resid_freq_check=10, max_iter=10000, eps=1e-10,

@@ -255,6 +255,20 @@ int glx_dev_cg_fused(glx_lattice lat, int kind, double mass, double* x_dev, cons
  * per enqueued group (default 8) and whether a group is captured once as a
  * hipGraph and replayed (default 1). */
 int glx_set_tuning(int group, int use_graph);
+/* Launch shape of the fused normal operator D^dag D (no reference
+ * counterpart; operators.cpp:444-453 is the operator it computes): rows per
+ * block of the row-marching kernel for an lx x ly lattice and vector element
+ * size esize (16 = complex FP64, 8 = complex FP32); 0 selects the tiled
+ * kernel.  lx = ly = 0 sets the default for sizes not in the table (16).
+ * The row-marching kernel needs lx % 64 == 0, ly % 4 == 0, ly >= 8; other
+ * sizes always use the tiled kernel. */
+int glx_set_normal_rows(int lx, int ly, int esize, int rows);
+int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
+/* Autotuner: times D^dag D on the lattice for every candidate launch shape
+ * (tiled; 8..64 rows per block), in FP64 and (if the lattice has FP32 links)
+ * FP32, records the fastest in the table above and returns it with its time
+ * per application in ms.  Any output pointer may be NULL. */
+int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, double* ms64, double* ms32);
 /* out = A in on device vectors; fp32 selects complex<float> vectors with the
  * FP32 (link_prec 1) or FP16 (link_prec 2) links. */
 int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int link_prec, void* out_dev,
