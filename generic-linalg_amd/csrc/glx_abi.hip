@@ -12,6 +12,7 @@
 #include "../../include/glx.h"
 #include "glx_solvers.h"
 
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <iostream>
@@ -829,31 +830,30 @@ int glx_get_normal_rows(int lx, int ly, int esize, int* rows) {
 int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, double* ms64, double* ms32) {
   return guarded([&] {
     if (!lat || reps <= 0) throw ArgError("bad arguments");
+    // the timed work is the solvers' own inner loop: `reps` iterations of
+    // the single-pass FP64 CG and of the mixed-precision CG (eps = 0, so
+    // neither stops early) on b = 1, x0 = 0
     DevOperator o = lattice_op(lat, OP_STAG_NORMAL, 0.1, 1);
     const int n = o.size();
-    DevBuf in(size_t(n) * sizeof(c128)), out(size_t(n) * sizeof(c128));
-    GLX_HIP(hipMemset(in.get(), 0, size_t(n) * sizeof(c128)));
+    std::vector<c128> ones(n, c128{1.0, 0.0});
+    DevBuf b(size_t(n) * sizeof(c128)), x(size_t(n) * sizeof(c128));
+    GLX_HIP(hipMemcpy(b.get(), ones.data(), size_t(n) * sizeof(c128), hipMemcpyHostToDevice));
     const int cands[] = {0, 8, 12, 16, 24, 32, 48, 64};
-    hipEvent_t a, b;
-    GLX_HIP(hipEventCreate(&a));
-    GLX_HIP(hipEventCreate(&b));
-    RedSpace red;
-    red.init(kMaxPointBlocks, nullptr);
+    SolveOpts so;
+    so.max_iter = reps;
+    so.eps = 0.0;
+    so.group = g_group;
+    so.use_graph = g_use_graph != 0;
+    MixedOpts mo;
+    mo.link_prec = LINKS_F32;
     auto time_one = [&](int fp32) {
-      float ms = 0.f;
-      GLX_HIP(hipEventRecord(a, nullptr));
-      for (int k = 0; k < reps; k++) {
-        if (fp32)
-          run_stencil<c64>(nullptr, o, LINKS_F32, InPlain<c64>{in.as<c64>()}, out.as<c64>(), EpiNone<c64>{},
-                           FinNone{}, red.ws(), kNoGate);
-        else
-          run_stencil<c128>(nullptr, o, LINKS_F64, InPlain<c128>{in.as<c128>()}, out.as<c128>(), EpiNone<c128>{},
-                            FinNone{}, red.ws(), kNoGate);
-      }
-      GLX_HIP(hipEventRecord(b, nullptr));
-      GLX_HIP(hipEventSynchronize(b));
-      GLX_HIP(hipEventElapsedTime(&ms, a, b));
-      return double(ms) / reps;
+      GLX_HIP(hipMemset(x.get(), 0, size_t(n) * sizeof(c128)));
+      GLX_HIP(hipDeviceSynchronize());
+      const auto t0 = std::chrono::steady_clock::now();
+      if (fp32) (void)dev_cg_mixed(o, x.get(), b.get(), so, mo);
+      else (void)dev_cg_fused(o, x.get(), b.get(), so);
+      GLX_HIP(hipDeviceSynchronize());
+      return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
     };
     for (int fp32 = 0; fp32 < 2; fp32++) {
       if (fp32 && !o.u32) continue;
@@ -861,10 +861,10 @@ int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, dou
       int best = 0;
       double best_ms = 1e300;
       for (int c : cands) {
-        if (c > 0 && !normal_fits(o.lx, o.ly, c)) continue;
+        if (c > 0 && !normal_fits(o.lx, o.ly, c, fp32 ? normal_cpl<c64>() : normal_cpl<c128>())) continue;
         set_normal_rows(o.lx, o.ly, esize, c);
         time_one(fp32);  // warm-up
-        const double t = time_one(fp32);
+        const double t = std::min(time_one(fp32), time_one(fp32));
         if (t < best_ms) {
           best_ms = t;
           best = c;
@@ -879,8 +879,15 @@ int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, dou
         if (ms64) *ms64 = best_ms;
       }
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+  });
+}
+
+int glx_dev_time_cg_kernel(glx_lattice lat, double mass, int prec, int reps, double* ms_per_launch,
+                           double* bytes_per_site) {
+  return guarded([&] {
+    if (!ms_per_launch) throw ArgError("ms_per_launch is NULL");
+    DevOperator o = lattice_op(lat, OP_STAG_NORMAL, mass, 1);
+    *ms_per_launch = time_cg_kernel(o, prec, reps, bytes_per_site);
   });
 }
 

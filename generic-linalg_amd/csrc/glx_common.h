@@ -180,6 +180,28 @@ __device__ __forceinline__ void block_sum(double (&v)[NR], double (*sm)[kWaves])
   }
 }
 
+// Sum NR values over the block; every thread gets the same totals (the
+// per-wave sums are added in the same fixed order in every thread).
+template <int NR>
+__device__ __forceinline__ void block_sum_all(double (&v)[NR], double (*sm)[kWaves]) {
+#pragma unroll
+  for (int k = 0; k < NR; k++) v[k] = wave_sum(v[k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NR; k++) sm[k][wave] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) s += sm[k][w];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
 // Reduction workspace: partial sums [nblocks * NR].
 struct RedWs {
   double* partials;
@@ -242,6 +264,21 @@ __global__ __launch_bounds__(kBlock) void k_finalize(int nblk, Fin fin, RedWs ws
   }
   block_sum<NR>(acc, sm);
   if (threadIdx.x == 0) fin(acc);
+}
+
+// Two adjacent elements as one aligned access (16 bytes for complex FP32):
+// the caller guarantees p + j is aligned to 2 * sizeof(T).
+template <class T>
+struct alignas(2 * sizeof(T)) Pair2 {
+  T a, b;
+};
+template <class T>
+__device__ __forceinline__ Pair2<T> ld2(const T* p, int j) {
+  return *reinterpret_cast<const Pair2<T>*>(p + j);
+}
+template <class T>
+__device__ __forceinline__ void st2(T* p, int j, T a, T b) {
+  *reinterpret_cast<Pair2<T>*>(p + j) = Pair2<T>{a, b};
 }
 
 // -------------------------------------------------------------- geometry --

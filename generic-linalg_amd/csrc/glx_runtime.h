@@ -130,25 +130,41 @@ constexpr bool kind_valid_for(int k) {
 int normal_rows_for(int lx, int ly, int esize);
 void set_normal_rows(int lx, int ly, int esize, int rows);  // lx = 0: set the default
 
+// Whether a launch of D^dag D goes to the row-marching kernel (otherwise the
+// tiled one): complex vectors, a launch shape in the table that fits the
+// lattice, and aligned paired accesses.
+template <class T, class In>
+bool normal_path(const DevOperator& o, const In& in, const T* out) {
+  if constexpr (!is_cx<T>::value) {
+    return false;
+  } else {
+    if (o.kind != OP_STAG_NORMAL) return false;
+    const int sr = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
+    return sr > 0 && normal_fits(o.lx, o.ly, sr, normal_cpl<T>()) && normal_pairs_aligned<T>(in, out);
+  }
+}
+
 // Launch the tiled stencil kernel for a runtime operator kind.
 template <class T, class LK, class In, class Epi, class Fin>
 void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T* out, Epi epi, Fin fin, RedWs ws,
-                   Gate done) {
+                   Gate done, bool finalize = true) {
   const Geom g = make_geom(o.lx, o.ly);
   const OpDev<LK> od = make_opdev(o, lk);
-  if constexpr (is_cx<T>::value) {
-    if (o.kind == OP_STAG_NORMAL) {
-      const int sr = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
-      if (sr > 0 && normal_fits(o.lx, o.ly, sr)) {
-        launch_normal<T, LK, In, Epi, Fin>(st, make_ngeom(o.lx, o.ly, sr), od, in, out, epi, fin, ws, done);
-        return;
-      }
+  if (normal_path<T>(o, in, out)) {
+    if constexpr (is_cx<T>::value) {
+      launch_normal<T, LK, In, Epi, Fin>(st, make_ngeom(o.lx, o.ly, normal_rows_for(o.lx, o.ly, int(sizeof(T))),
+                                                        normal_cpl<T>()),
+                                         od, in, out, epi, fin, ws, done, has_selffin<In>::value ? false : finalize);
+      return;
     }
   }
+  if constexpr (has_selffin<In>::value) {
+    throw ArgError("a self-finalising iteration needs the row-marching normal-operator kernel");
+  } else {
 #define GLX_CASE(K)                                                                     \
   case K:                                                                               \
     if constexpr (kind_valid_for<T>(K)) {                                               \
-      launch_stencil<K, T, LK, In, Epi, Fin>(st, g, od, in, out, epi, fin, ws, done);   \
+      launch_stencil<K, T, LK, In, Epi, Fin>(st, g, od, in, out, epi, fin, ws, done, finalize); \
       return;                                                                           \
     }                                                                                   \
     break;
@@ -172,32 +188,33 @@ void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T*
       break;
   }
 #undef GLX_CASE
-  throw ArgError("operator kind " + std::to_string(o.kind) + " is not available for this element type");
+    throw ArgError("operator kind " + std::to_string(o.kind) + " is not available for this element type");
+  }
 }
 
 // Stencil launch with the link precision chosen for T: FP64 links for FP64
 // vectors; FP32 or FP16 links for FP32 vectors.
 template <class T, class In, class Epi, class Fin>
 void run_stencil(hipStream_t st, const DevOperator& o, int link_prec, In in, T* out, Epi epi, Fin fin, RedWs ws,
-                 Gate done) {
+                 Gate done, bool finalize = true) {
   if (op_links(o.kind)) {
     if constexpr (std::is_same<typename Real<T>::type, double>::value) {
       if (!o.u64) throw ArgError("operator has no FP64 links on the device");
-      dispatch_kind<T>(st, o, LinksF64{o.u64}, in, out, epi, fin, ws, done);
+      dispatch_kind<T>(st, o, LinksF64{o.u64}, in, out, epi, fin, ws, done, finalize);
     } else {
       if (link_prec == LINKS_F16) {
         if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");
-        dispatch_kind<T>(st, o, LinksF16{o.u16, o.sc16}, in, out, epi, fin, ws, done);
+        dispatch_kind<T>(st, o, LinksF16{o.u16, o.sc16}, in, out, epi, fin, ws, done, finalize);
       } else {
         if (!o.u32) throw ArgError("operator has no FP32 links on the device");
-        dispatch_kind<T>(st, o, LinksF32{o.u32}, in, out, epi, fin, ws, done);
+        dispatch_kind<T>(st, o, LinksF32{o.u32}, in, out, epi, fin, ws, done, finalize);
       }
     }
   } else {
     if constexpr (std::is_same<typename Real<T>::type, double>::value) {
-      dispatch_kind<T>(st, o, LinksF64{nullptr}, in, out, epi, fin, ws, done);
+      dispatch_kind<T>(st, o, LinksF64{nullptr}, in, out, epi, fin, ws, done, finalize);
     } else {
-      dispatch_kind<T>(st, o, LinksF32{nullptr}, in, out, epi, fin, ws, done);
+      dispatch_kind<T>(st, o, LinksF32{nullptr}, in, out, epi, fin, ws, done, finalize);
     }
   }
 }

@@ -68,6 +68,12 @@ SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const So
 // solve rather than the reference's exact sequence of operations.
 SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const SolveOpts& o);
 
+// Live timing (HIP events) of the single-pass CG's fused iteration kernel on
+// the staggered normal operator: prec 0 = FP64, LINKS_F32 / LINKS_F16 = the
+// mixed solver's FP32 kernel with that link storage.  Returns ms per launch
+// and the algorithmic HBM bytes per lattice site of one launch.
+double time_cg_kernel(const DevOperator& op, int prec, int reps, double* bytes_per_site);
+
 // Plain operator application on device vectors.
 void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st);
 

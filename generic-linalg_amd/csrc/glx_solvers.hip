@@ -100,7 +100,7 @@ struct SolveState {
   int max_iter, resid_freq, worst_first, n_shift;
   double eps, bsqrt, rsq, rsqNew;
   double beta_r, resSq, maxrr, delta;
-  int reliable, pad1, pad2, pad3;
+  int reliable, pending, pad2, pad3;  // pending: self-finalising single-pass CG
   c128 alpha, alpha_prev, beta, beta_prev, rho, omega, pap;
   double* hist;
   double* resvec;
@@ -1229,17 +1229,19 @@ struct InFast {
   T* apn;
   typename Scal<T>::type a;
   double beta;
-  __device__ void init() {
-    const int k = s->k, o = (k + 1) & 1, n = k & 1;
+  // buffers of iteration k and the scalars of its prologue
+  __device__ void setup(int k, c128 alpha, double beta_r) {
+    const int o = (k + 1) & 1, n = k & 1;
     ro = b.rs(o);
     apo = b.aps(o);
     po = b.ps(o);
     rn = b.rs(n);
     pn = b.ps(n);
     apn = b.aps(n);
-    a = as_scal<T>(s->alpha);
-    beta = s->beta_r;
+    a = as_scal<T>(alpha);
+    beta = beta_r;
   }
+  __device__ void init() { setup(s->k, s->alpha, s->beta_r); }
   __device__ T* target(T*) const { return apn; }
   struct Raw {
     T po, ro, apo;
@@ -1271,6 +1273,40 @@ struct InFast {
     return p;
   }
   __device__ T load(int j, bool own, T& aux) const { return finish(fetch(j, own), j, own, aux); }
+  // paired form for two adjacent sites (16-byte accesses for complex FP32)
+  static constexpr bool kPair = true;
+  struct RawPair {
+    Pair2<T> po, ro, apo;
+    Pair2<TX> xv;
+  };
+  __host__ bool aligned2() const {
+    auto al = [](const void* q, size_t e) { return reinterpret_cast<uintptr_t>(q) % (2 * e) == 0; };
+    return al(b.r, sizeof(T)) && al(b.p, sizeof(T)) && al(b.ap, sizeof(T)) && al(b.rs(1), sizeof(T)) &&
+           al(x, sizeof(TX));
+  }
+  __device__ RawPair fetch_pair(int j, bool own) const {
+    RawPair w;
+    w.po = ld2(po, j);
+    w.ro = ld2(ro, j);
+    w.apo = ld2(apo, j);
+    w.xv = own ? ld2(x, j) : Pair2<TX>{zero_of<TX>(), zero_of<TX>()};
+    return w;
+  }
+  __device__ void finish_pair(const RawPair& w, int j, bool own, T& v0, T& v1, T& aux0, T& aux1) const {
+    const T r0 = w.ro.a - scal(a, w.apo.a), r1 = w.ro.b - scal(a, w.apo.b);
+    v0 = r0 + rscal(beta, w.po.a);
+    v1 = r1 + rscal(beta, w.po.b);
+    if (own) {
+      st2(rn, j, r0, r1);
+      st2(pn, j, v0, v1);
+      aux0 = r0;
+      aux1 = r1;
+      TX d0, d1;
+      cvt(d0, scal(a, w.po.a));
+      cvt(d1, scal(a, w.po.b));
+      st2(x, j, w.xv.a + d0, w.xv.b + d1);
+    }
+  }
 };
 
 // <p,Ap>, |Ap|^2, <Ap,r>, |r|^2
@@ -1295,52 +1331,173 @@ struct EpiFast {
   }
 };
 
+// Scalar part of one single-pass CG step: from iteration k's sums <p,Ap>,
+// |Ap|^2, <Ap,r>, |r|^2 form alpha_k, |r_{k+1}|^2 by the recurrence, beta_k,
+// and decide whether to stop (FP64: converged or out of iterations; mixed:
+// also when the residual fell by delta since the last reliable update).
+struct FastStep {
+  c128 alpha;
+  double rsq, rsqNew, beta_r, maxrr, relres;
+  bool stop;
+};
+template <class T, bool MIXED>
+__device__ FastStep fast_step(const double* v, int k, double maxrr, const SolveState* s) {
+  c128 pap, apr;
+  double apap, rr;
+  if constexpr (is_cx<T>::value) {
+    pap = {v[0], v[1]};
+    apap = v[2];
+    apr = {v[3], v[4]};
+    rr = v[5];
+  } else {
+    pap = {v[0], 0.0};
+    apap = v[1];
+    apr = {v[2], 0.0};
+    rr = v[3];
+  }
+  FastStep f;
+  if constexpr (is_cx<T>::value) f.alpha = cdiv(c128{rr, 0.0}, pap);
+  else f.alpha = {rr / pap.re, 0.0};
+  // Re(conj(alpha) <Ap, r>)
+  const double re = f.alpha.re * apr.re + f.alpha.im * apr.im;
+  double rn = rr - 2.0 * re + (f.alpha.re * f.alpha.re + f.alpha.im * f.alpha.im) * apap;
+  if (!(rn > 0.0)) rn = 0.0;
+  f.rsq = rr;
+  f.rsqNew = rn;
+  const double rnorm = sqrt(rn);
+  f.relres = rnorm / s->bsqrt;
+  f.maxrr = maxrr;
+  if constexpr (MIXED) {
+    if (rnorm > f.maxrr) f.maxrr = rnorm;
+    f.stop = rnorm < s->delta * f.maxrr || rnorm < s->eps * s->bsqrt || k == s->max_iter - 1;
+  } else {
+    f.stop = rnorm < s->eps * s->bsqrt || k == s->max_iter - 1;
+  }
+  f.beta_r = rn / rr;
+  return f;
+}
+
+// Finaliser of the two-kernel form (stencil + k_finalize).
 template <class T, bool MIXED>
 struct FinFast {
   SolveState* s;
   __device__ void operator()(const double* v) const {
-    c128 pap, apr;
-    double apap, rr;
-    if constexpr (is_cx<T>::value) {
-      pap = {v[0], v[1]};
-      apap = v[2];
-      apr = {v[3], v[4]};
-      rr = v[5];
-    } else {
-      pap = {v[0], 0.0};
-      apap = v[1];
-      apr = {v[2], 0.0};
-      rr = v[3];
-    }
     const int k = s->k;
-    c128 alpha;
-    if constexpr (is_cx<T>::value) alpha = cdiv(c128{rr, 0.0}, pap);
-    else alpha = {rr / pap.re, 0.0};
-    // Re(conj(alpha) <Ap, r>)
-    const double re = alpha.re * apr.re + alpha.im * apr.im;
-    double rn = rr - 2.0 * re + (alpha.re * alpha.re + alpha.im * alpha.im) * apap;
-    if (!(rn > 0.0)) rn = 0.0;
-    s->alpha = alpha;
-    s->rsq = rr;
-    s->rsqNew = rn;
-    const double rnorm = sqrt(rn);
-    if (s->hist) s->hist[k] = rnorm / s->bsqrt;
-    if constexpr (MIXED) {
-      if (rnorm > s->maxrr) s->maxrr = rnorm;
-      if (rnorm < s->delta * s->maxrr || rnorm < s->eps * s->bsqrt || k == s->max_iter - 1) {
-        s->kfinal = k;
-        s->done = 2;
-        return;
+    const FastStep f = fast_step<T, MIXED>(v, k, s->maxrr, s);
+    s->alpha = f.alpha;
+    s->rsq = f.rsq;
+    s->rsqNew = f.rsqNew;
+    if (s->hist) s->hist[k] = f.relres;
+    if constexpr (MIXED) s->maxrr = f.maxrr;
+    if (f.stop) {
+      s->kfinal = k;
+      s->done = MIXED ? 2 : 1;
+      return;
+    }
+    s->beta_r = f.beta_r;
+    s->k = k + 1;
+  }
+};
+
+// Self-finalising form (row-marching kernel only): the iteration kernel
+// itself reduces the previous launch's partial sums in every block (the
+// grid is the same every launch and the order fixed, so every block computes
+// identical scalars), and block 0 writes the resulting state.  No finaliser
+// launch and no device-wide atomics per iteration.  Launches alternate
+// between two state copies and two partial-sum buffers by a launch parity q
+// fixed on the host: launch q reads state `rd` and buffer q ^ 1, writes state
+// `wr` and buffer q; nothing a launch reads is written during that launch.
+// Groups hold an even number of launches, so the state between groups is
+// always the home copy (parity 1 writes it), which the host mirror, the
+// reliable-update kernels and the final x update use.
+//   pending = 1: the partial sums of iteration k (in buffer q ^ 1) await
+//                reduction; this launch performs iteration k + 1
+//   pending = 0: alpha / beta_r are final (start, after a reliable update);
+//                this launch performs iteration k
+// A launch that finds done != 0 only carries the state over to `wr`.
+template <class T, class TX, bool MIXED>
+struct InFastSF : InFast<T, TX> {
+  static constexpr bool kSelfFin = true;
+  static constexpr int NR = EpiFast<T>::NR;
+  const SolveState* rd;
+  SolveState* wr;
+  int q;
+  int pending, k0;
+  bool gated;
+  FastStep f;
+  __device__ void init() {
+    gated = rd->done != 0;
+    pending = rd->pending;
+    k0 = rd->k;
+    // buffers of the iteration this launch performs; its scalars are final
+    // only after prelude()
+    this->setup(pending ? k0 + 1 : k0, rd->alpha, rd->beta_r);
+  }
+  // Block-wide; every thread returns the same decision (true = iterate).
+  __device__ bool prelude(RedWs ws) {
+    if (gated) return false;
+    if (!pending) return true;
+    __shared__ double sm[NR][kWaves];
+    const int nblk = gridDim.x;
+    const double* P = ws.partials + (q ^ 1) * NR * nblk;
+    double acc[NR];
+#pragma unroll
+    for (int k = 0; k < NR; k++) acc[k] = 0.0;
+    // two terms per thread per sweep (512 blocks in one memory round trip)
+    // keeps the registers of this prelude small
+    constexpr int U = 2;
+    for (int b0 = threadIdx.x; b0 < nblk; b0 += kBlock * U) {
+      double v[U][NR];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int b = b0 + u * kBlock;
+#pragma unroll
+        for (int k = 0; k < NR; k++) v[u][k] = b < nblk ? P[k * nblk + b] : 0.0;
       }
-    } else {
-      if (rnorm < s->eps * s->bsqrt || k == s->max_iter - 1) {
-        s->kfinal = k;
-        s->done = 1;
-        return;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+#pragma unroll
+        for (int k = 0; k < NR; k++) acc[k] += v[u][k];
       }
     }
-    s->beta_r = rn / rr;
-    s->k = k + 1;
+    block_sum_all<NR>(acc, sm);
+    f = fast_step<T, MIXED>(acc, k0, rd->maxrr, rd);
+    if (f.stop) return false;
+    this->a = as_scal<T>(f.alpha);
+    this->beta = f.beta_r;
+    return true;
+  }
+  __device__ RedWs out_ws(RedWs ws) const { return RedWs{ws.partials + q * NR * gridDim.x}; }
+  // At the end of the kernel: block 0 writes the state of the next launch.
+  __device__ void arrive(bool iterated) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    // carried over unless changed below
+    wr->k = rd->k;
+    wr->done = rd->done;
+    wr->kfinal = rd->kfinal;
+    wr->pending = rd->pending;
+    wr->alpha = rd->alpha;
+    wr->beta_r = rd->beta_r;
+    wr->rsq = rd->rsq;
+    wr->rsqNew = rd->rsqNew;
+    wr->maxrr = rd->maxrr;
+    if (gated) return;
+    if (pending) {
+      wr->alpha = f.alpha;
+      wr->rsq = f.rsq;
+      wr->rsqNew = f.rsqNew;
+      if (rd->hist) rd->hist[k0] = f.relres;
+      if constexpr (MIXED) wr->maxrr = f.maxrr;
+      if (f.stop) {
+        wr->kfinal = k0;
+        wr->done = MIXED ? 2 : 1;
+        wr->pending = 0;
+        return;
+      }
+      wr->beta_r = f.beta_r;
+      wr->k = k0 + 1;
+    }
+    if (iterated) wr->pending = 1;
   }
 };
 
@@ -1376,6 +1533,7 @@ struct FinFastInit {
     s->beta_r = 0.0;
     s->k = 0;
     s->done = 0;
+    s->pending = 0;
   }
 };
 
@@ -1437,6 +1595,7 @@ struct FinFastReliable {
     s->alpha = {0.0, 0.0};
     s->maxrr = sqrt(rt);
     s->k = k + 1;
+    s->pending = 0;
     s->done = 0;
   }
 };
@@ -1468,6 +1627,62 @@ struct OwnedStream {
     }
   }
 };
+
+// Live timing of the single-pass CG's iteration kernel alone, in the form the
+// solver launches it (self-finalising where the row-marching kernel takes
+// the lattice, else the stencil without its finaliser): `reps` launches on
+// one stream between two HIP events, on zeroed vectors (no data-dependent
+// control flow in the kernel).  Returns ms per launch.
+template <class TD, class TS, bool MIXED>
+double time_fast_kernel_impl(const DevOperator& o, int link_prec, int reps) {
+  const int n = o.size();
+  OwnedStream own;
+  hipStream_t st = own.st;
+  Work w(st, stencil_blocks(o.lx, o.ly), 0, false);
+  std::vector<DevBuf> keep;
+  auto vec = [&](size_t bytes) {
+    keep.emplace_back(bytes);
+    GLX_HIP(hipMemsetAsync(keep.back().get(), 0, bytes, st));
+    return keep.back().get();
+  };
+  FastBufs<TS> fb;
+  fb.n = n;
+  fb.r = static_cast<TS*>(vec(2 * n * sizeof(TS)));
+  fb.p = static_cast<TS*>(vec(2 * n * sizeof(TS)));
+  fb.ap = static_cast<TS*>(vec(2 * n * sizeof(TS)));
+  using TXv = typename std::conditional<MIXED, TS, TD>::type;
+  TXv* xv = static_cast<TXv*>(vec(n * sizeof(TXv)));
+  const bool sf = normal_path<TS>(o, InFast<TS, TXv>{fb, xv, w.ds}, fb.aps(0));
+  DevBuf shadow_buf(sizeof(SolveState));
+  SolveState* shadow = shadow_buf.as<SolveState>();
+  GLX_HIP(hipMemcpyAsync(shadow, w.ds, sizeof(SolveState), hipMemcpyDeviceToDevice, st));
+  int q = 0;
+  auto launch = [&]() {
+    if (sf) {
+      const SolveState* rd = q == 0 ? w.ds : shadow;
+      SolveState* wr = q == 0 ? shadow : w.ds;
+      run_stencil<TS>(st, o, link_prec, InFastSF<TS, TXv, MIXED>{{fb, xv, rd}, rd, wr, q}, fb.aps(0),
+                      EpiFast<TS>{}, FinNone{}, w.ws(), kNoGate, false);
+      q ^= 1;
+    } else {
+      run_stencil<TS>(st, o, link_prec, InFast<TS, TXv>{fb, xv, w.ds}, fb.aps(0), EpiFast<TS>{},
+                      FinFast<TS, MIXED>{w.ds}, w.ws(), kNoGate, false);
+    }
+  };
+  launch();  // warm-up
+  hipEvent_t a, b;
+  GLX_HIP(hipEventCreate(&a));
+  GLX_HIP(hipEventCreate(&b));
+  GLX_HIP(hipEventRecord(a, st));
+  for (int k = 0; k < reps; k++) launch();
+  GLX_HIP(hipEventRecord(b, st));
+  GLX_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  GLX_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return double(ms) / reps;
+}
 
 template <class TD, class TS, bool MIXED>
 SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpts& opt, const MixedOpts& mo) {
@@ -1503,15 +1718,28 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
   apply_typed<TD>(st, o, LINKS_F64, t64, x, w.ws(), kNoGate);
   launch_point(st, n, PtFastInit<TD, TS>{fb.rs(1), fb.aps(1), fb.ps(1), b, t64}, FinFastInit{w.ds}, w.ws(), kNoGate);
   const int link_prec = MIXED ? mo.link_prec : LINKS_F64;
-  auto enq_iter = [&]() {
-    if constexpr (MIXED) {
-      run_stencil<TS>(st, o, link_prec, InFast<TS, TS>{fb, xs, w.ds}, fb.aps(0), EpiFast<TS>{}, FinFast<TS, true>{w.ds},
-                      w.ws(), w.done());
+  using TXv = typename std::conditional<MIXED, TS, TD>::type;
+  TXv* xv;
+  if constexpr (MIXED) xv = xs;
+  else xv = x;
+  // self-finalising iteration (no finaliser launch) where the row-marching
+  // kernel takes the lattice; the stencil + finaliser pair otherwise
+  const bool sf = normal_path<TS>(o, InFast<TS, TXv>{fb, xv, w.ds}, fb.aps(0));
+  DevBuf shadow_buf(sf ? sizeof(SolveState) : 1);
+  SolveState* shadow = shadow_buf.as<SolveState>();
+  auto enq_iter = [&](int q) {
+    if (sf) {
+      const SolveState* rd = q == 0 ? w.ds : shadow;
+      SolveState* wr = q == 0 ? shadow : w.ds;
+      run_stencil<TS>(st, o, link_prec, InFastSF<TS, TXv, MIXED>{{fb, xv, rd}, rd, wr, q}, fb.aps(0),
+                      EpiFast<TS>{}, FinNone{}, w.ws(), kNoGate, false);
     } else {
-      run_stencil<TS>(st, o, link_prec, InFast<TS, TD>{fb, x, w.ds}, fb.aps(0), EpiFast<TS>{}, FinFast<TS, false>{w.ds},
-                      w.ws(), w.done());
+      run_stencil<TS>(st, o, link_prec, InFast<TS, TXv>{fb, xv, w.ds}, fb.aps(0), EpiFast<TS>{},
+                      FinFast<TS, MIXED>{w.ds}, w.ws(), w.done());
     }
   };
+  // the shadow copy starts as the initial state (constants included)
+  if (sf) GLX_HIP(hipMemcpyAsync(shadow, w.ds, sizeof(SolveState), hipMemcpyDeviceToDevice, st));
   auto enq_reliable = [&]() {
     if constexpr (MIXED) {
       launch_point(st, n, PtFastAccum<TD, TS>{x, xs, fb, w.ds}, FinNone{}, w.ws(), w.paused());
@@ -1519,9 +1747,11 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
       launch_point(st, n, PtFastTrue<TD, TS>{fb, b, t64, w.ds}, FinFastReliable{w.ds}, w.ws(), w.paused());
     }
   };
-  const int group = std::max(1, opt.group);
+  // the self-finalising form alternates state copies: an even number of
+  // launches per group ends every group on the home copy
+  const int group = sf ? 2 * ((std::max(1, opt.group) + 1) / 2) : std::max(1, opt.group);
   auto enq_group = [&]() {
-    for (int g = 0; g < group; g++) enq_iter();
+    for (int g = 0; g < group; g++) enq_iter(g & 1);
     enq_reliable();
   };
   // optional: one captured graph of a group, replayed
@@ -1627,6 +1857,24 @@ SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const So
   if (real_op(op))
     return fastcg_impl<double, double, false>(op, static_cast<double*>(x), static_cast<const double*>(b), o, m);
   return fastcg_impl<c128, c128, false>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
+}
+
+double time_cg_kernel(const DevOperator& o, int prec, int reps, double* bytes_per_site) {
+  if (!o.tiled() || o.kind != OP_STAG_NORMAL) throw ArgError("timing is offered for the staggered normal operator");
+  if (reps <= 0) throw ArgError("reps must be positive");
+  if (prec == 0) {
+    // reads p, r, Ap (old slots), x; writes r, p, Ap (new slots), x; links
+    if (bytes_per_site) *bytes_per_site = 3 * 16 + 16 + 3 * 16 + 16 + 2 * 16;
+    return time_fast_kernel_impl<c128, c128, false>(o, LINKS_F64, reps);
+  }
+  if (prec == LINKS_F16) {
+    if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");
+    if (bytes_per_site) *bytes_per_site = 3 * 8 + 8 + 3 * 8 + 8 + 2 * 4 + 4;
+    return time_fast_kernel_impl<c128, c64, true>(o, LINKS_F16, reps);
+  }
+  if (!o.u32) throw ArgError("operator has no FP32 links on the device");
+  if (bytes_per_site) *bytes_per_site = 3 * 8 + 8 + 3 * 8 + 8 + 2 * 8;
+  return time_fast_kernel_impl<c128, c64, true>(o, LINKS_F32, reps);
 }
 
 void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st) {

@@ -105,6 +105,14 @@ struct LinksF16 {
     float sc;
   };
   __device__ __forceinline__ Raw fetch(int s) const { return Raw{u[s], scale[s]}; }
+  // sites s and s + 1 (s even): one 16-byte and one 8-byte load
+  static constexpr bool kPair = true;
+  __device__ __forceinline__ void fetch_pair(int s, Raw& r0, Raw& r1) const {
+    const Pair2<HalfLink> h = ld2(u, s);
+    const Pair2<float> c = ld2(scale, s);
+    r0 = Raw{h.a, c.a};
+    r1 = Raw{h.b, c.b};
+  }
   __device__ __forceinline__ static void finish(const Raw& r, C& ux, C& uy) {
     const float2 a = __half22float2(r.h.x), b = __half22float2(r.h.y);
     ux = {r.sc * a.x, r.sc * a.y};
@@ -146,6 +154,10 @@ struct OpDev {
 // reads of a later row before it computes the current one: fetch(j, own)
 // performs only the reads into a Raw record, finish(raw, j, own, aux) the
 // arithmetic and the owned-site stores; load = finish(fetch).
+//
+// Optionally (kPair) a prologue offers the same for the two sites j, j + 1
+// (j even, all its vectors aligned to two elements) with paired accesses:
+// fetch_pair(j, own) -> RawPair, finish_pair(raw, j, own, v0, v1, aux0, aux1).
 template <class T>
 struct InPlain {
   static constexpr bool kAux = false;
@@ -156,6 +168,14 @@ struct InPlain {
   __device__ __forceinline__ void init() {}
   __device__ __forceinline__ Raw fetch(int j, bool) const { return Raw{p[j]}; }
   __device__ __forceinline__ T finish(const Raw& r, int, bool, T&) const { return r.v; }
+  static constexpr bool kPair = true;
+  using RawPair = Pair2<T>;
+  __device__ __forceinline__ RawPair fetch_pair(int j, bool) const { return ld2(p, j); }
+  __device__ __forceinline__ void finish_pair(const RawPair& r, int, bool, T& v0, T& v1, T&, T&) const {
+    v0 = r.a;
+    v1 = r.b;
+  }
+  __host__ __device__ bool aligned2() const { return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0; }
   __device__ __forceinline__ T load(int j, bool, T&) const { return p[j]; }
   __device__ __forceinline__ T* target(T* out) const { return out; }
 };
@@ -489,11 +509,12 @@ inline Geom make_geom(int lx, int ly) {
 
 template <int KIND, class T, class LK, class In, class Epi, class Fin>
 void launch_stencil(hipStream_t st, const Geom& g, const OpDev<LK>& op, In in, T* out, Epi epi, Fin fin, RedWs ws,
-                    Gate gate) {
+                    Gate gate, bool finalize = true) {
   hipLaunchKernelGGL((k_stencil<KIND, T, LK, In, Epi, Fin>), dim3(g.nblk), dim3(kBlock), 0, st, g, op, in, out, epi,
                      fin, ws, gate);
   GLX_CHECK_LAUNCH();
   if constexpr (Epi::NR > 0) {
+    if (!finalize) return;
     hipLaunchKernelGGL((k_finalize<Epi::NR, Fin>), dim3(1), dim3(kBlock), 0, st, g.nblk, fin, ws, gate);
     GLX_CHECK_LAUNCH();
   }

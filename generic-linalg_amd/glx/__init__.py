@@ -158,6 +158,8 @@ def lib():
         L.glx_dev_cg_fused.argtypes = [vp, i, d, vp, vp, i, d, VP, IP]
         L.glx_set_tuning.restype = i
         L.glx_set_tuning.argtypes = [i, i]
+        L.glx_dev_time_cg_kernel.restype = i
+        L.glx_dev_time_cg_kernel.argtypes = [vp, d, i, i, C.POINTER(d), C.POINTER(d)]
         L.glx_set_normal_rows.restype = i
         L.glx_set_normal_rows.argtypes = [i, i, i, i]
         L.glx_get_normal_rows.restype = i
@@ -518,6 +520,15 @@ def time_apply(lat, op: str, mass: float, out: DeviceArray, inp: DeviceArray, re
     _check(lib().glx_dev_time_apply(lat.h, OP_KINDS[op], mass, int(fp32), link_prec, out.ptr, inp.ptr, reps,
                                     C.byref(ms)), "glx_dev_time_apply")
     return ms.value
+
+
+def time_cg_kernel(lat, mass: float, prec: int = 0, reps: int = 100):
+    """(ms per launch, algorithmic bytes per site) of the single-pass CG's
+    fused iteration kernel: prec 0 FP64, LINKS_F32 / LINKS_F16 mixed FP32."""
+    ms, by = C.c_double(0.0), C.c_double(0.0)
+    _check(lib().glx_dev_time_cg_kernel(lat.h, mass, prec, reps, C.byref(ms), C.byref(by)),
+           "glx_dev_time_cg_kernel")
+    return ms.value, by.value
 
 
 def synchronize():

@@ -264,10 +264,12 @@ int glx_set_tuning(int group, int use_graph);
  * sizes always use the tiled kernel. */
 int glx_set_normal_rows(int lx, int ly, int esize, int rows);
 int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
-/* Autotuner: times D^dag D on the lattice for every candidate launch shape
- * (tiled; 8..64 rows per block), in FP64 and (if the lattice has FP32 links)
- * FP32, records the fastest in the table above and returns it with its time
- * per application in ms.  Any output pointer may be NULL. */
+/* Autotuner: for every candidate launch shape (tiled; 8..64 rows per block)
+ * times `reps` iterations of the single-pass FP64 CG (glx_dev_cg_fused) and,
+ * if the lattice has FP32 links, of the mixed-precision CG on D^dag D of the
+ * lattice, records the fastest shape per precision in the table above and
+ * returns it with its time per iteration in ms.  Any output pointer may be
+ * NULL. */
 int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, double* ms64, double* ms32);
 /* out = A in on device vectors; fp32 selects complex<float> vectors with the
  * FP32 (link_prec 1) or FP16 (link_prec 2) links. */
@@ -279,6 +281,13 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
                        const void* in_dev, int reps, double* ms_per_apply);
 
 /* Device memory helpers (so that callers need no HIP headers). */
+/* Live timing (HIP events around `reps` back-to-back launches on one stream)
+ * of the single-pass CG's fused iteration kernel on D^dag D of the lattice:
+ * prec 0 = FP64 (glx_dev_cg_fused), 1 / 2 = the mixed-precision CG's FP32
+ * kernel with FP32 / FP16 links.  Returns ms per launch and the algorithmic
+ * HBM bytes per site one launch moves.  No reference counterpart. */
+int glx_dev_time_cg_kernel(glx_lattice lat, double mass, int prec, int reps, double* ms_per_launch,
+                           double* bytes_per_site);
 int glx_dev_alloc(void** ptr, size_t bytes);
 int glx_dev_free(void* ptr);
 int glx_memcpy_htod(void* dst_dev, const void* src_host, size_t bytes);
