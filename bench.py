@@ -123,6 +123,9 @@ def main():
     ap.add_argument("--links", choices=["f32", "f16"], default="f32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true", help="skip the 4096^2 stencil measurement")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
+                         "for scaling rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the solver path has no multi-GPU code; "
@@ -130,6 +133,15 @@ def main():
         return 2
 
     import glx
+
+    if args.copy_mib > 0:
+        n = args.copy_mib << 16  # complex128 elements (16 bytes)
+        src = glx.DeviceArray(np.ones(n, np.complex128))
+        dst = glx.DeviceArray(np.zeros(n, np.complex128))
+        for _ in range(3):
+            glx.copy_device(dst, src)
+        glx.synchronize()
+        del src, dst
 
     L, mass = args.L, args.mass
     V = L * L

@@ -25,8 +25,8 @@ using namespace glx;
 // ================================================================ errors ==
 static thread_local std::string g_last_error;
 // launch tuning of the single-pass and mixed-precision CG (glx_set_tuning)
-static int g_group = 8;
-static int g_use_graph = 1;
+static int g_group = 16;
+static int g_use_graph = 0;
 
 template <class F>
 static int guarded(F&& f) {
@@ -905,6 +905,9 @@ int glx_memcpy_htod(void* dst, const void* src, size_t bytes) {
 }
 int glx_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
   return guarded([&] { GLX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int glx_memcpy_dtod(void* dst, const void* src, size_t bytes) {
+  return guarded([&] { GLX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); });
 }
 int glx_device_synchronize(void) {
   return guarded([&] { GLX_HIP(hipDeviceSynchronize()); });

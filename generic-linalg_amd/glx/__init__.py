@@ -178,6 +178,8 @@ def lib():
         L.glx_memcpy_htod.argtypes = [vp, vp, C.c_size_t]
         L.glx_memcpy_dtoh.restype = i
         L.glx_memcpy_dtoh.argtypes = [vp, vp, C.c_size_t]
+        L.glx_memcpy_dtod.restype = i
+        L.glx_memcpy_dtod.argtypes = [vp, vp, C.c_size_t]
         L.glx_device_synchronize.restype = i
         L.glx_device_count.restype = i
         L.glx_device_count.argtypes = [C.POINTER(C.c_int)]
@@ -447,7 +449,7 @@ def dev_cg_fused(lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_
     return _info(ci, tx.text())
 
 
-def set_tuning(group: int = 8, use_graph: bool = True):
+def set_tuning(group: int = 16, use_graph: bool = False):
     _check(lib().glx_set_tuning(group, int(use_graph)), "glx_set_tuning")
 
 
@@ -529,6 +531,13 @@ def time_cg_kernel(lat, mass: float, prec: int = 0, reps: int = 100):
     _check(lib().glx_dev_time_cg_kernel(lat.h, mass, prec, reps, C.byref(ms), C.byref(by)),
            "glx_dev_time_cg_kernel")
     return ms.value, by.value
+
+
+def copy_device(dst: DeviceArray, src: DeviceArray):
+    """dst = src on the device (hipMemcpy device to device; sizes must match)."""
+    if dst.nbytes != src.nbytes:
+        raise GlxError("copy_device: sizes differ")
+    _check(lib().glx_memcpy_dtod(dst.ptr, src.ptr, src.nbytes), "glx_memcpy_dtod")
 
 
 def synchronize():

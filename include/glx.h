@@ -252,16 +252,18 @@ int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, cons
 int glx_dev_cg_fused(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
                      double eps, glx_verbose* verbosity, glx_inversion_info* info);
 /* Launch tuning of the single-pass CG and the mixed-precision CG: iterations
- * per enqueued group (default 8) and whether a group is captured once as a
- * hipGraph and replayed (default 1). */
+ * per enqueued group (default 16; rounded up to even where the iteration is
+ * self-finalising) and whether a group is captured once as a hipGraph and
+ * replayed (default 0: measured slightly slower than direct launches). */
 int glx_set_tuning(int group, int use_graph);
 /* Launch shape of the fused normal operator D^dag D (no reference
  * counterpart; operators.cpp:444-453 is the operator it computes): rows per
  * block of the row-marching kernel for an lx x ly lattice and vector element
  * size esize (16 = complex FP64, 8 = complex FP32); 0 selects the tiled
  * kernel.  lx = ly = 0 sets the default for sizes not in the table (16).
- * The row-marching kernel needs lx % 64 == 0, ly % 4 == 0, ly >= 8; other
- * sizes always use the tiled kernel. */
+ * The row-marching kernel needs lx % 64 == 0 (FP64) or lx % 128 == 0 (FP32,
+ * two sites per lane), ly % 4 == 0, ly >= 8; other sizes always use the
+ * tiled kernel. */
 int glx_set_normal_rows(int lx, int ly, int esize, int rows);
 int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
 /* Autotuner: for every candidate launch shape (tiled; 8..64 rows per block)
@@ -280,7 +282,6 @@ int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int 
 int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int link_prec, void* out_dev,
                        const void* in_dev, int reps, double* ms_per_apply);
 
-/* Device memory helpers (so that callers need no HIP headers). */
 /* Live timing (HIP events around `reps` back-to-back launches on one stream)
  * of the single-pass CG's fused iteration kernel on D^dag D of the lattice:
  * prec 0 = FP64 (glx_dev_cg_fused), 1 / 2 = the mixed-precision CG's FP32
@@ -288,10 +289,13 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
  * HBM bytes per site one launch moves.  No reference counterpart. */
 int glx_dev_time_cg_kernel(glx_lattice lat, double mass, int prec, int reps, double* ms_per_launch,
                            double* bytes_per_site);
+
+/* Device memory helpers (so that callers need no HIP headers). */
 int glx_dev_alloc(void** ptr, size_t bytes);
 int glx_dev_free(void* ptr);
 int glx_memcpy_htod(void* dst_dev, const void* src_host, size_t bytes);
 int glx_memcpy_dtoh(void* dst_host, const void* src_dev, size_t bytes);
+int glx_memcpy_dtod(void* dst_dev, const void* src_dev, size_t bytes);
 int glx_device_synchronize(void);
 int glx_device_count(int* count);
 /* Frees the library's cached device buffers. */
