@@ -97,6 +97,38 @@ def stencil_beyond_mall(glx, L=4096):
             "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def gradient_timing(glx, L=512, mass=0.05, reps=3):
+    """Gradient of S = phi^dag (D^dag D)^-1 phi on the 512^2 workload
+    (theta ~ N(0, 0.4^2), phi complex Gaussian): adjoint solve to 1e-10 +
+    D chi + per-link force, FP64 and FP32-bulk, gradient left in HBM."""
+    rng = np.random.default_rng(21)
+    theta = rng.normal(0.0, 0.4, 2 * L * L)
+    phi = rng.normal(size=L * L) + 1j * rng.normal(size=L * L)
+    lat = glx.Lattice(np.exp(1j * theta), L, L)
+    glx.autotune_normal(lat, reps=20)
+    dphi = glx.DeviceArray(phi)
+    g = glx.DeviceArray(nbytes=2 * L * L * 8, dtype=np.float64, shape=(2 * L * L,))
+    out = {"L": L, "mass": mass, "tol": 1e-10}
+    ref = None
+    for name, prec in (("fp64", 0), ("fp32_bulk", glx.LINKS_F32)):
+        glx.staggered_grad(lat, mass, dphi, precision=prec, grad=g, download=False)  # warm-up
+        ts = []
+        for _ in range(reps):
+            glx.synchronize()
+            t0 = time.perf_counter()
+            r = glx.staggered_grad(lat, mass, dphi, precision=prec, grad=g, download=False)
+            glx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        gv = g.download()
+        if ref is None:
+            ref = gv
+        out[name] = {"ms": round(float(np.median(ts)) * 1e3, 3), "iters": r.info.iter, "dS_dm": r.dS_dm,
+                     "relerr_vs_fp64": float(np.linalg.norm(gv - ref) / np.linalg.norm(ref))}
+    del lat
+    glx.release_cache()
+    return out
+
+
 def traffic_from_profile(links: str, alg_bytes: float):
     """HBM bytes per launch of the dominant kernel as measured by rocprofv3
     FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of known size), read from
@@ -123,6 +155,7 @@ def main():
     ap.add_argument("--links", choices=["f32", "f16"], default="f32")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true", help="skip the 4096^2 stencil measurement")
+    ap.add_argument("--no-grad", action="store_true", help="skip the 512^2 gradient timing")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
                          "for scaling rocprofv3 byte counters)")
@@ -234,6 +267,8 @@ def main():
     if not args.no_big:
         big = stencil_beyond_mall(glx)
 
+    grad = None if args.no_grad else gradient_timing(glx)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -294,6 +329,7 @@ def main():
             "bytes_per_site": it_bps,
             "ms_per_launch": it_ms,
         },
+        "gradient": grad,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

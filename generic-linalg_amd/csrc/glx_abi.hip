@@ -738,6 +738,29 @@ int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, cons
   });
 }
 
+int glx_dev_staggered_grad(glx_lattice lat, double mass, const double* phi_dev, int precision, int max_iter,
+                           double eps, double* grad_dev, double* chi_dev, double* dS_dm, double* S, double* S_robust,
+                           glx_inversion_info* info) {
+  return guarded([&] {
+    if (!phi_dev || !grad_dev) throw ArgError("phi_dev and grad_dev must not be NULL");
+    if (precision < 0 || precision > 2) throw ArgError("precision must be 0 (FP64), 1 (FP32 links) or 2 (FP16 links)");
+    DevOperator normal = lattice_op(lat, OP_STAG_NORMAL, mass, 1);
+    DevOperator dop = lattice_op(lat, OP_STAG, mass, 1);
+    if (precision == LINKS_F16 && !normal.u16) throw ArgError("lattice was created without GLX_LINKS_F16");
+    if (precision == LINKS_F32 && !normal.u32) throw ArgError("lattice was created without GLX_LINKS_F32");
+    SolveOpts so;
+    so.max_iter = max_iter;
+    so.eps = eps;
+    so.group = g_group;
+    so.use_graph = g_use_graph != 0;
+    GradResult g = dev_staggered_grad(normal, dop, phi_dev, precision, so, grad_dev, chi_dev);
+    if (dS_dm) *dS_dm = g.dS_dm;
+    if (S) *S = g.S;
+    if (S_robust) *S_robust = g.S_robust;
+    fill_info(info, g.solve, g.solve.name.c_str());
+  });
+}
+
 int glx_set_tuning(int group, int use_graph) {
   return guarded([&] {
     if (group < 1 || group > 1024) throw ArgError("group must be in [1, 1024]");

@@ -74,6 +74,20 @@ SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const So
 // and the algorithmic HBM bytes per lattice site of one launch.
 double time_cg_kernel(const DevOperator& op, int prec, int reps, double* bytes_per_site);
 
+// Gradient of S(theta, m) = phi^dag (D^dag D)^-1 phi (no reference
+// counterpart): one solve (precision 0: FP64 single-pass CG; LINKS_F32 /
+// LINKS_F16: mixed-precision CG), psi = D chi, one per-link force pass.
+// normal = the staggered normal operator, dop = D, both on the same links.
+// grad: device, 2V doubles in the links' layout; chi_out: device or null.
+struct GradResult {
+  double S = 0.0;         // Re phi^dag chi
+  double S_robust = 0.0;  // 2 Re phi^dag chi - |D chi|^2 (error second order in the residual)
+  double dS_dm = 0.0;     // -2 m |chi|^2
+  SolveResult solve;
+};
+GradResult dev_staggered_grad(const DevOperator& normal, const DevOperator& dop, const void* phi, int precision,
+                              const SolveOpts& o, double* grad, void* chi_out);
+
 // Plain operator application on device vectors.
 void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st);
 

@@ -1859,6 +1859,79 @@ SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const So
   return fastcg_impl<c128, c128, false>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
 }
 
+// ----------------------------------------------------------- gradient --
+// dS/dtheta_mu(x) = -eta_mu(x) Im[psi(x)^* U_mu(x) chi(x+mu) + psi(x+mu)^* U_mu(x)^* chi(x)]
+// for S = phi^dag (D^dag D)^-1 phi, chi = (D^dag D)^-1 phi, psi = D chi
+// (dU/dtheta = i U; eta_x = 1, eta_y(x) = (-1)^x, so eta_mu(x + mu) = eta_mu(x)).
+// Fused sums: |chi|^2 (dS/dm = -2 m |chi|^2), Re phi^dag chi (S), |psi|^2
+// (= chi^dag D^dag D chi, for the estimator 2 Re phi^dag chi - |D chi|^2).
+struct PtForce {
+  static constexpr int NR = 3;
+  int lx, ly;
+  const c128* chi;
+  const c128* psi;
+  const c128* phi;
+  const c128* u;
+  double* grad;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const int y = i / lx, x = i - y * lx;
+    const int nx = y * lx + (x + 1 == lx ? 0 : x + 1);
+    const int ny = (y + 1 == ly ? 0 : y + 1) * lx + x;
+    const c128 ci = chi[i], pi = psi[i];
+    auto term = [&](c128 U, int nb) {
+      const c128 a = cmulc(pi, U * chi[nb]);      // psi(x)^* U chi(x+mu)
+      const c128 b = cmulc(psi[nb], cmulc(U, ci));  // psi(x+mu)^* U^* chi(x)
+      return a.im + b.im;
+    };
+    grad[2 * i] = -term(u[2 * i], nx);
+    const double ty = term(u[2 * i + 1], ny);
+    grad[2 * i + 1] = (x & 1) ? ty : -ty;
+    part[0] += norm2(ci);
+    part[1] += dotc(phi[i], ci).re;
+    part[2] += norm2(pi);
+  }
+};
+struct FinStore3 {
+  double* out;
+  __device__ void operator()(const double* v) const {
+    out[0] = v[0];
+    out[1] = v[1];
+    out[2] = v[2];
+  }
+};
+
+GradResult dev_staggered_grad(const DevOperator& normal, const DevOperator& dop, const void* phi, int precision,
+                              const SolveOpts& o, double* grad, void* chi_out) {
+  if (normal.kind != OP_STAG_NORMAL || dop.kind != OP_STAG || !normal.u64)
+    throw ArgError("the gradient is defined for the staggered normal operator");
+  const int n = normal.size();
+  DevBuf chiB(chi_out ? 1 : size_t(n) * sizeof(c128)), psiB(size_t(n) * sizeof(c128)), sums(4 * sizeof(double));
+  c128* chi = chi_out ? static_cast<c128*>(chi_out) : chiB.as<c128>();
+  GLX_HIP(hipMemsetAsync(chi, 0, size_t(n) * sizeof(c128), o.stream));
+  GradResult g;
+  if (precision == 0) {
+    g.solve = dev_cg_fused(normal, chi, phi, o);
+  } else {
+    MixedOpts mo;
+    mo.link_prec = precision == LINKS_F16 ? LINKS_F16 : LINKS_F32;
+    g.solve = dev_cg_mixed(normal, chi, phi, o, mo);
+  }
+  dev_apply(dop, psiB.get(), chi, LINKS_F64, 0, o.stream);
+  RedSpace red;
+  red.init(kMaxPointBlocks, o.stream);
+  launch_point(o.stream, n,
+               PtForce{normal.lx, normal.ly, chi, psiB.as<c128>(), static_cast<const c128*>(phi), normal.u64, grad},
+               FinStore3{sums.as<double>()}, red.ws(), kNoGate);
+  double h[3];
+  GLX_HIP(hipMemcpyAsync(h, sums.get(), 3 * sizeof(double), hipMemcpyDeviceToHost, o.stream));
+  GLX_HIP(hipStreamSynchronize(o.stream));
+  g.dS_dm = -2.0 * normal.mass * h[0];
+  g.S = h[1];
+  g.S_robust = 2.0 * h[1] - h[2];
+  return g;
+}
+
 double time_cg_kernel(const DevOperator& o, int prec, int reps, double* bytes_per_site) {
   if (!o.tiled() || o.kind != OP_STAG_NORMAL) throw ArgError("timing is offered for the staggered normal operator");
   if (reps <= 0) throw ArgError("reps must be positive");

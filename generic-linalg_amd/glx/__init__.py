@@ -160,6 +160,8 @@ def lib():
         L.glx_set_tuning.argtypes = [i, i]
         L.glx_dev_time_cg_kernel.restype = i
         L.glx_dev_time_cg_kernel.argtypes = [vp, d, i, i, C.POINTER(d), C.POINTER(d)]
+        L.glx_dev_staggered_grad.restype = i
+        L.glx_dev_staggered_grad.argtypes = [vp, d, vp, i, i, d, vp, vp, C.POINTER(d), C.POINTER(d), C.POINTER(d), IP]
         L.glx_set_normal_rows.restype = i
         L.glx_set_normal_rows.argtypes = [i, i, i, i]
         L.glx_get_normal_rows.restype = i
@@ -447,6 +449,32 @@ def dev_cg_fused(lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_
     _check(lib().glx_dev_cg_fused(lat.h, OP_KINDS[op], mass, x.ptr, b.ptr, max_iter, eps, C.byref(tx.v),
                                   C.byref(ci)), "glx_dev_cg_fused")
     return _info(ci, tx.text())
+
+
+@dataclass
+class Grad:
+    grad: np.ndarray      # dS/dtheta, 2V doubles in the links' layout
+    dS_dm: float
+    S: float              # Re phi^dag chi
+    S_robust: float       # 2 Re phi^dag chi - |D chi|^2
+    info: "Info"
+
+
+def staggered_grad(lat, mass: float, phi: DeviceArray, precision: int = 0, max_iter: int = 100000,
+                   eps: float = 1e-10, grad: DeviceArray | None = None, chi: DeviceArray | None = None,
+                   download: bool = True) -> Grad:
+    """Gradient of S = phi^dag (D^dag D)^-1 phi with respect to the link angles
+    and the mass (precision 0 FP64; LINKS_F32 / LINKS_F16 mixed-precision
+    solve).  Runs on the device; the gradient is downloaded unless
+    download=False (then Grad.grad is None and `grad` holds it)."""
+    V = lat.lx * lat.ly
+    g = grad if grad is not None else DeviceArray(nbytes=2 * V * 8, dtype=np.float64, shape=(2 * V,))
+    dm, S, Sr = C.c_double(0.0), C.c_double(0.0), C.c_double(0.0)
+    ci = InversionInfo()
+    _check(lib().glx_dev_staggered_grad(lat.h, mass, phi.ptr, precision, max_iter, eps, g.ptr,
+                                        None if chi is None else chi.ptr, C.byref(dm), C.byref(S), C.byref(Sr),
+                                        C.byref(ci)), "glx_dev_staggered_grad")
+    return Grad(g.download() if download else None, dm.value, S.value, Sr.value, _info(ci, ""))
 
 
 def set_tuning(group: int = 16, use_graph: bool = False):

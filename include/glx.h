@@ -251,6 +251,20 @@ int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, cons
  * differs from the reference's sequence of operations. */
 int glx_dev_cg_fused(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
                      double eps, glx_verbose* verbosity, glx_inversion_info* info);
+/* Gradient of S(theta, m) = phi^dag (D^dag D)^-1 phi for the U(1) staggered
+ * operator of the lattice (no reference counterpart; the operator is
+ * operators.cpp:444-453, links U = exp(i theta)): chi = (D^dag D)^-1 phi by
+ * CG to eps (precision 0: FP64 single-pass CG; 1 / 2: mixed-precision CG with
+ * FP32 / FP16 links), psi = D chi, then per link
+ *   dS/dtheta_mu(x) = -eta_mu(x) Im[psi(x)^* U_mu(x) chi(x+mu) + psi(x+mu)^* U_mu(x)^* chi(x)]
+ * into grad_dev[2*(y*lx + x) + mu] (2*lx*ly doubles, the links' layout) and
+ * dS/dm = -2 m |chi|^2.  S = Re phi^dag chi; S_robust = 2 Re phi^dag chi -
+ * |D chi|^2 differs from the exact S only at second order in the solve
+ * error.  phi_dev: lx*ly complex; chi_dev (may be NULL) receives chi.
+ * dS_dm, S, S_robust, info may be NULL. */
+int glx_dev_staggered_grad(glx_lattice lat, double mass, const double* phi_dev, int precision, int max_iter,
+                           double eps, double* grad_dev, double* chi_dev, double* dS_dm, double* S, double* S_robust,
+                           glx_inversion_info* info);
 /* Launch tuning of the single-pass CG and the mixed-precision CG: iterations
  * per enqueued group (default 16; rounded up to even where the iteration is
  * self-finalising) and whether a group is captured once as a hipGraph and
