@@ -710,6 +710,37 @@ int glx_dev_cg_m(glx_lattice lat, int kind, double mass, int nc, double** x_dev,
   });
 }
 
+int glx_dev_cg_m_batched(int n_sys, const glx_batch_system* sys, int n_shift, const double* shifts,
+                         int resid_freq_check, int max_iter, double eps, int worst_first, int use_graph,
+                         glx_inversion_info* infos) {
+  return guarded([&] {
+    if (n_sys < 0 || (n_sys > 0 && !sys) || !shifts) throw ArgError("sys and shifts must not be NULL");
+    std::vector<BatchSystem> bs(n_sys);
+    std::vector<std::vector<void*>> xp(n_sys);
+    for (int s = 0; s < n_sys; s++) {
+      if (!sys[s].lat || !sys[s].b_dev || !sys[s].x_dev) throw ArgError("batched system with a NULL member");
+      bs[s].op = lattice_op(sys[s].lat, OP_STAG_NORMAL, sys[s].mass, 1);
+      bs[s].b = sys[s].b_dev;
+      xp[s].assign(sys[s].x_dev, sys[s].x_dev + n_shift);
+      bs[s].xs = xp[s].data();
+    }
+    SolveOpts so;
+    so.max_iter = max_iter;
+    so.eps = eps;
+    so.group = g_group;
+    so.use_graph = use_graph != 0;
+    std::vector<SolveResult> r = dev_cg_m_batched(bs, n_shift, shifts, resid_freq_check, worst_first != 0, so);
+    if (infos) {
+      for (int s = 0; s < n_sys; s++) {
+        fill_info(&infos[s], r[s], "CG-M");
+        if (infos[s].resSqmrhs)
+          for (int k = 0; k < n_shift && (infos[s].n_rhs <= 0 || k < infos[s].n_rhs); k++)
+            infos[s].resSqmrhs[k] = r[s].resSqmrhs[k];
+      }
+    }
+  });
+}
+
 int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
                      double eps, double delta, int link_prec, glx_verbose* verb, glx_inversion_info* info) {
   return guarded([&] {

@@ -214,15 +214,20 @@ struct RedWs {
 // in block order -- deterministic for a given launch shape -- and hands the
 // totals to the solver's finaliser.  Kernel boundaries make the partials
 // visible across XCDs, which avoids an L2 write-back per block.
+// The explicit form stores block `idx` of a reduction over `stride` blocks
+// (a batched launch reduces each system over its own block range).
 template <int NR>
-__device__ __forceinline__ void block_partials(double (&v)[NR], RedWs ws, int bid) {
+__device__ __forceinline__ void block_partials_ex(double (&v)[NR], RedWs ws, int idx, int stride) {
   __shared__ double sm[NR > 0 ? NR : 1][kWaves];
   block_sum<NR>(v, sm);
   if (threadIdx.x == 0) {
-    const int nb = gridDim.x;
 #pragma unroll
-    for (int k = 0; k < NR; k++) ws.partials[k * nb + bid] = v[k];
+    for (int k = 0; k < NR; k++) ws.partials[k * stride + idx] = v[k];
   }
+}
+template <int NR>
+__device__ __forceinline__ void block_partials(double (&v)[NR], RedWs ws, int bid) {
+  block_partials_ex<NR>(v, ws, bid, int(gridDim.x));
 }
 
 // A kernel's gate: it runs only while the solver's control word equals
@@ -242,8 +247,14 @@ __device__ __forceinline__ bool gate_closed(Gate g) {
 // memory latency instead of kFinUnroll.
 constexpr int kFinUnroll = 8;
 template <int NR, class Fin>
+__device__ __forceinline__ void finalize_body(int nblk, const Fin& fin, RedWs ws);
+template <int NR, class Fin>
 __global__ __launch_bounds__(kBlock) void k_finalize(int nblk, Fin fin, RedWs ws, Gate gate) {
   if (gate_closed(gate)) return;
+  finalize_body<NR, Fin>(nblk, fin, ws);
+}
+template <int NR, class Fin>
+__device__ __forceinline__ void finalize_body(int nblk, const Fin& fin, RedWs ws) {
   __shared__ double sm[NR][kWaves];
   double acc[NR];
 #pragma unroll

@@ -133,9 +133,11 @@ struct has_selffin : std::false_type {};
 template <class X>
 struct has_selffin<X, std::void_t<decltype(X::kSelfFin)>> : std::integral_constant<bool, X::kSelfFin> {};
 
-template <int CPL, class T, class LK, class In, class Epi, class Fin>
-__global__ __launch_bounds__(kBlock) void k_normal(NGeom g, OpDev<LK> op, In in, T* __restrict__ out_arg, Epi epi,
-                                                   Fin fin, RedWs ws, Gate gate) {
+// One block's share of a launch: block `bid` of the g.nblk blocks over this
+// lattice (a batched launch passes each system's local block index).
+template <int CPL, class T, class LK, class In, class Epi>
+__device__ __forceinline__ void normal_body(const NGeom& g, const OpDev<LK>& op, In& in, T* __restrict__ out_arg,
+                                            const Epi& epi, RedWs ws, int bid) {
   using C = typename LK::C;
   constexpr int NXB = kNX * CPL;  // block columns
   constexpr int PW = NXB + 4;     // p columns x0-2 .. x0+NXB+1
@@ -143,7 +145,6 @@ __global__ __launch_bounds__(kBlock) void k_normal(NGeom g, OpDev<LK> op, In in,
   constexpr int QW = NXB + 2;     // q columns x0-1 .. x0+NXB
   constexpr int RM = kNRing - 1;
 
-  if (gate_closed(gate)) return;
   in.init();
   T* __restrict__ out = in.target(out_arg);
 
@@ -154,7 +155,6 @@ __global__ __launch_bounds__(kBlock) void k_normal(NGeom g, OpDev<LK> op, In in,
   __shared__ T sA[In::kAux ? kNRing : 1][In::kAux ? NXB : 1];
 
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int bid = blockIdx.x;
   const int tile = xcd_tile(bid, g.nblk);
   double part[Epi::NR > 0 ? Epi::NR : 1];
 #pragma unroll
@@ -282,7 +282,14 @@ __global__ __launch_bounds__(kBlock) void k_normal(NGeom g, OpDev<LK> op, In in,
       return;
     }
   }
-  if constexpr (Epi::NR > 0) block_partials<Epi::NR>(part, ws, bid);
+  if constexpr (Epi::NR > 0) block_partials_ex<Epi::NR>(part, ws, bid, g.nblk);
+}
+
+template <int CPL, class T, class LK, class In, class Epi, class Fin>
+__global__ __launch_bounds__(kBlock) void k_normal(NGeom g, OpDev<LK> op, In in, T* __restrict__ out_arg, Epi epi,
+                                                   Fin, RedWs ws, Gate gate) {
+  if (gate_closed(gate)) return;
+  normal_body<CPL, T, LK, In, Epi>(g, op, in, out_arg, epi, ws, blockIdx.x);
 }
 
 template <class T, class LK, class In, class Epi, class Fin>

@@ -222,16 +222,20 @@ void run_stencil(hipStream_t st, const DevOperator& o, int link_prec, In in, T* 
 // ------------------------------------------------------ pointwise kernel --
 // f(i, part) updates element i and adds its share to the F::NR partial sums
 // (part has exactly that many entries).
-template <int NR, class F, class Fin>
-__global__ __launch_bounds__(kBlock) void k_point(int n, F f, Fin fin, RedWs ws, Gate gate) {
-  if (gate_closed(gate)) return;
+template <int NR, class F>
+__device__ __forceinline__ void point_body(int n, F& f, RedWs ws, int lb, int nblk) {
   f.init();
   double part[NR > 0 ? NR : 1];
 #pragma unroll
   for (int k = 0; k < NR; k++) part[k] = 0.0;
-  const int stride = gridDim.x * kBlock;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) f(i, part);
-  if constexpr (NR > 0) block_partials<NR>(part, ws, blockIdx.x);
+  const int stride = nblk * kBlock;
+  for (int i = lb * kBlock + threadIdx.x; i < n; i += stride) f(i, part);
+  if constexpr (NR > 0) block_partials_ex<NR>(part, ws, lb, nblk);
+}
+template <int NR, class F, class Fin>
+__global__ __launch_bounds__(kBlock) void k_point(int n, F f, Fin fin, RedWs ws, Gate gate) {
+  if (gate_closed(gate)) return;
+  point_body<NR, F>(n, f, ws, blockIdx.x, gridDim.x);
 }
 
 inline int point_blocks(int n) {

@@ -53,6 +53,21 @@ SolveResult dev_gcr(const DevOperator& op, void* x, const void* b, const SolveOp
 SolveResult dev_cg_m(const DevOperator& op, void* const* xs, const void* b, int n_shift, const double* shifts,
                      int resid_freq_check, bool worst_first, const SolveOpts& o);
 
+// Many independent multishift CG solves advanced together (batched CG-M; no
+// reference counterpart as a batch, each system follows generic_cg_m.cpp):
+// op = a staggered normal operator (its own lattice, links and mass), b its
+// right-hand side, xs its n_shift solution vectors (device).  All systems use
+// the same shifts.  Each phase of an iteration is one launch over all
+// unconverged systems; groups of iterations replay one captured hipGraph
+// when o.use_graph.
+struct BatchSystem {
+  DevOperator op;
+  const void* b;
+  void* const* xs;
+};
+std::vector<SolveResult> dev_cg_m_batched(const std::vector<BatchSystem>& sys, int n_shift, const double* shifts,
+                                          int resid_freq_check, bool worst_first, const SolveOpts& o);
+
 // Mixed-precision CG with reliable updates (no reference counterpart):
 // FP32 Krylov vectors and FP32 or FP16 links, FP64 true residual and
 // solution accumulation whenever the iterated residual drops by `delta`.

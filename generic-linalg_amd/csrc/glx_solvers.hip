@@ -1859,6 +1859,377 @@ SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const So
   return fastcg_impl<c128, c128, false>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
 }
 
+// ================================================== batched multishift ===
+// Many independent CG-M solves (generic_cg_m.cpp:312-600 semantics per
+// system: own lattice size, links, mass, right-hand side, the same shifts)
+// advanced together.  Every phase of a CG-M iteration is ONE launch over all
+// systems: the grid is the concatenation of the systems' block ranges (each
+// starting at a multiple of 8, so the XCD-aware tile order of a system is
+// kept), a block finds its system in a table, and a system whose solve has
+// converged (its own control word) returns at once -- a per-system
+// convergence mask.  The finalisers are one block per system.  A group of
+// iterations is captured once as a hipGraph and replayed; the host reads all
+// control words once per group.  Requires D^dag D lattices the row-marching
+// kernel takes (lx % 64 == 0, ly % 4 == 0, ly >= 8).
+
+struct BatchMap {
+  const int* sys_of_blk;  // system of each block of the launch
+  const int* off;         // first block of each system
+};
+
+template <class F>
+struct PtItem {
+  F f;
+  int n, nblk;
+  RedWs ws;
+  Gate gate;
+};
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_point_batch(const PtItem<F>* items, BatchMap bm) {
+  const int s = bm.sys_of_blk[blockIdx.x];
+  const int lb = blockIdx.x - bm.off[s];
+  const PtItem<F> it = items[s];
+  if (lb >= it.nblk || gate_closed(it.gate)) return;
+  F f = it.f;
+  point_body<F::NR, F>(it.n, f, it.ws, lb, it.nblk);
+}
+
+template <class Fin>
+struct FinItem {
+  Fin fin;
+  int nblk;
+  RedWs ws;
+  Gate gate;
+};
+template <int NR, class Fin>
+__global__ __launch_bounds__(kBlock) void k_finalize_batch(const FinItem<Fin>* items) {
+  const FinItem<Fin> it = items[blockIdx.x];
+  if (gate_closed(it.gate)) return;
+  finalize_body<NR, Fin>(it.nblk, it.fin, it.ws);
+}
+
+template <class T, class In, class Epi>
+struct NrmItem {
+  NGeom g;
+  OpDev<LinksF64> op;
+  In in;
+  T* out;
+  Epi epi;
+  RedWs ws;
+  Gate gate;
+};
+template <class T, class In, class Epi>
+__global__ __launch_bounds__(kBlock) void k_normal_batch(const NrmItem<T, In, Epi>* items, BatchMap bm) {
+  static_assert(!has_selffin<In>::value, "a batched launch has no self-finalising form");
+  const int s = bm.sys_of_blk[blockIdx.x];
+  const int lb = blockIdx.x - bm.off[s];
+  const NGeom g = items[s].g;
+  if (lb >= g.nblk || gate_closed(items[s].gate)) return;
+  const OpDev<LinksF64> op = items[s].op;
+  In in = items[s].in;
+  const Epi epi = items[s].epi;
+  normal_body<normal_cpl<T>(), T, LinksF64, In, Epi>(g, op, in, items[s].out, epi, items[s].ws, lb);
+}
+
+// r = p = p_s = b, x_s = 0 for every shift; |b|^2
+template <class T>
+struct PtCgmStart {
+  static constexpr int NR = 1;
+  const T* b;
+  T* r;
+  T* p;
+  T* const* ptab;
+  T* const* xtab;
+  int n_shift;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const T bi = b[i];
+    r[i] = bi;
+    p[i] = bi;
+    for (int k = 0; k < n_shift; k++) {
+      ptab[k][i] = bi;
+      xtab[k][i] = zero_of<T>();
+    }
+    part[0] += norm2(bi);
+  }
+};
+
+// Device copies of one launch kind's items and its block table.
+struct BatchLaunch {
+  DevBuf items, sys_of_blk, off;
+  int nblk_total = 0;
+  BatchMap map() const { return BatchMap{sys_of_blk.as<int>(), off.as<int>()}; }
+};
+template <class Item>
+static BatchLaunch make_batch_launch(const std::vector<Item>& items, const std::vector<int>& nblk) {
+  BatchLaunch b;
+  std::vector<int> sob, off(items.size());
+  for (size_t s = 0; s < items.size(); s++) {
+    off[s] = int(sob.size());
+    const int nb = ((nblk[s] + 7) / 8) * 8;
+    for (int k = 0; k < nb; k++) sob.push_back(int(s));
+  }
+  b.nblk_total = int(sob.size());
+  b.items = DevBuf(items.size() * sizeof(Item));
+  b.sys_of_blk = DevBuf(sob.size() * sizeof(int));
+  b.off = DevBuf(off.size() * sizeof(int));
+  GLX_HIP(hipMemcpy(b.items.get(), items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice));
+  GLX_HIP(hipMemcpy(b.sys_of_blk.get(), sob.data(), sob.size() * sizeof(int), hipMemcpyHostToDevice));
+  GLX_HIP(hipMemcpy(b.off.get(), off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+  return b;
+}
+
+template <class T>
+std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, int n_shift, const double* shifts,
+                                         int resid_freq, bool worst_first, const SolveOpts& opt) {
+  static_assert(std::is_same<T, c128>::value, "the batched multishift solver is complex FP64");
+  if (sys.empty()) return {};
+  if (n_shift < 1 || n_shift > kMaxShift) throw ArgError("n_shift must be in [1, 64]");
+  if (resid_freq < 1) throw ArgError("resid_freq_check must be >= 1");
+  const int ns = int(sys.size());
+  std::vector<int> nsite(ns), rows(ns);
+  for (int s = 0; s < ns; s++) {
+    const DevOperator& o = sys[s].op;
+    if (o.kind != OP_STAG_NORMAL || !o.u64) throw ArgError("batched systems must be staggered normal operators");
+    rows[s] = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
+    if (rows[s] <= 0 || !normal_fits(o.lx, o.ly, rows[s], normal_cpl<T>())) rows[s] = 16;
+    if (!normal_fits(o.lx, o.ly, rows[s], normal_cpl<T>()))
+      throw ArgError("batched system " + std::to_string(s) + ": lattice " + std::to_string(o.lx) + "x" +
+                     std::to_string(o.ly) + " needs lx % 64 == 0, ly % 4 == 0, ly >= 8");
+    if (!sys[s].b || !sys[s].xs) throw ArgError("batched system " + std::to_string(s) + ": b and x are required");
+    nsite[s] = o.size();
+  }
+  OwnedStream own;
+  hipStream_t st = opt.stream ? opt.stream : own.st;
+  // one allocation for every system's r, p, Ap and p_s
+  size_t total = 0;
+  std::vector<size_t> vbase(ns);
+  for (int s = 0; s < ns; s++) {
+    vbase[s] = total;
+    total += size_t(3 + n_shift) * nsite[s];
+  }
+  DevBuf vecs(total * sizeof(T));
+  DevBuf states(size_t(ns) * sizeof(SolveState));
+  DevBuf resv(size_t(ns) * n_shift * sizeof(double));
+  DevBuf tabs(size_t(ns) * 2 * n_shift * sizeof(T*));
+  SolveState* ds = states.as<SolveState>();
+  T* V = vecs.as<T>();
+  T** tab = tabs.as<T*>();
+  std::vector<T*> htab(size_t(ns) * 2 * n_shift);
+  std::vector<NGeom> geo(ns);
+  std::vector<int> nb_pt(ns), nb_st(ns);
+  int max_nb = 0;
+  for (int s = 0; s < ns; s++) {
+    const DevOperator& o = sys[s].op;
+    geo[s] = make_ngeom(o.lx, o.ly, rows[s], normal_cpl<T>());
+    nb_pt[s] = point_blocks(nsite[s]);
+    nb_st[s] = geo[s].nblk;
+    max_nb = std::max(max_nb, std::max(nb_pt[s], nb_st[s]));
+    for (int k = 0; k < n_shift; k++) {
+      htab[(size_t(s) * 2 + 0) * n_shift + k] = static_cast<T*>(sys[s].xs[k]);
+      htab[(size_t(s) * 2 + 1) * n_shift + k] = V + vbase[s] + size_t(3 + k) * nsite[s];
+    }
+  }
+  // partial sums: each system owns 2 * max_nb doubles (NR <= 2 everywhere)
+  DevBuf partB(size_t(ns) * 2 * max_nb * sizeof(double));
+  GLX_HIP(hipMemcpy(tab, htab.data(), htab.size() * sizeof(T*), hipMemcpyHostToDevice));
+  std::vector<SolveState> hs(ns);
+  for (int s = 0; s < ns; s++) {
+    SolveState& h = hs[s];
+    std::memset(&h, 0, sizeof(h));
+    h.max_iter = opt.max_iter;
+    h.eps = opt.eps;
+    h.hist = nullptr;
+    h.n_rem = n_shift;
+    h.n_shift = n_shift;
+    h.resid_freq = resid_freq;
+    h.worst_first = worst_first ? 1 : 0;
+    h.beta = {1.0, 0.0};
+    h.alpha = {0.0, 0.0};
+    for (int k = 0; k < n_shift; k++) {
+      h.zeta[k] = h.zeta_p[k] = h.beta_s[k] = {1.0, 0.0};
+      h.alpha_s[k] = {0.0, 0.0};
+      h.shifts[k] = shifts[k];
+      h.mapping[k] = k;
+      h.vec[k] = k;
+    }
+    h.resvec = resv.as<double>() + size_t(s) * n_shift;
+  }
+  GLX_HIP(hipMemcpy(ds, hs.data(), size_t(ns) * sizeof(SolveState), hipMemcpyHostToDevice));
+
+  auto ws_of = [&](int s) { return RedWs{partB.as<double>() + size_t(s) * 2 * max_nb}; };
+  auto gate_of = [&](int s) { return Gate{&ds[s].done, 0}; };
+  auto rvec = [&](int s) { return V + vbase[s]; };
+  auto pvec = [&](int s) { return V + vbase[s] + nsite[s]; };
+  auto apvec = [&](int s) { return V + vbase[s] + 2 * size_t(nsite[s]); };
+  T* const* xtab_of_base = tab;
+  auto xtab = [&](int s) { return xtab_of_base + (size_t(s) * 2 + 0) * n_shift; };
+  auto ptab = [&](int s) { return xtab_of_base + (size_t(s) * 2 + 1) * n_shift; };
+
+  std::vector<PtItem<PtCgmStart<T>>> i_start(ns);
+  std::vector<FinItem<FinCgmInit>> f_start(ns);
+  std::vector<NrmItem<T, InPlain<T>, EpiDotInOut<T>>> i_st(ns);
+  std::vector<FinItem<FinCgmBeta<T>>> f_st(ns);
+  std::vector<PtItem<PtCgmX<T>>> i_x(ns);
+  std::vector<FinItem<FinCgmR<T>>> f_x(ns);
+  std::vector<PtItem<PtCgmP<T>>> i_p(ns);
+  for (int s = 0; s < ns; s++) {
+    const Gate none = kNoGate;
+    i_start[s] = {PtCgmStart<T>{static_cast<const T*>(sys[s].b), rvec(s), pvec(s), ptab(s), xtab(s), n_shift},
+                  nsite[s], nb_pt[s], ws_of(s), none};
+    f_start[s] = {FinCgmInit{ds + s}, nb_pt[s], ws_of(s), none};
+    i_st[s] = {geo[s], make_opdev(sys[s].op, LinksF64{sys[s].op.u64}), InPlain<T>{pvec(s)}, apvec(s),
+               EpiDotInOut<T>{}, ws_of(s), gate_of(s)};
+    f_st[s] = {FinCgmBeta<T>{ds + s}, nb_st[s], ws_of(s), gate_of(s)};
+    i_x[s] = {PtCgmX<T>{xtab(s), ptab(s), rvec(s), apvec(s), ds + s}, nsite[s], nb_pt[s], ws_of(s), gate_of(s)};
+    f_x[s] = {FinCgmR<T>{ds + s}, nb_pt[s], ws_of(s), gate_of(s)};
+    i_p[s] = {PtCgmP<T>{ptab(s), pvec(s), rvec(s), ds + s}, nsite[s], nb_pt[s], ws_of(s), gate_of(s)};
+  }
+  BatchLaunch L_start = make_batch_launch(i_start, nb_pt);
+  BatchLaunch F_start = make_batch_launch(f_start, std::vector<int>(ns, 1));
+  BatchLaunch L_st = make_batch_launch(i_st, nb_st);
+  BatchLaunch F_st = make_batch_launch(f_st, std::vector<int>(ns, 1));
+  BatchLaunch L_x = make_batch_launch(i_x, nb_pt);
+  BatchLaunch F_x = make_batch_launch(f_x, std::vector<int>(ns, 1));
+  BatchLaunch L_p = make_batch_launch(i_p, nb_pt);
+
+  auto launch_stencil_phase = [&]() {
+    hipLaunchKernelGGL((k_normal_batch<T, InPlain<T>, EpiDotInOut<T>>), dim3(L_st.nblk_total), dim3(kBlock), 0, st,
+                       L_st.items.as<NrmItem<T, InPlain<T>, EpiDotInOut<T>>>(), L_st.map());
+    GLX_CHECK_LAUNCH();
+    hipLaunchKernelGGL((k_finalize_batch<EpiDotInOut<T>::NR, FinCgmBeta<T>>), dim3(ns), dim3(kBlock), 0, st,
+                       F_st.items.as<FinItem<FinCgmBeta<T>>>());
+    GLX_CHECK_LAUNCH();
+  };
+  auto enq_iter = [&]() {
+    hipLaunchKernelGGL((k_point_batch<PtCgmX<T>>), dim3(L_x.nblk_total), dim3(kBlock), 0, st,
+                       L_x.items.as<PtItem<PtCgmX<T>>>(), L_x.map());
+    GLX_CHECK_LAUNCH();
+    hipLaunchKernelGGL((k_finalize_batch<1, FinCgmR<T>>), dim3(ns), dim3(kBlock), 0, st,
+                       F_x.items.as<FinItem<FinCgmR<T>>>());
+    GLX_CHECK_LAUNCH();
+    hipLaunchKernelGGL((k_point_batch<PtCgmP<T>>), dim3(L_p.nblk_total), dim3(kBlock), 0, st,
+                       L_p.items.as<PtItem<PtCgmP<T>>>(), L_p.map());
+    GLX_CHECK_LAUNCH();
+    launch_stencil_phase();
+  };
+  // start: r = p = p_s = b, x_s = 0, |b|; then Ap, beta, zeta (the first
+  // stencil of generic_cg_m.cpp:100-122 precedes the loop)
+  hipLaunchKernelGGL((k_point_batch<PtCgmStart<T>>), dim3(L_start.nblk_total), dim3(kBlock), 0, st,
+                     L_start.items.as<PtItem<PtCgmStart<T>>>(), L_start.map());
+  GLX_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_finalize_batch<1, FinCgmInit>), dim3(ns), dim3(kBlock), 0, st,
+                     F_start.items.as<FinItem<FinCgmInit>>());
+  GLX_CHECK_LAUNCH();
+  launch_stencil_phase();
+
+  const int group = std::max(1, opt.group);
+  hipGraphExec_t exec = nullptr;
+  if (opt.use_graph) {
+    hipGraph_t graph = nullptr;
+    GLX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int g = 0; g < group; g++) enq_iter();
+    GLX_HIP(hipStreamEndCapture(st, &graph));
+    GLX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    GLX_HIP(hipGraphDestroy(graph));
+  }
+  // control words of all systems: the first 16 bytes of each state, strided
+  Mirror* pin = nullptr;
+  GLX_HIP(hipHostMalloc(reinterpret_cast<void**>(&pin), 2 * size_t(ns) * sizeof(Mirror), hipHostMallocDefault));
+  hipEvent_t ev[2];
+  GLX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+  GLX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+  auto cleanup = [&]() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipStreamSynchronize(st);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    (void)hipHostFree(pin);
+  };
+  auto submit = [&](int slot) {
+    if (exec) GLX_HIP(hipGraphLaunch(exec, st));
+    else
+      for (int g = 0; g < group; g++) enq_iter();
+    GLX_HIP(hipMemcpy2DAsync(pin + size_t(slot) * ns, sizeof(Mirror), ds, sizeof(SolveState), sizeof(Mirror), ns,
+                             hipMemcpyDeviceToHost, st));
+    GLX_HIP(hipEventRecord(ev[slot], st));
+  };
+  try {
+    if (opt.max_iter > 0) {
+      const long max_groups = 4L * (opt.max_iter / group + 2) + 16;
+      long groups = 0;
+      int cur = 0;
+      submit(cur);
+      groups++;
+      for (;;) {
+        submit(cur ^ 1);
+        groups++;
+        GLX_HIP(hipEventSynchronize(ev[cur]));
+        bool all = true;
+        for (int s = 0; s < ns && all; s++) all = pin[size_t(cur) * ns + s].done == 1;
+        if (all) break;
+        if (groups > max_groups) throw HipError("batched multishift CG made no progress");
+        cur ^= 1;
+      }
+    }
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+
+  // per system and shift: |(A + sigma_k) x_k - b|^2 (generic_cg_m.cpp:271-282)
+  {
+    std::vector<NrmItem<T, InPlain<T>, EpiNone<T>>> i_ax(ns);
+    std::vector<PtItem<PtShiftRes<T>>> i_res(ns);
+    std::vector<FinItem<FinResSq>> f_res(ns);
+    for (int k = 0; k < n_shift; k++) {
+      for (int s = 0; s < ns; s++) {
+        T* xk = static_cast<T*>(sys[s].xs[k]);
+        i_ax[s] = {geo[s], make_opdev(sys[s].op, LinksF64{sys[s].op.u64}), InPlain<T>{xk}, apvec(s), EpiNone<T>{},
+                   ws_of(s), kNoGate};
+        i_res[s] = {PtShiftRes<T>{apvec(s), xk, static_cast<const T*>(sys[s].b), shifts[k]}, nsite[s], nb_pt[s],
+                    ws_of(s), kNoGate};
+        f_res[s] = {FinResSq{ds + s, k}, nb_pt[s], ws_of(s), kNoGate};
+      }
+      BatchLaunch La = make_batch_launch(i_ax, nb_st);
+      BatchLaunch Lr = make_batch_launch(i_res, nb_pt);
+      BatchLaunch Fr = make_batch_launch(f_res, std::vector<int>(ns, 1));
+      hipLaunchKernelGGL((k_normal_batch<T, InPlain<T>, EpiNone<T>>), dim3(La.nblk_total), dim3(kBlock), 0, st,
+                         La.items.as<NrmItem<T, InPlain<T>, EpiNone<T>>>(), La.map());
+      GLX_CHECK_LAUNCH();
+      hipLaunchKernelGGL((k_point_batch<PtShiftRes<T>>), dim3(Lr.nblk_total), dim3(kBlock), 0, st,
+                         Lr.items.as<PtItem<PtShiftRes<T>>>(), Lr.map());
+      GLX_CHECK_LAUNCH();
+      hipLaunchKernelGGL((k_finalize_batch<1, FinResSq>), dim3(ns), dim3(kBlock), 0, st,
+                         Fr.items.as<FinItem<FinResSq>>());
+      GLX_CHECK_LAUNCH();
+      GLX_HIP(hipStreamSynchronize(st));  // the item buffers of this shift are released next
+    }
+  }
+  GLX_HIP(hipMemcpy(hs.data(), ds, size_t(ns) * sizeof(SolveState), hipMemcpyDeviceToHost));
+  std::vector<double> rv(size_t(ns) * n_shift);
+  GLX_HIP(hipMemcpy(rv.data(), resv.get(), rv.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<SolveResult> out(ns);
+  for (int s = 0; s < ns; s++) {
+    const int kf = opt.max_iter > 0 ? hs[s].kfinal : 0;
+    SolveResult& r = out[s];
+    r.name = "CG-M";
+    r.success = opt.max_iter > 0 ? (kf != opt.max_iter - 1) : true;
+    r.iter = kf + 1;
+    r.ops_count = 1 + (opt.max_iter > 0 ? kf : 0) + n_shift;
+    r.resSq = 0.0;
+    r.bsqrt = hs[s].bsqrt;
+    r.resSqmrhs.assign(rv.begin() + size_t(s) * n_shift, rv.begin() + size_t(s + 1) * n_shift);
+  }
+  return out;
+}
+
+std::vector<SolveResult> dev_cg_m_batched(const std::vector<BatchSystem>& sys, int n_shift, const double* shifts,
+                                          int resid_freq_check, bool worst_first, const SolveOpts& o) {
+  return cg_m_batch_impl<c128>(sys, n_shift, shifts, resid_freq_check, worst_first, o);
+}
+
 // ----------------------------------------------------------- gradient --
 // dS/dtheta_mu(x) = -eta_mu(x) Im[psi(x)^* U_mu(x) chi(x+mu) + psi(x+mu)^* U_mu(x)^* chi(x)]
 // for S = phi^dag (D^dag D)^-1 phi, chi = (D^dag D)^-1 phi, psi = D chi

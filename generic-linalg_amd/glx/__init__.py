@@ -162,6 +162,8 @@ def lib():
         L.glx_dev_time_cg_kernel.argtypes = [vp, d, i, i, C.POINTER(d), C.POINTER(d)]
         L.glx_dev_staggered_grad.restype = i
         L.glx_dev_staggered_grad.argtypes = [vp, d, vp, i, i, d, vp, vp, C.POINTER(d), C.POINTER(d), C.POINTER(d), IP]
+        L.glx_dev_cg_m_batched.restype = i
+        L.glx_dev_cg_m_batched.argtypes = [i, vp, i, vp, i, i, d, i, i, vp]
         L.glx_set_normal_rows.restype = i
         L.glx_set_normal_rows.argtypes = [i, i, i, i]
         L.glx_get_normal_rows.restype = i
@@ -537,6 +539,37 @@ def dev_cg_m(lat, op: str, mass: float, xs, b: DeviceArray, shifts, resid_freq_c
     _check(lib().glx_dev_cg_m(None if lat is None else lat.h, OP_KINDS[op], mass, nc, ptrs, b.ptr, n, sh.ctypes.data,
                               resid_freq_check, max_iter, eps, int(worst_first), None, C.byref(ci)), "glx_dev_cg_m")
     return _info(ci, "", res)
+
+
+class BatchSystem(C.Structure):
+    _fields_ = [("lat", C.c_void_p), ("mass", C.c_double), ("b_dev", C.c_void_p), ("x_dev", C.c_void_p)]
+
+
+def dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=10000, eps=1e-10, worst_first=False,
+                     use_graph=True) -> list:
+    """Batched multishift CG on the staggered normal operator: systems is a
+    list of (lattice, mass, b: DeviceArray, xs: list of DeviceArray), all with
+    the same shifts.  Returns one Info per system (resSqmrhs per shift)."""
+    sh = np.ascontiguousarray(shifts, dtype=np.float64)
+    n = sh.size
+    ns = len(systems)
+    arr = (BatchSystem * max(ns, 1))()
+    keep = []
+    for s, (lat, mass, b, xs) in enumerate(systems):
+        if len(xs) != n:
+            raise GlxError("each system needs one solution vector per shift")
+        ptrs = (C.c_void_p * n)(*[x.ptr for x in xs])
+        keep.append(ptrs)
+        arr[s] = BatchSystem(lat.h, mass, b.ptr, C.cast(ptrs, C.c_void_p))
+    infos = (InversionInfo * max(ns, 1))()
+    res = [np.zeros(n) for _ in range(ns)]
+    for s in range(ns):
+        infos[s].resSqmrhs = res[s].ctypes.data
+        infos[s].n_rhs = n
+    _check(lib().glx_dev_cg_m_batched(ns, C.cast(arr, C.c_void_p), n, sh.ctypes.data, resid_freq_check, max_iter,
+                                      eps, int(worst_first), int(use_graph), C.cast(infos, C.c_void_p)),
+           "glx_dev_cg_m_batched")
+    return [_info(infos[s], "", res[s]) for s in range(ns)]
 
 
 def dev_apply(lat, op: str, mass: float, out: DeviceArray, inp: DeviceArray, fp32=False, link_prec=LINKS_F32, nc=1):

@@ -237,6 +237,25 @@ int glx_dev_solve(int solver, glx_lattice lat, int kind, double mass, int nc, do
 int glx_dev_cg_m(glx_lattice lat, int kind, double mass, int nc, double** x_dev, const double* b_dev, int n_shift,
                  const double* shifts, int resid_freq_check, int max_iter, double eps, int worst_first,
                  glx_verbose* verbosity, glx_inversion_info* info);
+/* Batched multishift CG (no reference counterpart as a batch; each system
+ * follows minv_vector_cg_m, generic_cg_m.cpp:312-600, on D^dag D of its own
+ * lattice and mass): n_sys independent solves advanced together, every phase
+ * of an iteration one launch over all unconverged systems, groups of
+ * iterations replayed from one captured hipGraph when use_graph.  All systems
+ * share n_shift and shifts.  x_dev of system s: n_shift device vectors
+ * (zeroed, then the solutions, in the order of shifts).  infos: n_sys
+ * records (may be NULL); a non-NULL resSqmrhs receives that system's
+ * per-shift squared true residuals.  Lattices need lx % 64 == 0,
+ * ly % 4 == 0, ly >= 8. */
+typedef struct {
+  glx_lattice lat;
+  double mass;
+  const double* b_dev;
+  double** x_dev;
+} glx_batch_system;
+int glx_dev_cg_m_batched(int n_sys, const glx_batch_system* sys, int n_shift, const double* shifts,
+                         int resid_freq_check, int max_iter, double eps, int worst_first, int use_graph,
+                         glx_inversion_info* infos);
 /* Mixed-precision CG (no reference counterpart): FP32 Krylov vectors with
  * FP32 (link_prec = 1) or FP16 + scale (link_prec = 2) links; FP64 true
  * residual r = b - A x and FP64 solution accumulation at every reliable
