@@ -129,6 +129,49 @@ def gradient_timing(glx, L=512, mass=0.05, reps=3):
     return out
 
 
+def batch_multishift(glx, n_lat=48, rhs=4, seq=True):
+    """48 staggered lattices, L uniform on {64, ..., 512}, m ~ U[0.01, 0.1],
+    theta ~ N(0, 0.4^2), 4 complex Gaussian right-hand sides each, 12 shifts
+    sigma_k = 10^(-3 + 0.5 k), tol 1e-10, resid_freq_check 10: the 192
+    multishift solves in one batched call (launch shapes autotuned per L),
+    and the same solves one call each for comparison."""
+    rng = np.random.default_rng(4)
+    Ls = rng.choice([64, 128, 192, 256, 384, 512], size=n_lat)
+    masses = rng.uniform(0.01, 0.1, size=n_lat)
+    shifts = 10.0 ** (-3.0 + 0.5 * np.arange(12))
+    tuned, systems = {}, []
+    for i, L in enumerate(Ls):
+        L = int(L)
+        lat = glx.Lattice(np.exp(1j * rng.normal(0.0, 0.4, 2 * L * L)), L, L, f32=False, f16=False)
+        if L not in tuned:
+            tuned[L] = glx.autotune_normal(lat, reps=10)["rows64"]
+        for _ in range(rhs):
+            b = rng.normal(size=L * L) + 1j * rng.normal(size=L * L)
+            xs = [glx.DeviceArray(nbytes=L * L * 16, shape=(L * L,)) for _ in shifts]
+            systems.append((lat, float(masses[i]), glx.DeviceArray(b), xs))
+    glx.dev_cg_m_batched(systems[:8], shifts, max_iter=20)  # warm-up (code paths, allocations)
+    glx.synchronize()
+    t0 = time.perf_counter()
+    infos = glx.dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=100000, eps=1e-10)
+    glx.synchronize()
+    t_batch = time.perf_counter() - t0
+    out = {"systems": len(systems), "lattices": n_lat, "rhs": rhs, "shifts": 12,
+           "sites_total": int(sum(int(L) * int(L) for L in Ls) * rhs),
+           "batch_s": round(t_batch, 3), "all_success": all(i.success for i in infos),
+           "iters_max": max(i.iter for i in infos), "iters_sum": sum(i.iter for i in infos),
+           "rows_by_L": {str(k): v for k, v in sorted(tuned.items())}}
+    if seq:
+        t0 = time.perf_counter()
+        for lat, m, db, xs in systems:
+            glx.dev_cg_m(lat, "staggered_normal", m, xs, db, shifts, resid_freq_check=10, max_iter=100000,
+                         eps=1e-10)
+        glx.synchronize()
+        out["one_call_each_s"] = round(time.perf_counter() - t0, 3)
+    del systems
+    glx.release_cache()
+    return out
+
+
 def traffic_from_profile(links: str, alg_bytes: float):
     """HBM bytes per launch of the dominant kernel as measured by rocprofv3
     FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of known size), read from
@@ -156,6 +199,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true", help="skip the 4096^2 stencil measurement")
     ap.add_argument("--no-grad", action="store_true", help="skip the 512^2 gradient timing")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multishift workload")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
                          "for scaling rocprofv3 byte counters)")
@@ -268,6 +312,7 @@ def main():
         big = stencil_beyond_mall(glx)
 
     grad = None if args.no_grad else gradient_timing(glx)
+    batch = None if args.no_batch else batch_multishift(glx)
 
     cpu = None
     if not args.no_cpu:
@@ -330,6 +375,7 @@ def main():
             "ms_per_launch": it_ms,
         },
         "gradient": grad,
+        "batch_multishift": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
