@@ -44,11 +44,14 @@ def make_inputs(L: int, sigma: float, seed: int):
     return links.astype(np.complex128), b.astype(np.complex128)
 
 
-def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0):
+def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0, grad=None, batch=None):
     """Reference CPU CG (oracle/_ref, built from the reference sources; the
     oracle restatement where that build is absent), one thread, on a bounded
     sample: a fixed number of iterations of the same 1024^2 solve, scaled to
-    the iteration count the FP64 solve needs."""
+    the iteration count the FP64 solve needs.  Also, one thread each: the
+    512^2 gradient (reference CG solve + D chi, the oracle's force; the force
+    has no reference counterpart) and one multishift solve of the batched
+    workload, projected to the whole batch by site-iterations."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc  # noqa: E402  (CPU baseline leg only)
 
@@ -67,7 +70,7 @@ def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0):
     dt = time.perf_counter() - t0
     per_it = dt / (info.iter + 3)
     projected_ms = per_it * (gpu_fp64_iters + 3) * 1e3
-    return {
+    out = {
         "value": round(projected_ms, 1),
         "unit": "ms (FP64 CG time to 1e-10, projected)",
         "cores": 1,
@@ -75,7 +78,48 @@ def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0):
         "sample": f"{info.iter} CG iterations of the same {L}x{L} solve on one host core "
                   f"({dt:.2f} s, {per_it * 1e3:.1f} ms/iteration), scaled to the {gpu_fp64_iters} iterations "
                   f"the FP64 solve takes",
+        "cpu_model": cpu_model(),
     }
+    if grad is not None:
+        Lg, mg = grad["L"], grad["mass"]
+        rng = np.random.default_rng(21)
+        theta = rng.normal(0.0, 0.4, 2 * Lg * Lg)
+        phi = rng.normal(size=Lg * Lg) + 1j * rng.normal(size=Lg * Lg)
+        lk = np.exp(1j * theta)
+        t0 = time.perf_counter()
+        chi, gi = orc.solve("cg", "staggered_normal", phi, None, links=lk, mass=mg, lx=Lg, ly=Lg, max_iter=100000,
+                            eps=1e-10, impl=impl)
+        psi = orc.apply("staggered", chi, lk, mg, Lg, Lg, impl=impl)
+        orc.force(chi, psi, lk, mg, Lg, Lg)
+        out["gradient_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        out["gradient_sample"] = (f"full {Lg}x{Lg} gradient on one core: {impl} CG ({gi.iter} iterations) + D chi, "
+                                  f"the oracle's force")
+    if batch is not None:
+        Lb, mb = 128, 0.05
+        rng = np.random.default_rng(8)
+        lk = np.exp(1j * rng.normal(0.0, 0.4, 2 * Lb * Lb))
+        bb = rng.normal(size=Lb * Lb) + 1j * rng.normal(size=Lb * Lb)
+        shifts = 10.0 ** (-3.0 + 0.5 * np.arange(12))
+        t0 = time.perf_counter()
+        _, ci, _, _ = orc.cg_m("staggered_normal", bb, shifts, links=lk, mass=mb, lx=Lb, ly=Lb,
+                               resid_freq_check=10, max_iter=100000, eps=1e-10, impl=impl)
+        dtb = time.perf_counter() - t0
+        per_site_it = dtb / (Lb * Lb * ci.iter)
+        out["batch_s"] = round(per_site_it * batch["site_iters"], 1)
+        out["batch_sample"] = (f"one {Lb}x{Lb} 12-shift CG-M on one core ({impl}, {ci.iter} iterations, {dtb:.2f} s), "
+                               f"projected to the batch's {batch['site_iters']:.3g} site-iterations")
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip() + f" ({os.cpu_count()} logical CPUs visible)"
+    except OSError:
+        pass
+    return None
 
 
 def stencil_beyond_mall(glx, L=4096):
@@ -155,10 +199,12 @@ def batch_multishift(glx, n_lat=48, rhs=4, seq=True):
     infos = glx.dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=100000, eps=1e-10)
     glx.synchronize()
     t_batch = time.perf_counter() - t0
+    site_iters = float(sum(lat.lx * lat.ly * inf.iter for (lat, _, _, _), inf in zip(systems, infos)))
     out = {"systems": len(systems), "lattices": n_lat, "rhs": rhs, "shifts": 12,
            "sites_total": int(sum(int(L) * int(L) for L in Ls) * rhs),
            "batch_s": round(t_batch, 3), "all_success": all(i.success for i in infos),
            "iters_max": max(i.iter for i in infos), "iters_sum": sum(i.iter for i in infos),
+           "site_iters": site_iters,
            "rows_by_L": {str(k): v for k, v in sorted(tuned.items())}}
     if seq:
         t0 = time.perf_counter()
@@ -317,7 +363,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(links, b, L, mass, info64.iter)
+            cpu = cpu_baseline(links, b, L, mass, info64.iter, grad=grad, batch=batch)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "unit": "ms", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
 

@@ -99,7 +99,7 @@ struct SolveState {
   int k, done, kfinal, n_rem;
   int max_iter, resid_freq, worst_first, n_shift;
   double eps, bsqrt, rsq, rsqNew;
-  double beta_r, resSq, maxrr, delta;
+  double beta_r, resSq, maxrr, delta;  // maxrr: single-pass CG: max of |r|^2 since the last reliable update
   int reliable, pending, pad2, pad3;  // pending: self-finalising single-pass CG
   c128 alpha, alpha_prev, beta, beta_prev, rho, omega, pap;
   double* hist;
@@ -385,7 +385,6 @@ struct FinCgInit {
   __device__ void operator()(const double* v) const {
     s->bsqrt = sqrt(v[0]);
     s->rsq = v[1];
-    s->maxrr = sqrt(v[1]);
   }
 };
 // p_new = r + beta p_old (all visited sites); owned: store p_new and
@@ -1334,14 +1333,17 @@ struct EpiFast {
 // Scalar part of one single-pass CG step: from iteration k's sums <p,Ap>,
 // |Ap|^2, <Ap,r>, |r|^2 form alpha_k, |r_{k+1}|^2 by the recurrence, beta_k,
 // and decide whether to stop (FP64: converged or out of iterations; mixed:
-// also when the residual fell by delta since the last reliable update).
+// also when |r| fell below delta times its maximum since the last reliable
+// update).  Tests compare squared norms (no square root on this path) and
+// alpha = rr conj(<p,Ap>) / |<p,Ap>|^2 takes one division.  maxrr2 is the
+// maximum of |r|^2 since the last reliable update.
 struct FastStep {
   c128 alpha;
-  double rsq, rsqNew, beta_r, maxrr, relres;
+  double rsq, rsqNew, beta_r, maxrr2;
   bool stop;
 };
 template <class T, bool MIXED>
-__device__ FastStep fast_step(const double* v, int k, double maxrr, const SolveState* s) {
+__device__ __forceinline__ FastStep fast_step(const double* v, int k, double maxrr2, const SolveState* s) {
   c128 pap, apr;
   double apap, rr;
   if constexpr (is_cx<T>::value) {
@@ -1356,22 +1358,21 @@ __device__ FastStep fast_step(const double* v, int k, double maxrr, const SolveS
     rr = v[3];
   }
   FastStep f;
-  if constexpr (is_cx<T>::value) f.alpha = cdiv(c128{rr, 0.0}, pap);
-  else f.alpha = {rr / pap.re, 0.0};
+  const double g = rr / (pap.re * pap.re + pap.im * pap.im);
+  f.alpha = {g * pap.re, -g * pap.im};
   // Re(conj(alpha) <Ap, r>)
   const double re = f.alpha.re * apr.re + f.alpha.im * apr.im;
   double rn = rr - 2.0 * re + (f.alpha.re * f.alpha.re + f.alpha.im * f.alpha.im) * apap;
   if (!(rn > 0.0)) rn = 0.0;
   f.rsq = rr;
   f.rsqNew = rn;
-  const double rnorm = sqrt(rn);
-  f.relres = rnorm / s->bsqrt;
-  f.maxrr = maxrr;
+  const double tol = s->eps * s->bsqrt;
+  f.maxrr2 = maxrr2;
   if constexpr (MIXED) {
-    if (rnorm > f.maxrr) f.maxrr = rnorm;
-    f.stop = rnorm < s->delta * f.maxrr || rnorm < s->eps * s->bsqrt || k == s->max_iter - 1;
+    if (rn > f.maxrr2) f.maxrr2 = rn;
+    f.stop = rn < (s->delta * s->delta) * f.maxrr2 || rn < tol * tol || k == s->max_iter - 1;
   } else {
-    f.stop = rnorm < s->eps * s->bsqrt || k == s->max_iter - 1;
+    f.stop = rn < tol * tol || k == s->max_iter - 1;
   }
   f.beta_r = rn / rr;
   return f;
@@ -1387,8 +1388,8 @@ struct FinFast {
     s->alpha = f.alpha;
     s->rsq = f.rsq;
     s->rsqNew = f.rsqNew;
-    if (s->hist) s->hist[k] = f.relres;
-    if constexpr (MIXED) s->maxrr = f.maxrr;
+    if (s->hist) s->hist[k] = sqrt(f.rsqNew) / s->bsqrt;
+    if constexpr (MIXED) s->maxrr = f.maxrr2;
     if (f.stop) {
       s->kfinal = k;
       s->done = MIXED ? 2 : 1;
@@ -1486,8 +1487,8 @@ struct InFastSF : InFast<T, TX> {
       wr->alpha = f.alpha;
       wr->rsq = f.rsq;
       wr->rsqNew = f.rsqNew;
-      if (rd->hist) rd->hist[k0] = f.relres;
-      if constexpr (MIXED) wr->maxrr = f.maxrr;
+      if (rd->hist) rd->hist[k0] = sqrt(f.rsqNew) / rd->bsqrt;
+      if constexpr (MIXED) wr->maxrr = f.maxrr2;
       if (f.stop) {
         wr->kfinal = k0;
         wr->done = MIXED ? 2 : 1;
@@ -1528,7 +1529,7 @@ struct FinFastInit {
   __device__ void operator()(const double* v) const {
     s->bsqrt = sqrt(v[0]);
     s->rsq = v[1];
-    s->maxrr = sqrt(v[1]);
+    s->maxrr = v[1];  // |r|^2
     s->alpha = {0.0, 0.0};
     s->beta_r = 0.0;
     s->k = 0;
@@ -1593,7 +1594,7 @@ struct FinFastReliable {
     // the next prologue forms r_{k+1} = r_true - 0 * Ap and p_{k+1} = r_true + beta p_k
     s->beta_r = rt / s->rsq;
     s->alpha = {0.0, 0.0};
-    s->maxrr = sqrt(rt);
+    s->maxrr = rt;  // |r|^2
     s->k = k + 1;
     s->pending = 0;
     s->done = 0;
