@@ -218,6 +218,28 @@ def batch_multishift(glx, n_lat=48, rhs=4, seq=True):
     return out
 
 
+def eo_solves(glx, lat, b, mass, reps=3):
+    """D x = b on the same lattice by the even/odd pipeline on compacted
+    vectors (prepare, CG on m^2 - D_eo D_oe to 1e-10, reconstruct): FP64
+    single-pass and mixed precision, with the full residual |D x - b| / |b|."""
+    out = {}
+    db = glx.DeviceArray(b)
+    dx = glx.DeviceArray(np.zeros_like(b))
+    bn = float(np.linalg.norm(b))
+    for method in ("cg", "cg_fused", "mixed"):
+        glx.dev_solve_staggered_eo(lat, mass, dx, db, method, eps=1e-10)  # warm-up
+        ts = []
+        for _ in range(reps):
+            glx.synchronize()
+            t0 = time.perf_counter()
+            info, r2 = glx.dev_solve_staggered_eo(lat, mass, dx, db, method, eps=1e-10)
+            glx.synchronize()
+            ts.append(time.perf_counter() - t0)
+        out[method] = {"ms": round(float(np.median(ts)) * 1e3, 3), "iters": info.iter,
+                       "full_relres": float(np.sqrt(r2) / bn)}
+    return out
+
+
 def traffic_from_profile(links: str, alg_bytes: float):
     """HBM bytes per launch of the dominant kernel as measured by rocprofv3
     FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of known size), read from
@@ -245,6 +267,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-big", action="store_true", help="skip the 4096^2 stencil measurement")
     ap.add_argument("--no-grad", action="store_true", help="skip the 512^2 gradient timing")
+    ap.add_argument("--no-eo", action="store_true", help="skip the even/odd D x = b solves")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multishift workload")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
@@ -357,6 +380,7 @@ def main():
     if not args.no_big:
         big = stencil_beyond_mall(glx)
 
+    eo = None if args.no_eo else eo_solves(glx, lat, b, mass)
     grad = None if args.no_grad else gradient_timing(glx)
     batch = None if args.no_batch else batch_multishift(glx)
 
@@ -420,6 +444,7 @@ def main():
             "bytes_per_site": it_bps,
             "ms_per_launch": it_ms,
         },
+        "eo_solve_D": eo,
         "gradient": grad,
         "batch_multishift": batch,
         "cpu_baseline": cpu,

@@ -538,7 +538,7 @@ DevOperator lattice_op(glx_lattice lat, int kind, double mass, int nc) {
   o.mass = mass;
   o.nc = (kind == OP_LAPLACE || kind == OP_LAPLACE_REAL) ? std::max(nc, 1) : 1;
   if (is_stencil_kind(kind)) throw ArgError("stencil_2d operators take coefficients: use the host entry points");
-  if (kind < 0 || kind > OP_M2MDEODOE) throw ArgError("unknown operator kind");
+  if (kind < 0 || (kind > OP_M2MDEODOE && kind != OP_STAG_EO_SCHUR)) throw ArgError("unknown operator kind");
   if (op_links(kind) && !lat) throw ArgError("this operator needs a lattice with links");
   if (lat) {
     o.lx = lat->lx;
@@ -549,6 +549,33 @@ DevOperator lattice_op(glx_lattice lat, int kind, double mass, int nc) {
     o.sc16 = (lat->flags & GLX_LINKS_F16) ? lat->sc16.as<float>() : nullptr;
   }
   return o;
+}
+}  // namespace
+
+namespace {
+template <class F, class FR>
+int dispatch_unpreconditioned(F plain, FR restarted, double* lhs, double* rhs, int size, int type,
+                              const glx_inverter_params* p, glx_matvec mv, void* extra, glx_verbose* verb,
+                              glx_inversion_info* info) {
+  if (!p) {
+    g_last_error = "params is NULL";
+    return GLX_ERR_ARG;
+  }
+  if (type == -1) {  // MINV_INVALID: the reference returns a default inversion_info
+    if (info) {
+      std::memset(info, 0, sizeof(*info));
+      info->n_rhs = -1;
+    }
+    return GLX_OK;
+  }
+  if (type != 0 && type != 2 && type != 3) {
+    g_last_error = "minv_unpreconditioned: solver " + std::to_string(type) +
+                   " is not part of this library (CG 0, GCR 2 and BiCGStab 3 are)";
+    return GLX_ERR_ARG;
+  }
+  const int k = type == 0 ? 0 : (type == 2 ? 2 : 1);  // index into {CG, BiCGStab, GCR}
+  return p->restart ? restarted[k](lhs, rhs, size, p->max_iters, p->tol, p->restart_freq, mv, extra, verb, info)
+                    : plain[k](lhs, rhs, size, p->max_iters, p->tol, mv, extra, verb, info);
 }
 }  // namespace
 
@@ -792,6 +819,36 @@ int glx_dev_staggered_grad(glx_lattice lat, double mass, const double* phi_dev, 
   });
 }
 
+int glx_dev_solve_staggered_eo(glx_lattice lat, double mass, double* x_dev, const double* b_dev, int method,
+                               int max_iter, double eps, double* full_resSq, glx_inversion_info* info) {
+  return guarded([&] {
+    if (!x_dev || !b_dev) throw ArgError("x_dev and b_dev must not be NULL");
+    if (method != 0 && method != 1 && method != 2 && method != 3) throw ArgError("method must be 0, 1, 2 or 3");
+    DevOperator o = lattice_op(lat, OP_STAG, mass, 1);
+    if (method == LINKS_F16 && !o.u16) throw ArgError("lattice was created without GLX_LINKS_F16");
+    if (method == LINKS_F32 && !o.u32) throw ArgError("lattice was created without GLX_LINKS_F32");
+    SolveOpts so;
+    so.max_iter = max_iter;
+    so.eps = eps;
+    so.group = g_group;
+    so.use_graph = g_use_graph != 0;
+    SolveResult r = dev_solve_staggered_eo(o, x_dev, b_dev, method, so, full_resSq);
+    fill_info(info, r, r.name);
+  });
+}
+int glx_dev_eo_prepare(glx_lattice lat, double mass, double* be_dev, const double* b_dev) {
+  return guarded([&] {
+    if (!be_dev || !b_dev) throw ArgError("be_dev and b_dev must not be NULL");
+    dev_eo_prepare(lattice_op(lat, OP_STAG, mass, 1), be_dev, b_dev, nullptr);
+  });
+}
+int glx_dev_eo_reconstruct(glx_lattice lat, double mass, double* x_dev, const double* xe_dev, const double* b_dev) {
+  return guarded([&] {
+    if (!x_dev || !xe_dev || !b_dev) throw ArgError("x_dev, xe_dev and b_dev must not be NULL");
+    dev_eo_reconstruct(lattice_op(lat, OP_STAG, mass, 1), x_dev, xe_dev, b_dev, nullptr);
+  });
+}
+
 int glx_set_tuning(int group, int use_graph) {
   return guarded([&] {
     if (group < 1 || group > 1024) throw ArgError("group must be in [1, 1024]");
@@ -977,6 +1034,25 @@ int glx_release_cache(void) {
   return guarded([&] { DevicePool::get().trim(); });
 }
 const char* glx_last_error(void) { return g_last_error.c_str(); }
+int glx_minv_unpreconditioned_r(double* lhs, double* rhs, int size, int type, const glx_inverter_params* params,
+                                glx_matvec mv, void* extra, glx_verbose* verb, glx_inversion_info* info) {
+  using P = int (*)(double*, double*, int, int, double, glx_matvec, void*, glx_verbose*, glx_inversion_info*);
+  using R = int (*)(double*, double*, int, int, double, int, glx_matvec, void*, glx_verbose*, glx_inversion_info*);
+  static const P plain[3] = {glx_minv_vector_cg_r, glx_minv_vector_bicgstab_r, glx_minv_vector_gcr_r};
+  static const R restarted[3] = {glx_minv_vector_cg_restart_r, glx_minv_vector_bicgstab_restart_r,
+                                 glx_minv_vector_gcr_restart_r};
+  return dispatch_unpreconditioned(plain, restarted, lhs, rhs, size, type, params, mv, extra, verb, info);
+}
+int glx_minv_unpreconditioned_c(double* lhs, double* rhs, int size, int type, const glx_inverter_params* params,
+                                glx_matvec mv, void* extra, glx_verbose* verb, glx_inversion_info* info) {
+  using P = int (*)(double*, double*, int, int, double, glx_matvec, void*, glx_verbose*, glx_inversion_info*);
+  using R = int (*)(double*, double*, int, int, double, int, glx_matvec, void*, glx_verbose*, glx_inversion_info*);
+  static const P plain[3] = {glx_minv_vector_cg_c, glx_minv_vector_bicgstab_c, glx_minv_vector_gcr_c};
+  static const R restarted[3] = {glx_minv_vector_cg_restart_c, glx_minv_vector_bicgstab_restart_c,
+                                 glx_minv_vector_gcr_restart_c};
+  return dispatch_unpreconditioned(plain, restarted, lhs, rhs, size, type, params, mv, extra, verb, info);
+}
+
 int glx_abi_version(void) { return GLX_ABI_VERSION; }
 
 }  // extern "C"

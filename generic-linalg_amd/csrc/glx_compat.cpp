@@ -219,3 +219,41 @@ struct RegisterCompat {
 };
 RegisterCompat g_register_compat;
 }  // namespace
+
+// ------------------------------------------------------------ dispatcher --
+// generic_inverter.cpp:18-188: the solver named by `type`, restarted when
+// params.restart (CG, GCR, BiCGStab); solvers outside this library throw.
+namespace {
+template <class V, class MV>
+inversion_info dispatch(V* lhs, V* rhs, int size, minv_inverter type, minv_inverter_params& p, MV mv, void* extra,
+                        inversion_verbose_struct* verb) {
+  switch (type) {
+    case MINV_CG:
+      return p.restart ? minv_vector_cg_restart(lhs, rhs, size, p.max_iters, p.tol, p.restart_freq, mv, extra, verb)
+                       : minv_vector_cg(lhs, rhs, size, p.max_iters, p.tol, mv, extra, verb);
+    case MINV_GCR:
+      return p.restart ? minv_vector_gcr_restart(lhs, rhs, size, p.max_iters, p.tol, p.restart_freq, mv, extra, verb)
+                       : minv_vector_gcr(lhs, rhs, size, p.max_iters, p.tol, mv, extra, verb);
+    case MINV_BICGSTAB:
+      return p.restart
+                 ? minv_vector_bicgstab_restart(lhs, rhs, size, p.max_iters, p.tol, p.restart_freq, mv, extra, verb)
+                 : minv_vector_bicgstab(lhs, rhs, size, p.max_iters, p.tol, mv, extra, verb);
+    case MINV_INVALID:
+      return inversion_info();
+    default:
+      throw std::runtime_error("minv_unpreconditioned: solver " + std::to_string(int(type)) +
+                               " is not part of this library (CG, GCR and BiCGStab are)");
+  }
+}
+}  // namespace
+
+inversion_info minv_unpreconditioned(double* lhs, double* rhs, int size, minv_inverter type,
+                                     minv_inverter_params& params, rmv mv, void* extra,
+                                     inversion_verbose_struct* verb) {
+  return dispatch(lhs, rhs, size, type, params, mv, extra, verb);
+}
+inversion_info minv_unpreconditioned(complex<double>* lhs, complex<double>* rhs, int size, minv_inverter type,
+                                     minv_inverter_params& params, cmv mv, void* extra,
+                                     inversion_verbose_struct* verb) {
+  return dispatch(lhs, rhs, size, type, params, mv, extra, verb);
+}

@@ -5,7 +5,7 @@
 // reduction.
 #pragma once
 
-#include "glx_normal.h"
+#include "glx_eo.h"
 
 #include <functional>
 #include <map>
@@ -95,7 +95,8 @@ struct DevOperator {
   bool host_complex = true;
   int size_ = 0;
 
-  int size() const { return host_cb ? size_ : lx * ly * nc; }
+  // vector length (compacted even sites for the even/odd Schur operator)
+  int size() const { return host_cb ? size_ : (kind == OP_STAG_EO_SCHUR ? lx * ly / 2 : lx * ly * nc); }
   bool is_host() const { return host_cb != nullptr; }
   bool tiled() const { return !host_cb && nc == 1; }
 };
@@ -130,18 +131,27 @@ constexpr bool kind_valid_for(int k) {
 int normal_rows_for(int lx, int ly, int esize);
 void set_normal_rows(int lx, int ly, int esize, int rows);  // lx = 0: set the default
 
-// Whether a launch of D^dag D goes to the row-marching kernel (otherwise the
-// tiled one): complex vectors, a launch shape in the table that fits the
-// lattice, and aligned paired accesses.
+// Whether a launch goes to a row-marching kernel: D^dag D (glx_normal.h;
+// otherwise the tiled kernel) or the compact even/odd Schur operator
+// (glx_eo.h; it has no other kernel).  Complex vectors, a launch shape in the
+// table that fits the lattice, and aligned paired accesses.
 template <class T, class In>
 bool normal_path(const DevOperator& o, const In& in, const T* out) {
   if constexpr (!is_cx<T>::value) {
     return false;
   } else {
-    if (o.kind != OP_STAG_NORMAL) return false;
-    const int sr = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
+    if (o.kind != OP_STAG_NORMAL && o.kind != OP_STAG_EO_SCHUR) return false;
+    int sr = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
+    if (o.kind == OP_STAG_EO_SCHUR) {
+      if (sr <= 0) sr = 16;  // "tiled" does not exist for this operator
+      return eo_fits(o.lx, o.ly, sr, normal_cpl<T>()) && normal_pairs_aligned<T>(in, out);
+    }
     return sr > 0 && normal_fits(o.lx, o.ly, sr, normal_cpl<T>()) && normal_pairs_aligned<T>(in, out);
   }
+}
+inline int eo_rows_for(int lx, int ly, int esize) {
+  const int sr = normal_rows_for(lx, ly, esize);
+  return sr > 0 ? sr : 16;
 }
 
 // Launch the tiled stencil kernel for a runtime operator kind.
@@ -152,12 +162,21 @@ void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T*
   const OpDev<LK> od = make_opdev(o, lk);
   if (normal_path<T>(o, in, out)) {
     if constexpr (is_cx<T>::value) {
-      launch_normal<T, LK, In, Epi, Fin>(st, make_ngeom(o.lx, o.ly, normal_rows_for(o.lx, o.ly, int(sizeof(T))),
-                                                        normal_cpl<T>()),
-                                         od, in, out, epi, fin, ws, done, has_selffin<In>::value ? false : finalize);
+      const bool fin_launch = has_selffin<In>::value ? false : finalize;
+      if (o.kind == OP_STAG_EO_SCHUR)
+        launch_eo<T, LK, In, Epi, Fin>(st, make_eogeom(o.lx, o.ly, eo_rows_for(o.lx, o.ly, int(sizeof(T))),
+                                                       normal_cpl<T>()),
+                                       od, in, out, epi, fin, ws, done, fin_launch);
+      else
+        launch_normal<T, LK, In, Epi, Fin>(st, make_ngeom(o.lx, o.ly, normal_rows_for(o.lx, o.ly, int(sizeof(T))),
+                                                          normal_cpl<T>()),
+                                           od, in, out, epi, fin, ws, done, fin_launch);
       return;
     }
   }
+  if (o.kind == OP_STAG_EO_SCHUR)
+    throw ArgError("the compact even/odd operator needs complex vectors, (lx/2) % 64 == 0 (FP64) or "
+                   "(lx/2) % 128 == 0 (FP32), ly % 4 == 0, ly >= 8, and aligned vectors");
   if constexpr (has_selffin<In>::value) {
     throw ArgError("a self-finalising iteration needs the row-marching normal-operator kernel");
   } else {

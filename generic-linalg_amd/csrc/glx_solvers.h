@@ -103,6 +103,18 @@ struct GradResult {
 GradResult dev_staggered_grad(const DevOperator& normal, const DevOperator& dop, const void* phi, int precision,
                               const SolveOpts& o, double* grad, void* chi_out);
 
+// Even/odd solve of D x = b for the staggered operator on compacted even-site
+// vectors (operators.cpp:543-616 keep full-length vectors): prepare
+// b_e' = m b_e - D_eo b_o, CG on m^2 - D_eo D_oe (method 0: reference-order
+// CG; 3: single-pass FP64 CG; LINKS_F32 / LINKS_F16: mixed-precision CG),
+// reconstruct x_o = (b_o - D_oe x_e) / m.  x, b: full-lattice device
+// vectors; the result describes the even-site solve; full_resSq (host, may
+// be null) receives |D x - b|^2.
+SolveResult dev_solve_staggered_eo(const DevOperator& op, void* x, const void* b, int method, const SolveOpts& o,
+                                   double* full_resSq);
+void dev_eo_prepare(const DevOperator& op, void* be, const void* b, hipStream_t st);
+void dev_eo_reconstruct(const DevOperator& op, void* x, const void* xe, const void* b, hipStream_t st);
+
 // Plain operator application on device vectors.
 void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st);
 

@@ -2231,6 +2231,130 @@ std::vector<SolveResult> dev_cg_m_batched(const std::vector<BatchSystem>& sys, i
   return cg_m_batch_impl<c128>(sys, n_shift, shifts, resid_freq_check, worst_first, o);
 }
 
+// ----------------------------------------------- even/odd (compacted) ---
+// Even-site vectors are compacted: site (x, y), x + y even, is element
+// y * (lx / 2) + (x >> 1).  With D = m - K:
+//   prepare      b_e' = m b_e - D_eo b_o = m b_e + (K b)_e     (operators.cpp:543-563)
+//   solve        (m^2 - D_eo D_oe) x_e = b_e'                 (operators.cpp:566-588)
+//   reconstruct  x_o = (b_o - D_oe x_e) / m = (b_o + (K x_e)_o) / m   (operators.cpp:592-616)
+__device__ __forceinline__ int eo_wrap_up(int v, int n) { return v + 1 == n ? 0 : v + 1; }
+__device__ __forceinline__ int eo_wrap_dn(int v, int n) { return v == 0 ? n - 1 : v - 1; }
+
+struct PtEoPrepare {
+  static constexpr int NR = 0;
+  int lx, ly;
+  double m;
+  const c128* b;  // full lattice
+  const c128* u;  // FP64 links, AoS
+  c128* be;       // compact even
+  __device__ void init() {}
+  __device__ void operator()(int ce, double*) {
+    const int W = lx >> 1;
+    const int y = ce / W, c = ce - y * W;
+    const int x = 2 * c + (y & 1);
+    const int i = y * lx + x;
+    const int xp = y * lx + eo_wrap_up(x, lx), xm = y * lx + eo_wrap_dn(x, lx);
+    const int yp = eo_wrap_up(y, ly) * lx + x, ym = eo_wrap_dn(y, ly) * lx + x;
+    const c128 kb = khop<c128, c128>(b[xp], b[xm], b[yp], b[ym], u[2 * i], u[2 * xm], u[2 * i + 1], u[2 * ym + 1],
+                                     (x & 1) != 0);
+    be[ce] = rscal(m, b[i]) + kb;
+  }
+};
+struct PtEoReconstruct {
+  static constexpr int NR = 0;
+  int lx, ly;
+  double m;
+  const c128* xe;  // compact even
+  const c128* b;   // full lattice
+  const c128* u;
+  c128* x;  // full lattice
+  __device__ void init() {}
+  __device__ void operator()(int i, double*) {
+    const int W = lx >> 1;
+    const int y = i / lx, xx = i - y * lx;
+    if (((xx + y) & 1) == 0) {
+      x[i] = xe[y * W + (xx >> 1)];
+      return;
+    }
+    const int xr = eo_wrap_up(xx, lx), xl = eo_wrap_dn(xx, lx);
+    const int yu = eo_wrap_up(y, ly), yd = eo_wrap_dn(y, ly);
+    const c128 kx = khop<c128, c128>(xe[y * W + (xr >> 1)], xe[y * W + (xl >> 1)], xe[yu * W + (xx >> 1)],
+                                     xe[yd * W + (xx >> 1)], u[2 * i], u[2 * (y * lx + xl)], u[2 * i + 1],
+                                     u[2 * (yd * lx + xx) + 1], (xx & 1) != 0);
+    const c128 v = b[i] + kx;
+    x[i] = c128{v.re / m, v.im / m};
+  }
+};
+// |a - b|^2
+struct PtDiff2 {
+  static constexpr int NR = 1;
+  const c128* a;
+  const c128* b;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) { part[0] += norm2(a[i] - b[i]); }
+};
+struct FinStore1 {
+  double* out;
+  __device__ void operator()(const double* v) const { out[0] = v[0]; }
+};
+
+void dev_eo_prepare(const DevOperator& op, void* be, const void* b, hipStream_t st) {
+  if (!op.u64 || op.lx % 2 || op.ly % 2) throw ArgError("even/odd preparation needs FP64 links and even lx, ly");
+  RedSpace red;
+  red.init(kMaxPointBlocks, st);
+  launch_point(st, op.lx * op.ly / 2,
+               PtEoPrepare{op.lx, op.ly, op.mass, static_cast<const c128*>(b), op.u64, static_cast<c128*>(be)},
+               FinNone{}, red.ws(), kNoGate);
+  GLX_HIP(hipStreamSynchronize(st));
+}
+void dev_eo_reconstruct(const DevOperator& op, void* x, const void* xe, const void* b, hipStream_t st) {
+  if (!op.u64 || op.lx % 2 || op.ly % 2) throw ArgError("even/odd reconstruction needs FP64 links and even lx, ly");
+  if (!(op.mass != 0.0)) throw ArgError("even/odd reconstruction divides by the mass: m must not be 0");
+  RedSpace red;
+  red.init(kMaxPointBlocks, st);
+  launch_point(st, op.lx * op.ly,
+               PtEoReconstruct{op.lx, op.ly, op.mass, static_cast<const c128*>(xe), static_cast<const c128*>(b),
+                               op.u64, static_cast<c128*>(x)},
+               FinNone{}, red.ws(), kNoGate);
+  GLX_HIP(hipStreamSynchronize(st));
+}
+
+SolveResult dev_solve_staggered_eo(const DevOperator& op, void* x, const void* b, int method, const SolveOpts& o,
+                                   double* full_resSq) {
+  if (!op.u64 || op.lx % 2 || op.ly % 2) throw ArgError("even/odd solve needs FP64 links and even lx, ly");
+  if (!(op.mass != 0.0)) throw ArgError("even/odd solve divides by the mass: m must not be 0");
+  DevOperator schur = op;
+  schur.kind = OP_STAG_EO_SCHUR;
+  const int n = op.lx * op.ly, ne = n / 2;
+  DevBuf be(size_t(ne) * sizeof(c128)), xe(size_t(ne) * sizeof(c128));
+  GLX_HIP(hipMemset(xe.get(), 0, size_t(ne) * sizeof(c128)));
+  dev_eo_prepare(op, be.get(), b, nullptr);
+  SolveResult r;
+  if (method == 0) {
+    r = dev_cg(schur, xe.get(), be.get(), o);
+  } else if (method == 3) {
+    r = dev_cg_fused(schur, xe.get(), be.get(), o);
+  } else {
+    MixedOpts mo;
+    mo.link_prec = method == LINKS_F16 ? LINKS_F16 : LINKS_F32;
+    r = dev_cg_mixed(schur, xe.get(), be.get(), o, mo);
+  }
+  dev_eo_reconstruct(op, x, xe.get(), b, nullptr);
+  if (full_resSq) {
+    // |D x - b|^2 of the full system, D with FP64 links
+    DevOperator d = op;
+    d.kind = OP_STAG;
+    DevBuf dx(size_t(n) * sizeof(c128)), sum(sizeof(double));
+    dev_apply(d, dx.get(), x, LINKS_F64, 0, nullptr);
+    RedSpace red;
+    red.init(kMaxPointBlocks, nullptr);
+    launch_point(nullptr, n, PtDiff2{dx.as<c128>(), static_cast<const c128*>(b)}, FinStore1{sum.as<double>()},
+                 red.ws(), kNoGate);
+    GLX_HIP(hipMemcpy(full_resSq, sum.get(), sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return r;
+}
+
 // ----------------------------------------------------------- gradient --
 // dS/dtheta_mu(x) = -eta_mu(x) Im[psi(x)^* U_mu(x) chi(x+mu) + psi(x+mu)^* U_mu(x)^* chi(x)]
 // for S = phi^dag (D^dag D)^-1 phi, chi = (D^dag D)^-1 phi, psi = D chi

@@ -47,6 +47,7 @@ enum OpKind : int {
   OP_STENCIL_EO = 14,    // coarse_stencil.cpp:395-459
   OP_STENCIL_OE = 15,    // coarse_stencil.cpp:560-624
   OP_STENCIL_M2MDEODOE = 16,  // operators_stencil.cpp:195-214
+  OP_STAG_EO_SCHUR = 17,      // m^2 - D_eo D_oe (operators.cpp:566-588) on compacted even-site vectors
 };
 
 __host__ __device__ constexpr int op_halo(int k) {
@@ -54,7 +55,7 @@ __host__ __device__ constexpr int op_halo(int k) {
 }
 __host__ __device__ constexpr bool op_links(int k) {
   return k == OP_LAPLACE_U1 || k == OP_STAG || k == OP_STAG_DAG || k == OP_STAG_NORMAL || k == OP_STAG_G5 ||
-         k == OP_DEO || k == OP_DOE || k == OP_M2MDEODOE;
+         k == OP_DEO || k == OP_DOE || k == OP_M2MDEODOE || k == OP_STAG_EO_SCHUR;
 }
 __host__ __device__ constexpr bool op_coeffs(int k) {
   return k == OP_STENCIL || k == OP_STENCIL_EO || k == OP_STENCIL_OE || k == OP_STENCIL_M2MDEODOE;

@@ -37,6 +37,7 @@ OP_KINDS = {
     "stencil_eo": 14,
     "stencil_oe": 15,
     "stencil_m2mdeodoe": 16,
+    "staggered_eo_schur": 17,
 }
 # C function implementing each operator callback (include/glx.h)
 OP_SYMBOLS = {
@@ -164,6 +165,12 @@ def lib():
         L.glx_dev_staggered_grad.argtypes = [vp, d, vp, i, i, d, vp, vp, C.POINTER(d), C.POINTER(d), C.POINTER(d), IP]
         L.glx_dev_cg_m_batched.restype = i
         L.glx_dev_cg_m_batched.argtypes = [i, vp, i, vp, i, i, d, i, i, vp]
+        L.glx_dev_solve_staggered_eo.restype = i
+        L.glx_dev_solve_staggered_eo.argtypes = [vp, d, vp, vp, i, i, d, C.POINTER(d), IP]
+        L.glx_dev_eo_prepare.restype = i
+        L.glx_dev_eo_prepare.argtypes = [vp, d, vp, vp]
+        L.glx_dev_eo_reconstruct.restype = i
+        L.glx_dev_eo_reconstruct.argtypes = [vp, d, vp, vp, vp]
         L.glx_set_normal_rows.restype = i
         L.glx_set_normal_rows.argtypes = [i, i, i, i]
         L.glx_get_normal_rows.restype = i
@@ -539,6 +546,28 @@ def dev_cg_m(lat, op: str, mass: float, xs, b: DeviceArray, shifts, resid_freq_c
     _check(lib().glx_dev_cg_m(None if lat is None else lat.h, OP_KINDS[op], mass, nc, ptrs, b.ptr, n, sh.ctypes.data,
                               resid_freq_check, max_iter, eps, int(worst_first), None, C.byref(ci)), "glx_dev_cg_m")
     return _info(ci, "", res)
+
+
+EO_METHODS = {"cg": 0, "mixed": LINKS_F32, "mixed_f16": LINKS_F16, "cg_fused": 3}
+
+
+def dev_solve_staggered_eo(lat, mass: float, x: DeviceArray, b: DeviceArray, method: str = "cg",
+                           max_iter: int = 100000, eps: float = 1e-10):
+    """D x = b by the even/odd pipeline on compacted vectors; returns
+    (Info of the even-site solve, |D x - b|^2)."""
+    r2 = C.c_double(0.0)
+    ci = InversionInfo()
+    _check(lib().glx_dev_solve_staggered_eo(lat.h, mass, x.ptr, b.ptr, EO_METHODS[method], max_iter, eps,
+                                            C.byref(r2), C.byref(ci)), "glx_dev_solve_staggered_eo")
+    return _info(ci), r2.value
+
+
+def dev_eo_prepare(lat, mass: float, be: DeviceArray, b: DeviceArray):
+    _check(lib().glx_dev_eo_prepare(lat.h, mass, be.ptr, b.ptr), "glx_dev_eo_prepare")
+
+
+def dev_eo_reconstruct(lat, mass: float, x: DeviceArray, xe: DeviceArray, b: DeviceArray):
+    _check(lib().glx_dev_eo_reconstruct(lat.h, mass, x.ptr, xe.ptr, b.ptr), "glx_dev_eo_reconstruct")
 
 
 class BatchSystem(C.Structure):

@@ -157,7 +157,12 @@ enum {
   GLX_OP_STENCIL = 11,
   GLX_OP_STENCIL_EO = 14,
   GLX_OP_STENCIL_OE = 15,
-  GLX_OP_STENCIL_M2MDEODOE = 16
+  GLX_OP_STENCIL_M2MDEODOE = 16,
+  /* m^2 - D_eo D_oe (operators.cpp:566-588) on COMPACTED even-site vectors of
+   * lx*ly/2 elements, site (x, y) with x + y even at y*(lx/2) + (x >> 1);
+   * device API only (glx_dev_*); needs (lx/2) % 64 == 0 (FP64) or % 128 == 0
+   * (FP32 vectors), ly % 4 == 0, ly >= 8 */
+  GLX_OP_STAGGERED_EO_SCHUR = 17
 };
 
 /* Bind a caller's own host callback to a device operator, so that solves
@@ -204,6 +209,28 @@ int glx_minv_vector_gcr_restart_r(double* phi, double* phi0, int size, int max_i
 int glx_minv_vector_gcr_restart_c(double* phi, double* phi0, int size, int max_iter, double eps, int restart_freq,
                                   glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
                                   glx_inversion_info* info);
+/* The reference's dispatcher minv_unpreconditioned (generic_inverter.cpp:18-188,
+ * generic_inverters.h:70-111): type is the reference's minv_inverter value
+ * (MINV_CG 0, MINV_GCR 2, MINV_BICGSTAB 3 run here, restarted when
+ * params->restart; MINV_INVALID -1 returns a zeroed info; the reference's
+ * other solvers -- CR 1, BiCGStab(l) 4, GMRES 5, SOR 6, MinRes 7 -- are not
+ * part of this library and return GLX_ERR_ARG).  glx_inverter_params mirrors
+ * minv_inverter_params field by field. */
+typedef struct {
+  double tol;
+  int max_iters;
+  int restart;
+  int restart_freq;
+  double sor_omega;
+  double minres_omega;
+  int bicgstabl_l;
+} glx_inverter_params;
+int glx_minv_unpreconditioned_r(double* lhs, double* rhs, int size, int type, const glx_inverter_params* params,
+                                glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
+                                glx_inversion_info* info);
+int glx_minv_unpreconditioned_c(double* lhs, double* rhs, int size, int type, const glx_inverter_params* params,
+                                glx_matvec matrix_vector, void* extra_info, glx_verbose* verbosity,
+                                glx_inversion_info* info);
 /* Multishift CG: phi[n_shift] host vectors (zeroed, then solutions);
  * shifts[n_shift] is read (left unchanged on return, as the reference
  * restores it); info->resSqmrhs (n_shift doubles) receives the per-shift
@@ -284,6 +311,19 @@ int glx_dev_cg_fused(glx_lattice lat, int kind, double mass, double* x_dev, cons
 int glx_dev_staggered_grad(glx_lattice lat, double mass, const double* phi_dev, int precision, int max_iter,
                            double eps, double* grad_dev, double* chi_dev, double* dS_dm, double* S, double* S_robust,
                            glx_inversion_info* info);
+/* Even/odd solve of D x = b for the staggered operator (the pipeline of
+ * operators.cpp:543-616 -- prepare, CG on m^2 - D_eo D_oe, reconstruct -- on
+ * compacted even-site vectors): x_dev, b_dev full-lattice complex vectors;
+ * method 0: reference-order CG, 3: single-pass FP64 CG, 1 / 2: mixed
+ * precision with FP32 / FP16 links.  info describes the even-site solve
+ * (its eps is relative to |b_e'|); full_resSq (may be NULL) receives
+ * |D x - b|^2.  Needs m != 0 and the GLX_OP_STAGGERED_EO_SCHUR sizes. */
+int glx_dev_solve_staggered_eo(glx_lattice lat, double mass, double* x_dev, const double* b_dev, int method,
+                               int max_iter, double eps, double* full_resSq, glx_inversion_info* info);
+/* The two ends of that pipeline: be_dev (compact, lx*ly/2) = m b_e - D_eo b_o;
+ * x_dev (full) = (x_e scattered, (b_o - D_oe x_e) / m on odd sites). */
+int glx_dev_eo_prepare(glx_lattice lat, double mass, double* be_dev, const double* b_dev);
+int glx_dev_eo_reconstruct(glx_lattice lat, double mass, double* x_dev, const double* xe_dev, const double* b_dev);
 /* Launch tuning of the single-pass CG and the mixed-precision CG: iterations
  * per enqueued group (default 16; rounded up to even where the iteration is
  * self-finalising) and whether a group is captured once as a hipGraph and

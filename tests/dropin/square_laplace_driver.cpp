@@ -11,6 +11,7 @@
 #include <iomanip>
 #include <iostream>
 #include <random>
+#include <stdexcept>
 
 #include "generic_inverters.h"
 #include "operators.h"
@@ -109,5 +110,32 @@ int main() {
   }
   std::cout << "[TEST]: host_callback iter " << invif.iter << " success " << invif.success << " relres "
             << std::sqrt(rr / bb) << "\n";
+
+  // 5. the dispatcher (generic_inverter.cpp:18-188)
+  minv_inverter_params params;
+  params.tol = 1e-10;
+  params.max_iters = 10000;
+  params.restart = false;
+  params.restart_freq = 64;
+  params.sor_omega = 1.0;
+  params.minres_omega = 1.0;
+  params.bicgstabl_l = 4;
+  stagif.mass = 0.05;
+  verb.verbosity = VERB_NONE;
+  for (int i = 0; i < V; i++) x[i] = 0.0;
+  invif = minv_unpreconditioned(x, b, V, MINV_CG, params, square_staggered_normal_u1, &stagif, &verb);
+  std::cout << "[TEST]: dispatch cg " << invif.name << " success " << invif.success << "\n";
+  params.restart = true;
+  for (int i = 0; i < V; i++) x[i] = 0.0;
+  invif = minv_unpreconditioned(x, b, V, MINV_BICGSTAB, params, square_staggered_u1, &stagif, &verb);
+  std::cout << "[TEST]: dispatch bicgstab_restart " << invif.name << " success " << invif.success << "\n";
+  invif = minv_unpreconditioned(x, b, V, MINV_INVALID, params, square_staggered_u1, &stagif, &verb);
+  std::cout << "[TEST]: dispatch invalid iter " << invif.iter << "\n";
+  try {
+    minv_unpreconditioned(x, b, V, MINV_CR, params, square_staggered_u1, &stagif, &verb);
+    std::cout << "[TEST]: dispatch cr ran\n";
+  } catch (const std::runtime_error& e) {
+    std::cout << "[TEST]: dispatch cr refused\n";
+  }
   return 0;
 }

@@ -65,6 +65,38 @@ def test_argument_errors_without_device():
     assert g(None, 3, None) == 1
 
 
+class _Params(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("restart", C.c_int), ("restart_freq", C.c_int),
+                ("sor_omega", C.c_double), ("minres_omega", C.c_double), ("bicgstabl_l", C.c_int)]
+
+
+class _Info(C.Structure):
+    _fields_ = [("resSq", C.c_double), ("iter", C.c_int), ("success", C.c_int), ("ops_count", C.c_int),
+                ("name", C.c_char * 32), ("resSqmrhs", C.c_void_p), ("n_rhs", C.c_int),
+                ("reliable_updates", C.c_int)]
+
+
+def test_dispatcher_without_device():
+    """minv_unpreconditioned (generic_inverter.cpp:18-188): MINV_INVALID
+    returns a default info as the reference does; solvers outside this
+    library are refused with a message, before any device work."""
+    lib = C.CDLL(LIB)
+    lib.glx_last_error.restype = C.c_char_p
+    vp = C.c_void_p
+    for f in (lib.glx_minv_unpreconditioned_r, lib.glx_minv_unpreconditioned_c):
+        f.restype = C.c_int
+        f.argtypes = [vp, vp, C.c_int, C.c_int, C.POINTER(_Params), vp, vp, vp, C.POINTER(_Info)]
+        p = _Params(1e-10, 100, 0, 10, 1.0, 1.0, 2)
+        info = _Info()
+        info.iter = 77
+        assert f(None, None, 4, -1, C.byref(p), None, None, None, C.byref(info)) == 0
+        assert info.iter == 0 and info.success == 0 and info.name == b""
+        for t in (1, 4, 5, 6, 7):
+            assert f(None, None, 4, t, C.byref(p), None, None, None, C.byref(info)) == 1
+            assert b"not part of this library" in lib.glx_last_error()
+        assert f(None, None, 4, 0, None, None, None, None, C.byref(info)) == 1
+
+
 def test_dropin_driver_builds(tmp_path):
     out = subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "dropin"), "-B"], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout + out.stderr

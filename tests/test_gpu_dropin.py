@@ -28,3 +28,8 @@ def test_dropin_driver_runs(glx):
     assert m and float(m.group(1)) == 0.0 and float(m.group(2)) == 0.01 and float(m.group(3)) == 0.005
     m = re.search(r"host_callback iter \d+ success 1 relres (\S+)", t)
     assert m and float(m.group(1)) < 1e-10
+    # minv_unpreconditioned (generic_inverter.cpp:18-188)
+    assert "[TEST]: dispatch cg CG success 1" in t
+    assert re.search(r"\[TEST\]: dispatch bicgstab_restart BiCGStab\(64\) success [01]", t)
+    assert "[TEST]: dispatch invalid iter 0" in t
+    assert "[TEST]: dispatch cr refused" in t
