@@ -205,6 +205,9 @@ def ref_lib():
         lib.ref_plaquette_u1.argtypes = [vp, i, i, vp]
         lib.ref_read_gauge_u1.argtypes = [vp, i, i, C.c_char_p]
         lib.ref_write_gauge_u1.argtypes = [vp, i, i, C.c_char_p]
+        lib.ref_ape_smear_u1.argtypes = [vp, vp, i, i, d, i]
+        lib.ref_topo_u1.argtypes = [vp, i, i]
+        lib.ref_topo_u1.restype = d
         _ref = lib
     return _ref
 

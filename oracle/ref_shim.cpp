@@ -326,5 +326,9 @@ void ref_read_gauge_u1(double* u, int lx, int ly, const char* path) {
 void ref_write_gauge_u1(double* u, int lx, int ly, const char* path) {
   write_gauge_u1(reinterpret_cast<cd*>(u), lx, ly, std::string(path));
 }
+void ref_ape_smear_u1(double* out, double* u, int lx, int ly, double alpha, int n_iter) {
+  apply_ape_smear_u1(reinterpret_cast<cd*>(out), reinterpret_cast<cd*>(u), lx, ly, alpha, n_iter);
+}
+double ref_topo_u1(double* u, int lx, int ly) { return get_topo_u1(reinterpret_cast<cd*>(u), lx, ly); }
 
 }  // extern "C"
