@@ -519,9 +519,9 @@ __global__ void k_links_lowp(int n, const c128* __restrict__ u, c64* __restrict_
   }
 }
 
-void lattice_fill(glx_lattice_s* L, const double* links) {
+// Refresh the FP32 / FP16 copies from the FP64 links.
+void lattice_lowp(glx_lattice_s* L) {
   const size_t V = size_t(L->lx) * L->ly;
-  GLX_HIP(hipMemcpy(L->u64.get(), links, 2 * V * 16, hipMemcpyHostToDevice));
   if (L->flags & (GLX_LINKS_F32 | GLX_LINKS_F16)) {
     hipLaunchKernelGGL(k_links_lowp, dim3(point_blocks(int(V))), dim3(kBlock), 0, nullptr, int(V),
                        L->u64.as<c128>(), (L->flags & GLX_LINKS_F32) ? L->u32.as<c64>() : nullptr,
@@ -530,6 +530,29 @@ void lattice_fill(glx_lattice_s* L, const double* links) {
     GLX_CHECK_LAUNCH();
   }
   GLX_HIP(hipDeviceSynchronize());
+}
+
+void lattice_fill(glx_lattice_s* L, const double* links) {
+  const size_t V = size_t(L->lx) * L->ly;
+  GLX_HIP(hipMemcpy(L->u64.get(), links, 2 * V * 16, hipMemcpyHostToDevice));
+  lattice_lowp(L);
+}
+
+std::unique_ptr<glx_lattice_s> lattice_alloc(int lx, int ly, int flags) {
+  if (lx <= 0 || ly <= 0) throw ArgError("lattice dimensions must be positive");
+  if (double(lx) * ly > 2147483647.0) throw ArgError("lattice volume must fit a 32-bit site index");
+  auto L = std::make_unique<glx_lattice_s>();
+  L->lx = lx;
+  L->ly = ly;
+  L->flags = flags;
+  const size_t V = size_t(lx) * ly;
+  L->u64 = DevBuf(2 * V * 16);
+  if (flags & GLX_LINKS_F32) L->u32 = DevBuf(2 * V * 8);
+  if (flags & GLX_LINKS_F16) {
+    L->u16 = DevBuf(V * sizeof(HalfLink));
+    L->sc16 = DevBuf(V * sizeof(float));
+  }
+  return L;
 }
 
 DevOperator lattice_op(glx_lattice lat, int kind, double mass, int nc) {
@@ -673,20 +696,45 @@ int glx_minv_vector_cg_m_c(double** phi, double* phi0, int n_shift, int size, in
 int glx_lattice_create(glx_lattice* lat, const double* links, int lx, int ly, int flags) {
   return guarded([&] {
     if (!lat || !links) throw ArgError("lat and links must not be NULL");
-    if (lx <= 0 || ly <= 0) throw ArgError("lattice dimensions must be positive");
-    auto L = std::make_unique<glx_lattice_s>();
-    L->lx = lx;
-    L->ly = ly;
-    L->flags = flags;
-    const size_t V = size_t(lx) * ly;
-    L->u64 = DevBuf(2 * V * 16);
-    if (flags & GLX_LINKS_F32) L->u32 = DevBuf(2 * V * 8);
-    if (flags & GLX_LINKS_F16) {
-      L->u16 = DevBuf(V * sizeof(HalfLink));
-      L->sc16 = DevBuf(V * sizeof(float));
-    }
+    auto L = lattice_alloc(lx, ly, flags);
     lattice_fill(L.get(), links);
     *lat = L.release();
+  });
+}
+int glx_lattice_create_gaussian(glx_lattice* lat, int lx, int ly, int flags, double beta, unsigned long long seed,
+                                unsigned int stream) {
+  return guarded([&] {
+    if (!lat) throw ArgError("lat must not be NULL");
+    if (!std::isfinite(beta)) throw ArgError("beta must be finite");
+    auto L = lattice_alloc(lx, ly, flags);
+    dev_gauss_links(L->u64.get(), lx, ly, beta, seed, stream, nullptr);
+    lattice_lowp(L.get());
+    *lat = L.release();
+  });
+}
+int glx_lattice_gaussian_links(glx_lattice lat, double beta, unsigned long long seed, unsigned int stream) {
+  return guarded([&] {
+    if (!lat) throw ArgError("lat must not be NULL");
+    if (!std::isfinite(beta)) throw ArgError("beta must be finite");
+    GLX_HIP(hipDeviceSynchronize());
+    dev_gauss_links(lat->u64.get(), lat->lx, lat->ly, beta, seed, stream, nullptr);
+    lattice_lowp(lat);
+  });
+}
+int glx_lattice_get_links(glx_lattice lat, double* links) {
+  return guarded([&] {
+    if (!lat || !links) throw ArgError("lat and links must not be NULL");
+    GLX_HIP(hipDeviceSynchronize());
+    GLX_HIP(hipMemcpy(links, lat->u64.get(), size_t(lat->lx) * lat->ly * 32, hipMemcpyDeviceToHost));
+  });
+}
+int glx_lattice_plaquette(glx_lattice lat, double* plaq) {
+  return guarded([&] {
+    if (!lat || !plaq) throw ArgError("lat and plaq must not be NULL");
+    GLX_HIP(hipDeviceSynchronize());
+    const std::complex<double> p = dev_plaquette(lat->u64.get(), lat->lx, lat->ly, nullptr);
+    plaq[0] = p.real();
+    plaq[1] = p.imag();
   });
 }
 int glx_lattice_update_links(glx_lattice lat, const double* links) {

@@ -13,6 +13,8 @@
 #include <string>
 #include <vector>
 
+#include <complex>
+
 namespace glx {
 
 enum Verb : int { VERB_PASS_THROUGH = -1, VERB_NONE = 0, VERB_SUMMARY = 1, VERB_RESTART_DETAIL = 2, VERB_DETAIL = 3 };
@@ -114,6 +116,11 @@ SolveResult dev_solve_staggered_eo(const DevOperator& op, void* x, const void* b
                                    double* full_resSq);
 void dev_eo_prepare(const DevOperator& op, void* be, const void* b, hipStream_t st);
 void dev_eo_reconstruct(const DevOperator& op, void* x, const void* xe, const void* b, hipStream_t st);
+
+// Gauge fields on the device: Philox4x32-10 Gaussian (beta > 0) or uniform
+// (beta = 0) link phases, and the mean plaquette.
+void dev_gauss_links(void* u, int lx, int ly, double beta, unsigned long long seed, unsigned stream, hipStream_t st);
+std::complex<double> dev_plaquette(const void* u, int lx, int ly, hipStream_t st);
 
 // Plain operator application on device vectors.
 void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, int fp32, hipStream_t st);

@@ -2459,4 +2459,109 @@ void dev_apply(const DevOperator& op, void* out, const void* in, int link_prec, 
   GLX_HIP(hipStreamSynchronize(st));
 }
 
+// ------------------------------------------------------------ gauge fields --
+// Counter-based Philox4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11): ten rounds of 32x32->64 multiplies by
+// M0 = 0xD2511F53 and M1 = 0xCD9E8D57 with the key bumped by the Weyl
+// constants 0x9E3779B9 / 0xBB67AE85 between rounds.  Every site draws from its
+// own counter, so a field depends only on (seed, stream), not on the launch.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+    const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// Site s: counter {s, stream, 0, 0}, key {seed lo, seed hi}.  Two 53-bit
+// uniforms a, b from the four words; beta > 0: Box-Muller pair
+// sigma sqrt(-2 ln(1 - a)) (cos, sin)(2 pi b) for the x and y links;
+// beta = 0: uniform phases -pi + 2 pi (a, b).
+struct PtGaussLinks {
+  static constexpr int NR = 0;
+  uint32_t k0, k1, stream;
+  double sigma;  // 1/sqrt(|beta|), 0 for the uniform (hot) field
+  c128* u;
+  __device__ void init() {}
+  __device__ void operator()(int s, double*) {
+    uint32_t c[4] = {uint32_t(s), stream, 0u, 0u};
+    philox4x32_10(c, k0, k1);
+    constexpr double kTwoM53 = 1.1102230246251565404e-16;  // 2^-53
+    constexpr double kTwoPi = 6.283185307179586232;
+    const double a = double(((uint64_t(c[1]) << 32) | c[0]) >> 11) * kTwoM53;
+    const double b = double(((uint64_t(c[3]) << 32) | c[2]) >> 11) * kTwoM53;
+    double t0, t1;
+    if (sigma > 0.0) {
+      const double r = sigma * sqrt(-2.0 * log(1.0 - a));
+      double sn, cs;
+      sincos(kTwoPi * b, &sn, &cs);
+      t0 = r * cs;
+      t1 = r * sn;
+    } else {
+      t0 = kTwoPi * a - 3.141592653589793116;
+      t1 = kTwoPi * b - 3.141592653589793116;
+    }
+    double s0, c0, s1, c1;
+    sincos(t0, &s0, &c0);
+    sincos(t1, &s1, &c1);
+    u[2 * s] = {c0, s0};
+    u[2 * s + 1] = {c1, s1};
+  }
+};
+
+// Mean plaquette: U_x(s) U_y(s + x) U_x^*(s + y) U_y^*(s), summed per site.
+struct PtPlaquette {
+  static constexpr int NR = 2;
+  int lx, ly;
+  const c128* u;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const int y = i / lx, x = i - y * lx;
+    const int nx = y * lx + (x + 1 == lx ? 0 : x + 1);
+    const int ny = (y + 1 == ly ? 0 : y + 1) * lx + x;
+    const c128 p = cmulc(u[2 * ny], cmulc(u[2 * i + 1], u[2 * i] * u[2 * nx + 1]));
+    part[0] += p.re;
+    part[1] += p.im;
+  }
+};
+struct FinStore2 {
+  double* out;
+  __device__ void operator()(const double* v) const {
+    out[0] = v[0];
+    out[1] = v[1];
+  }
+};
+
+void dev_gauss_links(void* u, int lx, int ly, double beta, unsigned long long seed, unsigned stream,
+                     hipStream_t st) {
+  if (lx <= 0 || ly <= 0 || double(lx) * ly > 2147483647.0) throw ArgError("lattice dimensions out of range");
+  const double b = beta < 0 ? -beta : beta;
+  RedSpace red;
+  red.init(1, st);
+  launch_point(st, lx * ly,
+               PtGaussLinks{uint32_t(seed), uint32_t(seed >> 32), stream, b > 0 ? 1.0 / sqrt(b) : 0.0,
+                            static_cast<c128*>(u)},
+               FinNone{}, red.ws(), kNoGate);
+  GLX_HIP(hipStreamSynchronize(st));
+}
+
+std::complex<double> dev_plaquette(const void* u, int lx, int ly, hipStream_t st) {
+  RedSpace red;
+  red.init(kMaxPointBlocks, st);
+  DevBuf out(2 * sizeof(double));
+  launch_point(st, lx * ly, PtPlaquette{lx, ly, static_cast<const c128*>(u)}, FinStore2{out.as<double>()},
+               red.ws(), kNoGate);
+  double h[2];
+  GLX_HIP(hipMemcpyAsync(h, out.get(), sizeof h, hipMemcpyDeviceToHost, st));
+  GLX_HIP(hipStreamSynchronize(st));
+  return {h[0] / (double(lx) * ly), h[1] / (double(lx) * ly)};
+}
+
 }  // namespace glx

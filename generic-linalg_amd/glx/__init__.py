@@ -147,6 +147,14 @@ def lib():
         L.glx_lattice_create.argtypes = [C.POINTER(vp), vp, i, i, i]
         L.glx_lattice_update_links.restype = i
         L.glx_lattice_update_links.argtypes = [vp, vp]
+        L.glx_lattice_create_gaussian.restype = i
+        L.glx_lattice_create_gaussian.argtypes = [C.POINTER(vp), i, i, i, C.c_double, C.c_ulonglong, C.c_uint]
+        L.glx_lattice_gaussian_links.restype = i
+        L.glx_lattice_gaussian_links.argtypes = [vp, C.c_double, C.c_ulonglong, C.c_uint]
+        L.glx_lattice_get_links.restype = i
+        L.glx_lattice_get_links.argtypes = [vp, vp]
+        L.glx_lattice_plaquette.restype = i
+        L.glx_lattice_plaquette.argtypes = [vp, vp]
         L.glx_lattice_destroy.restype = i
         L.glx_lattice_destroy.argtypes = [vp]
         L.glx_dev_solve.restype = i
@@ -416,6 +424,35 @@ class Lattice:
         flags = (LINKS_F32 if f32 else 0) | (LINKS_F16 if f16 else 0)
         _check(lib().glx_lattice_create(C.byref(h), self.links.ctypes.data, lx, ly, flags), "glx_lattice_create")
         self.h = h.value
+
+    @classmethod
+    def gaussian(cls, lx: int, ly: int, beta: float, seed: int, stream: int = 0, f32: bool = True,
+                 f16: bool = True):
+        """Links generated in HBM: theta ~ N(0, 1/|beta|) (uniform for beta = 0)
+        from Philox4x32-10 with counter {site, stream, 0, 0} and key seed."""
+        self = cls.__new__(cls)
+        self.lx, self.ly, self.links = lx, ly, None
+        h = C.c_void_p()
+        flags = (LINKS_F32 if f32 else 0) | (LINKS_F16 if f16 else 0)
+        _check(lib().glx_lattice_create_gaussian(C.byref(h), lx, ly, flags, float(beta), int(seed), int(stream)),
+               "glx_lattice_create_gaussian")
+        self.h = h.value
+        return self
+
+    def regenerate(self, beta: float, seed: int, stream: int = 0):
+        self.links = None
+        _check(lib().glx_lattice_gaussian_links(self.h, float(beta), int(seed), int(stream)),
+               "glx_lattice_gaussian_links")
+
+    def get_links(self):
+        out = np.empty(2 * self.lx * self.ly, np.complex128)
+        _check(lib().glx_lattice_get_links(self.h, out.ctypes.data), "glx_lattice_get_links")
+        return out
+
+    def plaquette(self) -> complex:
+        out = np.zeros(2)
+        _check(lib().glx_lattice_plaquette(self.h, out.ctypes.data), "glx_lattice_plaquette")
+        return complex(out[0], out[1])
 
     def update(self, links):
         self.links = np.ascontiguousarray(links, dtype=np.complex128)

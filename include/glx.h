@@ -253,6 +253,22 @@ typedef struct glx_lattice_s* glx_lattice;
 int glx_lattice_create(glx_lattice* lat, const double* links, int lx, int ly, int flags);
 int glx_lattice_update_links(glx_lattice lat, const double* links);
 int glx_lattice_destroy(glx_lattice lat);
+/* Gauge fields generated and measured in HBM: the device counterparts of
+ * gauss_gauge_u1 / rand_gauge_u1 and get_plaquette_u1 (u1_utils.cpp:79-112,
+ * 190-207), for setting up many lattices without host traffic.  Each site s
+ * draws from the counter-based Philox4x32-10 generator (counter
+ * {s, stream, 0, 0}, key = seed), so a field depends only on (seed, stream).
+ * theta ~ N(0, 1/|beta|) by Box-Muller; beta = 0 gives uniform phases on
+ * [-pi, pi) (a hot start).  The host std::mt19937 stream of the reference is
+ * not reproduced: for fields equal to the reference's, generate them on the
+ * host (compat/u1_utils.h) and use glx_lattice_create. */
+int glx_lattice_create_gaussian(glx_lattice* lat, int lx, int ly, int flags, double beta, unsigned long long seed,
+                                unsigned int stream);
+int glx_lattice_gaussian_links(glx_lattice lat, double beta, unsigned long long seed, unsigned int stream);
+/* Host copy of the FP64 links (2*lx*ly complex values). */
+int glx_lattice_get_links(glx_lattice lat, double* links);
+/* Mean plaquette, plaq[0] + i plaq[1] (get_plaquette_u1, u1_utils.cpp:190-207). */
+int glx_lattice_plaquette(glx_lattice lat, double* plaq);
 
 /* Solver selection for glx_dev_solve. */
 enum { GLX_SOLVER_CG = 0, GLX_SOLVER_BICGSTAB = 1, GLX_SOLVER_GCR = 2 };

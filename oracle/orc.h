@@ -182,6 +182,13 @@ void orc_apply_gauge_trans_u1(orc_c* u, const orc_c* t, int lx, int ly);    /* u
 orc_c orc_plaquette_u1(const orc_c* u, int lx, int ly);                     /* u1_utils.cpp:190-207 */
 int orc_read_gauge_u1(orc_c* u, int lx, int ly, const char* path);          /* u1_utils.cpp:17-38 */
 int orc_write_gauge_u1(const orc_c* u, int lx, int ly, const char* path);   /* u1_utils.cpp:44-65 */
+/* The library's device field generator (no reference counterpart; the
+ * reference draws from std::mt19937 on the host): Philox4x32-10 with
+ * counter {s, stream, 0, 0} and key {seed lo, seed hi} per site, Box-Muller
+ * for beta != 0, uniform phases for beta = 0.  Pinned by the published
+ * Random123 known-answer vectors and by distribution tests. */
+void orc_philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1);
+void orc_philox_gauge_u1(orc_c* u, int lx, int ly, double beta, uint64_t seed, uint32_t stream);
 
 /* ---- gradient of S = Re phi^dag (D^dag D)^-1 phi (no reference counterpart):
  *      FP64 CPU gradient from chi = (D^dag D)^-1 phi, psi = D chi. ---- */

@@ -148,6 +148,8 @@ def orc_lib():
         lib.orc_gauss_gauge_u1.argtypes = [vp, i, i, C.POINTER(OrcMT), d]
         lib.orc_rand_trans_u1.argtypes = [vp, i, i, C.POINTER(OrcMT)]
         lib.orc_apply_gauge_trans_u1.argtypes = [vp, vp, i, i]
+        lib.orc_philox4x32_10.argtypes = [vp, C.c_uint32, C.c_uint32]
+        lib.orc_philox_gauge_u1.argtypes = [vp, i, i, d, C.c_uint64, C.c_uint32]
         lib.orc_plaquette_u1.restype = C2
         lib.orc_plaquette_u1.argtypes = [vp, i, i]
         lib.orc_read_gauge_u1.restype = i
@@ -484,6 +486,20 @@ class Rng:
     def canonical(self) -> float:
         assert self.impl == "orc"
         return orc_lib().orc_canonical(C.byref(self.mt))
+
+
+def philox4x32_10(counter, key):
+    """Philox4x32-10 block: four 32-bit counter words, two key words."""
+    c = np.array(counter, dtype=np.uint32)
+    orc_lib().orc_philox4x32_10(c.ctypes.data, int(key[0]), int(key[1]))
+    return [int(v) for v in c]
+
+
+def philox_gauge(lx, ly, beta, seed, stream=0):
+    """The library's device field generator, restated on the CPU."""
+    u = np.zeros(2 * lx * ly, np.complex128)
+    orc_lib().orc_philox_gauge_u1(u.ctypes.data, lx, ly, float(beta), int(seed), int(stream))
+    return u
 
 
 def unit_gauge(lx, ly):

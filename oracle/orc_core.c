@@ -313,3 +313,44 @@ int orc_write_gauge_u1(const orc_c* u, int lx, int ly, const char* path)
   fclose(f);
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and the device field     */
+/* generator built on it (see orc.h).                                          */
+void orc_philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = (uint32_t)p1;
+    c[2] = n2;
+    c[3] = (uint32_t)p0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+void orc_philox_gauge_u1(orc_c* u, int lx, int ly, double beta, uint64_t seed, uint32_t stream)
+{
+  const double two_m53 = 1.1102230246251565404e-16, two_pi = 6.283185307179586232, pi = 3.141592653589793116;
+  if (beta < 0) beta = -beta;
+  const double sigma = beta > 0 ? 1.0 / sqrt(beta) : 0.0;
+  for (int s = 0; s < lx * ly; s++) {
+    uint32_t c[4] = {(uint32_t)s, stream, 0u, 0u};
+    orc_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double a = (double)((((uint64_t)c[1] << 32) | c[0]) >> 11) * two_m53;
+    double b = (double)((((uint64_t)c[3] << 32) | c[2]) >> 11) * two_m53;
+    double t0, t1;
+    if (sigma > 0) {
+      double r = sigma * sqrt(-2.0 * log(1.0 - a));
+      t0 = r * cos(two_pi * b);
+      t1 = r * sin(two_pi * b);
+    } else {
+      t0 = two_pi * a - pi;
+      t1 = two_pi * b - pi;
+    }
+    u[2 * s] = CMPLX(cos(t0), sin(t0));
+    u[2 * s + 1] = CMPLX(cos(t1), sin(t1));
+  }
+}
