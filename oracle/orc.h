@@ -138,6 +138,36 @@ void orc_stencil_eoprec_prepare(orc_c* rhs_e, orc_c* rhs_orig, void* stencil);  
 void orc_stencil_m2mdeodoe(orc_c* lhs, orc_c* rhs, void* stencil);                       /* operators_stencil.cpp:195-214 */
 void orc_stencil_eoprec_reconstruct(orc_c* lhs_full, orc_c* lhs_e, orc_c* rhs_o, void* stencil); /* :218-236 */
 
+/* ---- stencil_2d, any Nc, up to two links (coarse_stencil.h:14-154,
+ *      coarse_stencil.cpp:12-1610) ----
+ * Coefficient (term t, site s, row r, column c) of a block array is at
+ * ((t V + s) nc + r) nc + c; terms: clover; hopping {+x, +y, -x, -y};
+ * two_link {+2x, +x+y, +2y, -x+y, -2x, -x-y, -2y, +x-y}.  sdir follows the
+ * reference's stencil_dir (0 = all terms, 1 = clover, 2..5 hopping,
+ * 6..13 two-link). */
+enum { ORC_PIECE_ALL = 0, ORC_PIECE_EO = 1, ORC_PIECE_OE = 2, ORC_PIECE_TB = 3, ORC_PIECE_BT = 4 };
+typedef struct {
+  int lx, ly, nc;
+  int has_two;
+  int sdir;
+  orc_c* clover;    /* V nc^2 */
+  orc_c* hopping;   /* 4 V nc^2 */
+  orc_c* two_link;  /* 8 V nc^2 (has_two) */
+  orc_c shift, eo_shift, dof_shift;
+} orc_cstencil;
+/* piece: ALL coarse_stencil.cpp:12-392, EO :395-557, OE :560-722,
+ * TB :725-1117, BT :1120-1512 */
+void orc_cstencil_apply(orc_c* lhs, orc_c* rhs, const orc_cstencil* s, int piece);
+/* callback form of the full stencil (extra = orc_cstencil*) */
+void orc_cstencil_apply_all(orc_c* lhs, orc_c* rhs, void* s);
+/* operators_stencil.cpp:179-236 on any Nc (index_is_even of the full index) */
+void orc_cstencil_eoprec_prepare(orc_c* rhs_e, orc_c* rhs_orig, const orc_cstencil* s);
+void orc_cstencil_m2mdeodoe(orc_c* lhs, orc_c* rhs, const orc_cstencil* s);
+void orc_cstencil_eoprec_reconstruct(orc_c* lhs_full, orc_c* lhs_e, orc_c* rhs_o, const orc_cstencil* s);
+/* generate_stencil_2d (coarse_stencil.cpp:1515-1627): probe mv with unit
+ * vectors and accumulate (+=) into the coefficient arrays */
+void orc_cstencil_generate(orc_cstencil* s, void (*mv)(orc_c*, orc_c*, void*), void* extra);
+
 /* ---- Krylov solvers (generic_cg.cpp, generic_cg_m.cpp, generic_bicgstab.cpp,
  *      generic_gcr.cpp).  `hist`, if non-null, receives the iterated relative
  *      residual after each iteration (max_iter entries). ---- */

@@ -72,6 +72,12 @@ class OrcStencil(C.Structure):
                 ("shift", C.c_double * 2), ("eo_shift", C.c_double * 2), ("dof_shift", C.c_double * 2)]
 
 
+class OrcCStencil(C.Structure):
+    _fields_ = [("lx", C.c_int), ("ly", C.c_int), ("nc", C.c_int), ("has_two", C.c_int), ("sdir", C.c_int),
+                ("clover", C.c_void_p), ("hopping", C.c_void_p), ("two_link", C.c_void_p),
+                ("shift", C.c_double * 2), ("eo_shift", C.c_double * 2), ("dof_shift", C.c_double * 2)]
+
+
 class C2(C.Structure):
     """double _Complex returned by value (two SSE registers)."""
     _fields_ = [("re", C.c_double), ("im", C.c_double)]
@@ -166,6 +172,11 @@ def orc_lib():
         lib.orc_staggered_eoprec_prepare.argtypes = [vp, vp, vp]
         lib.orc_staggered_eoprec_reconstruct.argtypes = [vp, vp, vp, vp]
         lib.orc_stencil_eoprec_prepare.argtypes = [vp, vp, vp]
+        lib.orc_cstencil_apply.argtypes = [vp, vp, vp, i]
+        lib.orc_cstencil_eoprec_prepare.argtypes = [vp, vp, vp]
+        lib.orc_cstencil_m2mdeodoe.argtypes = [vp, vp, vp]
+        lib.orc_cstencil_eoprec_reconstruct.argtypes = [vp, vp, vp, vp]
+        lib.orc_cstencil_generate.argtypes = [vp, vp, vp]
         lib.orc_stencil_eoprec_reconstruct.argtypes = [vp, vp, vp, vp]
         lib.orc_staggered_force.argtypes = [vp, vp, vp, C.POINTER(OrcStagOp)]
         lib.orc_staggered_dS_dm.restype = d
@@ -208,6 +219,10 @@ def ref_lib():
         lib.ref_read_gauge_u1.argtypes = [vp, i, i, C.c_char_p]
         lib.ref_write_gauge_u1.argtypes = [vp, i, i, C.c_char_p]
         lib.ref_ape_smear_u1.argtypes = [vp, vp, i, i, d, i]
+        lib.ref_cstencil_apply.argtypes = [i, i, i, i, i, i, vp, vp, vp, vp, vp, vp, vp]
+        lib.ref_cstencil_apply.restype = i
+        lib.ref_cstencil_generate.argtypes = [i, vp, d, i, i, i, i, vp, vp, vp, vp, i, vp, vp, vp]
+        lib.ref_cstencil_generate.restype = i
         lib.ref_topo_u1.argtypes = [vp, i, i]
         lib.ref_topo_u1.restype = d
         _ref = lib
@@ -577,3 +592,132 @@ def lattice_coord(dims, nc, i):
 def lattice_is_even(dims, nc, i):
     d = np.ascontiguousarray(dims, dtype=np.int32)
     return bool(orc_lib().orc_index_is_even(d.ctypes.data, nc, i))
+
+
+# --------------------------------------------------------------------------
+# stencil_2d for any Nc, up to two links (coarse_stencil.cpp:12-1627).
+PIECES = {"all": 0, "eo": 1, "oe": 2, "tb": 3, "bt": 4}
+SDIR = {"all": 0, "clover": 1, "+x": 2, "+y": 3, "-x": 4, "-y": 5, "+2x": 6, "+x+y": 7, "+2y": 8, "-x+y": 9,
+        "-2x": 10, "-x-y": 11, "-2y": 12, "+x-y": 13}
+
+
+class CStencil:
+    """Coefficients of a general stencil_2d in the reference layout:
+    clover (V nc nc), hopping (4 V nc nc), two_link (8 V nc nc or None);
+    element (term, site, row, column) at ((term V + site) nc + row) nc + column."""
+
+    def __init__(self, lx, ly, nc, clover=None, hopping=None, two_link=None, shift=0j, eo_shift=0j, dof_shift=0j,
+                 sdir=0, has_two=None):
+        self.lx, self.ly, self.nc = lx, ly, nc
+        blk = lx * ly * nc * nc
+        self.has_two = (two_link is not None) if has_two is None else bool(has_two)
+        z = lambda k: np.zeros(k * blk, np.complex128)
+        self.clover = z(1) if clover is None else np.ascontiguousarray(clover, dtype=np.complex128)
+        self.hopping = z(4) if hopping is None else np.ascontiguousarray(hopping, dtype=np.complex128)
+        self.two_link = (z(8) if two_link is None else np.ascontiguousarray(two_link, dtype=np.complex128)) \
+            if self.has_two else None
+        self.shift, self.eo_shift, self.dof_shift = complex(shift), complex(eo_shift), complex(dof_shift)
+        self.sdir = SDIR[sdir] if isinstance(sdir, str) else int(sdir)
+
+    @classmethod
+    def random(cls, rng, lx, ly, nc, has_two=False, shifts=(0j, 0j, 0j), sdir=0):
+        blk = lx * ly * nc * nc
+        g = lambda k: rng.standard_normal(k * blk) + 1j * rng.standard_normal(k * blk)
+        return cls(lx, ly, nc, g(1), g(4), g(8) if has_two else None, *shifts, sdir=sdir)
+
+    def shifts6(self):
+        return np.array([self.shift.real, self.shift.imag, self.eo_shift.real, self.eo_shift.imag,
+                         self.dof_shift.real, self.dof_shift.imag])
+
+    def struct(self):
+        st = OrcCStencil()
+        st.lx, st.ly, st.nc, st.has_two, st.sdir = self.lx, self.ly, self.nc, int(self.has_two), self.sdir
+        st.clover, st.hopping = self.clover.ctypes.data, self.hopping.ctypes.data
+        st.two_link = self.two_link.ctypes.data if self.has_two else None
+        st.shift[:] = [self.shift.real, self.shift.imag]
+        st.eo_shift[:] = [self.eo_shift.real, self.eo_shift.imag]
+        st.dof_shift[:] = [self.dof_shift.real, self.dof_shift.imag]
+        return st
+
+    @property
+    def n(self):
+        return self.lx * self.ly * self.nc
+
+
+def _ref_cstencil(piece, st, lhs, rhs, aux=None):
+    rc = ref_lib().ref_cstencil_apply(piece, st.lx, st.ly, st.nc, int(st.has_two), st.sdir, st.clover.ctypes.data,
+                                      st.hopping.ctypes.data, st.two_link.ctypes.data if st.has_two else None,
+                                      st.shifts6().ctypes.data, lhs.ctypes.data, rhs.ctypes.data,
+                                      None if aux is None else aux.ctypes.data)
+    assert rc == 0
+
+
+def cstencil_apply(st, rhs, piece="all", impl="orc", lhs=None):
+    """lhs = piece of the stencil applied to rhs; `lhs` (if given) is the
+    initial content, which some sdir/piece combinations leave untouched."""
+    rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+    out = np.zeros_like(rhs) if lhs is None else np.array(lhs, dtype=np.complex128, copy=True)
+    if impl == "ref":
+        _ref_cstencil(PIECES[piece], st, out, rhs)
+    else:
+        s = st.struct()
+        orc_lib().orc_cstencil_apply(out.ctypes.data, rhs.ctypes.data, C.byref(s), PIECES[piece])
+    return out
+
+
+def cstencil_m2mdeodoe(st, rhs, impl="orc"):
+    rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+    out = np.zeros_like(rhs)
+    if impl == "ref":
+        _ref_cstencil(5, st, out, rhs)
+    else:
+        s = st.struct()
+        orc_lib().orc_cstencil_m2mdeodoe(out.ctypes.data, rhs.ctypes.data, C.byref(s))
+    return out
+
+
+def cstencil_eoprec_prepare(st, rhs, impl="orc"):
+    rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+    out = np.zeros_like(rhs)
+    if impl == "ref":
+        _ref_cstencil(6, st, out, rhs)
+    else:
+        s = st.struct()
+        orc_lib().orc_cstencil_eoprec_prepare(out.ctypes.data, rhs.ctypes.data, C.byref(s))
+    return out
+
+
+def cstencil_eoprec_reconstruct(st, lhs_e, rhs_o, impl="orc"):
+    lhs_e = np.ascontiguousarray(lhs_e, dtype=np.complex128)
+    rhs_o = np.ascontiguousarray(rhs_o, dtype=np.complex128)
+    out = np.zeros_like(lhs_e)
+    if impl == "ref":
+        _ref_cstencil(7, st, out, lhs_e, rhs_o)
+    else:
+        s = st.struct()
+        orc_lib().orc_cstencil_eoprec_reconstruct(out.ctypes.data, lhs_e.ctypes.data, rhs_o.ctypes.data, C.byref(s))
+    return out
+
+
+def cstencil_generate(lx, ly, nc, has_two, op=None, links=None, mass=0.0, src=None, impl="orc"):
+    """generate_stencil_2d: probe an operator (a function-form op by name, or
+    another general stencil `src`) with unit vectors."""
+    out = CStencil(lx, ly, nc, two_link=np.zeros(8 * lx * ly * nc * nc, np.complex128) if has_two else None)
+    if impl == "ref":
+        lk = None if links is None else np.ascontiguousarray(links, dtype=np.complex128)
+        z6 = np.zeros(6)
+        ref_lib().ref_cstencil_generate(
+            OPS[op] if op is not None else -1, None if lk is None else lk.ctypes.data, mass, lx, ly, nc, int(has_two),
+            None if src is None else src.clover.ctypes.data, None if src is None else src.hopping.ctypes.data,
+            None if src is None or not src.has_two else src.two_link.ctypes.data,
+            (z6 if src is None else src.shifts6()).ctypes.data, 0 if src is None else int(src.has_two),
+            out.clover.ctypes.data, out.hopping.ctypes.data, out.two_link.ctypes.data if has_two else None)
+        return out
+    s = out.struct()
+    if src is not None:
+        ss = src.struct()
+        orc_lib().orc_cstencil_generate(C.byref(s), C.cast(orc_lib().orc_cstencil_apply_all, C.c_void_p), C.byref(ss))
+    else:
+        ctx = OpContext(op, links, mass, lx, ly, nc)
+        orc_lib().orc_cstencil_generate(C.byref(s), ctx.fn, ctx.extra)
+    return out

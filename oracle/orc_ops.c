@@ -325,6 +325,135 @@ void orc_stencil_eoprec_reconstruct(orc_c* lhs_full, orc_c* lhs_e, orc_c* rhs_o,
 }
 
 /* ------------------------------------------------------------------------ */
+/* stencil_2d for any Nc (coarse_stencil.cpp:12-1627).                          */
+/* term offsets (dx, dy): clover, hopping 0..3, two-link 0..7 */
+static const int cst_off[13][2] = {{0, 0},  {1, 0},  {0, 1},   {-1, 0}, {0, -1}, {2, 0}, {1, 1},
+                                   {0, 2},  {-1, 1}, {-2, 0},  {-1, -1}, {0, -2}, {1, -1}};
+
+static const orc_c* cst_block(const orc_cstencil* s, int t)
+{
+  const size_t blk = (size_t)s->lx * s->ly * s->nc * s->nc;
+  if (t == 0) return s->clover;
+  if (t < 5) return s->hopping + (size_t)(t - 1) * blk;
+  return s->two_link + (size_t)(t - 5) * blk;
+}
+
+/* Which terms a piece applies for the stencil's sdir (see orc.h). */
+static int cst_term_on(const orc_cstencil* s, int piece, int t)
+{
+  const int eo = piece == ORC_PIECE_EO || piece == ORC_PIECE_OE;
+  if (t >= 5 && !s->has_two) return 0;
+  if (s->sdir == 0) return eo ? (t >= 1 && t <= 4) : 1;
+  if (s->sdir == 1) return !eo && t == 0;
+  if (s->sdir <= 5) return t == s->sdir - 1;
+  return !eo && t == s->sdir - 1;
+}
+
+void orc_cstencil_apply(orc_c* lhs, orc_c* rhs, const orc_cstencil* s, int piece)
+{
+  const int lx = s->lx, ly = s->ly, nc = s->nc;
+  const int dims[2] = {lx, ly};
+  const int n = lx * ly * nc;
+  /* a two-link direction without two-link terms leaves lhs as it was
+   * (coarse_stencil.cpp:264-391 and the tb/bt copies) */
+  if (s->sdir >= 6 && !s->has_two && (piece == ORC_PIECE_ALL || piece == ORC_PIECE_TB || piece == ORC_PIECE_BT))
+    return;
+  const int c_lo = piece == ORC_PIECE_TB ? nc / 2 : 0;
+  const int c_hi = piece == ORC_PIECE_BT ? nc / 2 : nc;
+  const int shifts = piece == ORC_PIECE_ALL && (s->sdir == 0 || s->sdir == 1);
+  for (int i = 0; i < n; i++) {
+    int coord[2], color;
+    orc_index_to_coord(dims, nc, i, coord, &color);
+    const int even = orc_index_is_even(dims, nc, i);
+    orc_c acc = CMPLX(0.0, 0.0);
+    int row_on = 1;
+    if (piece == ORC_PIECE_EO) row_on = even;
+    if (piece == ORC_PIECE_OE) row_on = !even;
+    if (piece == ORC_PIECE_TB) row_on = color < nc / 2;
+    if (piece == ORC_PIECE_BT) row_on = color >= nc / 2;
+    if (row_on) {
+      for (int t = 0; t < 13; t++) {
+        if (!cst_term_on(s, piece, t)) continue;
+        int nb[2];
+        nb[0] = ((coord[0] + cst_off[t][0]) % lx + lx) % lx;
+        nb[1] = ((coord[1] + cst_off[t][1]) % ly + ly) % ly;
+        const orc_c* blk = cst_block(s, t);
+        for (int c = c_lo; c < c_hi; c++)
+          acc = orc_add(acc, orc_cmul(blk[(size_t)c + (size_t)nc * i], rhs[orc_coord_to_index(dims, nc, nb, c)]));
+      }
+      if (shifts) {
+        if (cabs(s->shift) != 0.0) acc = orc_add(acc, orc_cmul(s->shift, rhs[i]));
+        if (cabs(s->eo_shift) != 0.0) acc = orc_add(acc, orc_cmul(orc_rmul(even ? 1.0 : -1.0, s->eo_shift), rhs[i]));
+        if (cabs(s->dof_shift) != 0.0)
+          acc = orc_add(acc, orc_cmul(orc_rmul(color < nc / 2 ? 1.0 : -1.0, s->dof_shift), rhs[i]));
+      }
+    }
+    lhs[i] = acc;
+  }
+}
+
+void orc_cstencil_apply_all(orc_c* lhs, orc_c* rhs, void* s) { orc_cstencil_apply(lhs, rhs, (const orc_cstencil*)s, ORC_PIECE_ALL); }
+
+void orc_cstencil_eoprec_prepare(orc_c* rhs_e, orc_c* rhs_orig, const orc_cstencil* s)
+{
+  const int dims[2] = {s->lx, s->ly}, n = s->lx * s->ly * s->nc;
+  orc_cstencil_apply(rhs_e, rhs_orig, s, ORC_PIECE_EO);
+  for (int i = 0; i < n; i++)
+    if (orc_index_is_even(dims, s->nc, i)) rhs_e[i] = orc_sub(orc_cmul(s->shift, rhs_orig[i]), rhs_e[i]);
+}
+void orc_cstencil_m2mdeodoe(orc_c* lhs, orc_c* rhs, const orc_cstencil* s)
+{
+  const int dims[2] = {s->lx, s->ly}, n = s->lx * s->ly * s->nc;
+  orc_c* tmp = (orc_c*)malloc(sizeof(orc_c) * (size_t)n);
+  orc_cstencil_apply(tmp, rhs, s, ORC_PIECE_OE);
+  orc_cstencil_apply(lhs, tmp, s, ORC_PIECE_EO);
+  for (int i = 0; i < n; i++)
+    if (orc_index_is_even(dims, s->nc, i)) lhs[i] = orc_sub(orc_cmul(orc_cmul(s->shift, s->shift), rhs[i]), lhs[i]);
+  free(tmp);
+}
+void orc_cstencil_eoprec_reconstruct(orc_c* lhs_full, orc_c* lhs_e, orc_c* rhs_o, const orc_cstencil* s)
+{
+  const int dims[2] = {s->lx, s->ly}, n = s->lx * s->ly * s->nc;
+  const double inv_mass = 1.0 / creal(s->shift);
+  orc_cstencil_apply(lhs_full, lhs_e, s, ORC_PIECE_OE);
+  for (int i = 0; i < n; i++) {
+    if (!orc_index_is_even(dims, s->nc, i))
+      lhs_full[i] = orc_rmul(inv_mass, orc_sub(rhs_o[i], lhs_full[i]));
+    else
+      lhs_full[i] = lhs_e[i];
+  }
+}
+
+void orc_cstencil_generate(orc_cstencil* s, void (*mv)(orc_c*, orc_c*, void*), void* extra)
+{
+  const int lx = s->lx, ly = s->ly, nc = s->nc, dims[2] = {lx, ly};
+  const int n = lx * ly * nc;
+  orc_c* e = (orc_c*)malloc(sizeof(orc_c) * (size_t)n);
+  orc_c* col = (orc_c*)malloc(sizeof(orc_c) * (size_t)n);
+  const int nterms = s->has_two ? 13 : 5;
+  for (int i = 0; i < n; i++) {
+    int p[2], a;
+    orc_index_to_coord(dims, nc, i, p, &a);
+    for (int k = 0; k < n; k++) e[k] = CMPLX(0.0, 0.0);
+    e[i] = CMPLX(1.0, 0.0);
+    for (int k = 0; k < n; k++) col[k] = CMPLX(0.0, 0.0);
+    mv(col, e, extra);
+    for (int c = 0; c < nc; c++)
+      for (int t = 0; t < nterms; t++) {
+        /* the site whose term-t neighbour is p */
+        int q[2];
+        q[0] = ((p[0] - cst_off[t][0]) % lx + lx) % lx;
+        q[1] = ((p[1] - cst_off[t][1]) % ly + ly) % ly;
+        const int row = orc_coord_to_index(dims, nc, q, c);
+        orc_c* blk = (orc_c*)cst_block(s, t);
+        blk[(size_t)a + (size_t)nc * row] = orc_add(blk[(size_t)a + (size_t)nc * row], col[row]);
+      }
+  }
+  free(e);
+  free(col);
+}
+
+/* ------------------------------------------------------------------------ */
 /* Gradient of S = phi^dag (D^dag D)^-1 phi with D = m - K,
  * K psi(x) = 1/2 sum_mu eta_mu(x) [U_mu(x) psi(x+mu) - U_mu(x-mu)^* psi(x-mu)].
  * With chi = (D^dag D)^-1 phi and psi = D chi, dS = -2 Re psi^dag (dD) chi, and

@@ -198,6 +198,72 @@ int ref_stencil_coeffs(int opid, double* clover, double* hopping, double* shifts
   return 0;
 }
 
+// General stencil_2d (any Nc, two-link, sdir): a Lattice and stencil_2d built
+// from caller coefficient arrays in the reference layout.
+struct CStencil {
+  Lattice* lat;
+  stencil_2d* st;
+  CStencil(int lx, int ly, int nc, int has_two, int sdir, const double* clover, const double* hopping,
+           const double* two, const double* shifts6) {
+    int dims[2] = {lx, ly};
+    lat = new Lattice(2, dims, nc);
+    st = new stencil_2d(lat, has_two ? 2 : 1, cd(shifts6[0], shifts6[1]), cd(shifts6[2], shifts6[3]),
+                        cd(shifts6[4], shifts6[5]));
+    const size_t blk = size_t(lx) * ly * nc * nc;
+    if (clover) std::memcpy(st->clover, clover, sizeof(cd) * blk);
+    if (hopping) std::memcpy(st->hopping, hopping, sizeof(cd) * 4 * blk);
+    if (has_two && two) std::memcpy(st->two_link, two, sizeof(cd) * 8 * blk);
+    st->sdir = static_cast<stencil_dir>(sdir);
+  }
+  ~CStencil() {
+    delete st;
+    delete lat;
+  }
+};
+
+// piece: 0 all, 1 eo, 2 oe, 3 tb, 4 bt, 5 m2mdeodoe, 6 eoprec prepare,
+// 7 eoprec reconstruct (aux = rhs_o, rhs = lhs_e)
+int ref_cstencil_apply(int piece, int lx, int ly, int nc, int has_two, int sdir, const double* clover,
+                       const double* hopping, const double* two, const double* shifts6, double* lhs, double* rhs,
+                       double* aux) {
+  CStencil c(lx, ly, nc, has_two, sdir, clover, hopping, two, shifts6);
+  cd* L = reinterpret_cast<cd*>(lhs);
+  cd* R = reinterpret_cast<cd*>(rhs);
+  switch (piece) {
+    case 0: apply_stencil_2d(L, R, c.st); break;
+    case 1: apply_stencil_2d_eo(L, R, c.st); break;
+    case 2: apply_stencil_2d_oe(L, R, c.st); break;
+    case 3: apply_stencil_2d_tb(L, R, c.st); break;
+    case 4: apply_stencil_2d_bt(L, R, c.st); break;
+    case 5: apply_square_staggered_m2mdeodoe_stencil(L, R, c.st); break;
+    case 6: apply_square_staggered_eoprec_prepare_stencil(L, R, c.st); break;
+    case 7: apply_square_staggered_eoprec_reconstruct_stencil(L, R, reinterpret_cast<cd*>(aux), c.st); break;
+    default: return -1;
+  }
+  return 0;
+}
+
+// generate_stencil_2d from one of the function-form operators (nc = 1 links)
+// or from another general stencil (src_* arrays, same lattice and nc).
+int ref_cstencil_generate(int opid, double* links, double mass, int lx, int ly, int nc, int has_two,
+                          const double* src_clover, const double* src_hopping, const double* src_two,
+                          const double* src_shifts6, int src_has_two, double* clover, double* hopping, double* two) {
+  const double zero6[6] = {0, 0, 0, 0, 0, 0};
+  CStencil out(lx, ly, nc, has_two, 0, nullptr, nullptr, nullptr, zero6);
+  if (opid >= 0) {
+    Ctx c(links, mass, lx, ly, nc, opid);
+    generate_stencil_2d(out.st, complex_op(opid), c.extra(opid));
+  } else {
+    CStencil src(lx, ly, nc, src_has_two, 0, src_clover, src_hopping, src_two, src_shifts6);
+    generate_stencil_2d(out.st, apply_stencil_2d, src.st);
+  }
+  const size_t blk = size_t(lx) * ly * nc * nc;
+  std::memcpy(clover, out.st->clover, sizeof(cd) * blk);
+  std::memcpy(hopping, out.st->hopping, sizeof(cd) * 4 * blk);
+  if (has_two) std::memcpy(two, out.st->two_link, sizeof(cd) * 8 * blk);
+  return 0;
+}
+
 // solver: 0 CG, 1 BiCGStab, 2 GCR.  restart_freq > 0 selects the _restart form.
 // info = {resSq, iter, success, ops_count}.  Verbosity lines go to `out`.
 int ref_solve(int solver, int opid, double* phi, double* phi0, int size, int max_iter, double eps, int restart_freq,

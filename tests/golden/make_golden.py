@@ -25,6 +25,10 @@ OPS = ["laplace", "laplace_u1", "staggered", "staggered_dagger", "staggered_norm
        "deo", "doe", "m2mdeodoe", "stencil", "stencil_dagger", "stencil_gamma5", "stencil_eo", "stencil_oe",
        "stencil_m2mdeodoe"]
 
+# (piece, sdir) pairs of the general-stencil fixtures
+CST_CASES = [("all", 0), ("all", 1), ("all", 3), ("all", 7), ("eo", 0), ("oe", 4), ("eo", 9), ("tb", 0), ("bt", 0),
+             ("tb", 12), ("bt", 1)]
+
 
 def main():
     if not orc.ref_available():
@@ -72,6 +76,30 @@ def main():
     for w in ("dot", "norm2sq", "diffnorm2sq"):
         arrays[f"blas_{w}_c"] = np.array([orc.blas(w, v, v[::-1].copy(), impl="ref")])
         arrays[f"blas_{w}_r"] = np.array([orc.blas(w, vr, vr[::-1].copy(), impl="ref")])
+
+    # general stencil_2d: nc = 3 with two-link terms on 5x4, coefficients and
+    # vectors drawn from std::mt19937(13)
+    g = orc.Rng(13, "ref")
+    clx, cly, cnc = 5, 4, 3
+    blk = clx * cly * cnc * cnc
+    cst = orc.CStencil(clx, cly, cnc, g.gaussian_complex(blk), g.gaussian_complex(4 * blk),
+                       g.gaussian_complex(8 * blk), 0.3 + 0.1j, -0.2j, 0.7)
+    cv = g.gaussian_complex(clx * cly * cnc)
+    cw = g.gaussian_complex(clx * cly * cnc)
+    del g
+    arrays["cst_clover"], arrays["cst_hopping"], arrays["cst_two"] = cst.clover, cst.hopping, cst.two_link
+    arrays["cst_shifts"], arrays["cst_in"], arrays["cst_in2"] = cst.shifts6(), cv, cw
+    for piece, sdir in CST_CASES:
+        cst.sdir = sdir
+        arrays[f"cst_out_{piece}_{sdir}"] = orc.cstencil_apply(cst, cv, piece, impl="ref", lhs=cw)
+    cst.sdir = 0
+    arrays["cst_m2mdeodoe"] = orc.cstencil_m2mdeodoe(cst, cv, impl="ref")
+    arrays["cst_prepare"] = orc.cstencil_eoprec_prepare(cst, cv, impl="ref")
+    arrays["cst_reconstruct"] = orc.cstencil_eoprec_reconstruct(cst, cv, cw, impl="ref")
+    gen = orc.cstencil_generate(6, 5, 1, True, op="staggered_normal", links=orc.Rng(5, "ref").gauss_gauge(6, 5, 2.0),
+                                mass=0.2, impl="ref")
+    arrays["cst_gen_links"] = orc.Rng(5, "ref").gauss_gauge(6, 5, 2.0)
+    arrays["cst_gen_clover"], arrays["cst_gen_hopping"], arrays["cst_gen_two"] = gen.clover, gen.hopping, gen.two_link
 
     # solves (16x16 unless stated), with DETAIL verbosity text
     g = orc.Rng(11, "ref")
