@@ -27,6 +27,9 @@ enum op_type {
   STAGGERED_INDEX = 5,
 };
 
+// maximum number of hops of an operator's stencil (operators.cpp:8-23)
+int get_stencil_size(op_type opt);
+
 void square_laplace(complex<double>* lhs, complex<double>* rhs, void* extra_data);
 void square_laplace_u1(complex<double>* lhs, complex<double>* rhs, void* extra_data);
 void square_staggered_u1(complex<double>* lhs, complex<double>* rhs, void* extra_data);

@@ -10,15 +10,19 @@
 // restart wrappers do, and every verbosity line is reproduced
 // (verbosity.h:73-129).
 #include "../../include/glx.h"
+#include "glx_internal.h"
 #include "glx_solvers.h"
 
+#include <algorithm>
 #include <chrono>
+#include <complex>
 #include <cmath>
 #include <cstring>
 #include <iostream>
 #include <memory>
 #include <sstream>
 #include <unordered_map>
+#include <vector>
 
 using namespace glx;
 
@@ -126,6 +130,7 @@ struct RegEntry {
   int kind;
   bool has_params;
   glx_staggered_u1_op params;
+  int (*translate)(void*, glx_stencil_2d_nc*) = nullptr;  // general stencils given in another form
 };
 struct Registry {
   std::mutex mu;
@@ -134,9 +139,9 @@ struct Registry {
     static Registry* r = new Registry;
     return *r;
   }
-  void add(void* f, int kind, const glx_staggered_u1_op* p) {
+  void add(void* f, int kind, const glx_staggered_u1_op* p, int (*tr)(void*, glx_stencil_2d_nc*) = nullptr) {
     std::lock_guard<std::mutex> g(mu);
-    RegEntry e{kind, p != nullptr, {}};
+    RegEntry e{kind, p != nullptr, {}, tr};
     if (p) e.params = *p;
     map[f] = e;
   }
@@ -174,15 +179,77 @@ struct BuiltinInit {
     r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_eo), OP_STENCIL_EO, nullptr);
     r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_oe), OP_STENCIL_OE, nullptr);
     r.add(reinterpret_cast<void*>(&glx_apply_square_staggered_m2mdeodoe_stencil), OP_STENCIL_M2MDEODOE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc), OP_CST, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc_eo), OP_CST_EO, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc_oe), OP_CST_OE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc_tb), OP_CST_TB, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc_bt), OP_CST_BT, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc_m2mdeodoe), OP_CST_M2MDEODOE, nullptr);
   }
 };
 BuiltinInit g_builtin_init;
+}  // namespace
+
+void glx::register_cst_callback(void* fn, int kind, int (*translate)(void*, glx_stencil_2d_nc*)) {
+  if (!is_cst_kind(kind)) throw ArgError("register_cst_callback: not a general-stencil kind");
+  Registry::get().add(fn, kind, nullptr, translate);
+}
+
+namespace {
 
 // A device operator together with the device storage it points into.
 struct Resolved {
   DevOperator op;
-  DevBuf links, clover, hop;
+  DevBuf links, clover, hop, cs, cstmp;
 };
+
+// General stencil: checks the description, uploads the coefficients, and
+// transposes them into the device layout (glx_cstencil.h).  Fills the
+// operator's lattice, shifts and coefficient fields (not its kind).
+void cst_upload(const glx_stencil_2d_nc& st, DevOperator& o, DevBuf& coef, DevBuf& tmp) {
+  if (st.lx <= 0 || st.ly <= 0 || st.nc <= 0) throw ArgError("stencil dimensions and nc must be positive");
+  if (double(st.lx) * st.ly * st.nc > 2147483647.0) throw ArgError("stencil vector length must fit an int");
+  if (!st.clover || !st.hopping) throw ArgError("stencil needs clover and hopping coefficients");
+  if (st.has_two && !st.two_link) throw ArgError("has_two is set but two_link is NULL");
+  if (st.sdir < 0 || st.sdir > 13) throw ArgError("sdir must be a stencil_dir (0..13)");
+  const int V = st.lx * st.ly, nterms = st.has_two ? 13 : 5;
+  const size_t blk = size_t(V) * st.nc * st.nc * 16;
+  {
+    DevBuf stage(nterms * blk);
+    char* d = static_cast<char*>(stage.get());
+    GLX_HIP(hipMemcpy(d, st.clover, blk, hipMemcpyHostToDevice));
+    GLX_HIP(hipMemcpy(d + blk, st.hopping, 4 * blk, hipMemcpyHostToDevice));
+    if (st.has_two) GLX_HIP(hipMemcpy(d + 5 * blk, st.two_link, 8 * blk, hipMemcpyHostToDevice));
+    coef = DevBuf(nterms * blk);
+    hipLaunchKernelGGL(k_cstencil_transpose<0>, dim3(kMaxPointBlocks), dim3(kBlock), 0, nullptr, V, st.nc, nterms,
+                       stage.as<c128>(), coef.as<c128>());
+    GLX_CHECK_LAUNCH();
+    GLX_HIP(hipDeviceSynchronize());
+  }
+  tmp = DevBuf(size_t(V) * st.nc * 16);
+  o.lx = st.lx;
+  o.ly = st.ly;
+  o.nc = st.nc;
+  o.shift = {st.shift[0], st.shift[1]};
+  o.eo_shift = {st.eo_shift[0], st.eo_shift[1]};
+  o.dof_shift = {st.dof_shift[0], st.dof_shift[1]};
+  o.cs_coef = coef.as<c128>();
+  o.cs_tmp = tmp.as<c128>();
+  o.cs_sdir = st.sdir;
+  o.cs_has_two = st.has_two != 0;
+}
+
+// The general-stencil description behind a registered callback's extra_info.
+glx_stencil_2d_nc cst_desc(const RegEntry& e, void* extra) {
+  glx_stencil_2d_nc st{};
+  if (e.translate) {
+    if (e.translate(extra, &st) != 0) throw ArgError("stencil description could not be read");
+  } else {
+    if (!extra) throw ArgError("general stencil operator needs a glx_stencil_2d_nc");
+    st = *static_cast<const glx_stencil_2d_nc*>(extra);
+  }
+  return st;
+}
 
 size_t elem_bytes(bool cplx) { return cplx ? 16 : 8; }
 
@@ -206,7 +273,9 @@ std::unique_ptr<Resolved> resolve(glx_matvec mv, void* extra, int size, bool cpl
   if (real_kind(e.kind) == cplx) throw ArgError("operator and vector types do not match (real vs complex)");
   DevOperator& o = R->op;
   o.kind = e.kind;
-  if (is_stencil_kind(e.kind)) {
+  if (is_cst_kind(e.kind)) {
+    cst_upload(cst_desc(e, extra), o, R->cs, R->cstmp);
+  } else if (is_stencil_kind(e.kind)) {
     const glx_stencil_2d* st = static_cast<const glx_stencil_2d*>(extra);
     if (!st || !st->hopping || !st->clover) throw ArgError("stencil operator needs a glx_stencil_2d with coefficients");
     o.lx = st->lx;
@@ -487,6 +556,112 @@ void eo_reconstruct(glx_matvec doe_fn, double* full, double* lhs_e, double* rhs_
   if (rc != GLX_OK) std::fprintf(stderr, "glx: eoprec_reconstruct failed: %s\n", g_last_error.c_str());
 }
 
+// General stencil through a callback: a two-link sdir on a stencil without
+// two-link terms leaves lhs as it was (full / tb / bt), as the reference does.
+void cs_host_apply(glx_matvec self, double* lhs, double* rhs, void* extra) {
+  int rc = guarded([&] {
+    RegEntry e;
+    if (!Registry::get().find(reinterpret_cast<void*>(self), e)) throw ArgError("unregistered stencil callback");
+    const glx_stencil_2d_nc st = cst_desc(e, extra);
+    if (cst_piece(e.kind) >= 0 && cs_untouched(cst_piece(e.kind), st.sdir, st.has_two != 0)) return;
+    auto R = resolve(self, extra, -1, true);
+    const size_t bytes = size_t(R->op.size()) * 16;
+    DevVec in(rhs, bytes), out(nullptr, bytes);
+    dev_apply(R->op, out.get(), in.get(), LINKS_F64, 0, nullptr);
+    out.to_host(lhs);
+  });
+  if (rc != GLX_OK) {
+    std::fprintf(stderr, "glx: stencil application failed: %s\n", g_last_error.c_str());
+    RegEntry e;
+    if (Registry::get().find(reinterpret_cast<void*>(self), e)) {
+      glx_stencil_2d_nc st{};
+      bool ok = true;
+      try {
+        st = cst_desc(e, extra);
+      } catch (...) {
+        ok = false;
+      }
+      if (ok && st.lx > 0 && st.ly > 0 && st.nc > 0) {
+        const size_t n = size_t(st.lx) * st.ly * st.nc * 2;
+        for (size_t i = 0; i < n; i++) lhs[i] = std::nan("");
+      }
+    }
+  }
+}
+
+// eo-prec prepare / reconstruct on a general stencil (operators_stencil.cpp:
+// 179-192, 218-236): one stencil launch that also combines.
+void cs_eo_helper(bool prepare, double* out_h, double* a_h, double* b_h, void* extra) {
+  int rc = guarded([&] {
+    auto R = resolve(reinterpret_cast<glx_matvec>(prepare ? &glx_apply_stencil_2d_nc_eo : &glx_apply_stencil_2d_nc_oe),
+                     extra, -1, true);
+    const size_t bytes = size_t(R->op.size()) * 16;
+    DevVec a(a_h, bytes), out(nullptr, bytes);
+    DevVec b(prepare ? nullptr : b_h, prepare ? 16 : bytes);
+    CsDev d = cst_dev(R->op, prepare ? CS_EO : CS_OE);
+    if (prepare) {
+      d.comb_mode = CS_AXMY;
+      d.comb_a = R->op.shift;
+      d.comb = a.buf.as<c128>();
+    } else {
+      d.comb_mode = CS_RECON;
+      d.comb_inv = 1.0 / R->op.shift.re;
+      d.comb = b.buf.as<c128>();
+      d.comb2 = a.buf.as<c128>();
+    }
+    launch_cst_one(nullptr, d, a.buf.as<c128>(), out.buf.as<c128>(), kNoGate);
+    GLX_HIP(hipDeviceSynchronize());
+    out.to_host(out_h);
+  });
+  if (rc != GLX_OK) std::fprintf(stderr, "glx: stencil eoprec helper failed: %s\n", g_last_error.c_str());
+}
+
+// generate_stencil_2d (coarse_stencil.cpp:1515-1627): the operator applied to
+// each unit vector e_i, i = (site p, colour a); entry (q, c) of A e_i is the
+// term-t coefficient (site q, row c, column a) where p = q + off_t.  The
+// per-coefficient additions run in the reference's order.
+void cs_generate(glx_stencil_2d_nc* st, glx_matvec mv, void* extra) {
+  if (!st || !st->clover || !st->hopping || (st->has_two && !st->two_link)) throw ArgError("bad stencil arrays");
+  if (!mv) throw ArgError("matrix_vector is NULL");
+  const int lx = st->lx, ly = st->ly, nc = st->nc;
+  if (lx <= 0 || ly <= 0 || nc <= 0) throw ArgError("stencil dimensions and nc must be positive");
+  const int n = lx * ly * nc, nterms = st->has_two ? 13 : 5;
+  const size_t blk = size_t(lx) * ly * nc * nc;
+  std::vector<std::complex<double>> e(n), col(n);
+  auto R = resolve(mv, extra, n, true);
+  DevBuf de, dcol;
+  if (!R->op.is_host()) {
+    de = DevBuf(size_t(n) * 16);
+    dcol = DevBuf(size_t(n) * 16);
+    GLX_HIP(hipMemset(de.get(), 0, size_t(n) * 16));
+  }
+  const std::complex<double> one(1.0, 0.0), zero(0.0, 0.0);
+  for (int i = 0; i < n; i++) {
+    const int a = i % nc, p = i / nc, px = p % lx, py = p / lx;
+    if (R->op.is_host()) {
+      std::fill(e.begin(), e.end(), zero);
+      e[i] = one;
+      std::fill(col.begin(), col.end(), zero);
+      mv(reinterpret_cast<double*>(col.data()), reinterpret_cast<double*>(e.data()), extra);
+    } else {
+      GLX_HIP(hipMemcpy(de.as<c128>() + i, &one, 16, hipMemcpyHostToDevice));
+      dev_apply(R->op, dcol.get(), de.get(), LINKS_F64, 0, nullptr);
+      GLX_HIP(hipMemcpy(col.data(), dcol.get(), size_t(n) * 16, hipMemcpyDeviceToHost));
+      GLX_HIP(hipMemcpy(de.as<c128>() + i, &zero, 16, hipMemcpyHostToDevice));
+    }
+    for (int c = 0; c < nc; c++)
+      for (int t = 0; t < nterms; t++) {
+        int dx, dy;
+        cs_offset(t, dx, dy);
+        const int qx = ((px - dx) % lx + lx) % lx, qy = ((py - dy) % ly + ly) % ly;
+        const size_t row = size_t(qy * lx + qx) * nc + c;
+        double* base = t == 0 ? st->clover : (t < 5 ? st->hopping + 2 * (t - 1) * blk : st->two_link + 2 * (t - 5) * blk);
+        std::complex<double>* coef = reinterpret_cast<std::complex<double>*>(base);
+        coef[a + nc * row] += col[row];
+      }
+  }
+}
+
 }  // namespace
 
 // =========================================================== lattices ====
@@ -629,6 +804,31 @@ int glx_get_square_staggered_dagger_u1_stencil(glx_stencil_2d* st, const glx_sta
 void glx_apply_stencil_2d(double* lhs, double* rhs, void* st) { host_apply(&glx_apply_stencil_2d, lhs, rhs, st, true); }
 void glx_apply_stencil_2d_eo(double* lhs, double* rhs, void* st) { host_apply(&glx_apply_stencil_2d_eo, lhs, rhs, st, true); }
 void glx_apply_stencil_2d_oe(double* lhs, double* rhs, void* st) { host_apply(&glx_apply_stencil_2d_oe, lhs, rhs, st, true); }
+void glx_apply_stencil_2d_nc(double* lhs, double* rhs, void* st) { cs_host_apply(&glx_apply_stencil_2d_nc, lhs, rhs, st); }
+void glx_apply_stencil_2d_nc_eo(double* lhs, double* rhs, void* st) {
+  cs_host_apply(&glx_apply_stencil_2d_nc_eo, lhs, rhs, st);
+}
+void glx_apply_stencil_2d_nc_oe(double* lhs, double* rhs, void* st) {
+  cs_host_apply(&glx_apply_stencil_2d_nc_oe, lhs, rhs, st);
+}
+void glx_apply_stencil_2d_nc_tb(double* lhs, double* rhs, void* st) {
+  cs_host_apply(&glx_apply_stencil_2d_nc_tb, lhs, rhs, st);
+}
+void glx_apply_stencil_2d_nc_bt(double* lhs, double* rhs, void* st) {
+  cs_host_apply(&glx_apply_stencil_2d_nc_bt, lhs, rhs, st);
+}
+void glx_apply_stencil_2d_nc_m2mdeodoe(double* lhs, double* rhs, void* st) {
+  cs_host_apply(&glx_apply_stencil_2d_nc_m2mdeodoe, lhs, rhs, st);
+}
+void glx_stencil_2d_nc_eoprec_prepare(double* rhs_e, double* rhs_orig, void* st) {
+  cs_eo_helper(true, rhs_e, rhs_orig, nullptr, st);
+}
+void glx_stencil_2d_nc_eoprec_reconstruct(double* lhs_full, double* lhs_e, double* rhs_o, void* st) {
+  cs_eo_helper(false, lhs_full, lhs_e, rhs_o, st);
+}
+int glx_generate_stencil_2d_nc(glx_stencil_2d_nc* st, glx_matvec mv, void* extra) {
+  return guarded([&] { cs_generate(st, mv, extra); });
+}
 void glx_apply_square_staggered_m2mdeodoe_stencil(double* lhs, double* rhs, void* st) {
   host_apply(&glx_apply_square_staggered_m2mdeodoe_stencil, lhs, rhs, st, true);
 }
@@ -1047,6 +1247,82 @@ int glx_dev_time_cg_kernel(glx_lattice lat, double mass, int prec, int reps, dou
     if (!ms_per_launch) throw ArgError("ms_per_launch is NULL");
     DevOperator o = lattice_op(lat, OP_STAG_NORMAL, mass, 1);
     *ms_per_launch = time_cg_kernel(o, prec, reps, bytes_per_site);
+  });
+}
+
+// ------------------------------------------------ general stencil handle --
+struct glx_cstencil_s {
+  glx_stencil_2d_nc desc{};  // dimensions, sdir, has_two, shifts (pointers cleared)
+  DevOperator op;
+  DevBuf coef, tmp;
+};
+namespace {
+DevOperator cst_handle_op(glx_cstencil cs, int kind) {
+  if (!cs) throw ArgError("stencil handle is NULL");
+  if (!is_cst_kind(kind)) throw ArgError("kind must be one of GLX_OP_CSTENCIL*");
+  DevOperator o = cs->op;
+  o.kind = kind;
+  return o;
+}
+}  // namespace
+
+int glx_cstencil_create(glx_cstencil* out, const glx_stencil_2d_nc* st) {
+  return guarded([&] {
+    if (!out || !st) throw ArgError("cs and st must not be NULL");
+    auto h = std::make_unique<glx_cstencil_s>();
+    cst_upload(*st, h->op, h->coef, h->tmp);
+    h->op.kind = OP_CST;
+    h->desc = *st;
+    h->desc.clover = h->desc.hopping = h->desc.two_link = nullptr;
+    *out = h.release();
+  });
+}
+int glx_cstencil_destroy(glx_cstencil cs) {
+  return guarded([&] {
+    GLX_HIP(hipDeviceSynchronize());
+    delete cs;
+  });
+}
+int glx_dev_cstencil_apply(glx_cstencil cs, int kind, int sdir, double* out_dev, const double* in_dev) {
+  return guarded([&] {
+    if (!out_dev || !in_dev) throw ArgError("out_dev and in_dev must not be NULL");
+    if (sdir < 0 || sdir > 13) throw ArgError("sdir must be a stencil_dir (0..13)");
+    DevOperator o = cst_handle_op(cs, kind);
+    o.cs_sdir = sdir;
+    dev_apply(o, out_dev, in_dev, LINKS_F64, 0, nullptr);
+  });
+}
+int glx_dev_cstencil_solve(int solver, glx_cstencil cs, int kind, double* x_dev, const double* b_dev, int max_iter,
+                           double eps, glx_verbose* verb, glx_inversion_info* info) {
+  return guarded([&] {
+    if (!x_dev || !b_dev) throw ArgError("x_dev and b_dev must not be NULL");
+    DevOperator o = cst_handle_op(cs, kind);
+    SolveResult r = solve_and_print(solver, o, x_dev, b_dev, max_iter, eps, verb);
+    fill_info(info, r, solver_name(solver));
+  });
+}
+int glx_dev_time_cstencil(glx_cstencil cs, int kind, double* out_dev, const double* in_dev, int reps,
+                          double* ms_per_apply) {
+  return guarded([&] {
+    if (!out_dev || !in_dev || !ms_per_apply || reps <= 0) throw ArgError("bad arguments");
+    DevOperator o = cst_handle_op(cs, kind);
+    auto launch = [&] {
+      launch_cst(nullptr, o, reinterpret_cast<c128*>(out_dev), reinterpret_cast<const c128*>(in_dev), kNoGate);
+    };
+    launch();
+    GLX_HIP(hipDeviceSynchronize());
+    hipEvent_t a, b;
+    GLX_HIP(hipEventCreate(&a));
+    GLX_HIP(hipEventCreate(&b));
+    GLX_HIP(hipEventRecord(a, nullptr));
+    for (int k = 0; k < reps; k++) launch();
+    GLX_HIP(hipEventRecord(b, nullptr));
+    GLX_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    GLX_HIP(hipEventElapsedTime(&ms, a, b));
+    GLX_HIP(hipEventDestroy(a));
+    GLX_HIP(hipEventDestroy(b));
+    *ms_per_apply = double(ms) / reps;
   });
 }
 

@@ -17,6 +17,9 @@
 #include "../compat/generic_inverters.h"
 #include "../compat/operators.h"
 #include "../compat/u1_utils.h"
+#include "../compat/coarse_stencil.h"
+#include "../compat/operators_stencil.h"
+#include "glx_internal.h"
 
 namespace {
 
@@ -177,6 +180,19 @@ inversion_info minv_vector_cg_m(complex<double>** phi, complex<double>* phi0, in
 // ----------------------------------------------------------- operators ----
 void square_laplace(complex<double>* l, complex<double>* r, void* e) { glx_square_laplace(D(l), D(r), e); }
 void square_laplace_u1(complex<double>* l, complex<double>* r, void* e) { glx_square_laplace_u1(D(l), D(r), e); }
+int get_stencil_size(op_type opt) {
+  switch (opt) {
+    case STAGGERED:
+    case LAPLACE:
+    case LAPLACE_NC2:
+    case G5_STAGGERED:
+      return 1;
+    case STAGGERED_INDEX:
+    case STAGGERED_NORMAL:
+      return 2;
+  }
+  return 0;
+}
 void square_staggered_u1(complex<double>* l, complex<double>* r, void* e) { glx_square_staggered_u1(D(l), D(r), e); }
 void gamma_5(complex<double>* l, complex<double>* r, void* e) { glx_gamma_5(D(l), D(r), e); }
 void square_staggered_gamma5_u1(complex<double>* l, complex<double>* r, void* e) {
@@ -371,4 +387,116 @@ void apply_ape_smear_u1(complex<double>* out, complex<double>* u, int x_len, int
     for (size_t i = 0; i < nl; i++) cur[i] = std::polar(1.0, std::arg(out[i]));
   }
   std::copy(cur.begin(), cur.end(), out);
+}
+
+// ------------------------------------------------------------- stencil_2d --
+// coarse_stencil.cpp:12-1627 and operators_stencil.cpp:14-236 through the
+// general-stencil C ABI: the compat stencil_2d is described to the library
+// (pointers into its own arrays), and the compat apply functions are
+// registered so that a solve given one of them runs on the GPU.
+namespace {
+int describe(void* extra, glx_stencil_2d_nc* out) {
+  stencil_2d* s = static_cast<stencil_2d*>(extra);
+  if (!s || !s->lat || s->lat->get_nd() != 2) return -1;
+  out->lx = s->lat->get_lattice_dimension(0);
+  out->ly = s->lat->get_lattice_dimension(1);
+  out->nc = s->lat->get_nc();
+  out->has_two = s->has_two ? 1 : 0;
+  out->sdir = static_cast<int>(s->sdir);
+  out->clover = D(s->clover);
+  out->hopping = D(s->hopping);
+  out->two_link = s->has_two ? D(s->two_link) : nullptr;
+  out->shift[0] = s->shift.real();
+  out->shift[1] = s->shift.imag();
+  out->eo_shift[0] = s->eo_shift.real();
+  out->eo_shift[1] = s->eo_shift.imag();
+  out->dof_shift[0] = s->dof_shift.real();
+  out->dof_shift[1] = s->dof_shift.imag();
+  return 0;
+}
+glx_stencil_2d_nc described(void* extra) {
+  glx_stencil_2d_nc d{};
+  if (describe(extra, &d) != 0) throw std::runtime_error("stencil_2d: needs a two-dimensional Lattice");
+  return d;
+}
+
+struct StencilRegistration {
+  StencilRegistration() {
+    glx::register_cst_callback(reinterpret_cast<void*>(M(&apply_stencil_2d)), GLX_OP_CSTENCIL, &describe);
+    glx::register_cst_callback(reinterpret_cast<void*>(M(&apply_stencil_2d_eo)), GLX_OP_CSTENCIL_EO, &describe);
+    glx::register_cst_callback(reinterpret_cast<void*>(M(&apply_stencil_2d_oe)), GLX_OP_CSTENCIL_OE, &describe);
+    glx::register_cst_callback(reinterpret_cast<void*>(M(&apply_stencil_2d_tb)), GLX_OP_CSTENCIL_TB, &describe);
+    glx::register_cst_callback(reinterpret_cast<void*>(M(&apply_stencil_2d_bt)), GLX_OP_CSTENCIL_BT, &describe);
+    glx::register_cst_callback(reinterpret_cast<void*>(M(&apply_square_staggered_m2mdeodoe_stencil)),
+                               GLX_OP_CSTENCIL_M2MDEODOE, &describe);
+  }
+} g_stencil_registration;
+
+void staggered_builder(stencil_2d* st, staggered_u1_op* op,
+                       int (*build)(glx_stencil_2d*, const glx_staggered_u1_op*)) {
+  if (st->generated || st->lat->get_nc() != 1) return;  // operators_stencil.cpp:16-24
+  glx_stencil_2d c{};
+  c.lx = st->lat->get_lattice_dimension(0);
+  c.ly = st->lat->get_lattice_dimension(1);
+  c.clover = D(st->clover);
+  c.hopping = D(st->hopping);
+  glx_staggered_u1_op p{D(op->lattice), op->mass, op->x_fine, op->y_fine, op->Nc, op->wilson_coeff};
+  if (build(&c, &p) != GLX_OK) throw std::runtime_error(std::string("stencil builder: ") + glx_last_error());
+  st->shift = complex<double>(c.shift[0], c.shift[1]);
+  st->eo_shift = complex<double>(c.eo_shift[0], c.eo_shift[1]);
+  st->dof_shift = complex<double>(c.dof_shift[0], c.dof_shift[1]);
+  st->generated = true;
+}
+}  // namespace
+
+void apply_stencil_2d(complex<double>* lhs, complex<double>* rhs, void* extra) {
+  glx_stencil_2d_nc d = described(extra);
+  glx_apply_stencil_2d_nc(D(lhs), D(rhs), &d);
+}
+void apply_stencil_2d_eo(complex<double>* lhs, complex<double>* rhs, void* extra) {
+  glx_stencil_2d_nc d = described(extra);
+  glx_apply_stencil_2d_nc_eo(D(lhs), D(rhs), &d);
+}
+void apply_stencil_2d_oe(complex<double>* lhs, complex<double>* rhs, void* extra) {
+  glx_stencil_2d_nc d = described(extra);
+  glx_apply_stencil_2d_nc_oe(D(lhs), D(rhs), &d);
+}
+void apply_stencil_2d_tb(complex<double>* lhs, complex<double>* rhs, void* extra) {
+  glx_stencil_2d_nc d = described(extra);
+  glx_apply_stencil_2d_nc_tb(D(lhs), D(rhs), &d);
+}
+void apply_stencil_2d_bt(complex<double>* lhs, complex<double>* rhs, void* extra) {
+  glx_stencil_2d_nc d = described(extra);
+  glx_apply_stencil_2d_nc_bt(D(lhs), D(rhs), &d);
+}
+void generate_stencil_2d(stencil_2d* st, void (*mv)(complex<double>*, complex<double>*, void*), void* extra) {
+  if (st->generated || st->stencil_size > 2) return;
+  glx_stencil_2d_nc d = described(st);
+  if (glx_generate_stencil_2d_nc(&d, M(mv), extra) != GLX_OK)
+    throw std::runtime_error(std::string("generate_stencil_2d: ") + glx_last_error());
+  st->generated = true;
+}
+
+void get_square_staggered_u1_stencil(stencil_2d* st, staggered_u1_op* op) {
+  staggered_builder(st, op, &glx_get_square_staggered_u1_stencil);
+}
+void get_square_staggered_gamma5_u1_stencil(stencil_2d* st, staggered_u1_op* op) {
+  staggered_builder(st, op, &glx_get_square_staggered_gamma5_u1_stencil);
+}
+void get_square_staggered_dagger_u1_stencil(stencil_2d* st, staggered_u1_op* op) {
+  staggered_builder(st, op, &glx_get_square_staggered_dagger_u1_stencil);
+}
+void apply_square_staggered_eoprec_prepare_stencil(complex<double>* rhs_e, complex<double>* rhs_orig,
+                                                   stencil_2d* st) {
+  glx_stencil_2d_nc d = described(st);
+  glx_stencil_2d_nc_eoprec_prepare(D(rhs_e), D(rhs_orig), &d);
+}
+void apply_square_staggered_m2mdeodoe_stencil(complex<double>* lhs, complex<double>* rhs, void* extra) {
+  glx_stencil_2d_nc d = described(extra);
+  glx_apply_stencil_2d_nc_m2mdeodoe(D(lhs), D(rhs), &d);
+}
+void apply_square_staggered_eoprec_reconstruct_stencil(complex<double>* lhs_full, complex<double>* lhs_e,
+                                                       complex<double>* rhs_o, stencil_2d* st) {
+  glx_stencil_2d_nc d = described(st);
+  glx_stencil_2d_nc_eoprec_reconstruct(D(lhs_full), D(lhs_e), D(rhs_o), &d);
 }

@@ -112,9 +112,12 @@ __device__ __forceinline__ void eo_body(const EoGeom& eg, const OpDev<LK>& op, I
       if (hl) lh = op.links.fetch(wrap_coord(y0 + 4 * k - 2 + hl_row, ly) * lx + xhl);
     };
 
+    // the previous launch's partials are read first: loads complete in
+    // issue order, so the prelude's wait does not include the first rows
+    if constexpr (kSF) in.prelude_load(ws);
     if (active) fetch(0);
     bool run = true;
-    if constexpr (kSF) run = in.prelude(ws);  // overlaps the reads of the first rows
+    if constexpr (kSF) run = in.prelude_finish(ws);
     for (int k = 0; k < ((active && run) ? nsteps : 0); k++) {
       if (k > 0) __syncthreads();  // step k-1's readers are done with the slots reused below
       {

@@ -5,6 +5,7 @@
 // reduction.
 #pragma once
 
+#include "glx_cstencil.h"
 #include "glx_eo.h"
 
 #include <functional>
@@ -89,6 +90,13 @@ struct DevOperator {
   const c128* clover = nullptr;
   const c128* hop = nullptr;
   c128 shift{0, 0}, eo_shift{0, 0}, dof_shift{0, 0};
+  // general stencil_2d (kinds OP_CST*): column-major coefficients, the
+  // stencil's sdir and whether it has two-link terms, and a vector of scratch
+  // for the composite m^2 - D_eo D_oe
+  const c128* cs_coef = nullptr;
+  int cs_sdir = 0;
+  bool cs_has_two = false;
+  c128* cs_tmp = nullptr;
   // host callback fallback
   void (*host_cb)(double*, double*, void*) = nullptr;
   void* host_extra = nullptr;
@@ -98,7 +106,7 @@ struct DevOperator {
   // vector length (compacted even sites for the even/odd Schur operator)
   int size() const { return host_cb ? size_ : (kind == OP_STAG_EO_SCHUR ? lx * ly / 2 : lx * ly * nc); }
   bool is_host() const { return host_cb != nullptr; }
-  bool tiled() const { return !host_cb && nc == 1; }
+  bool tiled() const { return !host_cb && nc == 1 && !is_cst_kind(kind); }
 };
 
 enum LinkPrec : int { LINKS_F64 = 0, LINKS_F32 = 1, LINKS_F16 = 2 };
@@ -262,6 +270,34 @@ inline int point_blocks(int n) {
   if (b > kMaxPointBlocks) b = kMaxPointBlocks;
   if (b < 1) b = 1;
   return b;
+}
+
+// General stencil_2d launch (glx_cstencil.h) for a runtime kind; the
+// composite m^2 - D_eo D_oe is an oe launch into o.cs_tmp and an eo launch
+// that combines (operators_stencil.cpp:195-214).
+inline void launch_cst_one(hipStream_t st, const CsDev& d, const c128* in, c128* out, Gate done) {
+  const int n = d.lx * d.ly * d.nc;
+  hipLaunchKernelGGL(k_cstencil<0>, dim3(point_blocks(n)), dim3(kBlock), 0, st, d, in, out, done);
+  GLX_CHECK_LAUNCH();
+}
+inline CsDev cst_dev(const DevOperator& o, int piece) {
+  return make_csdev(o.lx, o.ly, o.nc, piece, o.cs_sdir, o.cs_has_two, o.cs_coef, o.shift, o.eo_shift, o.dof_shift);
+}
+inline void launch_cst(hipStream_t st, const DevOperator& o, c128* out, const c128* in, Gate done) {
+  if (!o.cs_coef) throw ArgError("general stencil without coefficients on the device");
+  if (o.kind == OP_CST_M2MDEODOE) {
+    if (!o.cs_tmp) throw ArgError("m^2 - D_eo D_oe needs a scratch vector");
+    launch_cst_one(st, cst_dev(o, CS_OE), in, o.cs_tmp, done);
+    CsDev e = cst_dev(o, CS_EO);
+    e.comb_mode = CS_AXMY;
+    e.comb_a = o.shift * o.shift;
+    e.comb = in;
+    launch_cst_one(st, e, o.cs_tmp, out, done);
+    return;
+  }
+  const int piece = cst_piece(o.kind);
+  if (cs_untouched(piece, o.cs_sdir, o.cs_has_two)) return;
+  launch_cst_one(st, cst_dev(o, piece), in, out, done);
 }
 
 // The number of partial sums is the functor's own F::NR, so a launch can

@@ -297,6 +297,14 @@ void apply_typed(hipStream_t st, const DevOperator& o, int link_prec, T* out, co
     GLX_HIP(hipStreamSynchronize(st));
     return;
   }
+  if (is_cst_kind(o.kind)) {
+    if constexpr (std::is_same<T, c128>::value) {
+      launch_cst(st, o, out, in, done);
+      return;
+    } else {
+      throw ArgError("the general stencil takes FP64 complex vectors");
+    }
+  }
   if (o.nc != 1) {
     if (o.kind != OP_LAPLACE && o.kind != OP_LAPLACE_REAL)
       throw ArgError("Nc > 1 is supported for the Laplacians only");
@@ -1434,8 +1442,25 @@ struct InFastSF : InFast<T, TX> {
     // only after prelude()
     this->setup(pending ? k0 + 1 : k0, rd->alpha, rd->beta_r);
   }
+  // The previous launch's partial sums, in two halves: prelude_load issues
+  // the first sweep of loads before the kernel's first row reads, so that
+  // waiting for them (loads complete in issue order) does not also wait for
+  // the rows; prelude_finish reduces them and takes the CG step.
   // Block-wide; every thread returns the same decision (true = iterate).
-  __device__ bool prelude(RedWs ws) {
+  static constexpr int kPreU = 2;  // terms per thread per sweep (512 blocks in one sweep)
+  double pv[kPreU][NR];
+  __device__ void prelude_load(RedWs ws) {
+    if (gated || !pending) return;
+    const int nblk = gridDim.x;
+    const double* P = ws.partials + (q ^ 1) * NR * nblk;
+#pragma unroll
+    for (int u = 0; u < kPreU; u++) {
+      const int b = threadIdx.x + u * kBlock;
+#pragma unroll
+      for (int k = 0; k < NR; k++) pv[u][k] = b < nblk ? P[k * nblk + b] : 0.0;
+    }
+  }
+  __device__ bool prelude_finish(RedWs ws) {
     if (gated) return false;
     if (!pending) return true;
     __shared__ double sm[NR][kWaves];
@@ -1443,20 +1468,22 @@ struct InFastSF : InFast<T, TX> {
     const double* P = ws.partials + (q ^ 1) * NR * nblk;
     double acc[NR];
 #pragma unroll
-    for (int k = 0; k < NR; k++) acc[k] = 0.0;
-    // two terms per thread per sweep (512 blocks in one memory round trip)
-    // keeps the registers of this prelude small
-    constexpr int U = 2;
-    for (int b0 = threadIdx.x; b0 < nblk; b0 += kBlock * U) {
-      double v[U][NR];
+    for (int k = 0; k < NR; k++) acc[k] = pv[0][k];
 #pragma unroll
-      for (int u = 0; u < U; u++) {
+    for (int u = 1; u < kPreU; u++) {
+#pragma unroll
+      for (int k = 0; k < NR; k++) acc[k] += pv[u][k];
+    }
+    for (int b0 = threadIdx.x + kBlock * kPreU; b0 < nblk; b0 += kBlock * kPreU) {
+      double v[kPreU][NR];
+#pragma unroll
+      for (int u = 0; u < kPreU; u++) {
         const int b = b0 + u * kBlock;
 #pragma unroll
         for (int k = 0; k < NR; k++) v[u][k] = b < nblk ? P[k * nblk + b] : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < U; u++) {
+      for (int u = 0; u < kPreU; u++) {
 #pragma unroll
         for (int k = 0; k < NR; k++) acc[k] += v[u][k];
       }

@@ -48,6 +48,13 @@ enum OpKind : int {
   OP_STENCIL_OE = 15,    // coarse_stencil.cpp:560-624
   OP_STENCIL_M2MDEODOE = 16,  // operators_stencil.cpp:195-214
   OP_STAG_EO_SCHUR = 17,      // m^2 - D_eo D_oe (operators.cpp:566-588) on compacted even-site vectors
+  // stencil_2d for any Nc with two-link terms (glx_cstencil.h)
+  OP_CST = 18,                // coarse_stencil.cpp:12-392
+  OP_CST_EO = 19,             // coarse_stencil.cpp:395-557
+  OP_CST_OE = 20,             // coarse_stencil.cpp:560-722
+  OP_CST_TB = 21,             // coarse_stencil.cpp:725-1117
+  OP_CST_BT = 22,             // coarse_stencil.cpp:1120-1512
+  OP_CST_M2MDEODOE = 23,      // operators_stencil.cpp:195-214 on any Nc
 };
 
 __host__ __device__ constexpr int op_halo(int k) {
@@ -57,6 +64,9 @@ __host__ __device__ constexpr bool op_links(int k) {
   return k == OP_LAPLACE_U1 || k == OP_STAG || k == OP_STAG_DAG || k == OP_STAG_NORMAL || k == OP_STAG_G5 ||
          k == OP_DEO || k == OP_DOE || k == OP_M2MDEODOE || k == OP_STAG_EO_SCHUR;
 }
+__host__ __device__ constexpr bool is_cst_kind(int k) { return k >= OP_CST && k <= OP_CST_M2MDEODOE; }
+// CsPiece of a general-stencil kind (the composite m^2 - D_eo D_oe has none)
+__host__ __device__ constexpr int cst_piece(int k) { return k == OP_CST_M2MDEODOE ? -1 : k - OP_CST; }
 __host__ __device__ constexpr bool op_coeffs(int k) {
   return k == OP_STENCIL || k == OP_STENCIL_EO || k == OP_STENCIL_OE || k == OP_STENCIL_M2MDEODOE;
 }

@@ -90,6 +90,12 @@ class Stencil2D(C.Structure):
                 ("shift", C.c_double * 2), ("eo_shift", C.c_double * 2), ("dof_shift", C.c_double * 2)]
 
 
+class Stencil2DNc(C.Structure):
+    _fields_ = [("lx", C.c_int), ("ly", C.c_int), ("nc", C.c_int), ("has_two", C.c_int), ("sdir", C.c_int),
+                ("clover", C.c_void_p), ("hopping", C.c_void_p), ("two_link", C.c_void_p),
+                ("shift", C.c_double * 2), ("eo_shift", C.c_double * 2), ("dof_shift", C.c_double * 2)]
+
+
 @dataclass
 class Info:
     """inversion_info (inverter_struct.h:14-89)."""
@@ -155,6 +161,26 @@ def lib():
         L.glx_lattice_get_links.argtypes = [vp, vp]
         L.glx_lattice_plaquette.restype = i
         L.glx_lattice_plaquette.argtypes = [vp, vp]
+        for f in ("glx_apply_stencil_2d_nc", "glx_apply_stencil_2d_nc_eo", "glx_apply_stencil_2d_nc_oe",
+                  "glx_apply_stencil_2d_nc_tb", "glx_apply_stencil_2d_nc_bt", "glx_apply_stencil_2d_nc_m2mdeodoe"):
+            getattr(L, f).restype = None
+            getattr(L, f).argtypes = [vp, vp, vp]
+        L.glx_stencil_2d_nc_eoprec_prepare.restype = None
+        L.glx_stencil_2d_nc_eoprec_prepare.argtypes = [vp, vp, vp]
+        L.glx_stencil_2d_nc_eoprec_reconstruct.restype = None
+        L.glx_stencil_2d_nc_eoprec_reconstruct.argtypes = [vp, vp, vp, vp]
+        L.glx_generate_stencil_2d_nc.restype = i
+        L.glx_generate_stencil_2d_nc.argtypes = [vp, vp, vp]
+        L.glx_cstencil_create.restype = i
+        L.glx_cstencil_create.argtypes = [C.POINTER(vp), vp]
+        L.glx_cstencil_destroy.restype = i
+        L.glx_cstencil_destroy.argtypes = [vp]
+        L.glx_dev_cstencil_apply.restype = i
+        L.glx_dev_cstencil_apply.argtypes = [vp, i, i, vp, vp]
+        L.glx_dev_cstencil_solve.restype = i
+        L.glx_dev_cstencil_solve.argtypes = [i, vp, i, vp, vp, i, C.c_double, vp, vp]
+        L.glx_dev_time_cstencil.restype = i
+        L.glx_dev_time_cstencil.argtypes = [vp, i, vp, vp, i, C.POINTER(C.c_double)]
         L.glx_lattice_destroy.restype = i
         L.glx_lattice_destroy.argtypes = [vp]
         L.glx_dev_solve.restype = i
@@ -386,6 +412,10 @@ class DeviceArray:
         self.dtype = np.dtype(dtype if arr is None else np.asarray(arr).dtype)
         self.shape = shape if arr is None else np.asarray(arr).shape
         self.nbytes = nbytes if arr is None else np.asarray(arr).nbytes
+        if self.shape is None:
+            self.shape = (self.nbytes // self.dtype.itemsize,)
+        if int(np.prod(self.shape)) * self.dtype.itemsize != self.nbytes:
+            raise ValueError("DeviceArray: shape and dtype do not cover nbytes")
         p = C.c_void_p()
         _check(lib().glx_dev_alloc(C.byref(p), max(self.nbytes, 1)), "glx_dev_alloc")
         self.ptr = p.value
@@ -399,6 +429,7 @@ class DeviceArray:
 
     def download(self):
         out = np.empty(self.shape, dtype=self.dtype)
+        assert out.nbytes == self.nbytes
         _check(lib().glx_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes), "glx_memcpy_dtoh")
         return out
 
@@ -673,3 +704,132 @@ def synchronize():
 
 def release_cache():
     _check(lib().glx_release_cache(), "glx_release_cache")
+
+
+# ----------------------------------------------------- general stencil_2d --
+CST_KINDS = {"all": 18, "eo": 19, "oe": 20, "tb": 21, "bt": 22, "m2mdeodoe": 23}
+CST_CALLBACKS = {"all": "glx_apply_stencil_2d_nc", "eo": "glx_apply_stencil_2d_nc_eo",
+                 "oe": "glx_apply_stencil_2d_nc_oe", "tb": "glx_apply_stencil_2d_nc_tb",
+                 "bt": "glx_apply_stencil_2d_nc_bt", "m2mdeodoe": "glx_apply_stencil_2d_nc_m2mdeodoe"}
+
+
+class StencilNc:
+    """stencil_2d for any nc (coarse_stencil.h:33-154): coefficient arrays in
+    the reference layout (clover V nc^2, hopping 4 V nc^2, two_link 8 V nc^2 or
+    None; element (term, site, row, column) at ((term V + site) nc + row) nc +
+    column), complex shifts and the stencil_dir `sdir`."""
+
+    def __init__(self, lx, ly, nc, clover, hopping, two_link=None, shift=0j, eo_shift=0j, dof_shift=0j, sdir=0):
+        self.lx, self.ly, self.nc, self.sdir = lx, ly, nc, int(sdir)
+        self.clover = np.ascontiguousarray(clover, dtype=np.complex128)
+        self.hopping = np.ascontiguousarray(hopping, dtype=np.complex128)
+        self.two_link = None if two_link is None else np.ascontiguousarray(two_link, dtype=np.complex128)
+        self.shift, self.eo_shift, self.dof_shift = complex(shift), complex(eo_shift), complex(dof_shift)
+        blk = lx * ly * nc * nc
+        if self.clover.size != blk or self.hopping.size != 4 * blk or \
+                (self.two_link is not None and self.two_link.size != 8 * blk):
+            raise ValueError("coefficient arrays do not match lx * ly * nc^2 blocks")
+
+    @property
+    def n(self):
+        return self.lx * self.ly * self.nc
+
+    def struct(self) -> Stencil2DNc:
+        st = Stencil2DNc()
+        st.lx, st.ly, st.nc, st.sdir = self.lx, self.ly, self.nc, self.sdir
+        st.has_two = int(self.two_link is not None)
+        st.clover, st.hopping = self.clover.ctypes.data, self.hopping.ctypes.data
+        st.two_link = None if self.two_link is None else self.two_link.ctypes.data
+        st.shift[:] = [self.shift.real, self.shift.imag]
+        st.eo_shift[:] = [self.eo_shift.real, self.eo_shift.imag]
+        st.dof_shift[:] = [self.dof_shift.real, self.dof_shift.imag]
+        return st
+
+
+def cstencil_apply(st: StencilNc, rhs, piece="all", lhs=None):
+    """The callback form (host vectors; the GPU does the work).  `lhs` is the
+    initial content of the output, which a two-link sdir on a stencil without
+    two-link terms leaves as it is."""
+    rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+    out = np.zeros_like(rhs) if lhs is None else np.array(lhs, dtype=np.complex128, copy=True)
+    cs = st.struct()
+    fn = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)(op_callback_addr(CST_CALLBACKS[piece]))
+    fn(out.ctypes.data, rhs.ctypes.data, C.cast(C.pointer(cs), C.c_void_p))
+    return out
+
+
+def op_callback_addr(symbol: str) -> int:
+    return C.cast(getattr(lib(), symbol), C.c_void_p).value
+
+
+def cstencil_eoprec_prepare(st: StencilNc, rhs):
+    rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+    out = np.zeros_like(rhs)
+    cs = st.struct()
+    lib().glx_stencil_2d_nc_eoprec_prepare(out.ctypes.data, rhs.ctypes.data, C.byref(cs))
+    return out
+
+
+def cstencil_eoprec_reconstruct(st: StencilNc, lhs_e, rhs_o):
+    lhs_e = np.ascontiguousarray(lhs_e, dtype=np.complex128)
+    rhs_o = np.ascontiguousarray(rhs_o, dtype=np.complex128)
+    out = np.zeros_like(lhs_e)
+    cs = st.struct()
+    lib().glx_stencil_2d_nc_eoprec_reconstruct(out.ctypes.data, lhs_e.ctypes.data, rhs_o.ctypes.data, C.byref(cs))
+    return out
+
+
+def cstencil_generate(lx, ly, nc, has_two, callback: int, extra) -> StencilNc:
+    """generate_stencil_2d: probe the operator at address `callback` (with
+    extra_info pointer `extra`) with unit vectors."""
+    blk = lx * ly * nc * nc
+    out = StencilNc(lx, ly, nc, np.zeros(blk, np.complex128), np.zeros(4 * blk, np.complex128),
+                    np.zeros(8 * blk, np.complex128) if has_two else None)
+    cs = out.struct()
+    _check(lib().glx_generate_stencil_2d_nc(C.byref(cs), callback, extra), "glx_generate_stencil_2d_nc")
+    return out
+
+
+def cstencil_solve(solver: str, st: StencilNc, b, piece="all", x0=None, max_iter=10000, eps=1e-10, restart_freq=0):
+    """minv_vector_* with a general-stencil callback: the solve runs on the GPU."""
+    cs = st.struct()
+    return solve(solver, "stencil", b, x0, max_iter=max_iter, eps=eps, restart_freq=restart_freq,
+                 callback=op_callback_addr(CST_CALLBACKS[piece]), extra=C.cast(C.pointer(cs), C.c_void_p).value)
+
+
+class CStencilDev:
+    """A general stencil resident in HBM (glx_cstencil_create)."""
+
+    def __init__(self, st: StencilNc):
+        self.lx, self.ly, self.nc, self.sdir = st.lx, st.ly, st.nc, st.sdir
+        h = C.c_void_p()
+        cs = st.struct()
+        _check(lib().glx_cstencil_create(C.byref(h), C.byref(cs)), "glx_cstencil_create")
+        self.h = h.value
+
+    def apply(self, out: DeviceArray, inp: DeviceArray, piece="all", sdir=None):
+        _check(lib().glx_dev_cstencil_apply(self.h, CST_KINDS[piece], self.sdir if sdir is None else int(sdir),
+                                            out.ptr, inp.ptr), "glx_dev_cstencil_apply")
+
+    def solve(self, solver: str, x: DeviceArray, b: DeviceArray, piece="all", max_iter=10000, eps=1e-10):
+        ci = InversionInfo()
+        _check(lib().glx_dev_cstencil_solve(SOLVER_IDS[solver], self.h, CST_KINDS[piece], x.ptr, b.ptr, max_iter, eps,
+                                            None, C.byref(ci)), "glx_dev_cstencil_solve")
+        return _info(ci)
+
+    def time(self, out: DeviceArray, inp: DeviceArray, piece="all", reps=100) -> float:
+        ms = C.c_double()
+        _check(lib().glx_dev_time_cstencil(self.h, CST_KINDS[piece], out.ptr, inp.ptr, reps, C.byref(ms)),
+               "glx_dev_time_cstencil")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().glx_cstencil_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -141,6 +141,45 @@ void glx_apply_square_staggered_m2mdeodoe_stencil(double* lhs, double* rhs, void
 void glx_apply_square_staggered_eoprec_prepare_stencil(double* rhs_e, double* rhs_orig, void* stencil);
 void glx_apply_square_staggered_eoprec_reconstruct_stencil(double* lhs_full, double* lhs_e, double* rhs_o, void* stencil);
 
+/* stencil_2d for any number of colours nc, with optional two-link terms
+ * (coarse_stencil.h:14-154), as plain data.  Coefficient (site s, row r,
+ * column c) of a block is at (s*nc + r)*nc + c (the reference's layout):
+ * clover[V nc^2]; hopping[4 V nc^2], blocks {+x, +y, -x, -y}; two_link
+ * [8 V nc^2], blocks {+2x, +x+y, +2y, -x+y, -2x, -x-y, -2y, +x-y}, read only
+ * when has_two.  sdir is the reference's stencil_dir (GLX_SDIR_*).  Vectors
+ * have lx*ly*nc complex entries, entry (y*lx + x)*nc + colour. */
+enum {
+  GLX_SDIR_ALL = 0, GLX_SDIR_CLOVER = 1, GLX_SDIR_XP1 = 2, GLX_SDIR_YP1 = 3, GLX_SDIR_XM1 = 4, GLX_SDIR_YM1 = 5,
+  GLX_SDIR_XP2 = 6, GLX_SDIR_XP1YP1 = 7, GLX_SDIR_YP2 = 8, GLX_SDIR_XM1YP1 = 9, GLX_SDIR_XM2 = 10,
+  GLX_SDIR_XM1YM1 = 11, GLX_SDIR_YM2 = 12, GLX_SDIR_XP1YM1 = 13
+};
+typedef struct {
+  int lx, ly, nc;
+  int has_two;
+  int sdir;
+  double* clover;
+  double* hopping;
+  double* two_link;
+  double shift[2];
+  double eo_shift[2];
+  double dof_shift[2];
+} glx_stencil_2d_nc;
+/* operators with extra_info = glx_stencil_2d_nc*; solvers given them run on
+ * the GPU with the coefficients uploaded once per solve */
+void glx_apply_stencil_2d_nc(double* lhs, double* rhs, void* stencil);     /* coarse_stencil.cpp:12-392 */
+void glx_apply_stencil_2d_nc_eo(double* lhs, double* rhs, void* stencil);  /* coarse_stencil.cpp:395-557 */
+void glx_apply_stencil_2d_nc_oe(double* lhs, double* rhs, void* stencil);  /* coarse_stencil.cpp:560-722 */
+void glx_apply_stencil_2d_nc_tb(double* lhs, double* rhs, void* stencil);  /* coarse_stencil.cpp:725-1117 */
+void glx_apply_stencil_2d_nc_bt(double* lhs, double* rhs, void* stencil);  /* coarse_stencil.cpp:1120-1512 */
+/* operators_stencil.cpp:179-236 on any nc */
+void glx_apply_stencil_2d_nc_m2mdeodoe(double* lhs, double* rhs, void* stencil);
+void glx_stencil_2d_nc_eoprec_prepare(double* rhs_e, double* rhs_orig, void* stencil);
+void glx_stencil_2d_nc_eoprec_reconstruct(double* lhs_full, double* lhs_e, double* rhs_o, void* stencil);
+/* generate_stencil_2d (coarse_stencil.cpp:1515-1627): applies matrix_vector to
+ * every unit vector and adds the entries into the (caller-zeroed) coefficient
+ * arrays; a registered device operator runs on the GPU */
+int glx_generate_stencil_2d_nc(glx_stencil_2d_nc* st, glx_matvec matrix_vector, void* extra_info);
+
 /* Operator kinds (numbering of the device kernels). */
 enum {
   GLX_OP_LAPLACE_REAL = 0,
@@ -162,7 +201,15 @@ enum {
    * lx*ly/2 elements, site (x, y) with x + y even at y*(lx/2) + (x >> 1);
    * device API only (glx_dev_*); needs (lx/2) % 64 == 0 (FP64) or % 128 == 0
    * (FP32 vectors), ly % 4 == 0, ly >= 8 */
-  GLX_OP_STAGGERED_EO_SCHUR = 17
+  GLX_OP_STAGGERED_EO_SCHUR = 17,
+  /* general stencil_2d (glx_stencil_2d_nc): full, eo, oe, tb, bt pieces and
+   * m^2 - D_eo D_oe */
+  GLX_OP_CSTENCIL = 18,
+  GLX_OP_CSTENCIL_EO = 19,
+  GLX_OP_CSTENCIL_OE = 20,
+  GLX_OP_CSTENCIL_TB = 21,
+  GLX_OP_CSTENCIL_BT = 22,
+  GLX_OP_CSTENCIL_M2MDEODOE = 23
 };
 
 /* Bind a caller's own host callback to a device operator, so that solves
@@ -378,6 +425,22 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
  * HBM bytes per site one launch moves.  No reference counterpart. */
 int glx_dev_time_cg_kernel(glx_lattice lat, double mass, int prec, int reps, double* ms_per_launch,
                            double* bytes_per_site);
+
+/* A general stencil resident in HBM (coefficients transposed for coalesced
+ * reads), for device vectors of lx*ly*nc complex entries.  kind is one of
+ * GLX_OP_CSTENCIL*; sdir overrides the stencil's own (GLX_SDIR_*).  A
+ * two-link sdir on a stencil without two-link terms leaves out_dev as it was
+ * (full, tb and bt pieces), as the reference leaves lhs. */
+typedef struct glx_cstencil_s* glx_cstencil;
+int glx_cstencil_create(glx_cstencil* cs, const glx_stencil_2d_nc* st);
+int glx_cstencil_destroy(glx_cstencil cs);
+int glx_dev_cstencil_apply(glx_cstencil cs, int kind, int sdir, double* out_dev, const double* in_dev);
+/* CG / BiCGStab / GCR (GLX_SOLVER_*) on device vectors with the stencil */
+int glx_dev_cstencil_solve(int solver, glx_cstencil cs, int kind, double* x_dev, const double* b_dev, int max_iter,
+                           double eps, glx_verbose* verbosity, glx_inversion_info* info);
+/* ms per application (HIP events around reps back-to-back launches) */
+int glx_dev_time_cstencil(glx_cstencil cs, int kind, double* out_dev, const double* in_dev, int reps,
+                          double* ms_per_apply);
 
 /* Device memory helpers (so that callers need no HIP headers). */
 int glx_dev_alloc(void** ptr, size_t bytes);

@@ -33,3 +33,27 @@ def test_dropin_driver_runs(glx):
     assert re.search(r"\[TEST\]: dispatch bicgstab_restart BiCGStab\(64\) success [01]", t)
     assert "[TEST]: dispatch invalid iter 0" in t
     assert "[TEST]: dispatch cr refused" in t
+
+
+def test_stencil_driver_runs(glx):
+    """tests/dropin/stencil_driver.cpp: Lattice, stencil_2d, the stencil
+    builders, eo-prec helpers, generate_stencil_2d and the nc = 2 pieces
+    through the compat headers (coarse_stencil.h:14-173,
+    operators_stencil.h:1-40), solves by pointer identity on the GPU."""
+    exe = os.path.join(ROOT, "tests", "dropin", "stencil_driver")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.dirname(exe), "stencil_driver"], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    t = out.stdout
+    val = lambda pat: float(re.search(pat, t).group(1))
+    assert val(r"stencil_vs_function (\S+)") < 1e-15
+    # even-site relres 1e-10 bounds the full |D x - b| / |b| by 1e-10 (m + 2) / m
+    m = re.search(r"eo_solve success (\d) iter (\d+) residual (\S+)", t)
+    assert m and m.group(1) == "1" and float(m.group(3)) < 1.01e-10 * (0.05 + 2) / 0.05
+    assert re.search(r"\[EO\]: CG Success Y", t)
+    assert val(r"generated_normal (\S+)") < 1e-14 and "generated 1" in t
+    assert val(r"directions_sum (\S+)") < 1e-14
+    assert val(r"nc2 tb_bottom_rows (\S+)") == 0.0
+    m = re.search(r"nc2_bicgstab success (\d) residual (\S+)", t)
+    assert m and m.group(1) == "1" and float(m.group(2)) < 1.01e-10
