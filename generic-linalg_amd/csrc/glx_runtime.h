@@ -302,10 +302,13 @@ inline void launch_cst(hipStream_t st, const DevOperator& o, c128* out, const c1
 
 // The number of partial sums is the functor's own F::NR, so a launch can
 // never disagree with what the functor accumulates.
+// Grid of a pointwise launch that is usually gated off (a smaller grid
+// dispatches and retires faster when every block just reads the gate).
+constexpr int kGatedBlocks = 512;
 template <class F, class Fin>
-void launch_point(hipStream_t st, int n, F f, Fin fin, RedWs ws, Gate done) {
+void launch_point(hipStream_t st, int n, F f, Fin fin, RedWs ws, Gate done, int max_blocks = kMaxPointBlocks) {
   static_assert(F::NR >= 0 && F::NR <= kMaxNR, "too many partial sums");
-  const int nb = point_blocks(n);
+  const int nb = std::min(point_blocks(n), std::max(1, max_blocks));
   hipLaunchKernelGGL((k_point<F::NR, F, Fin>), dim3(nb), dim3(kBlock), 0, st, n, f, fin, ws, done);
   GLX_CHECK_LAUNCH();
   if constexpr (F::NR > 0) {

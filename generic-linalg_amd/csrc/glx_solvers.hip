@@ -104,6 +104,9 @@ struct SolveState {
   c128 alpha, alpha_prev, beta, beta_prev, rho, omega, pap;
   double* hist;
   double* resvec;
+  // host-mapped copy of `done` (single-pass CG): the kernels that change
+  // `done` also store it here, so the host polls it without a copy command
+  int* hdone;
   // multishift
   c128 zeta[kMaxShift], zeta_p[kMaxShift], alpha_s[kMaxShift], beta_s[kMaxShift];
   double shifts[kMaxShift];
@@ -124,9 +127,18 @@ __device__ __forceinline__ typename Scal<T>::type as_scal(c128 a) {
 // Pinned slots for the control-word read-back (per host thread).
 struct PinnedSlots {
   Mirror* m = nullptr;
+  int* hdone = nullptr;      // host address of the single-pass CG's done flag
+  int* hdone_dev = nullptr;  // the same memory as the device addresses it
   hipEvent_t ev[2];
+  // events of the single-pass CG's groups, recorded without the system-scope
+  // fence: the done flag they guard is written to host memory by the kernels
+  hipEvent_t ev_nf[2];
   PinnedSlots() {
     GLX_HIP(hipHostMalloc(reinterpret_cast<void**>(&m), 2 * sizeof(Mirror), hipHostMallocDefault));
+    GLX_HIP(hipHostMalloc(reinterpret_cast<void**>(&hdone), 64, hipHostMallocMapped));
+    GLX_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&hdone_dev), hdone, 0));
+    for (int i = 0; i < 2; i++)
+      GLX_HIP(hipEventCreateWithFlags(&ev_nf[i], hipEventDisableTiming | hipEventDisableSystemFence));
     GLX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     GLX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   }
@@ -1401,6 +1413,7 @@ struct FinFast {
     if (f.stop) {
       s->kfinal = k;
       s->done = MIXED ? 2 : 1;
+      if (s->hdone) *s->hdone = s->done;
       return;
     }
     s->beta_r = f.beta_r;
@@ -1499,6 +1512,10 @@ struct InFastSF : InFast<T, TX> {
   // At the end of the kernel: block 0 writes the state of the next launch.
   __device__ void arrive(bool iterated) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    arrive_state(iterated);
+    if (rd->hdone) *rd->hdone = wr->done;
+  }
+  __device__ void arrive_state(bool iterated) {
     // carried over unless changed below
     wr->k = rd->k;
     wr->done = rd->done;
@@ -1562,6 +1579,7 @@ struct FinFastInit {
     s->k = 0;
     s->done = 0;
     s->pending = 0;
+    if (s->hdone) *s->hdone = 0;
   }
 };
 
@@ -1587,22 +1605,19 @@ struct PtFastAccum {
     xs[i] = zero_of<TS>();
   }
 };
-// step 3: r_true = b - A x in FP64, written (rounded) as the recurrence's
-// old residual r[k&1]; |r_true|^2 decides convergence
+// step 2 epilogue: r_true = b - A x in FP64, written (rounded) as the
+// recurrence's old residual r[k&1]; |r_true|^2 decides convergence
 template <class TD, class TS>
-struct PtFastTrue {
+struct EpiFastTrue {
   static constexpr int NR = 1;
   FastBufs<TS> bufs;
   const TD* b;
-  const TD* ax;
   const SolveState* s;
-  TS* r;
-  __device__ void init() { r = bufs.rs(s->k); }
-  __device__ void operator()(int i, double* part) {
-    const TD d = b[i] - ax[i];
+  __device__ __forceinline__ void acc(double* part, int i, TD, TD ax, TD) const {
+    const TD d = b[i] - ax;
     TS v;
     cvt(v, d);
-    r[i] = v;
+    bufs.rs(s->k)[i] = v;
     part[0] += norm2(d);
   }
 };
@@ -1616,6 +1631,7 @@ struct FinFastReliable {
     if (sqrt(rt) < s->eps * s->bsqrt || k >= s->max_iter - 1) {
       s->kfinal = k;
       s->done = 1;
+      if (s->hdone) *s->hdone = 1;
       return;
     }
     // the next prologue forms r_{k+1} = r_true - 0 * Ap and p_{k+1} = r_true + beta p_k
@@ -1625,6 +1641,7 @@ struct FinFastReliable {
     s->k = k + 1;
     s->pending = 0;
     s->done = 0;
+    if (s->hdone) *s->hdone = 0;
   }
 };
 // FP64 single-pass CG: the pending alpha p after convergence
@@ -1725,6 +1742,9 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
   h.eps = opt.eps;
   h.delta = mo.delta;
   h.hist = w.hist.as<double>();
+  PinnedSlots& pin = pinned();
+  __atomic_store_n(pin.hdone, 0, __ATOMIC_RELEASE);
+  h.hdone = pin.hdone_dev;
   w.upload(h);
   std::vector<DevBuf> keep;
   auto vec = [&](size_t bytes) {
@@ -1768,11 +1788,14 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
   };
   // the shadow copy starts as the initial state (constants included)
   if (sf) GLX_HIP(hipMemcpyAsync(shadow, w.ds, sizeof(SolveState), hipMemcpyDeviceToDevice, st));
+  // reliable update (gated on a pause): x64 += x32 + alpha p, then one FP64
+  // operator launch whose epilogue forms the true residual and its norm.
+  // Most groups skip it, so the gated pointwise launch uses a small grid.
   auto enq_reliable = [&]() {
     if constexpr (MIXED) {
-      launch_point(st, n, PtFastAccum<TD, TS>{x, xs, fb, w.ds}, FinNone{}, w.ws(), w.paused());
-      apply_typed<TD>(st, o, LINKS_F64, t64, x, w.ws(), w.paused());
-      launch_point(st, n, PtFastTrue<TD, TS>{fb, b, t64, w.ds}, FinFastReliable{w.ds}, w.ws(), w.paused());
+      launch_point(st, n, PtFastAccum<TD, TS>{x, xs, fb, w.ds}, FinNone{}, w.ws(), w.paused(), kGatedBlocks);
+      run_stencil<TD>(st, o, LINKS_F64, InPlain<TD>{x}, t64, EpiFastTrue<TD, TS>{fb, b, w.ds},
+                      FinFastReliable{w.ds}, w.ws(), w.paused());
     }
   };
   // the self-finalising form alternates state copies: an even number of
@@ -1796,25 +1819,44 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
     if (exec) GLX_HIP(hipGraphLaunch(exec, st));
     else enq_group();
   };
-  // pipelined: one group queued ahead of the one whose control word is read
-  PinnedSlots& pin = pinned();
+  // pipelined: one group queued ahead of the one being waited for.  With the
+  // self-finalising kernel the host reads the done flag the kernels store in
+  // host memory themselves, after an event recorded without the system-scope
+  // cache writeback (which would stall the following group); the other form
+  // copies the state back after each group.
   const long max_groups = 4L * (opt.max_iter / group + 2) + 16;
   long groups = 0;
-  Mirror m{0, 0, -1, 0};
-  int cur = 0;
-  auto submit = [&](int slot) {
-    launch();
-    groups++;
-    GLX_HIP(hipMemcpyAsync(&pin.m[slot], w.ds, sizeof(Mirror), hipMemcpyDeviceToHost, st));
-    GLX_HIP(hipEventRecord(pin.ev[slot], st));
-  };
-  if (opt.max_iter > 0) {
+  if (opt.max_iter > 0 && sf) {
+    int cur = 0;
+    auto submit = [&](int slot) {
+      launch();
+      groups++;
+      GLX_HIP(hipEventRecord(pin.ev_nf[slot], st));
+    };
+    submit(cur);
+    for (;;) {
+      submit(cur ^ 1);
+      GLX_HIP(hipEventSynchronize(pin.ev_nf[cur]));
+      if (__atomic_load_n(pin.hdone, __ATOMIC_ACQUIRE) == 1) break;
+      if (groups > max_groups) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        throw HipError("single-pass CG made no progress");
+      }
+      cur ^= 1;
+    }
+  } else if (opt.max_iter > 0) {
+    int cur = 0;
+    auto submit = [&](int slot) {
+      launch();
+      groups++;
+      GLX_HIP(hipMemcpyAsync(&pin.m[slot], w.ds, sizeof(Mirror), hipMemcpyDeviceToHost, st));
+      GLX_HIP(hipEventRecord(pin.ev[slot], st));
+    };
     submit(cur);
     for (;;) {
       submit(cur ^ 1);
       GLX_HIP(hipEventSynchronize(pin.ev[cur]));
-      m = pin.m[cur];
-      if (m.done == 1) break;
+      if (pin.m[cur].done == 1) break;
       if (groups > max_groups) {
         if (exec) (void)hipGraphExecDestroy(exec);
         throw HipError("single-pass CG made no progress");
