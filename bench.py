@@ -141,6 +141,71 @@ def stencil_beyond_mall(glx, L=4096):
             "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def cstencil_traffic(L: int, nc: int):
+    """HBM bytes per launch of k_cstencil for one bench shape, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary; None if absent."""
+    path = os.path.join(ROOT, "profiles", "k_cstencil_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    hit = False
+    with open(path) as f:
+        for line in f:
+            if line.startswith(f"L={L} nc={nc} "):
+                hit = True
+            elif hit and "traffic_bytes_per_launch:" in line:
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def general_stencil(glx):
+    """General stencil_2d (coarse_stencil.cpp:12-392) on the device: the
+    per-launch bytes are the coefficients (n_terms nc^2 16 B per site, read
+    once) plus the input and output vectors (2 nc 16 B per site).  Two shapes
+    of multigrid coarse operators: nc = 8 with two-link terms on 128^2
+    (218 MB of coefficients, beyond the last-level cache) and nc = 16
+    one-link on 64^2."""
+    out = []
+    rng = np.random.default_rng(9)
+    for L, nc, two in ((128, 8, True), (64, 16, False)):
+        V = L * L
+        blk = V * nc * nc
+        g = lambda k: (rng.standard_normal(k * blk) + 1j * rng.standard_normal(k * blk))
+        st = glx.StencilNc(L, L, nc, g(1), g(4), g(8) if two else None, 0.1)
+        h = glx.CStencilDev(st)
+        del st
+        v = (rng.standard_normal(V * nc) + 1j * rng.standard_normal(V * nc))
+        d_in, d_out = glx.DeviceArray(v), glx.DeviceArray(np.zeros_like(v))
+        ms = h.time(d_out, d_in, "all", reps=50)
+        nterms = 13 if two else 5
+        by = V * (nterms * nc * nc * 16 + 2 * nc * 16)
+        gbs = by / (ms * 1e-3) / 1e9
+        out.append({"L": L, "nc": nc, "two_link": two, "ms": ms, "bytes": by, "GBs": round(gbs, 1),
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": cstencil_traffic(L, nc)})
+        h.close()
+        del d_in, d_out
+    glx.release_cache()
+    return out
+
+
+def gauge_generation(glx, L=512, reps=10):
+    """Gaussian U(1) field generated in HBM (Philox4x32-10, Box-Muller) with
+    its FP32/FP16 copies, and the device mean plaquette, for 512^2."""
+    lat = glx.Lattice.gaussian(L, L, 6.25, seed=1)
+    glx.synchronize()
+    t0 = time.perf_counter()
+    for r in range(reps):
+        lat.regenerate(6.25, seed=r + 2)
+    glx.synchronize()
+    gen_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for r in range(reps):
+        p = lat.plaquette()
+    plaq_ms = (time.perf_counter() - t0) * 1e3 / reps
+    lat.close()
+    return {"L": L, "beta": 6.25, "generate_ms": round(gen_ms, 4), "plaquette_ms": round(plaq_ms, 4),
+            "plaquette": [p.real, p.imag], "note": "host-timed, each call synchronises"}
+
+
 def gradient_timing(glx, L=512, mass=0.05, reps=3):
     """Gradient of S = phi^dag (D^dag D)^-1 phi on the 512^2 workload
     (theta ~ N(0, 0.4^2), phi complex Gaussian): adjoint solve to 1e-10 +
@@ -269,6 +334,7 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the 512^2 gradient timing")
     ap.add_argument("--no-eo", action="store_true", help="skip the even/odd D x = b solves")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multishift workload")
+    ap.add_argument("--no-cstencil", action="store_true", help="skip the general stencil_2d timing")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
                          "for scaling rocprofv3 byte counters)")
@@ -383,6 +449,8 @@ def main():
     eo = None if args.no_eo else eo_solves(glx, lat, b, mass)
     grad = None if args.no_grad else gradient_timing(glx)
     batch = None if args.no_batch else batch_multishift(glx)
+    cst = None if args.no_cstencil else general_stencil(glx)
+    gauge = None if args.no_cstencil else gauge_generation(glx)
 
     cpu = None
     if not args.no_cpu:
@@ -447,6 +515,8 @@ def main():
         "eo_solve_D": eo,
         "gradient": grad,
         "batch_multishift": batch,
+        "general_stencil": cst,
+        "gauge_generation": gauge,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
