@@ -44,7 +44,7 @@ def make_inputs(L: int, sigma: float, seed: int):
     return links.astype(np.complex128), b.astype(np.complex128)
 
 
-def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0, grad=None, batch=None):
+def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0, grad=None, batch=None, small=None):
     """Reference CPU CG (oracle/_ref, built from the reference sources; the
     oracle restatement where that build is absent), one thread, on a bounded
     sample: a fixed number of iterations of the same 1024^2 solve, scaled to
@@ -94,6 +94,20 @@ def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0, grad=None, ba
         out["gradient_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
         out["gradient_sample"] = (f"full {Lg}x{Lg} gradient on one core: {impl} CG ({gi.iter} iterations) + D chi, "
                                   f"the oracle's force")
+    if small is not None:
+        gpu, inputs = small
+        L1, b1 = inputs["C1"]
+        L2, lk2, b2 = inputs["C2"]
+        for name, solver, op, mass, bb, lk, Ls in (("C1_laplace_cg", "cg", "laplace_real", 0.01, b1, None, L1),
+                                                   ("C2_normal_cg", "cg", "staggered_normal", 0.05, b2, lk2, L2),
+                                                   ("C2_bicgstab_D", "bicgstab", "staggered", 0.05, b2, lk2, L2)):
+            t0 = time.perf_counter()
+            xc, ci = orc.solve(solver, op, bb, None, links=lk, mass=mass, lx=Ls, ly=Ls, max_iter=100000,
+                               eps=1e-10, impl=impl)
+            dtc = time.perf_counter() - t0
+            xg = gpu[name]["x"]
+            out[name] = {"ms": round(dtc * 1e3, 2), "iters": ci.iter,
+                         "x_relerr_gpu_vs_cpu": float(np.linalg.norm(xg - xc) / np.linalg.norm(xc))}
     if batch is not None:
         Lb, mb = 128, 0.05
         rng = np.random.default_rng(8)
@@ -124,21 +138,78 @@ def cpu_model():
 
 def stencil_beyond_mall(glx, L=4096):
     """FP32 D^dag D (FP32 links) at 4096^2: 128 MiB per vector and 256 MiB of
-    links, beyond the 256 MB last-level cache, so the figure is HBM-bound."""
+    links, beyond the 256 MB last-level cache, so the figure is HBM-bound.
+    The launch shape of the operator alone is tuned here over rows per block
+    (the autotuner tunes the CG iteration kernel, which stages more data)."""
     V = L * L
     rng = np.random.default_rng(5)
     links = np.exp(1j * rng.normal(0.0, 0.5, size=2 * V)).astype(np.complex128)
     lat = glx.Lattice(links, L, L, f32=True, f16=False)
-    tuned = glx.autotune_normal(lat, reps=5)
     x = (rng.normal(size=V) + 1j * rng.normal(size=V)).astype(np.complex64)
     d_in = glx.DeviceArray(x)
     d_out = glx.DeviceArray(np.zeros_like(x))
-    ms = glx.time_apply(lat, "staggered_normal", 0.02, d_out, d_in, reps=50, fp32=True, link_prec=glx.LINKS_F32)
+    best = None
+    for rows in (32, 64, 96, 128):
+        glx.set_normal_rows(L, L, 8, rows)
+        ms = glx.time_apply(lat, "staggered_normal", 0.02, d_out, d_in, reps=50, fp32=True,
+                            link_prec=glx.LINKS_F32)
+        if best is None or ms < best[1]:
+            best = (rows, ms)
+    rows, ms = best
+    glx.set_normal_rows(L, L, 8, rows)
     gbs = 32 * V / (ms * 1e-3) / 1e9
+    del d_in, d_out
+    # the same operator in FP64 (64 B per site: 1 GiB per launch)
+    x64 = x.astype(np.complex128)
+    d_in, d_out = glx.DeviceArray(x64), glx.DeviceArray(np.zeros_like(x64))
+    best64 = None
+    for r64 in (32, 64, 128):
+        glx.set_normal_rows(L, L, 16, r64)
+        m64 = glx.time_apply(lat, "staggered_normal", 0.02, d_out, d_in, reps=30)
+        if best64 is None or m64 < best64[1]:
+            best64 = (r64, m64)
+    glx.set_normal_rows(L, L, 16, best64[0])
+    gbs64 = 64 * V / (best64[1] * 1e-3) / 1e9
     del lat, d_in, d_out
     glx.release_cache()
-    return {"L": L, "rows": tuned["rows32"], "fp32_ms": ms, "fp32_GBs": round(gbs, 1),
-            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return {"L": L, "rows": rows, "fp32_ms": ms, "fp32_GBs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "fp64_rows": best64[0], "fp64_ms": best64[1],
+            "fp64_GBs": round(gbs64, 1), "fp64_frac": round(gbs64 / HBM_PEAK_GBS, 4)}
+
+
+def small_configs(glx, reps=5):
+    """The reference's small workloads on the GPU (FP64, reference order of
+    operations): C1 real Laplace 64^2 (diagonal 4 + 0.01) with CG; C2
+    staggered 256^2 (m = 0.05, theta ~ N(0, 0.4^2)) with CG on D^dag D and
+    BiCGStab on D.  Host-timed solves on device-resident vectors (best of
+    `reps`); C1 is latency-bound, so no roofline fraction is given for it."""
+    out = {}
+    rng = np.random.default_rng(31)
+    L1 = 64
+    lat1 = glx.Lattice(np.ones(2 * L1 * L1, np.complex128), L1, L1, f32=False, f16=False)
+    b1 = rng.standard_normal(L1 * L1)
+    rng2 = np.random.default_rng(32)
+    L2 = 256
+    links2 = np.exp(1j * rng2.normal(0.0, 0.4, 2 * L2 * L2))
+    b2 = rng2.standard_normal(L2 * L2) + 1j * rng2.standard_normal(L2 * L2)
+    lat2 = glx.Lattice(links2, L2, L2, f32=False, f16=False)
+    cases = (("C1_laplace_cg", "cg", lat1, "laplace_real", 0.01, b1),
+             ("C2_normal_cg", "cg", lat2, "staggered_normal", 0.05, b2),
+             ("C2_bicgstab_D", "bicgstab", lat2, "staggered", 0.05, b2))
+    for name, solver, lat, op, mass, b in cases:
+        bd = glx.DeviceArray(b)
+        x = glx.DeviceArray(np.zeros_like(b))
+        best, info = 1e9, None
+        for _ in range(reps):
+            x.upload(np.zeros_like(b))
+            glx.synchronize()
+            t0 = time.perf_counter()
+            info = glx.dev_solve(solver, lat, op, mass, x, bd, max_iter=100000, eps=1e-10)
+            glx.synchronize()
+            best = min(best, time.perf_counter() - t0)
+        out[name] = {"ms": round(best * 1e3, 3), "iters": info.iter, "success": info.success,
+                     "L": lat.lx, "mass": mass, "x": x.download()}
+    return out, {"C1": (L1, b1), "C2": (L2, links2, b2)}
 
 
 def cstencil_traffic(L: int, nc: int):
@@ -450,12 +521,13 @@ def main():
     grad = None if args.no_grad else gradient_timing(glx)
     batch = None if args.no_batch else batch_multishift(glx)
     cst = None if args.no_cstencil else general_stencil(glx)
+    small = small_configs(glx)
     gauge = None if args.no_cstencil else gauge_generation(glx)
 
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(links, b, L, mass, info64.iter, grad=grad, batch=batch)
+            cpu = cpu_baseline(links, b, L, mass, info64.iter, grad=grad, batch=batch, small=small)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "unit": "ms", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
 
@@ -516,6 +588,7 @@ def main():
         "gradient": grad,
         "batch_multishift": batch,
         "general_stencil": cst,
+        "small_configs": {k: {kk: vv for kk, vv in v.items() if kk != "x"} for k, v in small[0].items()},
         "gauge_generation": gauge,
         "cpu_baseline": cpu,
     }

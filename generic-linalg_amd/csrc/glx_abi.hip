@@ -1197,7 +1197,7 @@ int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, dou
     std::vector<c128> ones(n, c128{1.0, 0.0});
     DevBuf b(size_t(n) * sizeof(c128)), x(size_t(n) * sizeof(c128));
     GLX_HIP(hipMemcpy(b.get(), ones.data(), size_t(n) * sizeof(c128), hipMemcpyHostToDevice));
-    const int cands[] = {0, 8, 12, 16, 24, 32, 48, 64};
+    const int cands[] = {0, 8, 12, 16, 24, 32, 48, 64, 96, 128};
     SolveOpts so;
     so.max_iter = reps;
     so.eps = 0.0;

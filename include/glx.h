@@ -402,7 +402,7 @@ int glx_set_tuning(int group, int use_graph);
  * tiled kernel. */
 int glx_set_normal_rows(int lx, int ly, int esize, int rows);
 int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
-/* Autotuner: for every candidate launch shape (tiled; 8..64 rows per block)
+/* Autotuner: for every candidate launch shape (tiled; 8..128 rows per block)
  * times `reps` iterations of the single-pass FP64 CG (glx_dev_cg_fused) and,
  * if the lattice has FP32 links, of the mixed-precision CG on D^dag D of the
  * lattice, records the fastest shape per precision in the table above and
