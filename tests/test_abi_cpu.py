@@ -98,9 +98,21 @@ def test_dispatcher_without_device():
 
 
 def test_dropin_driver_builds(tmp_path):
-    out = subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "dropin"), "-B"], capture_output=True, text=True)
+    d = os.path.join(ROOT, "tests", "dropin")
+    out = subprocess.run(["make", "-C", d, "-B"], capture_output=True, text=True)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert os.path.exists(os.path.join(ROOT, "tests", "dropin", "square_laplace_driver"))
+    for exe in ("square_laplace_driver", "u1_driver", "stencil_driver", "c_abi_check"):
+        assert os.path.exists(os.path.join(d, exe)), exe
+
+
+def test_header_is_plain_c():
+    """include/glx.h compiles as C99 with -pedantic -Werror (c_abi_check.c)
+    and the program runs without a device."""
+    d = os.path.join(ROOT, "tests", "dropin")
+    subprocess.run(["make", "-s", "-C", d, "c_abi_check"], check=True)
+    out = subprocess.run([os.path.join(d, "c_abi_check")], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert re.match(r"abi \d+ 1 1 1", out.stdout)
 
 
 def test_python_binding_imports():
