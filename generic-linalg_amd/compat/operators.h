@@ -48,4 +48,15 @@ void square_staggered_eoprec_reconstruct(complex<double>* lhs_full, complex<doub
 // square_laplace, tests/multishift/multishift.cpp:634-672).
 void square_laplace(double* lhs, double* rhs, void* extra_data);
 
+// the operators off the solver path (operators.cpp:127-846), applied on the
+// GPU as general stencils
+void square_staggered(complex<double>* lhs, complex<double>* rhs, void* extra_data);
+void square_staggered_gamma5(complex<double>* lhs, complex<double>* rhs, void* extra_data);
+// staggered D plus wilson_coeff times the two-link Laplacian
+void square_staggered_2linklaplace_u1(complex<double>* lhs, complex<double>* rhs, void* extra_data);
+void staggered_symmshift_x(complex<double>* lhs, complex<double>* rhs, void* extra_data);
+void staggered_symmshift_y(complex<double>* lhs, complex<double>* rhs, void* extra_data);
+// i D(m = 0) - m Gamma_5 with Gamma_5 = (i/2)(G_x G_y - G_y G_x) (arXiv 1410.5733)
+void staggered_index_operator(complex<double>* lhs, complex<double>* rhs, void* extra_data);
+
 #endif

@@ -185,6 +185,12 @@ struct BuiltinInit {
     r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc_tb), OP_CST_TB, nullptr);
     r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc_bt), OP_CST_BT, nullptr);
     r.add(reinterpret_cast<void*>(&glx_apply_stencil_2d_nc_m2mdeodoe), OP_CST_M2MDEODOE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered), GLX_OP_STAGGERED_FREE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_gamma5), GLX_OP_STAGGERED_GAMMA5_FREE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_square_staggered_2linklaplace_u1), GLX_OP_STAGGERED_2LINKLAPLACE, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_staggered_symmshift_x), GLX_OP_SYMMSHIFT_X, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_staggered_symmshift_y), GLX_OP_SYMMSHIFT_Y, nullptr);
+    r.add(reinterpret_cast<void*>(&glx_staggered_index_operator), GLX_OP_STAGGERED_INDEX, nullptr);
   }
 };
 BuiltinInit g_builtin_init;
@@ -258,6 +264,9 @@ void upload(DevBuf& buf, const void* host, size_t bytes) {
   GLX_HIP(hipMemcpy(buf.get(), host, bytes, hipMemcpyHostToDevice));
 }
 
+bool is_built_kind(int k);
+void build_op(int kind, const glx_staggered_u1_op& p, DevOperator& o, DevBuf& links, DevBuf& coef);
+
 // Build the device operator for a solve or a direct application.
 std::unique_ptr<Resolved> resolve(glx_matvec mv, void* extra, int size, bool cplx) {
   auto R = std::make_unique<Resolved>();
@@ -275,6 +284,10 @@ std::unique_ptr<Resolved> resolve(glx_matvec mv, void* extra, int size, bool cpl
   o.kind = e.kind;
   if (is_cst_kind(e.kind)) {
     cst_upload(cst_desc(e, extra), o, R->cs, R->cstmp);
+  } else if (is_built_kind(e.kind)) {
+    const glx_staggered_u1_op* p = e.has_params ? &e.params : static_cast<const glx_staggered_u1_op*>(extra);
+    if (!p) throw ArgError("operator needs extra_info (staggered_u1_op) or registered parameters");
+    build_op(e.kind, *p, o, R->links, R->cs);
   } else if (is_stencil_kind(e.kind)) {
     const glx_stencil_2d* st = static_cast<const glx_stencil_2d*>(extra);
     if (!st || !st->hopping || !st->clover) throw ArgError("stencil operator needs a glx_stencil_2d with coefficients");
@@ -505,6 +518,117 @@ int build_stencil(glx_stencil_2d* st, const glx_staggered_u1_op* op, double sg, 
     st->eo_shift[1] = 0.0;
     st->dof_shift[0] = st->dof_shift[1] = 0.0;
   });
+}
+
+// The operators of operators.h off the solver path, as nc = 1 general
+// stencils built on the device (13 terms, term-major: coef[t V + s]; for
+// nc = 1 this is also the kernel's column-major layout).  Term order:
+// clover, +x, +y, -x, -y, +2x, +x+y, +2y, -x+y, -2x, -x-y, -2y, +x-y.
+enum BuiltOp : int {
+  BUILT_STAG_FREE = GLX_OP_STAGGERED_FREE,        // operators.cpp:127-178
+  BUILT_G5_FREE = GLX_OP_STAGGERED_GAMMA5_FREE,   // operators.cpp:262-313
+  BUILT_2LINK = GLX_OP_STAGGERED_2LINKLAPLACE,    // operators.cpp:626-694
+  BUILT_SYMM_X = GLX_OP_SYMMSHIFT_X,              // operators.cpp:697-736
+  BUILT_SYMM_Y = GLX_OP_SYMMSHIFT_Y,              // operators.cpp:739-779
+  BUILT_INDEX = GLX_OP_STAGGERED_INDEX,           // operators.cpp:782-846
+};
+bool is_built_kind(int k) { return k >= GLX_OP_STAGGERED_FREE && k <= GLX_OP_STAGGERED_INDEX; }
+bool built_has_two(int k) { return k == BUILT_2LINK || k == BUILT_INDEX; }
+bool built_needs_links(int k) { return k != BUILT_STAG_FREE && k != BUILT_G5_FREE; }
+
+__global__ void k_build_op(int kind, int lx, int ly, const c128* __restrict__ u, double mass, double wc,
+                           c128* __restrict__ coef) {
+  const int V = lx * ly;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+    const int x = i % lx, y = i / lx;
+    const int xp = (x + 1) % lx, xm = (x - 1 + lx) % lx, yp = (y + 1) % ly, ym = (y - 1 + ly) % ly;
+    auto site = [&](int a, int b) { return b * lx + a; };
+    auto eta = [](int a) { return double(1 - 2 * (a % 2)); };
+    const c128 z{0.0, 0.0};
+    c128 t[kCsTerms];
+#pragma unroll
+    for (int k = 0; k < kCsTerms; k++) t[k] = z;
+    const double e = eta(x);
+    if (kind == BUILT_STAG_FREE || kind == BUILT_G5_FREE) {
+      const double eo = kind == BUILT_G5_FREE ? (((x + y) % 2 == 0) ? 1.0 : -1.0) : 1.0;
+      t[1] = {-0.5 * eo, 0.0};
+      t[2] = {-0.5 * eo * e, 0.0};
+      t[3] = {0.5 * eo, 0.0};
+      t[4] = {0.5 * eo * e, 0.0};
+    } else {
+      const c128 ux = u[2 * i], uy = u[2 * i + 1];
+      const c128 uxm = conj(u[2 * site(xm, y)]), uym = conj(u[2 * site(x, ym) + 1]);
+      if (kind == BUILT_SYMM_X) {
+        t[1] = 0.5 * ux;
+        t[3] = 0.5 * uxm;
+      } else if (kind == BUILT_SYMM_Y) {
+        t[2] = (0.5 * e) * uy;
+        t[4] = (0.5 * e) * uym;
+      } else if (kind == BUILT_2LINK) {
+        t[0] = {4.0 * wc, 0.0};
+        t[1] = -0.5 * ux;
+        t[2] = (-0.5 * e) * uy;
+        t[3] = 0.5 * uxm;
+        t[4] = (0.5 * e) * uym;
+        const int xm2 = (x - 2 + 2 * lx) % lx, ym2 = (y - 2 + 2 * ly) % ly;
+        t[5] = -wc * (ux * u[2 * site(xp, y)]);
+        t[7] = -wc * (uy * u[2 * site(x, yp) + 1]);
+        t[9] = -wc * (uxm * conj(u[2 * site(xm2, y)]));
+        t[11] = -wc * (uym * conj(u[2 * site(x, ym2) + 1]));
+      } else {  // BUILT_INDEX: i D(m = 0) - (i m / 2) (Gx Gy - Gy Gx), G the symmetric shifts
+        const c128 I{0.0, 1.0};
+        t[1] = I * (-0.5 * ux);
+        t[2] = I * ((-0.5 * e) * uy);
+        t[3] = I * (0.5 * uxm);
+        t[4] = I * ((0.5 * e) * uym);
+        // Gx Gy: the y shift acts at x +- 1 (its eta there), then the x shift
+        const c128 uxp_y = u[2 * site(xp, y) + 1], uxm_y = u[2 * site(xm, y) + 1];
+        const c128 uxp_ym = conj(u[2 * site(xp, ym) + 1]), uxm_ym = conj(u[2 * site(xm, ym) + 1]);
+        const double exp_ = eta(xp), exm = eta(xm);
+        const c128 a_pp = 0.25 * (ux * (exp_ * uxp_y)), a_pm = 0.25 * (ux * (exp_ * uxp_ym));
+        const c128 a_mp = 0.25 * (uxm * (exm * uxm_y)), a_mm = 0.25 * (uxm * (exm * uxm_ym));
+        // Gy Gx: the x shift acts at y +- 1, then the y shift (eta at x)
+        const c128 uyp_x = u[2 * site(x, yp)], uyp_xm = conj(u[2 * site(xm, yp)]);
+        const c128 uym_x = u[2 * site(x, ym)], uym_xm = conj(u[2 * site(xm, ym)]);
+        const c128 b_pp = 0.25 * ((e * uy) * uyp_x), b_mp = 0.25 * ((e * uy) * uyp_xm);
+        const c128 b_pm = 0.25 * ((e * uym) * uym_x), b_mm = 0.25 * ((e * uym) * uym_xm);
+        const c128 f = c128{0.0, -0.5 * mass};  // -(i m / 2)
+        t[6] = f * (a_pp - b_pp);   // +x+y
+        t[8] = f * (a_mp - b_mp);   // -x+y
+        t[10] = f * (a_mm - b_mm);  // -x-y
+        t[12] = f * (a_pm - b_pm);  // +x-y
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kCsTerms; k++) coef[size_t(k) * V + i] = t[k];
+  }
+}
+
+// Device operator for a built kind: links uploaded, coefficients built.
+void build_op(int kind, const glx_staggered_u1_op& p, DevOperator& o, DevBuf& links, DevBuf& coef) {
+  const int lx = p.x_fine, ly = p.y_fine;
+  if (lx <= 0 || ly <= 0) throw ArgError("lattice dimensions must be positive");
+  const size_t V = size_t(lx) * ly;
+  if (built_needs_links(kind)) {
+    if (!p.lattice) throw ArgError("operator needs links (staggered_u1_op.lattice)");
+    upload(links, p.lattice, 2 * V * 16);
+  }
+  coef = DevBuf(size_t(kCsTerms) * V * 16);
+  hipLaunchKernelGGL(k_build_op, dim3(point_blocks(int(V))), dim3(kBlock), 0, nullptr, kind, lx, ly,
+                     built_needs_links(kind) ? links.as<c128>() : nullptr, p.mass, p.wilson_coeff, coef.as<c128>());
+  GLX_CHECK_LAUNCH();
+  GLX_HIP(hipDeviceSynchronize());
+  o.kind = OP_CST;
+  o.lx = lx;
+  o.ly = ly;
+  o.nc = 1;
+  o.mass = p.mass;
+  o.cs_coef = coef.as<c128>();
+  o.cs_sdir = 0;
+  o.cs_has_two = built_has_two(kind);
+  o.shift = (kind == BUILT_STAG_FREE || kind == BUILT_2LINK) ? c128{p.mass, 0.0} : c128{0.0, 0.0};
+  o.eo_shift = kind == BUILT_G5_FREE ? c128{p.mass, 0.0} : c128{0.0, 0.0};
+  o.dof_shift = {0.0, 0.0};
 }
 
 // even/odd prepare and reconstruct on the device (operators.cpp:543-616,
@@ -805,6 +929,22 @@ void glx_apply_stencil_2d(double* lhs, double* rhs, void* st) { host_apply(&glx_
 void glx_apply_stencil_2d_eo(double* lhs, double* rhs, void* st) { host_apply(&glx_apply_stencil_2d_eo, lhs, rhs, st, true); }
 void glx_apply_stencil_2d_oe(double* lhs, double* rhs, void* st) { host_apply(&glx_apply_stencil_2d_oe, lhs, rhs, st, true); }
 void glx_apply_stencil_2d_nc(double* lhs, double* rhs, void* st) { cs_host_apply(&glx_apply_stencil_2d_nc, lhs, rhs, st); }
+void glx_square_staggered(double* lhs, double* rhs, void* e) { host_apply(&glx_square_staggered, lhs, rhs, e, true); }
+void glx_square_staggered_gamma5(double* lhs, double* rhs, void* e) {
+  host_apply(&glx_square_staggered_gamma5, lhs, rhs, e, true);
+}
+void glx_square_staggered_2linklaplace_u1(double* lhs, double* rhs, void* e) {
+  host_apply(&glx_square_staggered_2linklaplace_u1, lhs, rhs, e, true);
+}
+void glx_staggered_symmshift_x(double* lhs, double* rhs, void* e) {
+  host_apply(&glx_staggered_symmshift_x, lhs, rhs, e, true);
+}
+void glx_staggered_symmshift_y(double* lhs, double* rhs, void* e) {
+  host_apply(&glx_staggered_symmshift_y, lhs, rhs, e, true);
+}
+void glx_staggered_index_operator(double* lhs, double* rhs, void* e) {
+  host_apply(&glx_staggered_index_operator, lhs, rhs, e, true);
+}
 void glx_apply_stencil_2d_nc_eo(double* lhs, double* rhs, void* st) {
   cs_host_apply(&glx_apply_stencil_2d_nc_eo, lhs, rhs, st);
 }
@@ -842,7 +982,7 @@ void glx_apply_square_staggered_eoprec_reconstruct_stencil(double* lhs_full, dou
 int glx_register_operator(glx_matvec callback, int kind, const glx_staggered_u1_op* params) {
   return guarded([&] {
     if (!callback) throw ArgError("callback is NULL");
-    if (kind < 0 || kind > OP_STENCIL_M2MDEODOE) throw ArgError("unknown operator kind");
+    if ((kind < 0 || kind > OP_STENCIL_M2MDEODOE) && !is_built_kind(kind)) throw ArgError("unknown operator kind");
     Registry::get().add(reinterpret_cast<void*>(callback), kind, params);
   });
 }

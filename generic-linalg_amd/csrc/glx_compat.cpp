@@ -194,6 +194,22 @@ int get_stencil_size(op_type opt) {
   return 0;
 }
 void square_staggered_u1(complex<double>* l, complex<double>* r, void* e) { glx_square_staggered_u1(D(l), D(r), e); }
+void square_staggered(complex<double>* l, complex<double>* r, void* e) { glx_square_staggered(D(l), D(r), e); }
+void square_staggered_gamma5(complex<double>* l, complex<double>* r, void* e) {
+  glx_square_staggered_gamma5(D(l), D(r), e);
+}
+void square_staggered_2linklaplace_u1(complex<double>* l, complex<double>* r, void* e) {
+  glx_square_staggered_2linklaplace_u1(D(l), D(r), e);
+}
+void staggered_symmshift_x(complex<double>* l, complex<double>* r, void* e) {
+  glx_staggered_symmshift_x(D(l), D(r), e);
+}
+void staggered_symmshift_y(complex<double>* l, complex<double>* r, void* e) {
+  glx_staggered_symmshift_y(D(l), D(r), e);
+}
+void staggered_index_operator(complex<double>* l, complex<double>* r, void* e) {
+  glx_staggered_index_operator(D(l), D(r), e);
+}
 void gamma_5(complex<double>* l, complex<double>* r, void* e) { glx_gamma_5(D(l), D(r), e); }
 void square_staggered_gamma5_u1(complex<double>* l, complex<double>* r, void* e) {
   glx_square_staggered_gamma5_u1(D(l), D(r), e);
@@ -236,6 +252,12 @@ struct RegisterCompat {
     glx_register_operator(M(square_staggered_deo_u1), GLX_OP_DEO, nullptr);
     glx_register_operator(M(square_staggered_doe_u1), GLX_OP_DOE, nullptr);
     glx_register_operator(M(square_staggered_m2mdeodoe_u1), GLX_OP_M2MDEODOE, nullptr);
+    glx_register_operator(M(square_staggered), GLX_OP_STAGGERED_FREE, nullptr);
+    glx_register_operator(M(square_staggered_gamma5), GLX_OP_STAGGERED_GAMMA5_FREE, nullptr);
+    glx_register_operator(M(square_staggered_2linklaplace_u1), GLX_OP_STAGGERED_2LINKLAPLACE, nullptr);
+    glx_register_operator(M(staggered_symmshift_x), GLX_OP_SYMMSHIFT_X, nullptr);
+    glx_register_operator(M(staggered_symmshift_y), GLX_OP_SYMMSHIFT_Y, nullptr);
+    glx_register_operator(M(staggered_index_operator), GLX_OP_STAGGERED_INDEX, nullptr);
   }
 };
 RegisterCompat g_register_compat;

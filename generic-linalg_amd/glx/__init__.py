@@ -58,6 +58,12 @@ OP_SYMBOLS = {
     "stencil_eo": "glx_apply_stencil_2d_eo",
     "stencil_oe": "glx_apply_stencil_2d_oe",
     "stencil_m2mdeodoe": "glx_apply_square_staggered_m2mdeodoe_stencil",
+    "staggered_free": "glx_square_staggered",
+    "staggered_gamma5_free": "glx_square_staggered_gamma5",
+    "staggered_2linklaplace": "glx_square_staggered_2linklaplace_u1",
+    "symmshift_x": "glx_staggered_symmshift_x",
+    "symmshift_y": "glx_staggered_symmshift_y",
+    "index_operator": "glx_staggered_index_operator",
 }
 SOLVER_IDS = {"cg": 0, "bicgstab": 1, "gcr": 2}
 VERB_NONE, VERB_SUMMARY, VERB_RESTART_DETAIL, VERB_DETAIL = 0, 1, 2, 3
@@ -252,9 +258,9 @@ def op_callback(op: str) -> int:
 class _Extra:
     """Keeps the extra_info struct (and the arrays it points at) alive."""
 
-    def __init__(self, op: str, links, mass: float, lx: int, ly: int, nc: int = 1):
+    def __init__(self, op: str, links, mass: float, lx: int, ly: int, nc: int = 1, wilson: float = 0.0):
         self.links = None if links is None else np.ascontiguousarray(links, dtype=np.complex128)
-        self.st = StaggeredU1Op(None if self.links is None else self.links.ctypes.data, mass, lx, ly, nc, 0.0)
+        self.st = StaggeredU1Op(None if self.links is None else self.links.ctypes.data, mass, lx, ly, nc, wilson)
         self.ptr = C.cast(C.pointer(self.st), C.c_void_p)
         if op.startswith("stencil"):
             V = lx * ly
@@ -287,12 +293,13 @@ def _info(ci: InversionInfo, text: str = "", res=None) -> Info:
 
 
 # --------------------------------------------------------------------------
-def apply(op: str, rhs, links=None, mass: float = 0.0, lx: int = 0, ly: int = 0, nc: int = 1):
-    """lhs = A rhs on the GPU through the registered operator callback."""
+def apply(op: str, rhs, links=None, mass: float = 0.0, lx: int = 0, ly: int = 0, nc: int = 1, wilson: float = 0.0):
+    """lhs = A rhs on the GPU through the registered operator callback
+    (`wilson`: staggered_u1_op.wilson_coeff)."""
     real = op == "laplace_real"
     rhs = np.ascontiguousarray(rhs, dtype=np.float64 if real else np.complex128)
     lhs = np.empty_like(rhs)
-    ex = _Extra(op, links, mass, lx, ly, nc)
+    ex = _Extra(op, links, mass, lx, ly, nc, wilson)
     getattr(lib(), OP_SYMBOLS[op])(lhs.ctypes.data, rhs.ctypes.data, ex.ptr)
     if np.isnan(lhs).any():
         raise GlxError(f"{op} failed: {lib().glx_last_error().decode()}")
@@ -328,7 +335,7 @@ def stencil_coeffs(op: str, links, mass: float, lx: int, ly: int):
 
 
 def solve(solver: str, op: str, b, x0=None, *, links=None, mass=0.0, lx=0, ly=0, nc=1, max_iter=10000, eps=1e-10,
-          restart_freq=0, verb_level=0, prefix="", fmt="%g", callback=None, extra=None):
+          restart_freq=0, verb_level=0, prefix="", fmt="%g", callback=None, extra=None, wilson=0.0):
     """minv_vector_{cg,bicgstab,gcr}[_restart] with host vectors.
 
     `callback` / `extra` override the operator with a raw callback address and
@@ -337,7 +344,7 @@ def solve(solver: str, op: str, b, x0=None, *, links=None, mass=0.0, lx=0, ly=0,
     dt = np.float64 if real else np.complex128
     b = np.ascontiguousarray(b, dtype=dt)
     x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=dt, copy=True)
-    ex = None if callback is not None else _Extra(op, links, mass, lx, ly, nc)
+    ex = None if callback is not None else _Extra(op, links, mass, lx, ly, nc, wilson)
     cb = callback if callback is not None else op_callback(op)
     ep = extra if callback is not None else ex.ptr
     tx = _Text(verb_level, prefix, fmt)

@@ -113,6 +113,15 @@ void glx_square_staggered_normal_u1(double* lhs, double* rhs, void* extra_info);
 void glx_square_staggered_deo_u1(double* lhs, double* rhs, void* extra_info);     /* operators.cpp:456-496 */
 void glx_square_staggered_doe_u1(double* lhs, double* rhs, void* extra_info);     /* operators.cpp:499-539 */
 void glx_square_staggered_m2mdeodoe_u1(double* lhs, double* rhs, void* extra_info); /* operators.cpp:566-588 */
+/* the operators of operators.h off the solver path (operators.cpp:127-846),
+ * applied on the GPU as general stencils; wilson_coeff scales the two-link
+ * Laplacian term of the 2linklaplace operator */
+void glx_square_staggered(double* lhs, double* rhs, void* extra_info);
+void glx_square_staggered_gamma5(double* lhs, double* rhs, void* extra_info);
+void glx_square_staggered_2linklaplace_u1(double* lhs, double* rhs, void* extra_info);
+void glx_staggered_symmshift_x(double* lhs, double* rhs, void* extra_info);
+void glx_staggered_symmshift_y(double* lhs, double* rhs, void* extra_info);
+void glx_staggered_index_operator(double* lhs, double* rhs, void* extra_info);
 /* operators.cpp:543-563 and :592-616 */
 void glx_square_staggered_eoprec_prepare(double* rhs_e, double* rhs_orig, void* extra_info);
 void glx_square_staggered_eoprec_reconstruct(double* lhs_full, double* lhs_e, double* rhs_o, void* extra_info);
@@ -209,7 +218,15 @@ enum {
   GLX_OP_CSTENCIL_OE = 20,
   GLX_OP_CSTENCIL_TB = 21,
   GLX_OP_CSTENCIL_BT = 22,
-  GLX_OP_CSTENCIL_M2MDEODOE = 23
+  GLX_OP_CSTENCIL_M2MDEODOE = 23,
+  /* the other operators of operators.h (extra_info = glx_staggered_u1_op*),
+   * run as general stencils built on the device */
+  GLX_OP_STAGGERED_FREE = 24,          /* square_staggered, operators.cpp:127-178 */
+  GLX_OP_STAGGERED_GAMMA5_FREE = 25,   /* square_staggered_gamma5, :262-313 */
+  GLX_OP_STAGGERED_2LINKLAPLACE = 26,  /* square_staggered_2linklaplace_u1, :626-694 */
+  GLX_OP_SYMMSHIFT_X = 27,             /* staggered_symmshift_x, :697-736 */
+  GLX_OP_SYMMSHIFT_Y = 28,             /* staggered_symmshift_y, :739-779 */
+  GLX_OP_STAGGERED_INDEX = 29          /* staggered_index_operator, :782-846 */
 };
 
 /* Bind a caller's own host callback to a device operator, so that solves

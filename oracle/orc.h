@@ -115,6 +115,14 @@ void orc_staggered_eoprec_prepare(orc_c* rhs_e, orc_c* rhs_orig, void* extra);  
 void orc_staggered_m2mdeodoe_u1(orc_c* lhs, orc_c* rhs, void* extra);                     /* :566-588 */
 void orc_staggered_eoprec_reconstruct(orc_c* lhs_full, orc_c* lhs_e, orc_c* rhs_o, void* extra); /* :592-616 */
 
+/* ---- the other operators of operators.h (not on the solver path) ---- */
+void orc_staggered_free(orc_c* lhs, orc_c* rhs, void* extra);          /* operators.cpp:127-178 */
+void orc_staggered_gamma5_free(orc_c* lhs, orc_c* rhs, void* extra);   /* operators.cpp:262-313 */
+void orc_staggered_2linklaplace_u1(orc_c* lhs, orc_c* rhs, void* extra); /* operators.cpp:626-694 */
+void orc_staggered_symmshift_x(orc_c* lhs, orc_c* rhs, void* extra);   /* operators.cpp:697-736 */
+void orc_staggered_symmshift_y(orc_c* lhs, orc_c* rhs, void* extra);   /* operators.cpp:739-779 */
+void orc_staggered_index_operator(orc_c* lhs, orc_c* rhs, void* extra); /* operators.cpp:782-846 */
+
 /* ---- Lattice index math (lattice/lattice.h:52-139), nd = 2 ---- */
 int orc_coord_to_index(const int* dims, int nc, const int* coord, int color);
 void orc_index_to_coord(const int* dims, int nc, int i, int* coord, int* color);

@@ -42,6 +42,12 @@ OPS = {
     "stencil_eo": 14,
     "stencil_oe": 15,
     "stencil_m2mdeodoe": 16,
+    "staggered_free": 30,
+    "staggered_gamma5_free": 31,
+    "staggered_2linklaplace": 32,
+    "symmshift_x": 33,
+    "symmshift_y": 34,
+    "index_operator": 35,
 }
 SOLVERS = {"cg": 0, "bicgstab": 1, "gcr": 2}
 
@@ -219,6 +225,8 @@ def ref_lib():
         lib.ref_read_gauge_u1.argtypes = [vp, i, i, C.c_char_p]
         lib.ref_write_gauge_u1.argtypes = [vp, i, i, C.c_char_p]
         lib.ref_ape_smear_u1.argtypes = [vp, vp, i, i, d, i]
+        lib.ref_apply_wc.argtypes = [i, vp, vp, vp, d, d, i, i]
+        lib.ref_apply_wc.restype = i
         lib.ref_cstencil_apply.argtypes = [i, i, i, i, i, i, vp, vp, vp, vp, vp, vp, vp]
         lib.ref_cstencil_apply.restype = i
         lib.ref_cstencil_generate.argtypes = [i, vp, d, i, i, i, i, vp, vp, vp, vp, i, vp, vp, vp]
@@ -249,6 +257,12 @@ _ORC_OP_SYMS = {
     "stencil_eo": "orc_stencil_apply_eo",
     "stencil_oe": "orc_stencil_apply_oe",
     "stencil_m2mdeodoe": "orc_stencil_m2mdeodoe",
+    "staggered_free": "orc_staggered_free",
+    "staggered_gamma5_free": "orc_staggered_gamma5_free",
+    "staggered_2linklaplace": "orc_staggered_2linklaplace_u1",
+    "symmshift_x": "orc_staggered_symmshift_x",
+    "symmshift_y": "orc_staggered_symmshift_y",
+    "index_operator": "orc_staggered_index_operator",
 }
 
 
@@ -273,19 +287,26 @@ class OpContext:
             self.extra = C.cast(C.pointer(self.sten), C.c_void_p)
 
 
-def apply(op: str, rhs, links=None, mass: float = 0.0, lx: int = 0, ly: int = 0, nc: int = 1, impl: str = "orc"):
-    """lhs = A rhs with operator `op`; impl 'orc' (restatement) or 'ref'."""
+def apply(op: str, rhs, links=None, mass: float = 0.0, lx: int = 0, ly: int = 0, nc: int = 1, impl: str = "orc",
+          wilson: float = 0.0):
+    """lhs = A rhs with operator `op`; impl 'orc' (restatement) or 'ref';
+    `wilson` is staggered_u1_op.wilson_coeff (the two-link Laplacian term)."""
     real = op == "laplace_real"
     rhs = np.ascontiguousarray(rhs, dtype=np.float64 if real else np.complex128)
     lhs = np.empty_like(rhs)
     if impl == "ref":
         lib = ref_lib()
         lk = None if links is None else np.ascontiguousarray(links, dtype=np.complex128)
-        rc = lib.ref_apply(OPS[op], lhs.ctypes.data, rhs.ctypes.data, None if lk is None else lk.ctypes.data,
-                           mass, lx, ly, nc)
+        if wilson != 0.0:
+            rc = lib.ref_apply_wc(OPS[op], lhs.ctypes.data, rhs.ctypes.data, None if lk is None else lk.ctypes.data,
+                                  mass, wilson, lx, ly)
+        else:
+            rc = lib.ref_apply(OPS[op], lhs.ctypes.data, rhs.ctypes.data, None if lk is None else lk.ctypes.data,
+                               mass, lx, ly, nc)
         assert rc == 0
         return lhs
     ctx = OpContext(op, links, mass, lx, ly, nc)
+    ctx.st.wilson_coeff = wilson
     proto = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p, C.c_void_p)
     proto(ctx.fn.value)(lhs.ctypes.data, rhs.ctypes.data, ctx.extra)
     return lhs

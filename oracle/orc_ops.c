@@ -325,6 +325,124 @@ void orc_stencil_eoprec_reconstruct(orc_c* lhs_full, orc_c* lhs_e, orc_c* rhs_o,
 }
 
 /* ------------------------------------------------------------------------ */
+/* operators.cpp:127-846: the operators of operators.h off the solver path,  */
+/* each term added in the reference's order and association.                 */
+void orc_staggered_free(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, ly = op->y_fine;
+  for (int i = 0; i < lx * ly; i++) {
+    const int x = i % lx, y = i / lx;
+    const double eta1 = 1 - 2 * (x % 2);
+    orc_c v = CMPLX(0.0, 0.0);
+    v = orc_sub(v, rhs[y * lx + ((x + 1) % lx)]);
+    v = orc_add(v, rhs[y * lx + ((x + lx - 1) % lx)]);
+    v = orc_sub(v, orc_rmul(eta1, rhs[((y + 1) % ly) * lx + x]));
+    v = orc_add(v, orc_rmul(eta1, rhs[((y + ly - 1) % ly) * lx + x]));
+    v = orc_rmul(0.5, v);
+    lhs[i] = orc_add(v, orc_rmul(op->mass, rhs[i]));
+  }
+}
+void orc_staggered_gamma5_free(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int lx = op->x_fine, ly = op->y_fine;
+  for (int i = 0; i < lx * ly; i++) {
+    const int x = i % lx, y = i / lx;
+    const double eta1 = 1 - 2 * (x % 2);
+    const double eo = ((x + y) % 2 == 0) ? 1.0 : -1.0;
+    orc_c v = CMPLX(0.0, 0.0);
+    v = orc_sub(v, orc_rmul(eo, rhs[y * lx + ((x + 1) % lx)]));
+    v = orc_add(v, orc_rmul(eo, rhs[y * lx + ((x + lx - 1) % lx)]));
+    v = orc_sub(v, orc_rmul(eo * eta1, rhs[((y + 1) % ly) * lx + x]));
+    v = orc_add(v, orc_rmul(eo * eta1, rhs[((y + ly - 1) % ly) * lx + x]));
+    v = orc_rmul(0.5, v);
+    lhs[i] = orc_add(v, orc_rmul(eo * op->mass, rhs[i]));
+  }
+}
+void orc_staggered_2linklaplace_u1(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const orc_c* U = op->lattice;
+  const int lx = op->x_fine, ly = op->y_fine;
+  const double wc = op->wilson_coeff;
+  for (int i = 0; i < lx * ly; i++) {
+    const int x = i % lx, y = i / lx;
+    const double eta1 = 1 - 2 * (x % 2);
+    const int xp = (x + 1) % lx, xm = (x + lx - 1) % lx, xm2 = (x + lx - 2) % lx, xp2 = (x + 2) % lx;
+    const int yp = (y + 1) % ly, ym = (y + ly - 1) % ly, ym2 = (y + ly - 2) % ly, yp2 = (y + 2) % ly;
+    orc_c v = CMPLX(0.0, 0.0);
+    v = orc_sub(v, orc_cmul(U[y * lx * 2 + x * 2], rhs[y * lx + xp]));
+    v = orc_add(v, orc_cmul(orc_conj(U[y * lx * 2 + xm * 2]), rhs[y * lx + xm]));
+    v = orc_sub(v, orc_cmul(orc_rmul(eta1, U[y * lx * 2 + x * 2 + 1]), rhs[yp * lx + x]));
+    v = orc_add(v, orc_cmul(orc_rmul(eta1, orc_conj(U[ym * lx * 2 + x * 2 + 1])), rhs[ym * lx + x]));
+    v = orc_rmul(0.5, v);
+    v = orc_add(v, orc_rmul(op->mass, rhs[i]));
+    v = orc_add(v, orc_rmul(wc * 4.0, rhs[i]));
+    v = orc_sub(v, orc_cmul(orc_cmul(orc_rmul(wc, U[y * lx * 2 + x * 2]), U[y * lx * 2 + xp * 2]), rhs[y * lx + xp2]));
+    v = orc_sub(v, orc_cmul(orc_cmul(orc_rmul(wc, orc_conj(U[y * lx * 2 + xm * 2])), orc_conj(U[y * lx * 2 + xm2 * 2])),
+                            rhs[y * lx + xm2]));
+    v = orc_sub(v, orc_cmul(orc_cmul(orc_rmul(wc, U[y * lx * 2 + x * 2 + 1]), U[yp * lx * 2 + x * 2 + 1]),
+                            rhs[yp2 * lx + x]));
+    v = orc_sub(v, orc_cmul(orc_cmul(orc_rmul(wc, orc_conj(U[ym * lx * 2 + x * 2 + 1])),
+                                     orc_conj(U[ym2 * lx * 2 + x * 2 + 1])),
+                            rhs[ym2 * lx + x]));
+    lhs[i] = v;
+  }
+}
+void orc_staggered_symmshift_x(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const orc_c* U = op->lattice;
+  const int lx = op->x_fine, ly = op->y_fine;
+  for (int i = 0; i < lx * ly; i++) {
+    const int x = i % lx, y = i / lx;
+    const int xm = (x + lx - 1) % lx;
+    orc_c v = CMPLX(0.0, 0.0);
+    v = orc_add(v, orc_cmul(U[y * lx * 2 + x * 2], rhs[y * lx + ((x + 1) % lx)]));
+    v = orc_add(v, orc_cmul(orc_conj(U[y * lx * 2 + xm * 2]), rhs[y * lx + xm]));
+    lhs[i] = orc_rmul(0.5, v);
+  }
+}
+void orc_staggered_symmshift_y(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const orc_c* U = op->lattice;
+  const int lx = op->x_fine, ly = op->y_fine;
+  for (int i = 0; i < lx * ly; i++) {
+    const int x = i % lx, y = i / lx;
+    const double eta1 = 1 - 2 * (x % 2);
+    const int ym = (y + ly - 1) % ly;
+    orc_c v = CMPLX(0.0, 0.0);
+    v = orc_add(v, orc_cmul(orc_rmul(eta1, U[y * lx * 2 + x * 2 + 1]), rhs[((y + 1) % ly) * lx + x]));
+    v = orc_add(v, orc_cmul(orc_rmul(eta1, orc_conj(U[ym * lx * 2 + x * 2 + 1])), rhs[ym * lx + x]));
+    lhs[i] = orc_rmul(0.5, v);
+  }
+}
+void orc_staggered_index_operator(orc_c* lhs, orc_c* rhs, void* extra)
+{
+  const orc_stag_op* op = (const orc_stag_op*)extra;
+  const int V = op->x_fine * op->y_fine;
+  const orc_c ci = CMPLX(0.0, 1.0), cid2 = CMPLX(0.0, 0.5);
+  orc_c* tmp = (orc_c*)malloc(sizeof(orc_c) * (size_t)V);
+  orc_c* tmp2 = (orc_c*)malloc(sizeof(orc_c) * (size_t)V);
+  orc_stag_op nomass = *op;
+  nomass.mass = 0.0;
+  for (int i = 0; i < V; i++) lhs[i] = CMPLX(0.0, 0.0);
+  orc_staggered_u1(lhs, rhs, &nomass);
+  for (int i = 0; i < V; i++) lhs[i] = orc_cmul(lhs[i], ci);
+  orc_staggered_symmshift_y(tmp, rhs, extra);
+  orc_staggered_symmshift_x(tmp2, tmp, extra);
+  const orc_c mI = orc_rmul(op->mass, cid2);
+  for (int i = 0; i < V; i++) lhs[i] = orc_sub(lhs[i], orc_cmul(mI, tmp2[i]));
+  orc_staggered_symmshift_x(tmp, rhs, extra);
+  orc_staggered_symmshift_y(tmp2, tmp, extra);
+  for (int i = 0; i < V; i++) lhs[i] = orc_add(lhs[i], orc_cmul(mI, tmp2[i]));
+  free(tmp);
+  free(tmp2);
+}
+
+/* ------------------------------------------------------------------------ */
 /* stencil_2d for any Nc (coarse_stencil.cpp:12-1627).                          */
 /* term offsets (dx, dy): clover, hopping 0..3, two-link 0..7 */
 static const int cst_off[13][2] = {{0, 0},  {1, 0},  {0, 1},   {-1, 0}, {0, -1}, {2, 0}, {1, 1},

@@ -53,6 +53,13 @@ enum RefOp {
   OP_STENCIL_EO = 14,
   OP_STENCIL_OE = 15,
   OP_STENCIL_M2MDEODOE = 16,
+  // operators.h functions off the solver path
+  OP_STAG_FREE = 30,
+  OP_G5_FREE = 31,
+  OP_2LINK = 32,
+  OP_SYMM_X = 33,
+  OP_SYMM_Y = 34,
+  OP_INDEX = 35,
 };
 
 // Operator context: the staggered_u1_op the function-form operators read,
@@ -68,7 +75,7 @@ struct Ctx {
     op.y_fine = ly;
     op.Nc = nc;
     op.wilson_coeff = 0.0;
-    if (opid >= OP_STENCIL) {
+    if (opid >= OP_STENCIL && opid <= OP_STENCIL_M2MDEODOE) {
       int dims[2] = {lx, ly};
       lat = new Lattice(2, dims, 1);
       st = new stencil_2d(lat, 1);
@@ -84,7 +91,9 @@ struct Ctx {
     delete st;
     delete lat;
   }
-  void* extra(int opid) { return opid >= OP_STENCIL ? static_cast<void*>(st) : static_cast<void*>(&op); }
+  void* extra(int opid) {
+    return (opid >= OP_STENCIL && opid <= OP_STENCIL_M2MDEODOE) ? static_cast<void*>(st) : static_cast<void*>(&op);
+  }
 };
 
 typedef void (*cmv)(cd*, cd*, void*);
@@ -106,6 +115,12 @@ cmv complex_op(int opid) {
     case OP_STENCIL_EO: return apply_stencil_2d_eo;
     case OP_STENCIL_OE: return apply_stencil_2d_oe;
     case OP_STENCIL_M2MDEODOE: return apply_square_staggered_m2mdeodoe_stencil;
+    case OP_STAG_FREE: return square_staggered;
+    case OP_G5_FREE: return square_staggered_gamma5;
+    case OP_2LINK: return square_staggered_2linklaplace_u1;
+    case OP_SYMM_X: return staggered_symmshift_x;
+    case OP_SYMM_Y: return staggered_symmshift_y;
+    case OP_INDEX: return staggered_index_operator;
   }
   return nullptr;
 }
@@ -156,6 +171,16 @@ int ref_apply(int opid, double* lhs, double* rhs, double* links, double mass, in
     square_laplace(lhs, rhs, &c.op);
     return 0;
   }
+  cmv f = complex_op(opid);
+  if (!f) return -1;
+  f(reinterpret_cast<cd*>(lhs), reinterpret_cast<cd*>(rhs), c.extra(opid));
+  return 0;
+}
+
+// A function-form operator with a Wilson coefficient (the two-link Laplacian term).
+int ref_apply_wc(int opid, double* lhs, double* rhs, double* links, double mass, double wc, int lx, int ly) {
+  Ctx c(links, mass, lx, ly, 1, opid);
+  c.op.wilson_coeff = wc;
   cmv f = complex_op(opid);
   if (!f) return -1;
   f(reinterpret_cast<cd*>(lhs), reinterpret_cast<cd*>(rhs), c.extra(opid));

@@ -25,6 +25,10 @@ OPS = ["laplace", "laplace_u1", "staggered", "staggered_dagger", "staggered_norm
        "deo", "doe", "m2mdeodoe", "stencil", "stencil_dagger", "stencil_gamma5", "stencil_eo", "stencil_oe",
        "stencil_m2mdeodoe"]
 
+# operators.h functions off the solver path (wilson_coeff 0.7 for the two-link term)
+EXTRA_OPS = ["staggered_free", "staggered_gamma5_free", "staggered_2linklaplace", "symmshift_x", "symmshift_y",
+             "index_operator"]
+
 # (piece, sdir) pairs of the general-stencil fixtures
 CST_CASES = [("all", 0), ("all", 1), ("all", 3), ("all", 7), ("eo", 0), ("oe", 4), ("eo", 9), ("tb", 0), ("bt", 0),
              ("tb", 12), ("bt", 1)]
@@ -60,6 +64,8 @@ def main():
     arrays["op_in_cplx_nc2"] = vc2
     for op in OPS:
         arrays[f"op_out_{op}"] = orc.apply(op, v, links, m, lx, ly, impl="ref")
+    for op in EXTRA_OPS:
+        arrays[f"op_out_{op}"] = orc.apply(op, v, links, m, lx, ly, impl="ref", wilson=0.7)
     arrays["op_out_laplace_real_nc2"] = orc.apply("laplace_real", vr, None, m, lx, ly, 2, impl="ref")
     arrays["op_out_laplace_nc2"] = orc.apply("laplace", vc2, None, m, lx, ly, 2, impl="ref")
     for st in (False, True):
