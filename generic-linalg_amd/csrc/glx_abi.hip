@@ -1245,6 +1245,16 @@ int glx_set_tuning(int group, int use_graph) {
   });
 }
 
+int glx_set_sstep(int s, int rows) {
+  return guarded([&] { set_sstep(s, rows); });
+}
+int glx_get_sstep(int* s, int* rows) {
+  return guarded([&] {
+    if (s) *s = sstep_default();
+    if (rows) *rows = sstep_rows();
+  });
+}
+
 int glx_dev_cg_fused(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
                      double eps, glx_verbose* verb, glx_inversion_info* info) {
   return guarded([&] {

@@ -76,7 +76,17 @@ std::vector<SolveResult> dev_cg_m_batched(const std::vector<BatchSystem>& sys, i
 struct MixedOpts {
   int link_prec = LINKS_F32;
   double delta = 0.1;
+  // 2: two CG iterations per launch (s-step form) where the operator is the
+  // full-lattice D^dag D on a lattice that fits its strips; 1: one per
+  // launch; 0: the process default (set_sstep, initially 1)
+  int sstep = 0;
 };
+
+// Process-wide default of MixedOpts::sstep for the ABI's mixed solves, and the
+// rows per block of the s-step kernel (a multiple of 4, >= 8; 0 = 16).
+void set_sstep(int s, int rows);
+int sstep_default();
+int sstep_rows();
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m);
 
 // FP64 CG in the single-pass form the mixed solver uses (one fused stencil
@@ -87,8 +97,9 @@ SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const So
 
 // Live timing (HIP events) of the single-pass CG's fused iteration kernel on
 // the staggered normal operator: prec 0 = FP64, LINKS_F32 / LINKS_F16 = the
-// mixed solver's FP32 kernel with that link storage.  Returns ms per launch
-// and the algorithmic HBM bytes per lattice site of one launch.
+// mixed solver's FP32 kernel with that link storage, 2 + LINKS_F32 /
+// 2 + LINKS_F16 = the s-step kernel (two iterations per launch).  Returns ms
+// per launch and the algorithmic HBM bytes per lattice site of one launch.
 double time_cg_kernel(const DevOperator& op, int prec, int reps, double* bytes_per_site);
 
 // Gradient of S(theta, m) = phi^dag (D^dag D)^-1 phi (no reference

@@ -13,10 +13,12 @@
 // zeta, ...) are computed on the device by the reduction finalisers and read
 // by the next kernel; the host reads the 16-byte control word once per chunk.
 #include "glx_solvers.h"
+#include "glx_sstep.h"
 
 #include <tuple>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 
@@ -100,7 +102,7 @@ struct SolveState {
   int max_iter, resid_freq, worst_first, n_shift;
   double eps, bsqrt, rsq, rsqNew;
   double beta_r, resSq, maxrr, delta;  // maxrr: single-pass CG: max of |r|^2 since the last reliable update
-  int reliable, pending, pad2, pad3;  // pending: self-finalising single-pass CG
+  int reliable, pending, ko, pad3;  // pending: self-finalising single-pass CG; ko: s-step CG launches
   c128 alpha, alpha_prev, beta, beta_prev, rho, omega, pap;
   double* hist;
   double* resvec;
@@ -1577,6 +1579,7 @@ struct FinFastInit {
     s->alpha = {0.0, 0.0};
     s->beta_r = 0.0;
     s->k = 0;
+    s->ko = 0;
     s->done = 0;
     s->pending = 0;
     if (s->hdone) *s->hdone = 0;
@@ -1628,7 +1631,7 @@ struct FinFastReliable {
     const int k = s->k;
     s->reliable += 1;
     s->resSq = rt;
-    if (sqrt(rt) < s->eps * s->bsqrt || k >= s->max_iter - 1) {
+    if (rt == 0.0 || sqrt(rt) < s->eps * s->bsqrt || k >= s->max_iter - 1) {
       s->kfinal = k;
       s->done = 1;
       if (s->hdone) *s->hdone = 1;
@@ -1892,6 +1895,238 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
   return res;
 }
 
+
+// ====================================== s-step CG (mixed precision, s = 2) ===
+// The mixed solver with two CG iterations per launch (k_sscg, glx_sstep.h) on
+// the full-lattice D^dag D.  The reliable updates are the single-pass
+// solver's: x64 += x32, then one FP64 operator launch whose epilogue writes
+// the true residual into R0 of the basis slot the next launch reads.
+
+namespace {
+std::atomic<int> g_sstep{1};
+std::atomic<int> g_sstep_rows{16};
+}  // namespace
+void set_sstep(int s, int rows) {
+  if (s != 1 && s != 2) throw ArgError("s-step: s must be 1 or 2");
+  if (rows != 0 && (rows < 8 || rows % 4 != 0)) throw ArgError("s-step: rows must be 0 or a multiple of 4, >= 8");
+  g_sstep.store(s);
+  g_sstep_rows.store(rows ? rows : 16);
+}
+int sstep_default() { return g_sstep.load(); }
+int sstep_rows() { return g_sstep_rows.load(); }
+
+static bool sscg_fits(const DevOperator& o) {
+  return o.kind == OP_STAG_NORMAL && o.tiled() && o.lx >= 128 && o.lx % 128 == 0 && o.ly >= 8 && o.ly % 4 == 0;
+}
+
+// setup: R0 of slot 0 = b - A x in FP32, with |b|^2 and |r|^2
+struct PtSsInit {
+  static constexpr int NR = 2;
+  c64* r;
+  const c128* b;
+  const c128* ax;
+  __device__ void init() {}
+  __device__ void operator()(int i, double* part) {
+    const c128 bi = b[i];
+    const c128 ri = bi - ax[i];
+    c64 v;
+    cvt(v, ri);
+    r[i] = v;
+    part[0] += norm2(bi);
+    part[1] += norm2(ri);
+  }
+};
+// reliable update, step 1 (gated on a pause): x64 += x32; x32 = 0
+struct PtSsAccum {
+  static constexpr int NR = 0;
+  c128* x;
+  c64* xs;
+  __device__ void init() {}
+  __device__ void operator()(int i, double*) {
+    c128 d;
+    cvt(d, xs[i]);
+    x[i] = x[i] + d;
+    xs[i] = zero_of<c64>();
+  }
+};
+// step 2 epilogue: r_true = b - A x in FP64, written (rounded) as R0 of the
+// slot the next launch reads; |r_true|^2 decides convergence
+struct EpiSsTrue {
+  static constexpr int NR = 1;
+  SsBufs bf;
+  const c128* b;
+  const SolveState* s;
+  __device__ __forceinline__ void acc(double* part, int i, c128, c128 ax, c128) const {
+    const c128 d = b[i] - ax;
+    c64 v;
+    cvt(v, d);
+    bf.v(s->ko, 3)[i] = v;
+    part[0] += norm2(d);
+  }
+};
+
+// One s-step launch with the link storage of `link_prec`.
+static void launch_sscg(hipStream_t st, const DevOperator& o, int link_prec, const NGeom& g, SsBufs bf,
+                        const SolveState* rd, SolveState* wr, int q, RedWs ws) {
+  if (link_prec == LINKS_F16) {
+    if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");
+    const OpDev<LinksF16> od = make_opdev(o, LinksF16{o.u16, o.sc16});
+    hipLaunchKernelGGL((k_sscg<LinksF16, SolveState>), dim3(g.nblk), dim3(kBlock), 0, st, g, od, bf, rd, wr, q, ws);
+  } else {
+    if (!o.u32) throw ArgError("operator has no FP32 links on the device");
+    const OpDev<LinksF32> od = make_opdev(o, LinksF32{o.u32});
+    hipLaunchKernelGGL((k_sscg<LinksF32, SolveState>), dim3(g.nblk), dim3(kBlock), 0, st, g, od, bf, rd, wr, q, ws);
+  }
+  GLX_CHECK_LAUNCH();
+}
+
+// The s-step basis and FP32 solution increment of one solve, zeroed.
+struct SsSpace {
+  DevBuf y, xs;
+  SsBufs bf;
+  SsSpace(int n, hipStream_t st) : y(size_t(10) * n * sizeof(c64)), xs(size_t(n) * sizeof(c64)) {
+    GLX_HIP(hipMemsetAsync(y.get(), 0, size_t(10) * n * sizeof(c64), st));
+    GLX_HIP(hipMemsetAsync(xs.get(), 0, size_t(n) * sizeof(c64), st));
+    bf = SsBufs{y.as<c64>(), xs.as<c64>(), long(n)};
+  }
+};
+
+SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveOpts& opt, const MixedOpts& mo) {
+  const int n = o.size();
+  OwnedStream own;
+  hipStream_t st = opt.stream ? opt.stream : own.st;
+  const NGeom g = make_ngeom(o.lx, o.ly, sstep_rows(), 2);
+  // two buffers of 16 sums per block
+  Work w(st, std::max(2 * g.nblk, stencil_blocks(o.lx, o.ly)), opt.max_iter, opt.want_hist);
+  SolveState h;
+  std::memset(&h, 0, sizeof(h));
+  h.max_iter = opt.max_iter;
+  h.eps = opt.eps;
+  h.delta = mo.delta;
+  h.hist = w.hist.as<double>();
+  PinnedSlots& pin = pinned();
+  __atomic_store_n(pin.hdone, 0, __ATOMIC_RELEASE);
+  h.hdone = pin.hdone_dev;
+  w.upload(h);
+  SsSpace sp(n, st);
+  DevBuf t64buf(size_t(n) * sizeof(c128));
+  c128* t64 = t64buf.as<c128>();
+  // R0 of slot 0 = b - A x; P0 of slot 0 = 0 (the first launch is a restart with beta 0)
+  apply_typed<c128>(st, o, LINKS_F64, t64, x, w.ws(), kNoGate);
+  launch_point(st, n, PtSsInit{sp.bf.v(0, 3), b, t64}, FinFastInit{w.ds}, w.ws(), kNoGate);
+  DevBuf shadow_buf(sizeof(SolveState));
+  SolveState* shadow = shadow_buf.as<SolveState>();
+  GLX_HIP(hipMemcpyAsync(shadow, w.ds, sizeof(SolveState), hipMemcpyDeviceToDevice, st));
+  auto enq_iter = [&](int q) {
+    const SolveState* rd = q == 0 ? w.ds : shadow;
+    SolveState* wr = q == 0 ? shadow : w.ds;
+    launch_sscg(st, o, mo.link_prec, g, sp.bf, rd, wr, q, w.ws());
+  };
+  auto enq_reliable = [&]() {
+    launch_point(st, n, PtSsAccum{x, sp.bf.x}, FinNone{}, w.ws(), w.paused(), kGatedBlocks);
+    run_stencil<c128>(st, o, LINKS_F64, InPlain<c128>{x}, t64, EpiSsTrue{sp.bf, b, w.ds}, FinFastReliable{w.ds},
+                      w.ws(), w.paused());
+  };
+  // launches alternate state copies: an even number per group ends every
+  // group on the home copy, which the reliable update uses
+  const int group = 2 * ((std::max(1, opt.group) + 1) / 2);
+  auto enq_group = [&]() {
+    for (int k = 0; k < group; k++) enq_iter(k & 1);
+    enq_reliable();
+  };
+  hipGraphExec_t exec = nullptr;
+  if (opt.use_graph) {
+    hipGraph_t graph = nullptr;
+    GLX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enq_group();
+    GLX_HIP(hipStreamEndCapture(st, &graph));
+    GLX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    GLX_HIP(hipGraphDestroy(graph));
+  }
+  auto launch = [&]() {
+    if (exec) GLX_HIP(hipGraphLaunch(exec, st));
+    else enq_group();
+  };
+  // one group queued ahead of the one waited for; the kernels store the done
+  // flag into host memory
+  const long max_groups = 4L * (opt.max_iter / group + 2) + 16;
+  long groups = 0;
+  if (opt.max_iter > 0) {
+    int cur = 0;
+    auto submit = [&](int slot) {
+      launch();
+      groups++;
+      GLX_HIP(hipEventRecord(pin.ev_nf[slot], st));
+    };
+    submit(cur);
+    for (;;) {
+      submit(cur ^ 1);
+      GLX_HIP(hipEventSynchronize(pin.ev_nf[cur]));
+      if (__atomic_load_n(pin.hdone, __ATOMIC_ACQUIRE) == 1) break;
+      if (groups > max_groups) {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        throw HipError("s-step CG made no progress");
+      }
+      cur ^= 1;
+    }
+  }
+  GLX_HIP(hipStreamSynchronize(st));
+  if (exec) GLX_HIP(hipGraphExecDestroy(exec));
+  SolveState hs = w.download();
+  SolveResult res;
+  res.name = "CG-mixed";
+  if (opt.max_iter <= 0) {
+    res.iter = 1;
+    res.success = true;
+    return res;
+  }
+  res.success = hs.resSq == 0.0 || sqrt(hs.resSq) < opt.eps * hs.bsqrt;
+  res.ops_count = hs.kfinal + 2 + hs.reliable;
+  res.iter = hs.kfinal + 1;
+  res.resSq = hs.resSq;
+  res.bsqrt = hs.bsqrt;
+  res.reliable_updates = hs.reliable;
+  if (opt.want_hist) res.relres_hist = w.hist_host(res.iter);
+  return res;
+}
+
+// Live timing of the s-step kernel alone: `reps` launches between two HIP
+// events, from p = 0 and a constant r (every float 0x3c3c3c3c, about 0.0115)
+// with tolerance, delta and max_iter zero, so that no launch stops.  Returns
+// ms per launch.
+double time_sscg_kernel_impl(const DevOperator& o, int link_prec, int reps) {
+  const int n = o.size();
+  OwnedStream own;
+  hipStream_t st = own.st;
+  const NGeom g = make_ngeom(o.lx, o.ly, sstep_rows(), 2);
+  Work w(st, std::max(2 * g.nblk, stencil_blocks(o.lx, o.ly)), 0, false);
+  SsSpace sp(n, st);
+  GLX_HIP(hipMemsetAsync(sp.bf.v(0, 3), 0x3c, size_t(n) * sizeof(c64), st));
+  DevBuf shadow_buf(sizeof(SolveState));
+  SolveState* shadow = shadow_buf.as<SolveState>();
+  GLX_HIP(hipMemcpyAsync(shadow, w.ds, sizeof(SolveState), hipMemcpyDeviceToDevice, st));
+  int q = 0;
+  auto launch = [&]() {
+    const SolveState* rd = q == 0 ? w.ds : shadow;
+    SolveState* wr = q == 0 ? shadow : w.ds;
+    launch_sscg(st, o, link_prec, g, sp.bf, rd, wr, q, w.ws());
+    q ^= 1;
+  };
+  launch();  // warm-up (a restart; the rest read all five vectors)
+  hipEvent_t a, b;
+  GLX_HIP(hipEventCreate(&a));
+  GLX_HIP(hipEventCreate(&b));
+  GLX_HIP(hipEventRecord(a, st));
+  for (int k = 0; k < reps; k++) launch();
+  GLX_HIP(hipEventRecord(b, st));
+  GLX_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  GLX_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return double(ms) / reps;
+}
+
 // ============================================================ dispatch ====
 static bool real_op(const DevOperator& o) {
   return o.is_host() ? !o.host_complex : o.kind == OP_LAPLACE_REAL;
@@ -1918,6 +2153,8 @@ SolveResult dev_cg_m(const DevOperator& op, void* const* xs, const void* b, int 
                          resid_freq_check, worst_first, o);
 }
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m) {
+  if (!real_op(op) && (m.sstep ? m.sstep : sstep_default()) == 2 && sscg_fits(op))
+    return sscg_impl(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
   if (real_op(op))
     return fastcg_impl<double, float, true>(op, static_cast<double*>(x), static_cast<const double*>(b), o, m);
   return fastcg_impl<c128, c64, true>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
@@ -2504,6 +2741,13 @@ double time_cg_kernel(const DevOperator& o, int prec, int reps, double* bytes_pe
     // reads p, r, Ap (old slots), x; writes r, p, Ap (new slots), x; links
     if (bytes_per_site) *bytes_per_site = 3 * 16 + 16 + 3 * 16 + 16 + 2 * 16;
     return time_fast_kernel_impl<c128, c128, false>(o, LINKS_F64, reps);
+  }
+  if (prec == 2 + LINKS_F32 || prec == 2 + LINKS_F16) {
+    // reads the five basis vectors, x and the links; writes five vectors and x
+    if (!sscg_fits(o)) throw ArgError("the s-step kernel needs lx % 128 == 0, ly % 4 == 0, ly >= 8");
+    const int lp = prec - 2;
+    if (bytes_per_site) *bytes_per_site = 5 * 8 + 8 + (lp == LINKS_F16 ? 2 * 4 + 4 : 2 * 8) + 5 * 8 + 8;
+    return time_sscg_kernel_impl(o, lp, reps);
   }
   if (prec == LINKS_F16) {
     if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");

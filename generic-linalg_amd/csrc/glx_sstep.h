@@ -1,0 +1,534 @@
+// glx_sstep.h -- two CG iterations per launch: s-step CG (s = 2, the
+// communication-avoiding CG of Carson and Demmel with a monomial basis) on the
+// staggered normal operator A = D^dag D = m^2 - K^2 (operators.cpp:444-453),
+// complex FP32 vectors.  The mixed-precision solver runs it between FP64
+// reliable updates.
+//
+// Launch `ko` holds the basis of the previous launch in slot ko & 1,
+//   Y = [P0, P1, P2, R0, R1] = [p, A p, A^2 p, r, A r],
+// and writes the next basis into slot (ko + 1) & 1:
+//   prelude  every block sums the previous launch's 16 partial sums (in block
+//            order, so all blocks agree) into the Gram matrix G = Y^H Y, and
+//            performs two CG iterations on coordinate 5-vectors in FP64:
+//              p' = e0, r' = e3, x' = 0, delta = G33
+//              twice: w' = B p' (B shifts e0 -> e1 -> e2, e3 -> e4)
+//                     alpha = delta / p'^H G w'; x' += alpha p'; r' -= alpha w'
+//                     delta' = r'^H G r'; beta = delta' / delta; p' = r' + beta p'
+//            A Hermitian gives <A^a u, A^b v> = <u, A^(a+b) v>, so G comes
+//            from 5 + 4 + 3 sums (mu_d over p, nu_d over p and r, rho_d over r).
+//   stream   p = Y c_p, r = Y c_r and x += Y c_x site by site (FP32), then
+//            P1 = A p, R1 = A r and P2 = A P1 in one row-marching pass
+//   sums     mu_0..4, nu_0..3, rho_0..2 of the new basis, per block
+// A launch whose iterations stop (the residual fell by delta since the last
+// reliable update, or reached the tolerance or zero, or max_iter) forms only x and
+// the direction p of the stopping iteration (into P0 of the new slot); the
+// reliable update then recomputes r = b - A x in FP64 into R0 of that slot,
+// and the next launch restarts from P0 and R0 alone (c_p = e3 + beta e0,
+// c_r = e3, c_x = 0) without a CG step.
+//
+// Pass structure (a block of 4 waves owns a strip of 128 columns and sr rows;
+// lane l holds columns x0 + 2l, x0 + 2l + 1; each step of the march covers
+// four rows, one per wave, with a lag of one row per stage):
+//   S0  rows 4k-4 .. 4k-1   stage p, r (from Y) and the links      cols x0-4 .. x0+131
+//   S1  rows 4k-5 .. 4k-2   qp = K p, qr = K r                       cols x0-3 .. x0+130
+//   S2  rows 4k-6 .. 4k-3   P1 = m^2 p - K qp;  R1 = m^2 r - K qr    cols x0-2 .. x0+129; R1 own
+//   S3  rows 4k-7 .. 4k-4   q1 = K P1                                cols x0-1 .. x0+128
+//   S4  rows 4k-8 .. 4k-5   P2 = m^2 P1 - K q1, the sums             own columns
+// LDS rings of 8 rows hold every stage's rows (slot = (y + 8) & 7).  S4's
+// first row needs U_y of the row above the 8 in the ring (it shares a slot
+// with the newest staged row), so S0 saves that row's U_y into a one-row
+// buffer before overwriting it.  A strip takes sr/4 + 2 steps.
+#pragma once
+
+#include "glx_normal.h"
+
+namespace glx {
+
+constexpr int kSsNR = 16;  // partial sums per block
+
+// Two slots of the five basis vectors and the FP32 solution increment.
+struct SsBufs {
+  c64* y;  // slot s, vector j at y + (5 (s & 1) + j) n
+  c64* x;
+  long n;
+  __host__ __device__ __forceinline__ c64* v(int slot, int j) const { return y + (5 * (slot & 1) + j) * n; }
+};
+
+// <Y_i, Y_j> from the sums: 0..4 mu_d, 5..12 nu_d (re, im), 13..15 rho_d.
+struct SsGram {
+  const double* v;
+  __device__ __forceinline__ c128 at(int i, int j) const {
+    const bool pi = i < 3, pj = j < 3;
+    const int d = (pi ? i : i - 3) + (pj ? j : j - 3);
+    if (pi && pj) return {v[d], 0.0};
+    if (!pi && !pj) return {v[13 + d], 0.0};
+    const c128 nu{v[5 + 2 * d], v[6 + 2 * d]};
+    return pi ? nu : conj(nu);
+  }
+};
+// a^H G b
+__device__ __forceinline__ c128 ss_form(const SsGram& g, const c128 (&a)[5], const c128 (&b)[5]) {
+  c128 s{0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    c128 t{0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 5; j++) t = t + g.at(i, j) * b[j];
+    s = s + cmulc(a[i], t);
+  }
+  return s;
+}
+
+// What the prelude decided for this launch.
+struct SsPlan {
+  c128 cp[5], cr[5], cx[5];
+  bool stencil;  // false: the iterations stopped; form x and p only
+  int k, done, pending, steps;
+  double rsq, maxrr;
+  double hist[2];
+};
+
+// The two iterations on coordinates (v = the 16 sums of the basis in slot
+// ko & 1), from iteration k0 with maxrr2 = max |r|^2 since the last reliable
+// update.
+template <class St>
+__device__ __forceinline__ void ss_iterate(const double* v, const St* s, int k0, double maxrr2, SsPlan& pl) {
+  const SsGram g{v};
+  c128 pp[5], rr[5], xx[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    pp[i] = {i == 0 ? 1.0 : 0.0, 0.0};
+    rr[i] = {i == 3 ? 1.0 : 0.0, 0.0};
+    xx[i] = {0.0, 0.0};
+  }
+  double dl = v[13];
+  const double tol = s->eps * s->bsqrt;
+  const double dd = s->delta * s->delta;
+  pl.stencil = true;
+  pl.done = 0;
+  pl.pending = 1;
+  pl.steps = 0;
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const c128 w[5] = {{0.0, 0.0}, pp[0], pp[1], {0.0, 0.0}, pp[3]};
+    const c128 pw = ss_form(g, pp, w);
+    const double den = pw.re * pw.re + pw.im * pw.im;
+    const double gg = den > 0.0 ? dl / den : 0.0;
+    const c128 a{gg * pw.re, -gg * pw.im};  // dl / pw
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      xx[i] = xx[i] + a * pp[i];
+      rr[i] = rr[i] - a * w[i];
+    }
+    double dn = ss_form(g, rr, rr).re;
+    if (!(dn > 0.0)) dn = 0.0;
+    const int k = k0 + j;
+    pl.hist[j] = dn;
+    pl.steps = j + 1;
+    if (dn > maxrr2) maxrr2 = dn;
+    if (dn < dd * maxrr2 || dn < tol * tol || dn == 0.0 || k == s->max_iter - 1) {
+      // stop after iteration k: x and this iteration's direction are formed,
+      // the reliable update continues from them
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        pl.cp[i] = pp[i];
+        pl.cr[i] = rr[i];
+        pl.cx[i] = xx[i];
+      }
+      pl.stencil = false;
+      pl.done = 2;
+      pl.pending = 0;
+      pl.k = k;
+      pl.rsq = dl;
+      pl.maxrr = maxrr2;
+      return;
+    }
+    const double beta = dn / dl;
+#pragma unroll
+    for (int i = 0; i < 5; i++) pp[i] = rr[i] + beta * pp[i];
+    dl = dn;
+  }
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    pl.cp[i] = pp[i];
+    pl.cr[i] = rr[i];
+    pl.cx[i] = xx[i];
+  }
+  pl.k = k0 + 2;
+  pl.rsq = dl;
+  pl.maxrr = maxrr2;
+}
+
+// The fields a launch changes, carried from the state it read to the one it
+// writes (the constants were copied once when the solve started).
+template <class St>
+__device__ __forceinline__ void ss_carry(const St* rd, St* wr) {
+  wr->k = rd->k;
+  wr->ko = rd->ko;
+  wr->done = rd->done;
+  wr->kfinal = rd->kfinal;
+  wr->pending = rd->pending;
+  wr->rsq = rd->rsq;
+  wr->rsqNew = rd->rsqNew;
+  wr->maxrr = rd->maxrr;
+  wr->beta_r = rd->beta_r;
+  wr->alpha = rd->alpha;
+}
+
+__device__ __forceinline__ c64 ss_c64(c128 a) { return {float(a.re), float(a.im)}; }
+// sum_j c_j y_j over the basis (only P0 and R0 unless `full`)
+__device__ __forceinline__ c64 ss_comb(const c64 (&c)[5], const c64 (&y)[5], bool full) {
+  c64 s = c[0] * y[0] + c[3] * y[3];
+  if (full) s = s + c[1] * y[1] + c[2] * y[2] + c[4] * y[4];
+  return s;
+}
+
+template <class LK, class St>
+__global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q,
+                                                 RedWs ws) {
+  using T = c64;
+  using C = typename LK::C;
+  constexpr int CPL = 2;
+  constexpr int NXB = kNX * CPL;  // 128 columns per strip
+  constexpr int WP = NXB + 8;     // p, r: columns x0-4 .. x0+NXB+3
+  constexpr int WL = NXB + 8;     // links: columns x0-4 .. x0+NXB+2 (last slot unused)
+  constexpr int WQ = NXB + 6;     // qp, qr: x0-3 .. x0+NXB+2
+  constexpr int W1 = NXB + 4;     // P1: x0-2 .. x0+NXB+1
+  constexpr int W2 = NXB + 2;     // q1: x0-1 .. x0+NXB
+  __shared__ T sP[8][WP];
+  __shared__ T sR[8][WP];
+  __shared__ C sLx[8][WL];
+  __shared__ C sLy[8][WL];
+  __shared__ T sQp[8][WQ];
+  __shared__ T sQr[8][WQ];
+  __shared__ T sP1[8][W1];
+  __shared__ T sQ1[8][W2];
+  __shared__ T sR1[8][NXB];
+  __shared__ C sKeep[NXB];
+
+  // a finished solve (or a pending reliable update): carry the state over
+  if (rd->done != 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      ss_carry(rd, wr);
+      if (rd->hdone) *rd->hdone = rd->done;
+    }
+    return;
+  }
+  const int pending = rd->pending;
+  const bool full = pending != 0;  // the read slot holds all five vectors
+  const int k0 = rd->k, ko = rd->ko;
+  const int nblk = gridDim.x;
+
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int tile = xcd_tile(blockIdx.x, g.nblk);
+  const bool active = tile < g.ntiles;
+  const int tt = active ? tile : 0;  // grid padding only joins the sums
+  const int lx = g.lx, ly = g.ly;
+  const int x0 = (tt % g.ntx) * NXB;
+  const int y0 = (tt / g.ntx) * g.sr;
+  const int rows = min(g.sr, ly - y0);  // a multiple of 4
+  const int nsteps = rows / 4 + 2;
+  const int xm = x0 + CPL * l;
+  const int so = ko & 1, sn = so ^ 1;
+  const c64* Y[5] = {bf.v(so, 0), bf.v(so, 1), bf.v(so, 2), bf.v(so, 3), bf.v(so, 4)};
+  c64* N[5] = {bf.v(sn, 0), bf.v(sn, 1), bf.v(sn, 2), bf.v(sn, 3), bf.v(sn, 4)};
+
+  // the previous launch's sums: issued first (loads complete in order)
+  constexpr int kPreU = 2;
+  double pv[kPreU][kSsNR];
+  const double* PS = ws.partials + (q ^ 1) * kSsNR * nblk;
+  if (full) {
+#pragma unroll
+    for (int u = 0; u < kPreU; u++) {
+      const int bb = threadIdx.x + u * kBlock;
+#pragma unroll
+      for (int k = 0; k < kSsNR; k++) pv[u][k] = bb < nblk ? PS[k * nblk + bb] : 0.0;
+    }
+  }
+
+  // halo roles: wave 0 lanes 0..31 stage p, r on columns x0-4 .. x0-1 and
+  // x0+NXB .. x0+NXB+3 of the step's four rows; wave 1 lanes 0..27 stage the
+  // links on columns x0-4 .. x0-1 and x0+NXB .. x0+NXB+2
+  const bool hp = w == 0 && l < 32;
+  const bool hl = w == 1 && l < 28;
+  const int hp_row = l >> 3, hp_c = (l & 7) < 4 ? (l & 7) : NXB + (l & 7);
+  const int hl_row = l / 7, hl_c = (l % 7) < 4 ? (l % 7) : NXB + (l % 7);
+  const int xhp = wrap_coord(x0 - 4 + hp_c, lx);
+  const int xhl = wrap_coord(x0 - 4 + hl_c, lx);
+
+  Pair2<c64> ry[5], rx;
+  LinkRegs<LK, CPL> lm;
+  c64 hy[5];
+  typename LK::Raw lh;
+  auto fetch = [&](int k) {
+    const int yr = 4 * k - 4 + w;
+    const int j = wrap_coord(y0 + yr, ly) * lx + xm;
+    const bool own = yr >= 0 && yr < rows;
+    ry[0] = ld2(Y[0], j);
+    ry[3] = ld2(Y[3], j);
+    if (full) {
+      ry[1] = ld2(Y[1], j);
+      ry[2] = ld2(Y[2], j);
+      ry[4] = ld2(Y[4], j);
+      if (own) rx = ld2(bf.x, j);
+    }
+    lm.fetch(op.links, j);
+    if (hp) {
+      const int jh = wrap_coord(y0 + 4 * k - 4 + hp_row, ly) * lx + xhp;
+      hy[0] = Y[0][jh];
+      hy[3] = Y[3][jh];
+      if (full) {
+        hy[1] = Y[1][jh];
+        hy[2] = Y[2][jh];
+        hy[4] = Y[4][jh];
+      }
+    }
+    if (hl) lh = op.links.fetch(wrap_coord(y0 + 4 * k - 4 + hl_row, ly) * lx + xhl);
+  };
+  if (active) fetch(0);
+
+  // prelude: the coefficients of this launch
+  __shared__ double sm[kSsNR][kWaves];
+  SsPlan pl;
+  if (full) {
+    double acc[kSsNR];
+#pragma unroll
+    for (int k = 0; k < kSsNR; k++) acc[k] = pv[0][k];
+#pragma unroll
+    for (int u = 1; u < kPreU; u++) {
+#pragma unroll
+      for (int k = 0; k < kSsNR; k++) acc[k] += pv[u][k];
+    }
+    for (int b0 = threadIdx.x + kBlock * kPreU; b0 < nblk; b0 += kBlock) {
+#pragma unroll
+      for (int k = 0; k < kSsNR; k++) acc[k] += PS[k * nblk + b0];
+    }
+    block_sum_all<kSsNR>(acc, sm);
+    ss_iterate(acc, rd, k0, rd->maxrr, pl);
+  } else {
+    // restart from P0 (the last direction) and R0 (the true residual)
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      pl.cp[i] = {i == 3 ? 1.0 : (i == 0 ? rd->beta_r : 0.0), 0.0};
+      pl.cr[i] = {i == 3 ? 1.0 : 0.0, 0.0};
+      pl.cx[i] = {0.0, 0.0};
+    }
+    pl.stencil = true;
+    pl.done = 0;
+    pl.pending = 1;
+    pl.steps = 0;
+    pl.k = k0;
+    pl.rsq = rd->rsq;
+    pl.maxrr = rd->maxrr;
+  }
+  c64 cp[5], cr[5], cx[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    cp[i] = ss_c64(pl.cp[i]);
+    cr[i] = ss_c64(pl.cr[i]);
+    cx[i] = ss_c64(pl.cx[i]);
+  }
+
+  if (!pl.stencil) {
+    // stopped: x += Y c_x and P0' = Y c_p on the owned sites (a stop follows
+    // a launch that wrote all five vectors)
+    if (active) {
+      for (int yr = w; yr < rows; yr += 4) {
+        const int j = (y0 + yr) * lx + xm;
+        Pair2<c64> a[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) a[i] = ld2(Y[i], j);
+        const Pair2<c64> xo = ld2(bf.x, j);
+        const c64 y0v[5] = {a[0].a, a[1].a, a[2].a, a[3].a, a[4].a};
+        const c64 y1v[5] = {a[0].b, a[1].b, a[2].b, a[3].b, a[4].b};
+        st2(N[0], j, ss_comb(cp, y0v, true), ss_comb(cp, y1v, true));
+        st2(bf.x, j, xo.a + ss_comb(cx, y0v, true), xo.b + ss_comb(cx, y1v, true));
+      }
+    }
+  } else {
+    const float m2 = float(op.mass * op.mass);
+    double part[kSsNR];
+#pragma unroll
+    for (int k = 0; k < kSsNR; k++) part[k] = 0.0;
+    for (int k = 0; k < (active ? nsteps : 0); k++) {
+      if (k > 0) __syncthreads();  // step k-1's readers are done with the slots reused below
+      {
+        // S0: rows 4k-4 .. 4k-1
+        const int yr = 4 * k - 4 + w;
+        const int j = wrap_coord(y0 + yr, ly) * lx + xm;
+        const bool own = yr >= 0 && yr < rows;
+        const int sl = (yr + 8) & 7;
+        const c64 y0v[5] = {ry[0].a, ry[1].a, ry[2].a, ry[3].a, ry[4].a};
+        const c64 y1v[5] = {ry[0].b, ry[1].b, ry[2].b, ry[3].b, ry[4].b};
+        const c64 p0 = ss_comb(cp, y0v, full), p1 = ss_comb(cp, y1v, full);
+        const c64 r0 = ss_comb(cr, y0v, full), r1 = ss_comb(cr, y1v, full);
+        if (own) {
+          st2(N[0], j, p0, p1);
+          st2(N[3], j, r0, r1);
+          if (full) st2(bf.x, j, rx.a + ss_comb(cx, y0v, true), rx.b + ss_comb(cx, y1v, true));
+        }
+        sP[sl][CPL * l + 4] = p0;
+        sP[sl][CPL * l + 5] = p1;
+        sR[sl][CPL * l + 4] = r0;
+        sR[sl][CPL * l + 5] = r1;
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+          C ux, uy;
+          LK::finish(lm.r[c], ux, uy);
+          // wave 3 stages row 4k-1 into the slot of row 4k-9, whose U_y the
+          // first S4 row of this step still needs
+          if (w == 3) sKeep[CPL * l + c] = sLy[sl][CPL * l + c + 4];
+          sLx[sl][CPL * l + c + 4] = ux;
+          sLy[sl][CPL * l + c + 4] = uy;
+        }
+        if (hp) {
+          const int hs = (4 * k - 4 + hp_row + 8) & 7;
+          sP[hs][hp_c] = ss_comb(cp, hy, full);
+          sR[hs][hp_c] = ss_comb(cr, hy, full);
+        }
+        if (hl) {
+          const int hs = (4 * k - 4 + hl_row + 8) & 7;
+          C hx, hyy;
+          LK::finish(lh, hx, hyy);
+          sLx[hs][hl_c] = hx;
+          sLy[hs][hl_c] = hyy;
+        }
+      }
+      if (k + 1 < nsteps) fetch(k + 1);
+      __syncthreads();
+      {
+        // S1: qp = K p, qr = K r on rows 4k-5 .. 4k-2 (needed: -3 .. rows+2)
+        const int y = 4 * k - 5 + w;
+        if (y >= -3 && y <= rows + 2) {
+          const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
+          auto q1hop = [&](int qc) {
+            const int pc = qc + 1;  // p and link column of q column qc
+            const bool eta_neg = ((qc + 1) & 1) != 0;  // x = x0 - 3 + qc
+            sQp[s0][qc] = khop<T, C>(sP[s0][pc + 1], sP[s0][pc - 1], sP[sp1][pc], sP[sm1][pc], sLx[s0][pc],
+                                     sLx[s0][pc - 1], sLy[s0][pc], sLy[sm1][pc], eta_neg);
+            sQr[s0][qc] = khop<T, C>(sR[s0][pc + 1], sR[s0][pc - 1], sR[sp1][pc], sR[sm1][pc], sLx[s0][pc],
+                                     sLx[s0][pc - 1], sLy[s0][pc], sLy[sm1][pc], eta_neg);
+          };
+#pragma unroll
+          for (int c = 0; c < CPL; c++) q1hop(CPL * l + c + 3);
+          if (l < 6) q1hop(l < 3 ? l : NXB + l);
+        }
+      }
+      __syncthreads();
+      {
+        // S2: P1 = m^2 p - K qp (rows -2 .. rows+1), R1 = m^2 r - K qr (own)
+        const int y = 4 * k - 6 + w;
+        if (y >= -2 && y <= rows + 1) {
+          const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
+          const bool own = y >= 0 && y < rows;
+          auto p1hop = [&](const T (*Q)[WQ], const T (*F)[WP], int c1) {
+            const int qc = c1 + 1, pc = c1 + 2;  // x = x0 - 2 + c1
+            const T kq = khop<T, C>(Q[s0][qc + 1], Q[s0][qc - 1], Q[sp1][qc], Q[sm1][qc], sLx[s0][pc],
+                                    sLx[s0][pc - 1], sLy[s0][pc], sLy[sm1][pc], (c1 & 1) != 0);
+            return rscal(m2, F[s0][pc]) - kq;
+          };
+          T v1[CPL], v4[CPL];
+#pragma unroll
+          for (int c = 0; c < CPL; c++) {
+            const int c1 = CPL * l + c + 2;
+            v1[c] = p1hop(sQp, sP, c1);
+            sP1[s0][c1] = v1[c];
+          }
+          if (l < 4) {
+            const int c1 = l < 2 ? l : NXB + l;
+            sP1[s0][c1] = p1hop(sQp, sP, c1);
+          }
+          if (own) {
+#pragma unroll
+            for (int c = 0; c < CPL; c++) {
+              v4[c] = p1hop(sQr, sR, CPL * l + c + 2);
+              sR1[s0][CPL * l + c] = v4[c];
+            }
+            const int j = (y0 + y) * lx + xm;
+            st2(N[1], j, v1[0], v1[1]);
+            st2(N[4], j, v4[0], v4[1]);
+          }
+        }
+      }
+      __syncthreads();
+      {
+        // S3: q1 = K P1 on rows -1 .. rows
+        const int y = 4 * k - 7 + w;
+        if (y >= -1 && y <= rows) {
+          const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
+          auto q2hop = [&](int c2) {
+            const int lc = c2 + 3;  // x = x0 - 1 + c2; P1 column c2 + 1
+            sQ1[s0][c2] = khop<T, C>(sP1[s0][c2 + 2], sP1[s0][c2], sP1[sp1][c2 + 1], sP1[sm1][c2 + 1], sLx[s0][lc],
+                                     sLx[s0][lc - 1], sLy[s0][lc], sLy[sm1][lc], (c2 & 1) == 0);
+          };
+#pragma unroll
+          for (int c = 0; c < CPL; c++) q2hop(CPL * l + c + 1);
+          if (l < 2) q2hop(l == 0 ? 0 : NXB + 1);
+        }
+      }
+      __syncthreads();
+      {
+        // S4: P2 = m^2 P1 - K q1 and the sums on the own rows
+        const int y = 4 * k - 8 + w;
+        if (y >= 0 && y < rows) {
+          const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
+          T v2[CPL];
+#pragma unroll
+          for (int c = 0; c < CPL; c++) {
+            const int c2 = CPL * l + c + 1, lc = CPL * l + c + 4, c1 = CPL * l + c + 2;
+            const C uym = w == 0 ? sKeep[CPL * l + c] : sLy[sm1][lc];
+            const T kq = khop<T, C>(sQ1[s0][c2 + 1], sQ1[s0][c2 - 1], sQ1[sp1][c2], sQ1[sm1][c2], sLx[s0][lc],
+                                    sLx[s0][lc - 1], sLy[s0][lc], uym, c == 1);
+            const T pP1 = sP1[s0][c1];
+            const T p2 = rscal(m2, pP1) - kq;
+            v2[c] = p2;
+            const T pp = sP[s0][lc], rr = sR[s0][lc], r1 = sR1[s0][CPL * l + c];
+            part[0] += norm2(pp);
+            part[1] += dotc(pp, pP1).re;
+            part[2] += norm2(pP1);
+            part[3] += dotc(pP1, p2).re;
+            part[4] += norm2(p2);
+            const c128 n0 = dotc(pp, rr), n1 = dotc(pp, r1), n2 = dotc(pP1, r1), n3 = dotc(p2, r1);
+            part[5] += n0.re;
+            part[6] += n0.im;
+            part[7] += n1.re;
+            part[8] += n1.im;
+            part[9] += n2.re;
+            part[10] += n2.im;
+            part[11] += n3.re;
+            part[12] += n3.im;
+            part[13] += norm2(rr);
+            part[14] += dotc(rr, r1).re;
+            part[15] += norm2(r1);
+          }
+          st2(N[2], (y0 + y) * lx + xm, v2[0], v2[1]);
+        }
+      }
+    }
+    block_partials<kSsNR>(part, RedWs{ws.partials + q * kSsNR * nblk}, blockIdx.x);
+  }
+
+  // block 0 writes the state of the next launch
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ss_carry(rd, wr);
+    wr->k = pl.k;
+    wr->ko = ko + 1;
+    wr->rsq = pl.rsq;
+    wr->maxrr = pl.maxrr;
+    wr->pending = pl.pending;
+    // (constant indices only: a dynamically indexed member would move the
+    // plan to scratch memory)
+    if (rd->hist) {
+      if (pl.steps > 0) rd->hist[k0] = sqrt(pl.hist[0]) / rd->bsqrt;
+      if (pl.steps > 1) rd->hist[k0 + 1] = sqrt(pl.hist[1]) / rd->bsqrt;
+    }
+    if (pl.done) {
+      wr->kfinal = pl.k;
+      wr->done = pl.done;
+      wr->rsqNew = pl.steps > 1 ? pl.hist[1] : pl.hist[0];
+    }
+    if (rd->hdone) *rd->hdone = wr->done;
+  }
+}
+
+}  // namespace glx

@@ -199,6 +199,10 @@ def lib():
         L.glx_dev_cg_fused.argtypes = [vp, i, d, vp, vp, i, d, VP, IP]
         L.glx_set_tuning.restype = i
         L.glx_set_tuning.argtypes = [i, i]
+        L.glx_set_sstep.restype = i
+        L.glx_set_sstep.argtypes = [i, i]
+        L.glx_get_sstep.restype = i
+        L.glx_get_sstep.argtypes = [C.POINTER(i), C.POINTER(i)]
         L.glx_dev_time_cg_kernel.restype = i
         L.glx_dev_time_cg_kernel.argtypes = [vp, d, i, i, C.POINTER(d), C.POINTER(d)]
         L.glx_dev_staggered_grad.restype = i
@@ -565,6 +569,19 @@ def set_tuning(group: int = 16, use_graph: bool = False):
     _check(lib().glx_set_tuning(group, int(use_graph)), "glx_set_tuning")
 
 
+def set_sstep(s: int = 1, rows: int = 0):
+    """Mixed-precision CG on D^dag D: s = 2 runs two CG iterations per launch
+    (s-step form) where the lattice allows, s = 1 (default) one; rows per
+    block of the s-step kernel (0 = 16)."""
+    _check(lib().glx_set_sstep(s, rows), "glx_set_sstep")
+
+
+def get_sstep():
+    s, rows = C.c_int(), C.c_int()
+    _check(lib().glx_get_sstep(C.byref(s), C.byref(rows)), "glx_get_sstep")
+    return s.value, rows.value
+
+
 def set_normal_rows(lx: int, ly: int, esize: int, rows: int):
     """Rows per block of the row-marching D^dag D kernel for an lx x ly
     lattice and element size (16 complex FP64, 8 complex FP32); 0 = tiled
@@ -691,7 +708,9 @@ def time_apply(lat, op: str, mass: float, out: DeviceArray, inp: DeviceArray, re
 
 def time_cg_kernel(lat, mass: float, prec: int = 0, reps: int = 100):
     """(ms per launch, algorithmic bytes per site) of the single-pass CG's
-    fused iteration kernel: prec 0 FP64, LINKS_F32 / LINKS_F16 mixed FP32."""
+    fused iteration kernel: prec 0 FP64, LINKS_F32 / LINKS_F16 mixed FP32;
+    3 / 4 the mixed solver's s-step kernel (two iterations per launch) with
+    FP32 / FP16 links."""
     ms, by = C.c_double(0.0), C.c_double(0.0)
     _check(lib().glx_dev_time_cg_kernel(lat.h, mass, prec, reps, C.byref(ms), C.byref(by)),
            "glx_dev_time_cg_kernel")

@@ -409,6 +409,14 @@ int glx_dev_eo_reconstruct(glx_lattice lat, double mass, double* x_dev, const do
  * self-finalising) and whether a group is captured once as a hipGraph and
  * replayed (default 0: measured slightly slower than direct launches). */
 int glx_set_tuning(int group, int use_graph);
+/* Iterations per launch of the mixed-precision CG on D^dag D (no reference
+ * counterpart): s = 2 runs two CG iterations per launch in s-step form
+ * (basis p, Ap, A^2 p, r, Ar and its Gram matrix, FP32 vectors, FP64
+ * coordinates) where lx % 128 == 0, ly % 4 == 0 and ly >= 8; s = 1 (default)
+ * one iteration per launch.  rows: rows per block of the s-step kernel (a
+ * multiple of 4, >= 8; 0 = 16). */
+int glx_set_sstep(int s, int rows);
+int glx_get_sstep(int* s, int* rows);
 /* Launch shape of the fused normal operator D^dag D (no reference
  * counterpart; operators.cpp:444-453 is the operator it computes): rows per
  * block of the row-marching kernel for an lx x ly lattice and vector element
@@ -438,7 +446,8 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
 /* Live timing (HIP events around `reps` back-to-back launches on one stream)
  * of the single-pass CG's fused iteration kernel on D^dag D of the lattice:
  * prec 0 = FP64 (glx_dev_cg_fused), 1 / 2 = the mixed-precision CG's FP32
- * kernel with FP32 / FP16 links.  Returns ms per launch and the algorithmic
+ * kernel with FP32 / FP16 links, 3 / 4 = its s-step kernel (two iterations
+ * per launch) with FP32 / FP16 links.  Returns ms per launch and the algorithmic
  * HBM bytes per site one launch moves.  No reference counterpart. */
 int glx_dev_time_cg_kernel(glx_lattice lat, double mass, int prec, int reps, double* ms_per_launch,
                            double* bytes_per_site);

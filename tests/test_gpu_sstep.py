@@ -1,0 +1,136 @@
+"""Mixed-precision CG in s-step form: two CG iterations per launch (no
+reference counterpart; the operator is D^dag D of operators.cpp:444-453).
+
+The mixed solver takes this form on D^dag D when glx_set_sstep(2, rows)
+selected it and lx % 128 == 0, ly % 4 == 0 and ly >= 8.  Each launch
+builds the basis p, Ap, A^2 p, r, Ar in FP32 with its Gram matrix, and the two
+CG iterations run on coordinates in FP64; reliable updates recompute the FP64
+true residual as in the one-iteration form.  Checks per case:
+  * success, and the FP64 true relative residual <= tol recomputed with the
+    oracle's operator (and equal to the one the solver reports);
+  * the solution within cond(A) * 2 tol of the oracle's FP64 CG solution;
+  * the iteration count close to the one-iteration mixed solver's on the same
+    inputs (both are CG in exact arithmetic) and to the oracle's;
+  * max_iter stops at odd and even counts, frequent reliable updates
+    (delta = 0.5), a nonzero initial guess, b = 0, graph replay and other
+    rows per block.
+"""
+import numpy as np
+import pytest
+
+from helpers import lattice_inputs, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(glx, lat, mass, b, x0=None, sstep=2, rows=0, **kw):
+    glx.set_sstep(sstep, rows)
+    try:
+        db = glx.DeviceArray(b)
+        dx = glx.DeviceArray(np.zeros_like(b) if x0 is None else x0)
+        info = glx.dev_cg_mixed(lat, "staggered_normal", mass, dx, db, **kw)
+        return dx.download(), info
+    finally:
+        glx.set_sstep()
+
+
+def _check_solution(orc, links, b, x, info, L, Ly, mass, tol):
+    r = b - orc.apply("staggered_normal", x, links, mass, L, Ly)
+    relres = np.linalg.norm(r) / np.linalg.norm(b)
+    assert relres <= tol * 1.001, relres
+    assert abs(np.sqrt(info.resSq) / np.linalg.norm(b) - relres) <= 1e-3 * relres
+    return relres
+
+
+@pytest.mark.parametrize("link_prec", [1, 2])
+@pytest.mark.parametrize("L,Ly,mass,beta", [(128, 128, 0.05, 4.0), (256, 64, 0.1, 6.25), (128, 72, 0.05, 4.0)])
+def test_sstep_matches_oracle(glx, orc, L, Ly, mass, beta, link_prec):
+    links, bc, _ = lattice_inputs(orc, L, Ly, beta=beta, seed=71 + L + Ly)
+    lat = glx.Lattice(links, L, Ly)
+    x, info = _solve(glx, lat, mass, bc, max_iter=20000, eps=1e-10, delta=0.1, link_prec=link_prec)
+    assert info.success and info.reliable_updates >= 1
+    _check_solution(orc, links, bc, x, info, L, Ly, mass, 1e-10)
+    xo, io = orc.solve("cg", "staggered_normal", bc, None, links=links, mass=mass, lx=L, ly=Ly, max_iter=20000,
+                       eps=1e-10)
+    cond = (mass * mass + 2.0) / (mass * mass)
+    assert relerr(x, xo) <= 2e-10 * cond, (relerr(x, xo), cond)
+    assert info.iter <= 1.5 * io.iter + 20, (info.iter, io.iter)
+    # the one-iteration form on the same inputs: the same CG in exact arithmetic
+    x1, i1 = _solve(glx, lat, mass, bc, sstep=1, max_iter=20000, eps=1e-10, delta=0.1, link_prec=link_prec)
+    assert i1.success
+    assert abs(info.iter - i1.iter) <= max(6, 0.03 * i1.iter), (info.iter, i1.iter)
+    assert relerr(x, x1) <= 2e-10 * cond
+
+
+@pytest.mark.parametrize("max_iter", [9, 10, 1])
+def test_sstep_max_iter(glx, orc, max_iter):
+    """An unconverged solve stops after exactly max_iter iterations (odd and
+    even: the stop falls on either iteration of a launch) with success = false
+    and the true residual of the returned x."""
+    L = 128
+    links, bc, _ = lattice_inputs(orc, L, L, beta=4.0, seed=5)
+    lat = glx.Lattice(links, L, L)
+    x, info = _solve(glx, lat, 0.02, bc, max_iter=max_iter, eps=1e-10)
+    assert not info.success
+    assert info.iter == max_iter, info.iter
+    r = bc - orc.apply("staggered_normal", x, links, 0.02, L, L)
+    assert abs(np.sqrt(info.resSq) - np.linalg.norm(r)) <= 1e-6 * np.linalg.norm(r)
+    # the same number of CG iterations as the oracle's: the residual agrees
+    xo, io = orc.solve("cg", "staggered_normal", bc, None, links=links, mass=0.02, lx=L, ly=L, max_iter=max_iter,
+                       eps=1e-10)
+    ro = np.linalg.norm(bc - orc.apply("staggered_normal", xo, links, 0.02, L, L))
+    assert abs(np.linalg.norm(r) - ro) <= 1e-4 * ro, (np.linalg.norm(r), ro)
+
+
+def test_sstep_frequent_reliable_updates(glx, orc):
+    """delta = 0.5: a reliable update every few iterations, each followed by a
+    restart launch from the true residual; the solve still converges."""
+    L = 128
+    links, bc, _ = lattice_inputs(orc, L, L, beta=4.0, seed=8)
+    lat = glx.Lattice(links, L, L)
+    x, info = _solve(glx, lat, 0.05, bc, max_iter=20000, eps=1e-10, delta=0.5)
+    _, i01 = _solve(glx, lat, 0.05, bc, max_iter=20000, eps=1e-10, delta=0.1)
+    assert info.success and info.reliable_updates > 2 * i01.reliable_updates, (info.reliable_updates,
+                                                                              i01.reliable_updates)
+    _check_solution(orc, links, bc, x, info, L, L, 0.05, 1e-10)
+
+
+def test_sstep_initial_guess(glx, orc):
+    """The solve starts from r = b - A x0."""
+    L = 128
+    links, bc, _ = lattice_inputs(orc, L, L, beta=4.0, seed=9)
+    lat = glx.Lattice(links, L, L)
+    xo, _ = orc.solve("cg", "staggered_normal", bc, None, links=links, mass=0.05, lx=L, ly=L, max_iter=20000,
+                      eps=1e-4)
+    x, info = _solve(glx, lat, 0.05, bc, x0=xo, max_iter=20000, eps=1e-10)
+    _, i0 = _solve(glx, lat, 0.05, bc, max_iter=20000, eps=1e-10)
+    assert info.success and info.iter < i0.iter, (info.iter, i0.iter)
+    _check_solution(orc, links, bc, x, info, L, L, 0.05, 1e-10)
+    # an exact guess: the residual is zero, nothing changes
+    x2, info2 = _solve(glx, lat, 0.05, np.zeros_like(bc), max_iter=100, eps=1e-10)
+    assert info2.success and np.all(x2 == 0) and info2.resSq == 0.0
+
+
+@pytest.mark.parametrize("rows,graph", [(8, False), (32, False), (16, True)])
+def test_sstep_launch_shapes(glx, orc, rows, graph):
+    L = 256
+    links, bc, _ = lattice_inputs(orc, L, 96, beta=6.25, seed=12)
+    lat = glx.Lattice(links, L, 96)
+    glx.set_tuning(group=6, use_graph=graph)
+    try:
+        x, info = _solve(glx, lat, 0.1, bc, rows=rows, max_iter=20000, eps=1e-10)
+    finally:
+        glx.set_tuning()
+    assert info.success
+    _check_solution(orc, links, bc, x, info, L, 96, 0.1, 1e-10)
+
+
+def test_sstep_kernel_timing(glx, orc):
+    """The timing entry of the s-step kernel runs and reports its bytes."""
+    L = 256
+    links, _, _ = lattice_inputs(orc, L, L, beta=4.0, seed=1)
+    lat = glx.Lattice(links, L, L)
+    ms, by = glx.time_cg_kernel(lat, 0.05, prec=3, reps=20)
+    assert ms > 0 and by == 112
+    ms, by = glx.time_cg_kernel(lat, 0.05, prec=4, reps=20)
+    assert ms > 0 and by == 108
