@@ -376,11 +376,11 @@ def eo_solves(glx, lat, b, mass, reps=3):
     return out
 
 
-def traffic_from_profile(links: str, alg_bytes: float):
+def traffic_from_profile(kernel: str, links: str):
     """HBM bytes per launch of the dominant kernel as measured by rocprofv3
     FETCH_SIZE / WRITE_SIZE (scaled by a copy kernel of known size), read from
     the committed summary for this workload; None if there is none."""
-    path = os.path.join(ROOT, "profiles", f"k_normal_infast_fp32_{links}_L1024_pmc.txt")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_fp32_{links}_L1024_pmc.txt")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -454,18 +454,21 @@ def main():
         glx.synchronize()
         return time.perf_counter() - t0, info
 
-    def solve_mixed():
+    def solve_mixed(sstep=2):
+        glx.set_sstep(sstep)
         d_x.upload(zero)
         glx.synchronize()
         t0 = time.perf_counter()
         info = glx.dev_cg_mixed(lat, "staggered_normal", mass, d_x, d_b, max_iter=100000, eps=1e-10, delta=0.1,
                                 link_prec=link_prec)
         glx.synchronize()
+        glx.set_sstep()
         return time.perf_counter() - t0, info
 
     for _ in range(args.warmup):
         solve_fp64()
         solve_fused()
+        solve_mixed(1)
         solve_mixed()
     tfu = []
     for _ in range(max(1, args.steps)):
@@ -476,6 +479,10 @@ def main():
     for _ in range(max(1, args.steps)):
         dt, info64 = solve_fp64()
         t64.append(dt)
+    tm1 = []
+    for _ in range(max(1, args.steps)):
+        dt, infom1 = solve_mixed(1)
+        tm1.append(dt)
     tmx = []
     for _ in range(args.steps):
         dt, infomx = solve_mixed()
@@ -483,15 +490,18 @@ def main():
     ms64 = float(np.median(t64)) * 1e3
     msfu = float(np.median(tfu)) * 1e3
     msmx = float(np.median(tmx)) * 1e3
+    msm1 = float(np.median(tm1)) * 1e3
     best64 = min(ms64, msfu)
     x64 = d_x64.download()
     xmx = d_x.download()
     err_vs_fp64 = float(np.linalg.norm(xmx - x64) / np.linalg.norm(x64))
 
-    # live roofline of the mixed solve's dominant kernel: the fused iteration
-    # stencil (prologue r, p updates + D^dag D + six FP64 partial sums), HIP
-    # events around back-to-back launches on its own stream
-    it_ms, it_bps = glx.time_cg_kernel(lat, mass, prec=link_prec, reps=200)
+    # live roofline of the mixed solve's dominant kernel: the s-step launch
+    # (two CG iterations: p, r, x from the previous basis, then A p, A^2 p,
+    # A r and twelve FP64 partial sums), HIP events around back-to-back
+    # launches on its own stream; the one-iteration kernel beside it
+    it_ms, it_bps = glx.time_cg_kernel(lat, mass, prec=2 + link_prec, reps=200)
+    it1_ms, it1_bps = glx.time_cg_kernel(lat, mass, prec=link_prec, reps=200)
     it64_ms, it64_bps = glx.time_cg_kernel(lat, mass, prec=0, reps=200)
     it_gbs = it_bps * V / (it_ms * 1e-3) / 1e9
     it64_gbs = it64_bps * V / (it64_ms * 1e-3) / 1e9
@@ -554,6 +564,9 @@ def main():
         "speedup_vs_fp64": round(ms64 / msmx, 3),
         "speedup_basis": "plain FP64 GPU CG in the reference's order of operations (glx_dev_solve)",
         "speedup_vs_fp64_fused": round(msfu / msmx, 3),
+        "mixed_form": "s-step, two CG iterations per launch (glx_set_sstep(2, 16))",
+        "mixed_single_pass_ms": round(msm1, 3),
+        "mixed_single_pass_iters": infom1.iter,
         "fp64_iters": info64.iter,
         "fp64_fused_iters": infofu.iter,
         "fp64_fused_relerr_vs_fp64": float(np.linalg.norm(xfu - x64) / np.linalg.norm(x64)),
@@ -571,15 +584,17 @@ def main():
             "beyond_mall": big,
         },
         "cg_kernel_fp64": {"ms_per_launch": it64_ms, "bytes_per_site": it64_bps, "GBs": round(it64_gbs, 1)},
+        "cg_kernel_fp32_single_pass": {"ms_per_launch": it1_ms, "bytes_per_site": it1_bps,
+                                       "GBs": round(it1_bps * V / (it1_ms * 1e-3) / 1e9, 1)},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_normal<cx<float>, Links{args.links.upper()}, InFast, EpiFast, FinFast<true>> "
-                      f"(mixed CG iteration: r, p, x updates + D^dag D + 6 partial sums)",
+            "kernel": f"k_sscg<2, Links{args.links.upper()}> (s-step mixed CG launch = two CG iterations: "
+                      f"p, r, x from the previous basis + A p, A^2 p, A r + 12 partial sums)",
             "achieved": round(it_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(it_gbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile(args.links, it_bps * V),
+            "traffic": traffic_from_profile("k_sscg", args.links),
             "bytes_per_launch": it_bps * V,
             "bytes_per_site": it_bps,
             "ms_per_launch": it_ms,

@@ -1355,6 +1355,7 @@ int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, dou
     so.use_graph = g_use_graph != 0;
     MixedOpts mo;
     mo.link_prec = LINKS_F32;
+    mo.sstep = 1;  // the table shapes the one-iteration kernel; the s-step kernel has its own rows
     auto time_one = [&](int fp32) {
       GLX_HIP(hipMemset(x.get(), 0, size_t(n) * sizeof(c128)));
       GLX_HIP(hipDeviceSynchronize());

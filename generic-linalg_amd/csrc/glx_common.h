@@ -202,6 +202,53 @@ __device__ __forceinline__ void block_sum_all(double (&v)[NR], double (*sm)[kWav
   __syncthreads();
 }
 
+// Wave and block sums through DPP moves instead of LDS shuffles: within each
+// row of 16 lanes xor 1, xor 2 (quad permutations), then the half-row and row
+// mirrors; then the four row sums, read from lanes 0, 16, 32 and 48, added in
+// that order.  Every lane gets the same bits (a + b and b + a round alike).
+// All 64 lanes must be active.  Used by the self-finalising CG kernels, whose
+// per-launch reductions are a large share of their LDS instructions otherwise.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b), lane);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x141>(v);  // row_half_mirror
+  v += dpp_mov<0x140>(v);  // row_mirror
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+// Sum of NR values over the block in every thread (fixed order).
+template <int NR>
+__device__ __forceinline__ void block_sum_dpp(double (&v)[NR], double (*sm)[kWaves]) {
+#pragma unroll
+  for (int k = 0; k < NR; k++) v[k] = wave_sum_dpp(v[k]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NR; k++) sm[k][wave] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NR; k++) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) t += sm[k][w];
+    v[k] = t;
+  }
+  __syncthreads();
+}
+
 // Reduction workspace: partial sums [nblocks * NR].
 struct RedWs {
   double* partials;
@@ -228,6 +275,16 @@ __device__ __forceinline__ void block_partials_ex(double (&v)[NR], RedWs ws, int
 template <int NR>
 __device__ __forceinline__ void block_partials(double (&v)[NR], RedWs ws, int bid) {
   block_partials_ex<NR>(v, ws, bid, int(gridDim.x));
+}
+// The same store with the block sum of block_sum_dpp.
+template <int NR>
+__device__ __forceinline__ void block_partials_dpp(double (&v)[NR], RedWs ws, int bid) {
+  __shared__ double sm[NR > 0 ? NR : 1][kWaves];
+  block_sum_dpp<NR>(v, sm);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NR; k++) ws.partials[k * int(gridDim.x) + bid] = v[k];
+  }
 }
 
 // A kernel's gate: it runs only while the solver's control word equals

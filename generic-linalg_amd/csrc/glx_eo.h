@@ -203,7 +203,7 @@ __device__ __forceinline__ void eo_body(const EoGeom& eg, const OpDev<LK>& op, I
       }
     }
     if constexpr (kSF) {
-      if (run) block_partials<Epi::NR>(part, in.out_ws(ws), bid);
+      if (run) block_partials_dpp<Epi::NR>(part, in.out_ws(ws), bid);
       in.arrive(run);
       return;
     }

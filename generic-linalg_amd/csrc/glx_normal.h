@@ -280,7 +280,7 @@ __device__ __forceinline__ void normal_body(const NGeom& g, const OpDev<LK>& op,
       }
     }
     if constexpr (kSF) {
-      if (run) block_partials<Epi::NR>(part, in.out_ws(ws), bid);
+      if (run) block_partials_dpp<Epi::NR>(part, in.out_ws(ws), bid);
       in.arrive(run);
       return;
     }

@@ -78,7 +78,7 @@ struct MixedOpts {
   double delta = 0.1;
   // 2: two CG iterations per launch (s-step form) where the operator is the
   // full-lattice D^dag D on a lattice that fits its strips; 1: one per
-  // launch; 0: the process default (set_sstep, initially 1)
+  // launch; 0: the process default (set_sstep, initially 2)
   int sstep = 0;
 };
 

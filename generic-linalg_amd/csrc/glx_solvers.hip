@@ -1503,7 +1503,7 @@ struct InFastSF : InFast<T, TX> {
         for (int k = 0; k < NR; k++) acc[k] += v[u][k];
       }
     }
-    block_sum_all<NR>(acc, sm);
+    block_sum_dpp<NR>(acc, sm);
     f = fast_step<T, MIXED>(acc, k0, rd->maxrr, rd);
     if (f.stop) return false;
     this->a = as_scal<T>(f.alpha);
@@ -1903,7 +1903,7 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
 // the true residual into R0 of the basis slot the next launch reads.
 
 namespace {
-std::atomic<int> g_sstep{1};
+std::atomic<int> g_sstep{2};
 std::atomic<int> g_sstep_rows{16};
 }  // namespace
 void set_sstep(int s, int rows) {
@@ -1915,8 +1915,12 @@ void set_sstep(int s, int rows) {
 int sstep_default() { return g_sstep.load(); }
 int sstep_rows() { return g_sstep_rows.load(); }
 
+// Columns per lane of the s-step kernel (strips of 64 kSsCPL columns).
+constexpr int kSsCPL = 2;
+
 static bool sscg_fits(const DevOperator& o) {
-  return o.kind == OP_STAG_NORMAL && o.tiled() && o.lx >= 128 && o.lx % 128 == 0 && o.ly >= 8 && o.ly % 4 == 0;
+  constexpr int nx = kNX * kSsCPL;
+  return o.kind == OP_STAG_NORMAL && o.tiled() && o.lx >= nx && o.lx % nx == 0 && o.ly >= 8 && o.ly % 4 == 0;
 }
 
 // setup: R0 of slot 0 = b - A x in FP32, with |b|^2 and |r|^2
@@ -1971,11 +1975,11 @@ static void launch_sscg(hipStream_t st, const DevOperator& o, int link_prec, con
   if (link_prec == LINKS_F16) {
     if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");
     const OpDev<LinksF16> od = make_opdev(o, LinksF16{o.u16, o.sc16});
-    hipLaunchKernelGGL((k_sscg<LinksF16, SolveState>), dim3(g.nblk), dim3(kBlock), 0, st, g, od, bf, rd, wr, q, ws);
+    hipLaunchKernelGGL((k_sscg<kSsCPL, LinksF16, SolveState>), dim3(g.nblk), dim3(kBlock), 0, st, g, od, bf, rd, wr, q, ws);
   } else {
     if (!o.u32) throw ArgError("operator has no FP32 links on the device");
     const OpDev<LinksF32> od = make_opdev(o, LinksF32{o.u32});
-    hipLaunchKernelGGL((k_sscg<LinksF32, SolveState>), dim3(g.nblk), dim3(kBlock), 0, st, g, od, bf, rd, wr, q, ws);
+    hipLaunchKernelGGL((k_sscg<kSsCPL, LinksF32, SolveState>), dim3(g.nblk), dim3(kBlock), 0, st, g, od, bf, rd, wr, q, ws);
   }
   GLX_CHECK_LAUNCH();
 }
@@ -1995,7 +1999,7 @@ SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveO
   const int n = o.size();
   OwnedStream own;
   hipStream_t st = opt.stream ? opt.stream : own.st;
-  const NGeom g = make_ngeom(o.lx, o.ly, sstep_rows(), 2);
+  const NGeom g = make_ngeom(o.lx, o.ly, sstep_rows(), kSsCPL);
   // two buffers of 16 sums per block
   Work w(st, std::max(2 * g.nblk, stencil_blocks(o.lx, o.ly)), opt.max_iter, opt.want_hist);
   SolveState h;
@@ -2098,7 +2102,7 @@ double time_sscg_kernel_impl(const DevOperator& o, int link_prec, int reps) {
   const int n = o.size();
   OwnedStream own;
   hipStream_t st = own.st;
-  const NGeom g = make_ngeom(o.lx, o.ly, sstep_rows(), 2);
+  const NGeom g = make_ngeom(o.lx, o.ly, sstep_rows(), kSsCPL);
   Work w(st, std::max(2 * g.nblk, stencil_blocks(o.lx, o.ly)), 0, false);
   SsSpace sp(n, st);
   GLX_HIP(hipMemsetAsync(sp.bf.v(0, 3), 0x3c, size_t(n) * sizeof(c64), st));

@@ -15,10 +15,12 @@
 //                     alpha = delta / p'^H G w'; x' += alpha p'; r' -= alpha w'
 //                     delta' = r'^H G r'; beta = delta' / delta; p' = r' + beta p'
 //            A Hermitian gives <A^a u, A^b v> = <u, A^(a+b) v>, so G comes
-//            from 5 + 4 + 3 sums (mu_d over p, nu_d over p and r, rho_d over r).
+//            from 5 + 4 + 3 sums (mu_d over p, nu_d over p and r, rho_d over
+//            r); the CG scalars of a Hermitian A are real, so the coordinates
+//            are real and only the real parts of the sums enter.
 //   stream   p = Y c_p, r = Y c_r and x += Y c_x site by site (FP32), then
 //            P1 = A p, R1 = A r and P2 = A P1 in one row-marching pass
-//   sums     mu_0..4, nu_0..3, rho_0..2 of the new basis, per block
+//   sums     mu_0..4, Re nu_0..3, rho_0..2 of the new basis, per block
 // A launch whose iterations stop (the residual fell by delta since the last
 // reliable update, or reached the tolerance or zero, or max_iter) forms only x and
 // the direction p of the stopping iteration (into P0 of the new slot); the
@@ -26,14 +28,15 @@
 // and the next launch restarts from P0 and R0 alone (c_p = e3 + beta e0,
 // c_r = e3, c_x = 0) without a CG step.
 //
-// Pass structure (a block of 4 waves owns a strip of 128 columns and sr rows;
-// lane l holds columns x0 + 2l, x0 + 2l + 1; each step of the march covers
-// four rows, one per wave, with a lag of one row per stage):
-//   S0  rows 4k-4 .. 4k-1   stage p, r (from Y) and the links      cols x0-4 .. x0+131
-//   S1  rows 4k-5 .. 4k-2   qp = K p, qr = K r                       cols x0-3 .. x0+130
-//   S2  rows 4k-6 .. 4k-3   P1 = m^2 p - K qp;  R1 = m^2 r - K qr    cols x0-2 .. x0+129; R1 own
-//   S3  rows 4k-7 .. 4k-4   q1 = K P1                                cols x0-1 .. x0+128
-//   S4  rows 4k-8 .. 4k-5   P2 = m^2 P1 - K q1, the sums             own columns
+// Pass structure (a block of 4 waves owns a strip of W = 64 CPL columns and
+// sr rows; lane l holds columns x0 + CPL l .. x0 + CPL l + CPL - 1; each step
+// of the march covers four rows, one per wave, with a lag of one row per
+// stage):
+//   S0  rows 4k-4 .. 4k-1   stage p, r (from Y) and the links   cols x0-4 .. x0+W+3
+//   S1  rows 4k-5 .. 4k-2   qp = K p, qr = K r                    cols x0-3 .. x0+W+2
+//   S2  rows 4k-6 .. 4k-3   P1 = m^2 p - K qp                     cols x0-2 .. x0+W+1
+//   S3  rows 4k-7 .. 4k-4   q1 = K P1                             cols x0-1 .. x0+W
+//   S4  rows 4k-8 .. 4k-5   P2 = m^2 P1 - K q1, R1 = m^2 r - K qr, the sums (own)
 // LDS rings of 8 rows hold every stage's rows (slot = (y + 8) & 7).  S4's
 // first row needs U_y of the row above the 8 in the ring (it shares a slot
 // with the newest staged row), so S0 saves that row's U_y into a one-row
@@ -44,7 +47,7 @@
 
 namespace glx {
 
-constexpr int kSsNR = 16;  // partial sums per block
+constexpr int kSsNR = 12;  // partial sums per block
 
 // Two slots of the five basis vectors and the FP32 solution increment.
 struct SsBufs {
@@ -54,54 +57,55 @@ struct SsBufs {
   __host__ __device__ __forceinline__ c64* v(int slot, int j) const { return y + (5 * (slot & 1) + j) * n; }
 };
 
-// <Y_i, Y_j> from the sums: 0..4 mu_d, 5..12 nu_d (re, im), 13..15 rho_d.
+// The CG scalars of a Hermitian A are real, so the coordinate vectors stay
+// real and only Re <Y_i, Y_j> enters: from the sums 0..4 mu_d, 5..8 Re nu_d,
+// 9..11 rho_d.
 struct SsGram {
   const double* v;
-  __device__ __forceinline__ c128 at(int i, int j) const {
+  __device__ __forceinline__ double at(int i, int j) const {
     const bool pi = i < 3, pj = j < 3;
     const int d = (pi ? i : i - 3) + (pj ? j : j - 3);
-    if (pi && pj) return {v[d], 0.0};
-    if (!pi && !pj) return {v[13 + d], 0.0};
-    const c128 nu{v[5 + 2 * d], v[6 + 2 * d]};
-    return pi ? nu : conj(nu);
+    if (pi && pj) return v[d];
+    if (!pi && !pj) return v[9 + d];
+    return v[5 + d];
   }
 };
-// a^H G b
-__device__ __forceinline__ c128 ss_form(const SsGram& g, const c128 (&a)[5], const c128 (&b)[5]) {
-  c128 s{0.0, 0.0};
+// a^T Re(G) b
+__device__ __forceinline__ double ss_form(const SsGram& g, const double (&a)[5], const double (&b)[5]) {
+  double s = 0.0;
 #pragma unroll
   for (int i = 0; i < 5; i++) {
-    c128 t{0.0, 0.0};
+    double t = 0.0;
 #pragma unroll
-    for (int j = 0; j < 5; j++) t = t + g.at(i, j) * b[j];
-    s = s + cmulc(a[i], t);
+    for (int j = 0; j < 5; j++) t += g.at(i, j) * b[j];
+    s += a[i] * t;
   }
   return s;
 }
 
 // What the prelude decided for this launch.
 struct SsPlan {
-  c128 cp[5], cr[5], cx[5];
+  double cp[5], cr[5], cx[5];
   bool stencil;  // false: the iterations stopped; form x and p only
   int k, done, pending, steps;
   double rsq, maxrr;
   double hist[2];
 };
 
-// The two iterations on coordinates (v = the 16 sums of the basis in slot
+// The two iterations on coordinates (v = the sums of the basis in slot
 // ko & 1), from iteration k0 with maxrr2 = max |r|^2 since the last reliable
 // update.
 template <class St>
 __device__ __forceinline__ void ss_iterate(const double* v, const St* s, int k0, double maxrr2, SsPlan& pl) {
   const SsGram g{v};
-  c128 pp[5], rr[5], xx[5];
+  double pp[5], rr[5], xx[5];
 #pragma unroll
   for (int i = 0; i < 5; i++) {
-    pp[i] = {i == 0 ? 1.0 : 0.0, 0.0};
-    rr[i] = {i == 3 ? 1.0 : 0.0, 0.0};
-    xx[i] = {0.0, 0.0};
+    pp[i] = i == 0 ? 1.0 : 0.0;
+    rr[i] = i == 3 ? 1.0 : 0.0;
+    xx[i] = 0.0;
   }
-  double dl = v[13];
+  double dl = v[9];
   const double tol = s->eps * s->bsqrt;
   const double dd = s->delta * s->delta;
   pl.stencil = true;
@@ -110,17 +114,15 @@ __device__ __forceinline__ void ss_iterate(const double* v, const St* s, int k0,
   pl.steps = 0;
 #pragma unroll
   for (int j = 0; j < 2; j++) {
-    const c128 w[5] = {{0.0, 0.0}, pp[0], pp[1], {0.0, 0.0}, pp[3]};
-    const c128 pw = ss_form(g, pp, w);
-    const double den = pw.re * pw.re + pw.im * pw.im;
-    const double gg = den > 0.0 ? dl / den : 0.0;
-    const c128 a{gg * pw.re, -gg * pw.im};  // dl / pw
+    const double w[5] = {0.0, pp[0], pp[1], 0.0, pp[3]};
+    const double pw = ss_form(g, pp, w);
+    const double a = pw != 0.0 ? dl / pw : 0.0;
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-      xx[i] = xx[i] + a * pp[i];
-      rr[i] = rr[i] - a * w[i];
+      xx[i] += a * pp[i];
+      rr[i] -= a * w[i];
     }
-    double dn = ss_form(g, rr, rr).re;
+    double dn = ss_form(g, rr, rr);
     if (!(dn > 0.0)) dn = 0.0;
     const int k = k0 + j;
     pl.hist[j] = dn;
@@ -175,21 +177,23 @@ __device__ __forceinline__ void ss_carry(const St* rd, St* wr) {
   wr->alpha = rd->alpha;
 }
 
-__device__ __forceinline__ c64 ss_c64(c128 a) { return {float(a.re), float(a.im)}; }
+// Re(conj(a) b) in FP64
+__device__ __forceinline__ double redot(c64 a, c64 b) {
+  return double(a.re) * double(b.re) + double(a.im) * double(b.im);
+}
 // sum_j c_j y_j over the basis (only P0 and R0 unless `full`)
-__device__ __forceinline__ c64 ss_comb(const c64 (&c)[5], const c64 (&y)[5], bool full) {
+__device__ __forceinline__ c64 ss_comb(const float (&c)[5], const c64 (&y)[5], bool full) {
   c64 s = c[0] * y[0] + c[3] * y[3];
   if (full) s = s + c[1] * y[1] + c[2] * y[2] + c[4] * y[4];
   return s;
 }
 
-template <class LK, class St>
+template <int CPL, class LK, class St>
 __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q,
                                                  RedWs ws) {
   using T = c64;
   using C = typename LK::C;
-  constexpr int CPL = 2;
-  constexpr int NXB = kNX * CPL;  // 128 columns per strip
+  constexpr int NXB = kNX * CPL;  // columns per strip
   constexpr int WP = NXB + 8;     // p, r: columns x0-4 .. x0+NXB+3
   constexpr int WL = NXB + 8;     // links: columns x0-4 .. x0+NXB+2 (last slot unused)
   constexpr int WQ = NXB + 6;     // qp, qr: x0-3 .. x0+NXB+2
@@ -203,7 +207,6 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
   __shared__ T sQr[8][WQ];
   __shared__ T sP1[8][W1];
   __shared__ T sQ1[8][W2];
-  __shared__ T sR1[8][NXB];
   __shared__ C sKeep[NXB];
 
   // a finished solve (or a pending reliable update): carry the state over
@@ -232,6 +235,20 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
   const int so = ko & 1, sn = so ^ 1;
   const c64* Y[5] = {bf.v(so, 0), bf.v(so, 1), bf.v(so, 2), bf.v(so, 3), bf.v(so, 4)};
   c64* N[5] = {bf.v(sn, 0), bf.v(sn, 1), bf.v(sn, 2), bf.v(sn, 3), bf.v(sn, 4)};
+  // the CPL sites of a lane as one access (16 bytes for CPL = 2)
+  auto ldc = [](const c64* p, int j, c64 (&o)[CPL]) {
+    if constexpr (CPL == 2) {
+      const Pair2<c64> t = ld2(p, j);
+      o[0] = t.a;
+      o[1] = t.b;
+    } else {
+      o[0] = p[j];
+    }
+  };
+  auto stc = [](c64* p, int j, const c64 (&v)[CPL]) {
+    if constexpr (CPL == 2) st2(p, j, v[0], v[1]);
+    else p[j] = v[0];
+  };
 
   // the previous launch's sums: issued first (loads complete in order)
   constexpr int kPreU = 2;
@@ -256,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
   const int xhp = wrap_coord(x0 - 4 + hp_c, lx);
   const int xhl = wrap_coord(x0 - 4 + hl_c, lx);
 
-  Pair2<c64> ry[5], rx;
+  c64 ry[5][CPL], rx[CPL];
   LinkRegs<LK, CPL> lm;
   c64 hy[5];
   typename LK::Raw lh;
@@ -264,13 +281,13 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
     const int yr = 4 * k - 4 + w;
     const int j = wrap_coord(y0 + yr, ly) * lx + xm;
     const bool own = yr >= 0 && yr < rows;
-    ry[0] = ld2(Y[0], j);
-    ry[3] = ld2(Y[3], j);
+    ldc(Y[0], j, ry[0]);
+    ldc(Y[3], j, ry[3]);
     if (full) {
-      ry[1] = ld2(Y[1], j);
-      ry[2] = ld2(Y[2], j);
-      ry[4] = ld2(Y[4], j);
-      if (own) rx = ld2(bf.x, j);
+      ldc(Y[1], j, ry[1]);
+      ldc(Y[2], j, ry[2]);
+      ldc(Y[4], j, ry[4]);
+      if (own) ldc(bf.x, j, rx);
     }
     lm.fetch(op.links, j);
     if (hp) {
@@ -303,15 +320,15 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
 #pragma unroll
       for (int k = 0; k < kSsNR; k++) acc[k] += PS[k * nblk + b0];
     }
-    block_sum_all<kSsNR>(acc, sm);
+    block_sum_dpp<kSsNR>(acc, sm);
     ss_iterate(acc, rd, k0, rd->maxrr, pl);
   } else {
     // restart from P0 (the last direction) and R0 (the true residual)
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-      pl.cp[i] = {i == 3 ? 1.0 : (i == 0 ? rd->beta_r : 0.0), 0.0};
-      pl.cr[i] = {i == 3 ? 1.0 : 0.0, 0.0};
-      pl.cx[i] = {0.0, 0.0};
+      pl.cp[i] = i == 3 ? 1.0 : (i == 0 ? rd->beta_r : 0.0);
+      pl.cr[i] = i == 3 ? 1.0 : 0.0;
+      pl.cx[i] = 0.0;
     }
     pl.stencil = true;
     pl.done = 0;
@@ -321,12 +338,12 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
     pl.rsq = rd->rsq;
     pl.maxrr = rd->maxrr;
   }
-  c64 cp[5], cr[5], cx[5];
+  float cp[5], cr[5], cx[5];
 #pragma unroll
   for (int i = 0; i < 5; i++) {
-    cp[i] = ss_c64(pl.cp[i]);
-    cr[i] = ss_c64(pl.cr[i]);
-    cx[i] = ss_c64(pl.cx[i]);
+    cp[i] = float(pl.cp[i]);
+    cr[i] = float(pl.cr[i]);
+    cx[i] = float(pl.cx[i]);
   }
 
   if (!pl.stencil) {
@@ -335,14 +352,18 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
     if (active) {
       for (int yr = w; yr < rows; yr += 4) {
         const int j = (y0 + yr) * lx + xm;
-        Pair2<c64> a[5];
+        c64 a[5][CPL], xo[CPL], pn[CPL];
 #pragma unroll
-        for (int i = 0; i < 5; i++) a[i] = ld2(Y[i], j);
-        const Pair2<c64> xo = ld2(bf.x, j);
-        const c64 y0v[5] = {a[0].a, a[1].a, a[2].a, a[3].a, a[4].a};
-        const c64 y1v[5] = {a[0].b, a[1].b, a[2].b, a[3].b, a[4].b};
-        st2(N[0], j, ss_comb(cp, y0v, true), ss_comb(cp, y1v, true));
-        st2(bf.x, j, xo.a + ss_comb(cx, y0v, true), xo.b + ss_comb(cx, y1v, true));
+        for (int i = 0; i < 5; i++) ldc(Y[i], j, a[i]);
+        ldc(bf.x, j, xo);
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+          const c64 yv[5] = {a[0][c], a[1][c], a[2][c], a[3][c], a[4][c]};
+          pn[c] = ss_comb(cp, yv, true);
+          xo[c] = xo[c] + ss_comb(cx, yv, true);
+        }
+        stc(N[0], j, pn);
+        stc(bf.x, j, xo);
       }
     }
   } else {
@@ -358,19 +379,21 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
         const int j = wrap_coord(y0 + yr, ly) * lx + xm;
         const bool own = yr >= 0 && yr < rows;
         const int sl = (yr + 8) & 7;
-        const c64 y0v[5] = {ry[0].a, ry[1].a, ry[2].a, ry[3].a, ry[4].a};
-        const c64 y1v[5] = {ry[0].b, ry[1].b, ry[2].b, ry[3].b, ry[4].b};
-        const c64 p0 = ss_comb(cp, y0v, full), p1 = ss_comb(cp, y1v, full);
-        const c64 r0 = ss_comb(cr, y0v, full), r1 = ss_comb(cr, y1v, full);
-        if (own) {
-          st2(N[0], j, p0, p1);
-          st2(N[3], j, r0, r1);
-          if (full) st2(bf.x, j, rx.a + ss_comb(cx, y0v, true), rx.b + ss_comb(cx, y1v, true));
+        c64 pv2[CPL], rv2[CPL], xv2[CPL];
+#pragma unroll
+        for (int c = 0; c < CPL; c++) {
+          const c64 yv[5] = {ry[0][c], ry[1][c], ry[2][c], ry[3][c], ry[4][c]};
+          pv2[c] = ss_comb(cp, yv, full);
+          rv2[c] = ss_comb(cr, yv, full);
+          if (full) xv2[c] = rx[c] + ss_comb(cx, yv, true);
+          sP[sl][CPL * l + c + 4] = pv2[c];
+          sR[sl][CPL * l + c + 4] = rv2[c];
         }
-        sP[sl][CPL * l + 4] = p0;
-        sP[sl][CPL * l + 5] = p1;
-        sR[sl][CPL * l + 4] = r0;
-        sR[sl][CPL * l + 5] = r1;
+        if (own) {
+          stc(N[0], j, pv2);
+          stc(N[3], j, rv2);
+          if (full) stc(bf.x, j, xv2);
+        }
 #pragma unroll
         for (int c = 0; c < CPL; c++) {
           C ux, uy;
@@ -416,38 +439,28 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
       }
       __syncthreads();
       {
-        // S2: P1 = m^2 p - K qp (rows -2 .. rows+1), R1 = m^2 r - K qr (own)
+        // S2: P1 = m^2 p - K qp on rows -2 .. rows+1
         const int y = 4 * k - 6 + w;
         if (y >= -2 && y <= rows + 1) {
           const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
-          const bool own = y >= 0 && y < rows;
-          auto p1hop = [&](const T (*Q)[WQ], const T (*F)[WP], int c1) {
+          auto p1hop = [&](int c1) {
             const int qc = c1 + 1, pc = c1 + 2;  // x = x0 - 2 + c1
-            const T kq = khop<T, C>(Q[s0][qc + 1], Q[s0][qc - 1], Q[sp1][qc], Q[sm1][qc], sLx[s0][pc],
+            const T kq = khop<T, C>(sQp[s0][qc + 1], sQp[s0][qc - 1], sQp[sp1][qc], sQp[sm1][qc], sLx[s0][pc],
                                     sLx[s0][pc - 1], sLy[s0][pc], sLy[sm1][pc], (c1 & 1) != 0);
-            return rscal(m2, F[s0][pc]) - kq;
+            return rscal(m2, sP[s0][pc]) - kq;
           };
-          T v1[CPL], v4[CPL];
+          T v1[CPL];
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
             const int c1 = CPL * l + c + 2;
-            v1[c] = p1hop(sQp, sP, c1);
+            v1[c] = p1hop(c1);
             sP1[s0][c1] = v1[c];
           }
           if (l < 4) {
             const int c1 = l < 2 ? l : NXB + l;
-            sP1[s0][c1] = p1hop(sQp, sP, c1);
+            sP1[s0][c1] = p1hop(c1);
           }
-          if (own) {
-#pragma unroll
-            for (int c = 0; c < CPL; c++) {
-              v4[c] = p1hop(sQr, sR, CPL * l + c + 2);
-              sR1[s0][CPL * l + c] = v4[c];
-            }
-            const int j = (y0 + y) * lx + xm;
-            st2(N[1], j, v1[0], v1[1]);
-            st2(N[4], j, v4[0], v4[1]);
-          }
+          if (y >= 0 && y < rows) stc(N[1], (y0 + y) * lx + xm, v1);
         }
       }
       __syncthreads();
@@ -468,44 +481,53 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
       }
       __syncthreads();
       {
-        // S4: P2 = m^2 P1 - K q1 and the sums on the own rows
+        // S4 on the own rows: P2 = m^2 P1 - K q1, R1 = m^2 r - K qr (qr's
+        // ring still holds rows y-1 .. y+1), and the sums
         const int y = 4 * k - 8 + w;
         if (y >= 0 && y < rows) {
           const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
-          T v2[CPL];
+          T v2[CPL], v4[CPL];
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
-            const int c2 = CPL * l + c + 1, lc = CPL * l + c + 4, c1 = CPL * l + c + 2;
+            const int c2 = CPL * l + c + 1, lc = CPL * l + c + 4, c1 = CPL * l + c + 2, qc = CPL * l + c + 3;
+            const bool eta_neg = ((CPL * l + c) & 1) != 0;  // x = x0 + CPL l + c
+            const C ux = sLx[s0][lc], uxm = sLx[s0][lc - 1], uy = sLy[s0][lc];
             const C uym = w == 0 ? sKeep[CPL * l + c] : sLy[sm1][lc];
-            const T kq = khop<T, C>(sQ1[s0][c2 + 1], sQ1[s0][c2 - 1], sQ1[sp1][c2], sQ1[sm1][c2], sLx[s0][lc],
-                                    sLx[s0][lc - 1], sLy[s0][lc], uym, c == 1);
+            const T kq = khop<T, C>(sQ1[s0][c2 + 1], sQ1[s0][c2 - 1], sQ1[sp1][c2], sQ1[sm1][c2], ux, uxm, uy, uym,
+                                    eta_neg);
+            const T kr = khop<T, C>(sQr[s0][qc + 1], sQr[s0][qc - 1], sQr[sp1][qc], sQr[sm1][qc], ux, uxm, uy, uym,
+                                    eta_neg);
             const T pP1 = sP1[s0][c1];
+            const T pp = sP[s0][lc], rr = sR[s0][lc];
             const T p2 = rscal(m2, pP1) - kq;
+            const T r1 = rscal(m2, rr) - kr;
             v2[c] = p2;
-            const T pp = sP[s0][lc], rr = sR[s0][lc], r1 = sR1[s0][CPL * l + c];
+            v4[c] = r1;
             part[0] += norm2(pp);
-            part[1] += dotc(pp, pP1).re;
+            part[1] += redot(pp, pP1);
             part[2] += norm2(pP1);
-            part[3] += dotc(pP1, p2).re;
+            part[3] += redot(pP1, p2);
             part[4] += norm2(p2);
-            const c128 n0 = dotc(pp, rr), n1 = dotc(pp, r1), n2 = dotc(pP1, r1), n3 = dotc(p2, r1);
-            part[5] += n0.re;
-            part[6] += n0.im;
-            part[7] += n1.re;
-            part[8] += n1.im;
-            part[9] += n2.re;
-            part[10] += n2.im;
-            part[11] += n3.re;
-            part[12] += n3.im;
-            part[13] += norm2(rr);
-            part[14] += dotc(rr, r1).re;
-            part[15] += norm2(r1);
+            part[5] += redot(pp, rr);
+            part[6] += redot(pp, r1);
+            part[7] += redot(pP1, r1);
+            part[8] += redot(p2, r1);
+            part[9] += norm2(rr);
+            part[10] += redot(rr, r1);
+            part[11] += norm2(r1);
           }
-          st2(N[2], (y0 + y) * lx + xm, v2[0], v2[1]);
+          const int j = (y0 + y) * lx + xm;
+          stc(N[2], j, v2);
+          stc(N[4], j, v4);
         }
       }
     }
-    block_partials<kSsNR>(part, RedWs{ws.partials + q * kSsNR * nblk}, blockIdx.x);
+    block_sum_dpp<kSsNR>(part, sm);
+    if (threadIdx.x == 0) {
+      double* out = ws.partials + q * kSsNR * nblk;
+#pragma unroll
+      for (int k = 0; k < kSsNR; k++) out[k * nblk + blockIdx.x] = part[k];
+    }
   }
 
   // block 0 writes the state of the next launch

@@ -569,9 +569,9 @@ def set_tuning(group: int = 16, use_graph: bool = False):
     _check(lib().glx_set_tuning(group, int(use_graph)), "glx_set_tuning")
 
 
-def set_sstep(s: int = 1, rows: int = 0):
-    """Mixed-precision CG on D^dag D: s = 2 runs two CG iterations per launch
-    (s-step form) where the lattice allows, s = 1 (default) one; rows per
+def set_sstep(s: int = 2, rows: int = 0):
+    """Mixed-precision CG on D^dag D: s = 2 (default) runs two CG iterations
+    per launch (s-step form) where the lattice allows, s = 1 one; rows per
     block of the s-step kernel (0 = 16)."""
     _check(lib().glx_set_sstep(s, rows), "glx_set_sstep")
 

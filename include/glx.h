@@ -410,10 +410,10 @@ int glx_dev_eo_reconstruct(glx_lattice lat, double mass, double* x_dev, const do
  * replayed (default 0: measured slightly slower than direct launches). */
 int glx_set_tuning(int group, int use_graph);
 /* Iterations per launch of the mixed-precision CG on D^dag D (no reference
- * counterpart): s = 2 runs two CG iterations per launch in s-step form
- * (basis p, Ap, A^2 p, r, Ar and its Gram matrix, FP32 vectors, FP64
- * coordinates) where lx % 128 == 0, ly % 4 == 0 and ly >= 8; s = 1 (default)
- * one iteration per launch.  rows: rows per block of the s-step kernel (a
+ * counterpart): s = 2 (default) runs two CG iterations per launch in s-step
+ * form (basis p, Ap, A^2 p, r, Ar and its Gram matrix, FP32 vectors, FP64
+ * coordinates) where lx % 128 == 0, ly % 4 == 0 and ly >= 8; s = 1 one
+ * iteration per launch.  rows: rows per block of the s-step kernel (a
  * multiple of 4, >= 8; 0 = 16). */
 int glx_set_sstep(int s, int rows);
 int glx_get_sstep(int* s, int* rows);
@@ -429,8 +429,8 @@ int glx_set_normal_rows(int lx, int ly, int esize, int rows);
 int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
 /* Autotuner: for every candidate launch shape (tiled; 8..128 rows per block)
  * times `reps` iterations of the single-pass FP64 CG (glx_dev_cg_fused) and,
- * if the lattice has FP32 links, of the mixed-precision CG on D^dag D of the
- * lattice, records the fastest shape per precision in the table above and
+ * if the lattice has FP32 links, of the mixed-precision CG in its
+ * one-iteration form (glx_set_sstep(1, ...)) on D^dag D of the lattice, records the fastest shape per precision in the table above and
  * returns it with its time per iteration in ms.  Any output pointer may be
  * NULL. */
 int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, double* ms64, double* ms32);

@@ -127,10 +127,10 @@ def test_sstep_setting_without_device():
     lib = C.CDLL(LIB)
     lib.glx_last_error.restype = C.c_char_p
     s, rows = C.c_int(), C.c_int()
-    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (1, 16)
+    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (2, 16)
     assert lib.glx_set_sstep(3, 0) == 1 and b"s must be" in lib.glx_last_error()
     assert lib.glx_set_sstep(2, 6) == 1 and b"rows" in lib.glx_last_error()
-    assert lib.glx_set_sstep(2, 32) == 0
-    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (2, 32)
-    assert lib.glx_set_sstep(1, 0) == 0
-    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (1, 16)
+    assert lib.glx_set_sstep(1, 32) == 0
+    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (1, 32)
+    assert lib.glx_set_sstep(2, 0) == 0
+    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (2, 16)

@@ -1,8 +1,8 @@
 """Mixed-precision CG in s-step form: two CG iterations per launch (no
 reference counterpart; the operator is D^dag D of operators.cpp:444-453).
 
-The mixed solver takes this form on D^dag D when glx_set_sstep(2, rows)
-selected it and lx % 128 == 0, ly % 4 == 0 and ly >= 8.  Each launch
+The mixed solver takes this form on D^dag D (glx_set_sstep(2, rows), the
+default) when lx % 128 == 0, ly % 4 == 0 and ly >= 8.  Each launch
 builds the basis p, Ap, A^2 p, r, Ar in FP32 with its Gram matrix, and the two
 CG iterations run on coordinates in FP64; reliable updates recompute the FP64
 true residual as in the one-iteration form.  Checks per case:
