@@ -111,18 +111,21 @@ def test_sstep_initial_guess(glx, orc):
     assert info2.success and np.all(x2 == 0) and info2.resSq == 0.0
 
 
-@pytest.mark.parametrize("rows,graph", [(8, False), (32, False), (16, True)])
-def test_sstep_launch_shapes(glx, orc, rows, graph):
-    L = 256
-    links, bc, _ = lattice_inputs(orc, L, 96, beta=6.25, seed=12)
-    lat = glx.Lattice(links, L, 96)
+@pytest.mark.parametrize("L,Ly,rows,graph", [(256, 96, 8, False), (256, 96, 32, False), (256, 96, 16, True),
+                                             (128, 8, 16, False), (384, 12, 8, False)])
+def test_sstep_launch_shapes(glx, orc, L, Ly, rows, graph):
+    """Rows per block 8 / 16 / 32, graph replay, and the smallest lattices the
+    kernel takes (one strip row shorter than its rows per block; a strip of
+    8 rows whose 4-row halos wrap onto the other strip)."""
+    links, bc, _ = lattice_inputs(orc, L, Ly, beta=6.25, seed=12)
+    lat = glx.Lattice(links, L, Ly)
     glx.set_tuning(group=6, use_graph=graph)
     try:
         x, info = _solve(glx, lat, 0.1, bc, rows=rows, max_iter=20000, eps=1e-10)
     finally:
         glx.set_tuning()
     assert info.success
-    _check_solution(orc, links, bc, x, info, L, 96, 0.1, 1e-10)
+    _check_solution(orc, links, bc, x, info, L, Ly, 0.1, 1e-10)
 
 
 def test_sstep_kernel_timing(glx, orc):
