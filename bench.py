@@ -479,6 +479,7 @@ def main():
     lat = glx.Lattice(links, L, L, f32=True, f16=True)
     glx.set_tuning(group=16, use_graph=False)
     tuned = glx.autotune_normal(lat, reps=40)
+    tuned_ss = glx.autotune_sstep(lat, reps=40)
     d_b = glx.DeviceArray(b)
     d_x = glx.DeviceArray(np.zeros_like(b))
     d_x64 = glx.DeviceArray(np.zeros_like(b))
@@ -595,7 +596,8 @@ def main():
         "speedup_vs_fp64": round(ms64 / msmx, 3),
         "speedup_basis": "plain FP64 GPU CG in the reference's order of operations (glx_dev_solve)",
         "speedup_vs_fp64_fused": round(msfu / msmx, 3),
-        "mixed_form": "s-step, two CG iterations per launch (glx_set_sstep(2, 16))",
+        "mixed_form": (f"s-step, two CG iterations per launch, {tuned_ss['rows']} rows per block (autotuned)"
+                       if tuned_ss["rows"] > 0 else "one CG iteration per launch (autotuned)"),
         "mixed_single_pass_ms": round(msm1, 3),
         "mixed_single_pass_iters": infom1.iter,
         "fp64_iters": info64.iter,

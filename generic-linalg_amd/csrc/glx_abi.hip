@@ -1254,6 +1254,20 @@ int glx_get_sstep(int* s, int* rows) {
     if (rows) *rows = sstep_rows();
   });
 }
+int glx_set_sstep_rows(int lx, int ly, int rows) {
+  return guarded([&] {
+    if (lx <= 0 || ly <= 0) throw ArgError("bad lattice size");
+    if (rows > 0 && (rows < 8 || rows % 4 != 0)) throw ArgError("rows must be -1, 0 or a multiple of 4 >= 8");
+    set_sstep_rows(lx, ly, rows < 0 ? -1 : rows);
+  });
+}
+int glx_get_sstep_rows(int lx, int ly, int* rows) {
+  return guarded([&] {
+    if (!rows) throw ArgError("rows is NULL");
+    if (lx <= 0 || ly <= 0) throw ArgError("bad lattice size");
+    *rows = sstep_rows_for(lx, ly);
+  });
+}
 
 int glx_dev_cg_fused(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
                      double eps, glx_verbose* verb, glx_inversion_info* info) {
@@ -1389,6 +1403,60 @@ int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, dou
         if (ms64) *ms64 = best_ms;
       }
     }
+  });
+}
+
+int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iteration) {
+  return guarded([&] {
+    if (!lat || reps <= 0) throw ArgError("bad arguments");
+    // `reps` iterations of the mixed-precision CG (eps = 0: no early stop) on
+    // b = 1, x0 = 0, in its one-iteration form (candidate 0) and in s-step
+    // form at each rows per block; the fastest per iteration is installed
+    DevOperator o = lattice_op(lat, OP_STAG_NORMAL, 0.1, 1);
+    if (!o.u32) throw ArgError("lattice was created without GLX_LINKS_F32");
+    const int n = o.size();
+    std::vector<c128> ones(n, c128{1.0, 0.0});
+    DevBuf b(size_t(n) * sizeof(c128)), x(size_t(n) * sizeof(c128));
+    GLX_HIP(hipMemcpy(b.get(), ones.data(), size_t(n) * sizeof(c128), hipMemcpyHostToDevice));
+    SolveOpts so;
+    so.max_iter = reps;
+    so.eps = 0.0;
+    so.group = g_group;
+    so.use_graph = g_use_graph != 0;
+    MixedOpts mo;
+    mo.link_prec = LINKS_F32;
+    mo.sstep = 2;
+    const int forced = sstep_rows();
+    set_sstep(sstep_default(), 0);  // the table decides while timing
+    int best = 0;
+    double best_ms = 1e300;
+    try {
+      for (int c : {0, 8, 16, 32, 64}) {
+        if (c > 0 && (o.lx % 128 != 0 || o.ly < 8 || o.ly % 4 != 0)) break;
+        set_sstep_rows(o.lx, o.ly, c);
+        auto time_one = [&]() {
+          GLX_HIP(hipMemset(x.get(), 0, size_t(n) * sizeof(c128)));
+          GLX_HIP(hipDeviceSynchronize());
+          const auto t0 = std::chrono::steady_clock::now();
+          (void)dev_cg_mixed(o, x.get(), b.get(), so, mo);
+          GLX_HIP(hipDeviceSynchronize());
+          return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+        };
+        time_one();  // warm-up
+        const double t = std::min(time_one(), time_one());
+        if (t < best_ms) {
+          best_ms = t;
+          best = c;
+        }
+      }
+    } catch (...) {
+      set_sstep(sstep_default(), forced);
+      throw;
+    }
+    set_sstep(sstep_default(), forced);
+    set_sstep_rows(o.lx, o.ly, best);
+    if (rows) *rows = best;
+    if (ms_per_iteration) *ms_per_iteration = best_ms;
   });
 }
 

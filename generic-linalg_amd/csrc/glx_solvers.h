@@ -83,10 +83,16 @@ struct MixedOpts {
 };
 
 // Process-wide default of MixedOpts::sstep for the ABI's mixed solves, and the
-// rows per block of the s-step kernel (a multiple of 4, >= 8; 0 = 16).
+// rows per block of the s-step kernel (a multiple of 4, >= 8) for every
+// lattice size, or 0: per size (the table below, else sstep_rows_auto).
 void set_sstep(int s, int rows);
 int sstep_default();
 int sstep_rows();
+// Per lattice size: rows per block of the s-step kernel, or 0 for the
+// one-iteration form at that size (rows < 0 removes the entry).
+void set_sstep_rows(int lx, int ly, int rows);
+int sstep_rows_for(int lx, int ly);
+int sstep_rows_auto(int lx, int ly);
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m);
 
 // FP64 CG in the single-pass form the mixed solver uses (one fused stencil

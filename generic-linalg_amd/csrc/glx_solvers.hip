@@ -1904,16 +1904,45 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
 
 namespace {
 std::atomic<int> g_sstep{2};
-std::atomic<int> g_sstep_rows{16};
+std::atomic<int> g_sstep_rows{0};  // 0: per lattice size
+struct SsTable {
+  std::mutex mu;
+  std::map<std::pair<int, int>, int> rows;
+};
+SsTable& ss_table() {
+  static SsTable t;
+  return t;
+}
 }  // namespace
 void set_sstep(int s, int rows) {
   if (s != 1 && s != 2) throw ArgError("s-step: s must be 1 or 2");
   if (rows != 0 && (rows < 8 || rows % 4 != 0)) throw ArgError("s-step: rows must be 0 or a multiple of 4, >= 8");
   g_sstep.store(s);
-  g_sstep_rows.store(rows ? rows : 16);
+  g_sstep_rows.store(rows);
 }
 int sstep_default() { return g_sstep.load(); }
 int sstep_rows() { return g_sstep_rows.load(); }
+void set_sstep_rows(int lx, int ly, int rows) {
+  SsTable& t = ss_table();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (rows < 0) t.rows.erase(std::make_pair(lx, ly));
+  else t.rows[std::make_pair(lx, ly)] = rows;
+}
+// Without a table entry: 16 rows per block, 8 where that leaves fewer than
+// 256 blocks, 32 from 2048 blocks on (measured at 256^2 .. 2048^2: 256^2 and
+// 512^2 best at 8, 1024^2 at 16, 2048^2 at 32).
+int sstep_rows_auto(int lx, int ly) {
+  const long b16 = long(lx / (kNX * 2)) * ((ly + 15) / 16);
+  return b16 < 256 ? 8 : (b16 >= 2048 ? 32 : 16);
+}
+int sstep_rows_for(int lx, int ly) {
+  const int forced = g_sstep_rows.load();
+  if (forced > 0) return forced;
+  SsTable& t = ss_table();
+  std::lock_guard<std::mutex> g(t.mu);
+  auto it = t.rows.find(std::make_pair(lx, ly));
+  return it != t.rows.end() ? it->second : sstep_rows_auto(lx, ly);
+}
 
 // Columns per lane of the s-step kernel (strips of 64 kSsCPL columns).
 constexpr int kSsCPL = 2;
@@ -1995,11 +2024,12 @@ struct SsSpace {
   }
 };
 
-SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveOpts& opt, const MixedOpts& mo) {
+SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveOpts& opt, const MixedOpts& mo,
+                      int rows) {
   const int n = o.size();
   OwnedStream own;
   hipStream_t st = opt.stream ? opt.stream : own.st;
-  const NGeom g = make_ngeom(o.lx, o.ly, sstep_rows(), kSsCPL);
+  const NGeom g = make_ngeom(o.lx, o.ly, rows, kSsCPL);
   // two buffers of 16 sums per block
   Work w(st, std::max(2 * g.nblk, stencil_blocks(o.lx, o.ly)), opt.max_iter, opt.want_hist);
   SolveState h;
@@ -2098,11 +2128,11 @@ SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveO
 // events, from p = 0 and a constant r (every float 0x3c3c3c3c, about 0.0115)
 // with tolerance, delta and max_iter zero, so that no launch stops.  Returns
 // ms per launch.
-double time_sscg_kernel_impl(const DevOperator& o, int link_prec, int reps) {
+double time_sscg_kernel_impl(const DevOperator& o, int link_prec, int reps, int rows) {
   const int n = o.size();
   OwnedStream own;
   hipStream_t st = own.st;
-  const NGeom g = make_ngeom(o.lx, o.ly, sstep_rows(), kSsCPL);
+  const NGeom g = make_ngeom(o.lx, o.ly, rows, kSsCPL);
   Work w(st, std::max(2 * g.nblk, stencil_blocks(o.lx, o.ly)), 0, false);
   SsSpace sp(n, st);
   GLX_HIP(hipMemsetAsync(sp.bf.v(0, 3), 0x3c, size_t(n) * sizeof(c64), st));
@@ -2157,8 +2187,10 @@ SolveResult dev_cg_m(const DevOperator& op, void* const* xs, const void* b, int 
                          resid_freq_check, worst_first, o);
 }
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m) {
-  if (!real_op(op) && (m.sstep ? m.sstep : sstep_default()) == 2 && sscg_fits(op))
-    return sscg_impl(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
+  if (!real_op(op) && (m.sstep ? m.sstep : sstep_default()) == 2 && sscg_fits(op)) {
+    const int rows = sstep_rows_for(op.lx, op.ly);  // 0: the one-iteration form for this size
+    if (rows > 0) return sscg_impl(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m, rows);
+  }
   if (real_op(op))
     return fastcg_impl<double, float, true>(op, static_cast<double*>(x), static_cast<const double*>(b), o, m);
   return fastcg_impl<c128, c64, true>(op, static_cast<c128*>(x), static_cast<const c128*>(b), o, m);
@@ -2751,7 +2783,8 @@ double time_cg_kernel(const DevOperator& o, int prec, int reps, double* bytes_pe
     if (!sscg_fits(o)) throw ArgError("the s-step kernel needs lx % 128 == 0, ly % 4 == 0, ly >= 8");
     const int lp = prec - 2;
     if (bytes_per_site) *bytes_per_site = 5 * 8 + 8 + (lp == LINKS_F16 ? 2 * 4 + 4 : 2 * 8) + 5 * 8 + 8;
-    return time_sscg_kernel_impl(o, lp, reps);
+    const int rows = sstep_rows_for(o.lx, o.ly);
+    return time_sscg_kernel_impl(o, lp, reps, rows > 0 ? rows : sstep_rows_auto(o.lx, o.ly));
   }
   if (prec == LINKS_F16) {
     if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");

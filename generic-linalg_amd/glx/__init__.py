@@ -203,6 +203,12 @@ def lib():
         L.glx_set_sstep.argtypes = [i, i]
         L.glx_get_sstep.restype = i
         L.glx_get_sstep.argtypes = [C.POINTER(i), C.POINTER(i)]
+        L.glx_set_sstep_rows.restype = i
+        L.glx_set_sstep_rows.argtypes = [i, i, i]
+        L.glx_get_sstep_rows.restype = i
+        L.glx_get_sstep_rows.argtypes = [i, i, C.POINTER(i)]
+        L.glx_autotune_sstep.restype = i
+        L.glx_autotune_sstep.argtypes = [vp, i, C.POINTER(i), C.POINTER(d)]
         L.glx_dev_time_cg_kernel.restype = i
         L.glx_dev_time_cg_kernel.argtypes = [vp, d, i, i, C.POINTER(d), C.POINTER(d)]
         L.glx_dev_staggered_grad.restype = i
@@ -572,8 +578,27 @@ def set_tuning(group: int = 16, use_graph: bool = False):
 def set_sstep(s: int = 2, rows: int = 0):
     """Mixed-precision CG on D^dag D: s = 2 (default) runs two CG iterations
     per launch (s-step form) where the lattice allows, s = 1 one; rows per
-    block of the s-step kernel (0 = 16)."""
+    block of the s-step kernel for every size, or 0 (default) per size."""
     _check(lib().glx_set_sstep(s, rows), "glx_set_sstep")
+
+
+def set_sstep_rows(lx: int, ly: int, rows: int):
+    """Per-size entry: s-step rows per block, 0 = one-iteration form, -1 = remove."""
+    _check(lib().glx_set_sstep_rows(lx, ly, rows), "glx_set_sstep_rows")
+
+
+def get_sstep_rows(lx: int, ly: int) -> int:
+    r = C.c_int(0)
+    _check(lib().glx_get_sstep_rows(lx, ly, C.byref(r)), "glx_get_sstep_rows")
+    return r.value
+
+
+def autotune_sstep(lat, reps: int = 40) -> dict:
+    """Time the mixed CG on this lattice in one-iteration form and in s-step
+    form at 8 / 16 / 32 / 64 rows per block; install and return the fastest."""
+    r, ms = C.c_int(0), C.c_double(0.0)
+    _check(lib().glx_autotune_sstep(lat.h, reps, C.byref(r), C.byref(ms)), "glx_autotune_sstep")
+    return {"rows": r.value, "ms_per_iteration": ms.value}
 
 
 def get_sstep():

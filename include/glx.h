@@ -413,10 +413,23 @@ int glx_set_tuning(int group, int use_graph);
  * counterpart): s = 2 (default) runs two CG iterations per launch in s-step
  * form (basis p, Ap, A^2 p, r, Ar and its Gram matrix, FP32 vectors, FP64
  * coordinates) where lx % 128 == 0, ly % 4 == 0 and ly >= 8; s = 1 one
- * iteration per launch.  rows: rows per block of the s-step kernel (a
- * multiple of 4, >= 8; 0 = 16). */
+ * iteration per launch.  rows: rows per block of the s-step kernel for every
+ * lattice size (a multiple of 4, >= 8), or 0 (default): per size, from the
+ * table below, else 16 (8 where 16 leaves fewer than 256 blocks, 32 from 2048
+ * blocks on). */
 int glx_set_sstep(int s, int rows);
 int glx_get_sstep(int* s, int* rows);
+/* The per-size table: rows per block of the s-step kernel for an lx x ly
+ * lattice, or 0 = the one-iteration form at that size; -1 removes the entry.
+ * glx_get_sstep_rows returns what a solve of that size would use. */
+int glx_set_sstep_rows(int lx, int ly, int rows);
+int glx_get_sstep_rows(int lx, int ly, int* rows);
+/* Autotuner for the mixed-precision CG on D^dag D of the lattice: times
+ * `reps` iterations in the one-iteration form and in s-step form at 8, 16,
+ * 32 and 64 rows per block, installs the fastest in the table above and
+ * returns it (0 = one-iteration form) with its time per iteration in ms.
+ * Needs FP32 links.  No reference counterpart. */
+int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iteration);
 /* Launch shape of the fused normal operator D^dag D (no reference
  * counterpart; operators.cpp:444-453 is the operator it computes): rows per
  * block of the row-marching kernel for an lx x ly lattice and vector element

@@ -127,10 +127,22 @@ def test_sstep_setting_without_device():
     lib = C.CDLL(LIB)
     lib.glx_last_error.restype = C.c_char_p
     s, rows = C.c_int(), C.c_int()
-    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (2, 16)
+    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (2, 0)
     assert lib.glx_set_sstep(3, 0) == 1 and b"s must be" in lib.glx_last_error()
     assert lib.glx_set_sstep(2, 6) == 1 and b"rows" in lib.glx_last_error()
     assert lib.glx_set_sstep(1, 32) == 0
     assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (1, 32)
     assert lib.glx_set_sstep(2, 0) == 0
-    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (2, 16)
+    assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (2, 0)
+    # per size: the automatic choice by block count, the table, removal
+    r = C.c_int()
+    for (lx, ly), want in {(256, 256): 8, (512, 512): 8, (768, 768): 16, (1024, 1024): 16, (2048, 2048): 32}.items():
+        assert lib.glx_get_sstep_rows(lx, ly, C.byref(r)) == 0 and r.value == want, (lx, r.value)
+    assert lib.glx_set_sstep_rows(1024, 1024, 0) == 0
+    assert lib.glx_get_sstep_rows(1024, 1024, C.byref(r)) == 0 and r.value == 0
+    assert lib.glx_set_sstep_rows(1024, 1024, 6) == 1
+    assert lib.glx_set_sstep_rows(1024, 1024, -1) == 0
+    assert lib.glx_get_sstep_rows(1024, 1024, C.byref(r)) == 0 and r.value == 16
+    assert lib.glx_set_sstep(2, 24) == 0
+    assert lib.glx_get_sstep_rows(256, 256, C.byref(r)) == 0 and r.value == 24
+    assert lib.glx_set_sstep(2, 0) == 0

@@ -137,3 +137,21 @@ def test_sstep_kernel_timing(glx, orc):
     assert ms > 0 and by == 112
     ms, by = glx.time_cg_kernel(lat, 0.05, prec=4, reps=20)
     assert ms > 0 and by == 108
+
+
+def test_sstep_autotune(glx, orc):
+    """The autotuner installs one of its candidates for the lattice size
+    (0 = one-iteration form) and solves of that size use it."""
+    L = 256
+    links, bc, _ = lattice_inputs(orc, L, L, beta=4.0, seed=21)
+    lat = glx.Lattice(links, L, L)
+    try:
+        res = glx.autotune_sstep(lat, reps=10)
+        assert res["rows"] in (0, 8, 16, 32, 64) and res["ms_per_iteration"] > 0
+        assert glx.get_sstep_rows(L, L) == res["rows"]
+        x, info = _solve(glx, lat, 0.05, bc, max_iter=20000, eps=1e-10)
+        assert info.success
+        _check_solution(orc, links, bc, x, info, L, L, 0.05, 1e-10)
+    finally:
+        glx.set_sstep_rows(L, L, -1)
+    assert glx.get_sstep_rows(L, L) == 8
