@@ -188,6 +188,39 @@ __device__ __forceinline__ c64 ss_comb(const float (&c)[5], const c64 (&y)[5], b
   return s;
 }
 
+// The stencil steps of the march on the LDS rings (row slots s0 = y,
+// sm = y - 1, sp = y + 1; column indices as in the kernel's arrays).
+// qp, qr at q column qc (x = x0 - 3 + qc; p and link column qc + 1)
+template <class T, class C, int WP, int WL, int WQ>
+__device__ __forceinline__ void ss_q1hop(const T (*sP)[WP], const T (*sR)[WP], const C (*sLx)[WL],
+                                         const C (*sLy)[WL], T (*sQp)[WQ], T (*sQr)[WQ], int s0, int sm, int sp,
+                                         int qc) {
+  const int pc = qc + 1;
+  const bool eta_neg = ((qc + 1) & 1) != 0;
+  const C ux = sLx[s0][pc], uxm = sLx[s0][pc - 1], uy = sLy[s0][pc], uym = sLy[sm][pc];
+  sQp[s0][qc] = khop<T, C>(sP[s0][pc + 1], sP[s0][pc - 1], sP[sp][pc], sP[sm][pc], ux, uxm, uy, uym, eta_neg);
+  sQr[s0][qc] = khop<T, C>(sR[s0][pc + 1], sR[s0][pc - 1], sR[sp][pc], sR[sm][pc], ux, uxm, uy, uym, eta_neg);
+}
+// P1 = m^2 p - K qp at P1 column c1 (x = x0 - 2 + c1; q column c1 + 1, p and
+// link column c1 + 2)
+template <class T, class C, int WP, int WL, int WQ>
+__device__ __forceinline__ T ss_p1hop(const T (*sP)[WP], const C (*sLx)[WL], const C (*sLy)[WL],
+                                      const T (*sQp)[WQ], int s0, int sm, int sp, int c1, float m2) {
+  const int qc = c1 + 1, pc = c1 + 2;
+  const T kq = khop<T, C>(sQp[s0][qc + 1], sQp[s0][qc - 1], sQp[sp][qc], sQp[sm][qc], sLx[s0][pc], sLx[s0][pc - 1],
+                          sLy[s0][pc], sLy[sm][pc], (c1 & 1) != 0);
+  return rscal(m2, sP[s0][pc]) - kq;
+}
+// q1 = K P1 at q1 column c2 (x = x0 - 1 + c2; P1 column c2 + 1, link column
+// c2 + 3)
+template <class T, class C, int W1, int WL, int W2>
+__device__ __forceinline__ void ss_q2hop(const T (*sP1)[W1], const C (*sLx)[WL], const C (*sLy)[WL], T (*sQ1)[W2],
+                                         int s0, int sm, int sp, int c2) {
+  const int lc = c2 + 3;
+  sQ1[s0][c2] = khop<T, C>(sP1[s0][c2 + 2], sP1[s0][c2], sP1[sp][c2 + 1], sP1[sm][c2 + 1], sLx[s0][lc],
+                           sLx[s0][lc - 1], sLy[s0][lc], sLy[sm][lc], (c2 & 1) == 0);
+}
+
 template <int CPL, class LK, class St>
 __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q,
                                                  RedWs ws) {
@@ -263,10 +296,14 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
     }
   }
 
-  // halo roles: wave 0 lanes 0..31 stage p, r on columns x0-4 .. x0-1 and
-  // x0+NXB .. x0+NXB+3 of the step's four rows; wave 1 lanes 0..27 stage the
-  // links on columns x0-4 .. x0-1 and x0+NXB .. x0+NXB+2
-  const bool hp = w == 0 && l < 32;
+  // halo columns, all four rows of a step on one wave each (a pass with a
+  // few active lanes costs a wave its whole instruction stream):
+  //   S0  wave 3 lanes 0..31: p, r on columns x0-4 .. x0-1, x0+NXB .. x0+NXB+3
+  //       wave 1 lanes 0..27: the links on x0-4 .. x0-1, x0+NXB .. x0+NXB+2
+  //   S1  wave 0 lanes 0..23: qp, qr on x0-3 .. x0-1, x0+NXB .. x0+NXB+2
+  //   S2  wave 1 lanes 0..15: P1 on x0-2, x0-1, x0+NXB, x0+NXB+1
+  //   S3  wave 2 lanes 0..7:  q1 on x0-1, x0+NXB
+  const bool hp = w == 3 && l < 32;
   const bool hl = w == 1 && l < 28;
   const int hp_row = l >> 3, hp_c = (l & 7) < 4 ? (l & 7) : NXB + (l & 7);
   const int hl_row = l / 7, hl_c = (l % 7) < 4 ? (l % 7) : NXB + (l % 7);
@@ -424,17 +461,14 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
         const int y = 4 * k - 5 + w;
         if (y >= -3 && y <= rows + 2) {
           const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
-          auto q1hop = [&](int qc) {
-            const int pc = qc + 1;  // p and link column of q column qc
-            const bool eta_neg = ((qc + 1) & 1) != 0;  // x = x0 - 3 + qc
-            sQp[s0][qc] = khop<T, C>(sP[s0][pc + 1], sP[s0][pc - 1], sP[sp1][pc], sP[sm1][pc], sLx[s0][pc],
-                                     sLx[s0][pc - 1], sLy[s0][pc], sLy[sm1][pc], eta_neg);
-            sQr[s0][qc] = khop<T, C>(sR[s0][pc + 1], sR[s0][pc - 1], sR[sp1][pc], sR[sm1][pc], sLx[s0][pc],
-                                     sLx[s0][pc - 1], sLy[s0][pc], sLy[sm1][pc], eta_neg);
-          };
 #pragma unroll
-          for (int c = 0; c < CPL; c++) q1hop(CPL * l + c + 3);
-          if (l < 6) q1hop(l < 3 ? l : NXB + l);
+          for (int c = 0; c < CPL; c++) ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, s0, sm1, sp1, CPL * l + c + 3);
+        }
+        if (w == 0 && l < 24) {
+          const int yh = 4 * k - 5 + l / 6, hc = l % 6;
+          if (yh >= -3 && yh <= rows + 2)
+            ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7,
+                           hc < 3 ? hc : NXB + hc);
         }
       }
       __syncthreads();
@@ -443,24 +477,22 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
         const int y = 4 * k - 6 + w;
         if (y >= -2 && y <= rows + 1) {
           const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
-          auto p1hop = [&](int c1) {
-            const int qc = c1 + 1, pc = c1 + 2;  // x = x0 - 2 + c1
-            const T kq = khop<T, C>(sQp[s0][qc + 1], sQp[s0][qc - 1], sQp[sp1][qc], sQp[sm1][qc], sLx[s0][pc],
-                                    sLx[s0][pc - 1], sLy[s0][pc], sLy[sm1][pc], (c1 & 1) != 0);
-            return rscal(m2, sP[s0][pc]) - kq;
-          };
           T v1[CPL];
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
             const int c1 = CPL * l + c + 2;
-            v1[c] = p1hop(c1);
+            v1[c] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, s0, sm1, sp1, c1, m2);
             sP1[s0][c1] = v1[c];
           }
-          if (l < 4) {
-            const int c1 = l < 2 ? l : NXB + l;
-            sP1[s0][c1] = p1hop(c1);
-          }
           if (y >= 0 && y < rows) stc(N[1], (y0 + y) * lx + xm, v1);
+        }
+        if (w == 1 && l < 16) {
+          const int yh = 4 * k - 6 + (l >> 2), hc = l & 3;
+          if (yh >= -2 && yh <= rows + 1) {
+            const int c1 = hc < 2 ? hc : NXB + hc;
+            const int h0 = (yh + 8) & 7;
+            sP1[h0][c1] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, h0, (yh + 7) & 7, (yh + 9) & 7, c1, m2);
+          }
         }
       }
       __syncthreads();
@@ -469,14 +501,13 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
         const int y = 4 * k - 7 + w;
         if (y >= -1 && y <= rows) {
           const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
-          auto q2hop = [&](int c2) {
-            const int lc = c2 + 3;  // x = x0 - 1 + c2; P1 column c2 + 1
-            sQ1[s0][c2] = khop<T, C>(sP1[s0][c2 + 2], sP1[s0][c2], sP1[sp1][c2 + 1], sP1[sm1][c2 + 1], sLx[s0][lc],
-                                     sLx[s0][lc - 1], sLy[s0][lc], sLy[sm1][lc], (c2 & 1) == 0);
-          };
 #pragma unroll
-          for (int c = 0; c < CPL; c++) q2hop(CPL * l + c + 1);
-          if (l < 2) q2hop(l == 0 ? 0 : NXB + 1);
+          for (int c = 0; c < CPL; c++) ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, s0, sm1, sp1, CPL * l + c + 1);
+        }
+        if (w == 2 && l < 8) {
+          const int yh = 4 * k - 7 + (l >> 1);
+          if (yh >= -1 && yh <= rows)
+            ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, (l & 1) == 0 ? 0 : NXB + 1);
         }
       }
       __syncthreads();
