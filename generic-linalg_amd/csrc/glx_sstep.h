@@ -37,7 +37,8 @@
 //   S2  rows 4k-6 .. 4k-3   P1 = m^2 p - K qp                     cols x0-2 .. x0+W+1
 //   S3  rows 4k-7 .. 4k-4   q1 = K P1                             cols x0-1 .. x0+W
 //   S4  rows 4k-8 .. 4k-5   P2 = m^2 P1 - K q1, R1 = m^2 r - K qr, the sums (own)
-// LDS rings of 8 rows hold every stage's rows (slot = (y + 8) & 7).  S4's
+// LDS rings of 8 rows hold every stage's rows (slot = (y + 8) & 7); the
+// links are staged halved, which takes K's factor 1/2 out of every hop.  S4's
 // first row needs U_y of the row above the 8 in the ring (it shares a slot
 // with the newest staged row), so S0 saves that row's U_y into a one-row
 // buffer before overwriting it.  A strip takes sr/4 + 2 steps.
@@ -177,9 +178,18 @@ __device__ __forceinline__ void ss_carry(const St* rd, St* wr) {
   wr->alpha = rd->alpha;
 }
 
-// Re(conj(a) b) in FP64
-__device__ __forceinline__ double redot(c64 a, c64 b) {
-  return double(a.re) * double(b.re) + double(a.im) * double(b.im);
+// acc + Re(conj(a) b) in FP64 (fused multiply-adds)
+__device__ __forceinline__ double ss_acc(double acc, c64 a, c64 b) {
+  return fma(double(a.im), double(b.im), fma(double(a.re), double(b.re), acc));
+}
+// K at one point as khop, with the links already halved in LDS (a power-of-two
+// scale, so the result is khop's bit for bit)
+template <class T, class C>
+__device__ __forceinline__ T khop_h(T fxp, T fxm, T fyp, T fym, C ux, C uxm, C uy, C uym, bool eta_neg) {
+  T ax = umul(ux, fxp) - ucmul(uxm, fxm);
+  T ay = umul(uy, fyp) - ucmul(uym, fym);
+  if (eta_neg) ay = -ay;
+  return ax + ay;
 }
 // sum_j c_j y_j over the basis (only P0 and R0 unless `full`)
 __device__ __forceinline__ c64 ss_comb(const float (&c)[5], const c64 (&y)[5], bool full) {
@@ -198,8 +208,8 @@ __device__ __forceinline__ void ss_q1hop(const T (*sP)[WP], const T (*sR)[WP], c
   const int pc = qc + 1;
   const bool eta_neg = ((qc + 1) & 1) != 0;
   const C ux = sLx[s0][pc], uxm = sLx[s0][pc - 1], uy = sLy[s0][pc], uym = sLy[sm][pc];
-  sQp[s0][qc] = khop<T, C>(sP[s0][pc + 1], sP[s0][pc - 1], sP[sp][pc], sP[sm][pc], ux, uxm, uy, uym, eta_neg);
-  sQr[s0][qc] = khop<T, C>(sR[s0][pc + 1], sR[s0][pc - 1], sR[sp][pc], sR[sm][pc], ux, uxm, uy, uym, eta_neg);
+  sQp[s0][qc] = khop_h<T, C>(sP[s0][pc + 1], sP[s0][pc - 1], sP[sp][pc], sP[sm][pc], ux, uxm, uy, uym, eta_neg);
+  sQr[s0][qc] = khop_h<T, C>(sR[s0][pc + 1], sR[s0][pc - 1], sR[sp][pc], sR[sm][pc], ux, uxm, uy, uym, eta_neg);
 }
 // P1 = m^2 p - K qp at P1 column c1 (x = x0 - 2 + c1; q column c1 + 1, p and
 // link column c1 + 2)
@@ -207,7 +217,7 @@ template <class T, class C, int WP, int WL, int WQ>
 __device__ __forceinline__ T ss_p1hop(const T (*sP)[WP], const C (*sLx)[WL], const C (*sLy)[WL],
                                       const T (*sQp)[WQ], int s0, int sm, int sp, int c1, float m2) {
   const int qc = c1 + 1, pc = c1 + 2;
-  const T kq = khop<T, C>(sQp[s0][qc + 1], sQp[s0][qc - 1], sQp[sp][qc], sQp[sm][qc], sLx[s0][pc], sLx[s0][pc - 1],
+  const T kq = khop_h<T, C>(sQp[s0][qc + 1], sQp[s0][qc - 1], sQp[sp][qc], sQp[sm][qc], sLx[s0][pc], sLx[s0][pc - 1],
                           sLy[s0][pc], sLy[sm][pc], (c1 & 1) != 0);
   return rscal(m2, sP[s0][pc]) - kq;
 }
@@ -217,7 +227,7 @@ template <class T, class C, int W1, int WL, int W2>
 __device__ __forceinline__ void ss_q2hop(const T (*sP1)[W1], const C (*sLx)[WL], const C (*sLy)[WL], T (*sQ1)[W2],
                                          int s0, int sm, int sp, int c2) {
   const int lc = c2 + 3;
-  sQ1[s0][c2] = khop<T, C>(sP1[s0][c2 + 2], sP1[s0][c2], sP1[sp][c2 + 1], sP1[sm][c2 + 1], sLx[s0][lc],
+  sQ1[s0][c2] = khop_h<T, C>(sP1[s0][c2 + 2], sP1[s0][c2], sP1[sp][c2 + 1], sP1[sm][c2 + 1], sLx[s0][lc],
                            sLx[s0][lc - 1], sLy[s0][lc], sLy[sm][lc], (c2 & 1) == 0);
 }
 
@@ -435,6 +445,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
         for (int c = 0; c < CPL; c++) {
           C ux, uy;
           LK::finish(lm.r[c], ux, uy);
+          ux = 0.5f * ux;
+          uy = 0.5f * uy;
           // wave 3 stages row 4k-1 into the slot of row 4k-9, whose U_y the
           // first S4 row of this step still needs
           if (w == 3) sKeep[CPL * l + c] = sLy[sl][CPL * l + c + 4];
@@ -450,6 +462,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
           const int hs = (4 * k - 4 + hl_row + 8) & 7;
           C hx, hyy;
           LK::finish(lh, hx, hyy);
+          hx = 0.5f * hx;
+          hyy = 0.5f * hyy;
           sLx[hs][hl_c] = hx;
           sLy[hs][hl_c] = hyy;
         }
@@ -524,9 +538,9 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
             const bool eta_neg = ((CPL * l + c) & 1) != 0;  // x = x0 + CPL l + c
             const C ux = sLx[s0][lc], uxm = sLx[s0][lc - 1], uy = sLy[s0][lc];
             const C uym = w == 0 ? sKeep[CPL * l + c] : sLy[sm1][lc];
-            const T kq = khop<T, C>(sQ1[s0][c2 + 1], sQ1[s0][c2 - 1], sQ1[sp1][c2], sQ1[sm1][c2], ux, uxm, uy, uym,
+            const T kq = khop_h<T, C>(sQ1[s0][c2 + 1], sQ1[s0][c2 - 1], sQ1[sp1][c2], sQ1[sm1][c2], ux, uxm, uy, uym,
                                     eta_neg);
-            const T kr = khop<T, C>(sQr[s0][qc + 1], sQr[s0][qc - 1], sQr[sp1][qc], sQr[sm1][qc], ux, uxm, uy, uym,
+            const T kr = khop_h<T, C>(sQr[s0][qc + 1], sQr[s0][qc - 1], sQr[sp1][qc], sQr[sm1][qc], ux, uxm, uy, uym,
                                     eta_neg);
             const T pP1 = sP1[s0][c1];
             const T pp = sP[s0][lc], rr = sR[s0][lc];
@@ -534,18 +548,18 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
             const T r1 = rscal(m2, rr) - kr;
             v2[c] = p2;
             v4[c] = r1;
-            part[0] += norm2(pp);
-            part[1] += redot(pp, pP1);
-            part[2] += norm2(pP1);
-            part[3] += redot(pP1, p2);
-            part[4] += norm2(p2);
-            part[5] += redot(pp, rr);
-            part[6] += redot(pp, r1);
-            part[7] += redot(pP1, r1);
-            part[8] += redot(p2, r1);
-            part[9] += norm2(rr);
-            part[10] += redot(rr, r1);
-            part[11] += norm2(r1);
+            part[0] = ss_acc(part[0], pp, pp);
+            part[1] = ss_acc(part[1], pp, pP1);
+            part[2] = ss_acc(part[2], pP1, pP1);
+            part[3] = ss_acc(part[3], pP1, p2);
+            part[4] = ss_acc(part[4], p2, p2);
+            part[5] = ss_acc(part[5], pp, rr);
+            part[6] = ss_acc(part[6], pp, r1);
+            part[7] = ss_acc(part[7], pP1, r1);
+            part[8] = ss_acc(part[8], p2, r1);
+            part[9] = ss_acc(part[9], rr, rr);
+            part[10] = ss_acc(part[10], rr, r1);
+            part[11] = ss_acc(part[11], r1, r1);
           }
           const int j = (y0 + y) * lx + xm;
           stc(N[2], j, v2);
