@@ -161,22 +161,24 @@ __device__ __forceinline__ void eo_body(const EoGeom& eg, const OpDev<LK>& op, I
         // q = K_oe p on odd sites: step 0 rows -1, 0; step k > 0 rows 4k-3 .. 4k
         const int qa = k == 0 ? -1 : 4 * k - 3;
         const int yr = qa + w;
-        if (yr <= 4 * k) {
-          const int e = yr & 1;
-          const int s0 = (yr + 2) & RM, sm = (yr + 1) & RM, sp = (yr + 3) & RM;
-          // q column qc is compact odd column c0 - 1 + qc, full x = 2 c0 + 2 qc - 1 - e
+        // q column qc of row yq is compact odd column c0 - 1 + qc, full
+        // x = 2 c0 + 2 qc - 1 - e with e = yq & 1
+        auto qhop = [&](int yq, int qc) {
+          const int e = yq & 1;
+          const int s0 = (yq + 2) & RM, sm = (yq + 1) & RM, sp = (yq + 3) & RM;
           const bool eta_neg = e == 0;  // x odd
-          auto qhop = [&](int qc) {
-            const int li = 2 * qc + 2 - e;
-            return khop<T, C>(sP[s0][qc + 2 - e], sP[s0][qc + 1 - e], sP[sp][qc + 1], sP[sm][qc + 1], sLx[s0][li],
-                              sLx[s0][li - 1], sLy[s0][li], sLy[sm][li], eta_neg);
-          };
+          const int li = 2 * qc + 2 - e;
+          sQ[s0][qc] = khop<T, C>(sP[s0][qc + 2 - e], sP[s0][qc + 1 - e], sP[sp][qc + 1], sP[sm][qc + 1],
+                                  sLx[s0][li], sLx[s0][li - 1], sLy[s0][li], sLy[sm][li], eta_neg);
+        };
+        if (yr <= 4 * k) {
 #pragma unroll
-          for (int c = 0; c < CPL; c++) sQ[s0][CPL * l + c + 1] = qhop(CPL * l + c + 1);
-          if (l < 2) {
-            const int qc = l == 0 ? 0 : NXB + 1;
-            sQ[s0][qc] = qhop(qc);
-          }
+          for (int c = 0; c < CPL; c++) qhop(yr, CPL * l + c + 1);
+        }
+        // the two halo columns of all the step's q rows on wave 3
+        if (w == 3 && l < 8) {
+          const int yq = qa + (l >> 1);
+          if (yq <= 4 * k) qhop(yq, (l & 1) == 0 ? 0 : NXB + 1);
         }
       }
       __syncthreads();

@@ -241,21 +241,24 @@ __device__ __forceinline__ void normal_body(const NGeom& g, const OpDev<LK>& op,
         // q rows: step 0 rows -1, 0; step k > 0 rows 4k-3 .. 4k
         const int qa = k == 0 ? -1 : 4 * k - 3;
         const int yr = qa + w;
+        // q column c is x = x0 - 1 + c; its p column is c + 1, link column
+        // c + 1 (lx is even, so x's parity needs no wrap)
+        auto qhop = [&](int yq, int c) {
+          const int s0 = (yq + 2) & RM, sm = (yq + 1) & RM, sp = (yq + 3) & RM;
+          const int pc = c + 1;
+          const bool eta_neg = ((x0 - 1 + c) & 1) != 0;
+          sQ[s0][c] = khop<T, C>(sP[s0][pc + 1], sP[s0][pc - 1], sP[sp][pc], sP[sm][pc], sLx[s0][pc],
+                                 sLx[s0][pc - 1], sLy[s0][pc], sLy[sm][pc], eta_neg);
+        };
         if (yr <= 4 * k) {
-          const int s0 = (yr + 2) & RM, sm = (yr + 1) & RM, sp = (yr + 3) & RM;
-          // q column c is x = x0 - 1 + c; its p column is c + 1, link column c + 1
-          auto qhop = [&](int c) {
-            const int pc = c + 1;
-            const bool eta_neg = (wrap_coord(x0 - 1 + c, lx) & 1) != 0;
-            return khop<T, C>(sP[s0][pc + 1], sP[s0][pc - 1], sP[sp][pc], sP[sm][pc], sLx[s0][pc], sLx[s0][pc - 1],
-                              sLy[s0][pc], sLy[sm][pc], eta_neg);
-          };
 #pragma unroll
-          for (int c = 0; c < CPL; c++) sQ[s0][CPL * l + c + 1] = qhop(CPL * l + c + 1);
-          if (l < 2) {
-            const int c = l == 0 ? 0 : NXB + 1;
-            sQ[s0][c] = qhop(c);
-          }
+          for (int c = 0; c < CPL; c++) qhop(yr, CPL * l + c + 1);
+        }
+        // the two halo columns of all the step's q rows on wave 3 (a pass
+        // with two active lanes per wave would cost every wave a full pass)
+        if (w == 3 && l < 8) {
+          const int yq = qa + (l >> 1);
+          if (yq <= 4 * k) qhop(yq, (l & 1) == 0 ? 0 : NXB + 1);
         }
       }
       __syncthreads();
