@@ -1944,12 +1944,25 @@ int sstep_rows_for(int lx, int ly) {
   return it != t.rows.end() ? it->second : sstep_rows_auto(lx, ly);
 }
 
-// Columns per lane of the s-step kernel (strips of 64 kSsCPL columns).
-constexpr int kSsCPL = 2;
+// Columns per lane of the s-step kernel: D^dag D in strips of 128 columns
+// (two per lane); the even/odd Schur operator in strips of 64 compact
+// columns (one per lane: its links take twice the LDS), also 128 lattice
+// columns wide.
+constexpr int kSsCPL = 2, kSsCPLEo = 1;
 
 static bool sscg_fits(const DevOperator& o) {
-  constexpr int nx = kNX * kSsCPL;
-  return o.kind == OP_STAG_NORMAL && o.tiled() && o.lx >= nx && o.lx % nx == 0 && o.ly >= 8 && o.ly % 4 == 0;
+  if (!o.tiled() || o.ly < 8 || o.ly % 4 != 0) return false;
+  if (o.kind == OP_STAG_NORMAL) return o.lx >= kNX * kSsCPL && o.lx % (kNX * kSsCPL) == 0;
+  if (o.kind == OP_STAG_EO_SCHUR) return o.lx % 2 == 0 && o.lx / 2 >= kNX * kSsCPLEo && (o.lx / 2) % (kNX * kSsCPLEo) == 0;
+  return false;
+}
+// Launch geometry of the s-step kernel for the operator's lattice.
+static EoGeom ss_geom(const DevOperator& o, int rows) {
+  if (o.kind == OP_STAG_EO_SCHUR) return make_eogeom(o.lx, o.ly, rows, kSsCPLEo);
+  EoGeom e;
+  e.g = make_ngeom(o.lx, o.ly, rows, kSsCPL);
+  e.lx = o.lx;
+  return e;
 }
 
 // setup: R0 of slot 0 = b - A x in FP32, with |b|^2 and |r|^2
@@ -1999,18 +2012,27 @@ struct EpiSsTrue {
 };
 
 // One s-step launch with the link storage of `link_prec`.
-static void launch_sscg(hipStream_t st, const DevOperator& o, int link_prec, const NGeom& g, SsBufs bf,
+template <class LK>
+static void launch_sscg_lk(hipStream_t st, const DevOperator& o, const LK& lk, const EoGeom& eg, SsBufs bf,
+                           const SolveState* rd, SolveState* wr, int q, RedWs ws) {
+  const OpDev<LK> od = make_opdev(o, lk);
+  if (o.kind == OP_STAG_EO_SCHUR)
+    hipLaunchKernelGGL((k_sscg<kSsCPLEo, true, LK, SolveState>), dim3(eg.g.nblk), dim3(kBlock), 0, st, eg, od, bf,
+                       rd, wr, q, ws);
+  else
+    hipLaunchKernelGGL((k_sscg<kSsCPL, false, LK, SolveState>), dim3(eg.g.nblk), dim3(kBlock), 0, st, eg, od, bf,
+                       rd, wr, q, ws);
+  GLX_CHECK_LAUNCH();
+}
+static void launch_sscg(hipStream_t st, const DevOperator& o, int link_prec, const EoGeom& eg, SsBufs bf,
                         const SolveState* rd, SolveState* wr, int q, RedWs ws) {
   if (link_prec == LINKS_F16) {
     if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");
-    const OpDev<LinksF16> od = make_opdev(o, LinksF16{o.u16, o.sc16});
-    hipLaunchKernelGGL((k_sscg<kSsCPL, LinksF16, SolveState>), dim3(g.nblk), dim3(kBlock), 0, st, g, od, bf, rd, wr, q, ws);
+    launch_sscg_lk(st, o, LinksF16{o.u16, o.sc16}, eg, bf, rd, wr, q, ws);
   } else {
     if (!o.u32) throw ArgError("operator has no FP32 links on the device");
-    const OpDev<LinksF32> od = make_opdev(o, LinksF32{o.u32});
-    hipLaunchKernelGGL((k_sscg<kSsCPL, LinksF32, SolveState>), dim3(g.nblk), dim3(kBlock), 0, st, g, od, bf, rd, wr, q, ws);
+    launch_sscg_lk(st, o, LinksF32{o.u32}, eg, bf, rd, wr, q, ws);
   }
-  GLX_CHECK_LAUNCH();
 }
 
 // The s-step basis and FP32 solution increment of one solve, zeroed.
@@ -2029,9 +2051,9 @@ SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveO
   const int n = o.size();
   OwnedStream own;
   hipStream_t st = opt.stream ? opt.stream : own.st;
-  const NGeom g = make_ngeom(o.lx, o.ly, rows, kSsCPL);
-  // two buffers of 16 sums per block
-  Work w(st, std::max(2 * g.nblk, stencil_blocks(o.lx, o.ly)), opt.max_iter, opt.want_hist);
+  const EoGeom g = ss_geom(o, rows);
+  // two buffers of the sums per block
+  Work w(st, std::max(2 * g.g.nblk, stencil_blocks(o.lx, o.ly)), opt.max_iter, opt.want_hist);
   SolveState h;
   std::memset(&h, 0, sizeof(h));
   h.max_iter = opt.max_iter;
@@ -2132,8 +2154,8 @@ double time_sscg_kernel_impl(const DevOperator& o, int link_prec, int reps, int 
   const int n = o.size();
   OwnedStream own;
   hipStream_t st = own.st;
-  const NGeom g = make_ngeom(o.lx, o.ly, rows, kSsCPL);
-  Work w(st, std::max(2 * g.nblk, stencil_blocks(o.lx, o.ly)), 0, false);
+  const EoGeom g = ss_geom(o, rows);
+  Work w(st, std::max(2 * g.g.nblk, stencil_blocks(o.lx, o.ly)), 0, false);
   SsSpace sp(n, st);
   GLX_HIP(hipMemsetAsync(sp.bf.v(0, 3), 0x3c, size_t(n) * sizeof(c64), st));
   DevBuf shadow_buf(sizeof(SolveState));

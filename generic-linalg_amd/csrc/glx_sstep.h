@@ -44,7 +44,7 @@
 // buffer before overwriting it.  A strip takes sr/4 + 2 steps.
 #pragma once
 
-#include "glx_normal.h"
+#include "glx_eo.h"
 
 namespace glx {
 
@@ -231,14 +231,58 @@ __device__ __forceinline__ void ss_q2hop(const T (*sP1)[W1], const C (*sLx)[WL],
                            sLx[s0][lc - 1], sLy[s0][lc], sLy[sm][lc], (c2 & 1) == 0);
 }
 
-template <int CPL, class LK, class St>
-__global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q,
+// The same steps for the even/odd Schur operator m^2 - K_eo K_oe on compact
+// even-site vectors (glx_eo.h): p, r, P1 live on even sites, qp, qr, q1 on
+// odd sites, in compact columns; row y has parity e = y & 1 (y0 even), odd
+// site c sits at full column x = 2c + 1 - e with even x-neighbours c - e and
+// c + 1 - e, even site c at x = 2c + e with odd x-neighbours c + e - 1 and
+// c + e; y-neighbours share the compact column.  The links are staged by full
+// column, LDS link column = x - (2 x0 - 7).
+// qp, qr at odd column qc (compact c0 - 3 + qc)
+template <class T, class C, int WP, int WL, int WQ>
+__device__ __forceinline__ void ss_q1hop_eo(const T (*sP)[WP], const T (*sR)[WP], const C (*sLx)[WL],
+                                            const C (*sLy)[WL], T (*sQp)[WQ], T (*sQr)[WQ], int s0, int sm, int sp,
+                                            int qc, int e) {
+  const int li = 2 * qc + 2 - e;
+  const bool eta_neg = e == 0;
+  const C ux = sLx[s0][li], uxm = sLx[s0][li - 1], uy = sLy[s0][li], uym = sLy[sm][li];
+  sQp[s0][qc] = khop_h<T, C>(sP[s0][qc + 2 - e], sP[s0][qc + 1 - e], sP[sp][qc + 1], sP[sm][qc + 1], ux, uxm, uy,
+                             uym, eta_neg);
+  sQr[s0][qc] = khop_h<T, C>(sR[s0][qc + 2 - e], sR[s0][qc + 1 - e], sR[sp][qc + 1], sR[sm][qc + 1], ux, uxm, uy,
+                             uym, eta_neg);
+}
+// P1 = m^2 p - K_eo qp at even column c1 (compact c0 - 2 + c1; p column c1 + 2)
+template <class T, class C, int WP, int WL, int WQ>
+__device__ __forceinline__ T ss_p1hop_eo(const T (*sP)[WP], const C (*sLx)[WL], const C (*sLy)[WL],
+                                         const T (*sQp)[WQ], int s0, int sm, int sp, int c1, int e, float m2) {
+  const int li = 2 * c1 + e + 3;
+  const T kq = khop_h<T, C>(sQp[s0][c1 + e + 1], sQp[s0][c1 + e], sQp[sp][c1 + 1], sQp[sm][c1 + 1], sLx[s0][li],
+                            sLx[s0][li - 1], sLy[s0][li], sLy[sm][li], e != 0);
+  return rscal(m2, sP[s0][c1 + 2]) - kq;
+}
+// q1 = K_oe P1 at odd column c2 (compact c0 - 1 + c2)
+template <class T, class C, int W1, int WL, int W2>
+__device__ __forceinline__ void ss_q2hop_eo(const T (*sP1)[W1], const C (*sLx)[WL], const C (*sLy)[WL],
+                                            T (*sQ1)[W2], int s0, int sm, int sp, int c2, int e) {
+  const int li = 2 * c2 + 6 - e;
+  sQ1[s0][c2] = khop_h<T, C>(sP1[s0][c2 + 2 - e], sP1[s0][c2 + 1 - e], sP1[sp][c2 + 1], sP1[sm][c2 + 1], sLx[s0][li],
+                             sLx[s0][li - 1], sLy[s0][li], sLy[sm][li], e == 0);
+}
+
+// EO = false: D^dag D on full-lattice vectors (eg.lx = eg.g.lx); EO = true:
+// m^2 - K_eo K_oe on compact even-site vectors (eg.g over lx / 2 columns,
+// eg.lx the full width for the links).
+template <int CPL, bool EO, class LK, class St>
+__global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q,
                                                  RedWs ws) {
   using T = c64;
   using C = typename LK::C;
-  constexpr int NXB = kNX * CPL;  // columns per strip
+  const NGeom& g = eg.g;
+  constexpr int NXB = kNX * CPL;  // (compact) columns per strip
   constexpr int WP = NXB + 8;     // p, r: columns x0-4 .. x0+NXB+3
-  constexpr int WL = NXB + 8;     // links: columns x0-4 .. x0+NXB+2 (last slot unused)
+  // links: columns x0-4 .. x0+NXB+2 (last slot unused); EO: full columns
+  // 2 x0 - 7 .. 2 x0 + 2 NXB + 5 (last slot unused)
+  constexpr int WL = EO ? 2 * NXB + 14 : NXB + 8;
   constexpr int WQ = NXB + 6;     // qp, qr: x0-3 .. x0+NXB+2
   constexpr int W1 = NXB + 4;     // P1: x0-2 .. x0+NXB+1
   constexpr int W2 = NXB + 2;     // q1: x0-1 .. x0+NXB
@@ -269,7 +313,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
   const int tile = xcd_tile(blockIdx.x, g.nblk);
   const bool active = tile < g.ntiles;
   const int tt = active ? tile : 0;  // grid padding only joins the sums
-  const int lx = g.lx, ly = g.ly;
+  const int lx = g.lx, ly = g.ly;  // lx: vector row length (compact for EO)
+  const int lxf = eg.lx;            // link row length
   const int x0 = (tt % g.ntx) * NXB;
   const int y0 = (tt / g.ntx) * g.sr;
   const int rows = min(g.sr, ly - y0);  // a multiple of 4
@@ -310,18 +355,21 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
   // few active lanes costs a wave its whole instruction stream):
   //   S0  wave 3 lanes 0..31: p, r on columns x0-4 .. x0-1, x0+NXB .. x0+NXB+3
   //       wave 1 lanes 0..27: the links on x0-4 .. x0-1, x0+NXB .. x0+NXB+2
+  //       (EO lanes 0..51: full columns 2x0-7 .. 2x0-1, 2x0+2NXB .. 2x0+2NXB+5)
   //   S1  wave 0 lanes 0..23: qp, qr on x0-3 .. x0-1, x0+NXB .. x0+NXB+2
   //   S2  wave 1 lanes 0..15: P1 on x0-2, x0-1, x0+NXB, x0+NXB+1
   //   S3  wave 2 lanes 0..7:  q1 on x0-1, x0+NXB
   const bool hp = w == 3 && l < 32;
-  const bool hl = w == 1 && l < 28;
+  constexpr int kHL = EO ? 13 : 7;  // link halo columns per row
+  const bool hl = w == 1 && l < 4 * kHL;
   const int hp_row = l >> 3, hp_c = (l & 7) < 4 ? (l & 7) : NXB + (l & 7);
-  const int hl_row = l / 7, hl_c = (l % 7) < 4 ? (l % 7) : NXB + (l % 7);
+  const int hl_row = l / kHL, hl_j = l % kHL;
+  const int hl_c = EO ? (hl_j < 7 ? hl_j : 2 * NXB + hl_j) : (hl_j < 4 ? hl_j : NXB + hl_j);
   const int xhp = wrap_coord(x0 - 4 + hp_c, lx);
-  const int xhl = wrap_coord(x0 - 4 + hl_c, lx);
+  const int xhl = EO ? wrap_coord(2 * x0 - 7 + hl_c, lxf) : wrap_coord(x0 - 4 + hl_c, lx);
 
   c64 ry[5][CPL], rx[CPL];
-  LinkRegs<LK, CPL> lm;
+  LinkRegs<LK, EO ? 2 * CPL : CPL> lm;
   c64 hy[5];
   typename LK::Raw lh;
   auto fetch = [&](int k) {
@@ -336,7 +384,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
       ldc(Y[4], j, ry[4]);
       if (own) ldc(bf.x, j, rx);
     }
-    lm.fetch(op.links, j);
+    if constexpr (EO) lm.fetch(op.links, wrap_coord(y0 + yr, ly) * lxf + 2 * xm);
+    else lm.fetch(op.links, j);
     if (hp) {
       const int jh = wrap_coord(y0 + 4 * k - 4 + hp_row, ly) * lx + xhp;
       hy[0] = Y[0][jh];
@@ -347,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
         hy[4] = Y[4][jh];
       }
     }
-    if (hl) lh = op.links.fetch(wrap_coord(y0 + 4 * k - 4 + hl_row, ly) * lx + xhl);
+    if (hl) lh = op.links.fetch(wrap_coord(y0 + 4 * k - 4 + hl_row, ly) * lxf + xhl);
   };
   if (active) fetch(0);
 
@@ -441,17 +490,32 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
           stc(N[3], j, rv2);
           if (full) stc(bf.x, j, xv2);
         }
+        // wave 3 stages row 4k-1 into the slot of row 4k-9, whose U_y the
+        // first S4 row of this step still needs (for EO: at the full columns
+        // of that even row's sites, link columns 2b + 7)
+        if constexpr (EO) {
 #pragma unroll
-        for (int c = 0; c < CPL; c++) {
-          C ux, uy;
-          LK::finish(lm.r[c], ux, uy);
-          ux = 0.5f * ux;
-          uy = 0.5f * uy;
-          // wave 3 stages row 4k-1 into the slot of row 4k-9, whose U_y the
-          // first S4 row of this step still needs
-          if (w == 3) sKeep[CPL * l + c] = sLy[sl][CPL * l + c + 4];
-          sLx[sl][CPL * l + c + 4] = ux;
-          sLy[sl][CPL * l + c + 4] = uy;
+          for (int t = 0; t < 2 * CPL; t++) {
+            C ux, uy;
+            LK::finish(lm.r[t], ux, uy);
+            ux = 0.5f * ux;
+            uy = 0.5f * uy;
+            const int li = 2 * CPL * l + t + 7;
+            if (w == 3 && (t & 1) == 0) sKeep[CPL * l + t / 2] = sLy[sl][li];
+            sLx[sl][li] = ux;
+            sLy[sl][li] = uy;
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < CPL; c++) {
+            C ux, uy;
+            LK::finish(lm.r[c], ux, uy);
+            ux = 0.5f * ux;
+            uy = 0.5f * uy;
+            if (w == 3) sKeep[CPL * l + c] = sLy[sl][CPL * l + c + 4];
+            sLx[sl][CPL * l + c + 4] = ux;
+            sLy[sl][CPL * l + c + 4] = uy;
+          }
         }
         if (hp) {
           const int hs = (4 * k - 4 + hp_row + 8) & 7;
@@ -476,13 +540,19 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
         if (y >= -3 && y <= rows + 2) {
           const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
 #pragma unroll
-          for (int c = 0; c < CPL; c++) ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, s0, sm1, sp1, CPL * l + c + 3);
+          for (int c = 0; c < CPL; c++) {
+            if constexpr (EO) ss_q1hop_eo<T, C>(sP, sR, sLx, sLy, sQp, sQr, s0, sm1, sp1, CPL * l + c + 3, y & 1);
+            else ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, s0, sm1, sp1, CPL * l + c + 3);
+          }
         }
         if (w == 0 && l < 24) {
           const int yh = 4 * k - 5 + l / 6, hc = l % 6;
-          if (yh >= -3 && yh <= rows + 2)
-            ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7,
-                           hc < 3 ? hc : NXB + hc);
+          if (yh >= -3 && yh <= rows + 2) {
+            const int qc = hc < 3 ? hc : NXB + hc;
+            if constexpr (EO)
+              ss_q1hop_eo<T, C>(sP, sR, sLx, sLy, sQp, sQr, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, qc, yh & 1);
+            else ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, qc);
+          }
         }
       }
       __syncthreads();
@@ -495,7 +565,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
             const int c1 = CPL * l + c + 2;
-            v1[c] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, s0, sm1, sp1, c1, m2);
+            if constexpr (EO) v1[c] = ss_p1hop_eo<T, C>(sP, sLx, sLy, sQp, s0, sm1, sp1, c1, y & 1, m2);
+            else v1[c] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, s0, sm1, sp1, c1, m2);
             sP1[s0][c1] = v1[c];
           }
           if (y >= 0 && y < rows) stc(N[1], (y0 + y) * lx + xm, v1);
@@ -505,7 +576,9 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
           if (yh >= -2 && yh <= rows + 1) {
             const int c1 = hc < 2 ? hc : NXB + hc;
             const int h0 = (yh + 8) & 7;
-            sP1[h0][c1] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, h0, (yh + 7) & 7, (yh + 9) & 7, c1, m2);
+            if constexpr (EO)
+              sP1[h0][c1] = ss_p1hop_eo<T, C>(sP, sLx, sLy, sQp, h0, (yh + 7) & 7, (yh + 9) & 7, c1, yh & 1, m2);
+            else sP1[h0][c1] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, h0, (yh + 7) & 7, (yh + 9) & 7, c1, m2);
           }
         }
       }
@@ -516,12 +589,19 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
         if (y >= -1 && y <= rows) {
           const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
 #pragma unroll
-          for (int c = 0; c < CPL; c++) ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, s0, sm1, sp1, CPL * l + c + 1);
+          for (int c = 0; c < CPL; c++) {
+            if constexpr (EO) ss_q2hop_eo<T, C>(sP1, sLx, sLy, sQ1, s0, sm1, sp1, CPL * l + c + 1, y & 1);
+            else ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, s0, sm1, sp1, CPL * l + c + 1);
+          }
         }
         if (w == 2 && l < 8) {
           const int yh = 4 * k - 7 + (l >> 1);
-          if (yh >= -1 && yh <= rows)
-            ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, (l & 1) == 0 ? 0 : NXB + 1);
+          if (yh >= -1 && yh <= rows) {
+            const int c2 = (l & 1) == 0 ? 0 : NXB + 1;
+            if constexpr (EO)
+              ss_q2hop_eo<T, C>(sP1, sLx, sLy, sQ1, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, c2, yh & 1);
+            else ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, c2);
+          }
         }
       }
       __syncthreads();
@@ -534,16 +614,25 @@ __global__ __launch_bounds__(kBlock) void k_sscg(NGeom g, OpDev<LK> op, SsBufs b
           T v2[CPL], v4[CPL];
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
-            const int c2 = CPL * l + c + 1, lc = CPL * l + c + 4, c1 = CPL * l + c + 2, qc = CPL * l + c + 3;
-            const bool eta_neg = ((CPL * l + c) & 1) != 0;  // x = x0 + CPL l + c
+            const int b = CPL * l + c;
+            const int c1 = b + 2, pc = b + 4;
+            // full lattice: q1 column b + 1 +- 1, qr column b + 3 +- 1, link
+            // column b + 4; EO (even site, x = 2 (x0 + b) + e): q1 columns
+            // b + e, b + e + 1 (y: b + 1), qr b + e + 2, b + e + 3 (y: b + 3),
+            // link column 2b + e + 7
+            const int e = EO ? (y & 1) : 0;
+            const int q1m = EO ? b + e : b, q1p = EO ? b + e + 1 : b + 2;
+            const int qrm = EO ? b + e + 2 : b + 2, qrp = EO ? b + e + 3 : b + 4;
+            const int lc = EO ? 2 * b + e + 7 : b + 4;
+            const bool eta_neg = EO ? e != 0 : (b & 1) != 0;
             const C ux = sLx[s0][lc], uxm = sLx[s0][lc - 1], uy = sLy[s0][lc];
-            const C uym = w == 0 ? sKeep[CPL * l + c] : sLy[sm1][lc];
-            const T kq = khop_h<T, C>(sQ1[s0][c2 + 1], sQ1[s0][c2 - 1], sQ1[sp1][c2], sQ1[sm1][c2], ux, uxm, uy, uym,
-                                    eta_neg);
-            const T kr = khop_h<T, C>(sQr[s0][qc + 1], sQr[s0][qc - 1], sQr[sp1][qc], sQr[sm1][qc], ux, uxm, uy, uym,
-                                    eta_neg);
+            const C uym = w == 0 ? sKeep[b] : sLy[sm1][lc];
+            const T kq = khop_h<T, C>(sQ1[s0][q1p], sQ1[s0][q1m], sQ1[sp1][b + 1], sQ1[sm1][b + 1], ux, uxm, uy, uym,
+                                      eta_neg);
+            const T kr = khop_h<T, C>(sQr[s0][qrp], sQr[s0][qrm], sQr[sp1][b + 3], sQr[sm1][b + 3], ux, uxm, uy, uym,
+                                      eta_neg);
             const T pP1 = sP1[s0][c1];
-            const T pp = sP[s0][lc], rr = sR[s0][lc];
+            const T pp = sP[s0][pc], rr = sR[s0][pc];
             const T p2 = rscal(m2, pP1) - kq;
             const T r1 = rscal(m2, rr) - kr;
             v2[c] = p2;

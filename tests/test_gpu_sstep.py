@@ -155,3 +155,36 @@ def test_sstep_autotune(glx, orc):
     finally:
         glx.set_sstep_rows(L, L, -1)
     assert glx.get_sstep_rows(L, L) == 8
+
+
+@pytest.mark.parametrize("lx,ly,link", [(256, 64, "mixed"), (128, 8, "mixed"), (384, 40, "mixed_f16"), (512, 72, "mixed")])
+def test_sstep_eo_schur(glx, orc, lx, ly, link):
+    """The even/odd solve of D x = b (prepare, CG on m^2 - D_eo D_oe over the
+    compact even sites, reconstruct) with its mixed CG in s-step form (64
+    compact columns per strip): the full residual bound of the pipeline, the
+    solution of the reference-order FP64 pipeline, and an iteration count close
+    to the one-iteration form's (where that form takes the lattice: its FP32
+    kernel needs lx / 2 a multiple of 128)."""
+    m = 0.05
+    links, bc, _ = lattice_inputs(orc, lx, ly, beta=6.25, seed=41 + lx + ly)
+    lat = glx.Lattice(links, lx, ly)
+
+    def solve(method, sstep):
+        glx.set_sstep(sstep)
+        try:
+            dx = glx.DeviceArray(np.zeros_like(bc))
+            info, r2 = glx.dev_solve_staggered_eo(lat, m, dx, glx.DeviceArray(bc), method, max_iter=5000, eps=1e-10)
+            return dx.download(), info
+        finally:
+            glx.set_sstep()
+
+    x, info = solve(link, 2)
+    assert info.success
+    r = bc - orc.apply("staggered", x, links, m, lx, ly)
+    assert np.linalg.norm(r) <= 1e-10 * (m + 2.0) / m * np.linalg.norm(bc) * 1.01
+    x0, _ = solve("cg", 2)
+    kappa = (m * m + 4.0) / (m * m)
+    assert relerr(x, x0) < 2e-10 * kappa
+    if (lx // 2) % 128 == 0:
+        _, i1 = solve(link, 1)
+        assert abs(info.iter - i1.iter) <= max(6, 0.03 * i1.iter), (info.iter, i1.iter)
