@@ -1928,12 +1928,15 @@ void set_sstep_rows(int lx, int ly, int rows) {
   if (rows < 0) t.rows.erase(std::make_pair(lx, ly));
   else t.rows[std::make_pair(lx, ly)] = rows;
 }
-// Without a table entry: 16 rows per block, 8 where that leaves fewer than
-// 256 blocks, 32 from 2048 blocks on (measured at 256^2 .. 2048^2: 256^2 and
-// 512^2 best at 8, 1024^2 at 16, 2048^2 at 32).
+// Without a table entry: the fewest rows per block (8, 16, 32, 64, 128) that
+// keep the grid within 512 blocks, two per CU, else 128 (measured: 256^2 and
+// 512^2 best at 8, 768^2 and 1024^2 at 16, 2048^2 at 64, 4096^2 at 128; the
+// strips are 128 lattice columns wide for both operators).
 int sstep_rows_auto(int lx, int ly) {
-  const long b16 = long(lx / (kNX * 2)) * ((ly + 15) / 16);
-  return b16 < 256 ? 8 : (b16 >= 2048 ? 32 : 16);
+  const long strips = lx / (kNX * 2);
+  for (int r = 8; r <= 64; r *= 2)
+    if (strips * ((ly + r - 1) / r) <= 512) return r;
+  return 128;
 }
 int sstep_rows_for(int lx, int ly) {
   const int forced = g_sstep_rows.load();

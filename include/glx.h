@@ -415,8 +415,8 @@ int glx_set_tuning(int group, int use_graph);
  * coordinates) where lx % 128 == 0, ly % 4 == 0 and ly >= 8; s = 1 one
  * iteration per launch.  rows: rows per block of the s-step kernel for every
  * lattice size (a multiple of 4, >= 8), or 0 (default): per size, from the
- * table below, else 16 (8 where 16 leaves fewer than 256 blocks, 32 from 2048
- * blocks on). */
+ * table below, else the fewest of 8, 16, 32, 64 that keep the launch within
+ * 512 blocks of 128 lattice columns (128 beyond). */
 int glx_set_sstep(int s, int rows);
 int glx_get_sstep(int* s, int* rows);
 /* The per-size table: rows per block of the s-step kernel for an lx x ly

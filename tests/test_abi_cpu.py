@@ -136,7 +136,8 @@ def test_sstep_setting_without_device():
     assert lib.glx_get_sstep(C.byref(s), C.byref(rows)) == 0 and (s.value, rows.value) == (2, 0)
     # per size: the automatic choice by block count, the table, removal
     r = C.c_int()
-    for (lx, ly), want in {(256, 256): 8, (512, 512): 8, (768, 768): 16, (1024, 1024): 16, (2048, 2048): 32}.items():
+    for (lx, ly), want in {(256, 256): 8, (512, 512): 8, (768, 768): 16, (1024, 1024): 16, (2048, 2048): 64,
+                           (4096, 4096): 128}.items():
         assert lib.glx_get_sstep_rows(lx, ly, C.byref(r)) == 0 and r.value == want, (lx, r.value)
     assert lib.glx_set_sstep_rows(1024, 1024, 0) == 0
     assert lib.glx_get_sstep_rows(1024, 1024, C.byref(r)) == 0 and r.value == 0
