@@ -1431,7 +1431,7 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
     int best = 0;
     double best_ms = 1e300;
     try {
-      for (int c : {0, 8, 16, 32, 64}) {
+      for (int c : {0, 8, 16, 32, 64, 128}) {
         if (c > 0 && (o.lx % 128 != 0 || o.ly < 8 || o.ly % 4 != 0)) break;
         set_sstep_rows(o.lx, o.ly, c);
         auto time_one = [&]() {

@@ -595,7 +595,7 @@ def get_sstep_rows(lx: int, ly: int) -> int:
 
 def autotune_sstep(lat, reps: int = 40) -> dict:
     """Time the mixed CG on this lattice in one-iteration form and in s-step
-    form at 8 / 16 / 32 / 64 rows per block; install and return the fastest."""
+    form at 8 / 16 / 32 / 64 / 128 rows per block; install and return the fastest."""
     r, ms = C.c_int(0), C.c_double(0.0)
     _check(lib().glx_autotune_sstep(lat.h, reps, C.byref(r), C.byref(ms)), "glx_autotune_sstep")
     return {"rows": r.value, "ms_per_iteration": ms.value}

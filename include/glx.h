@@ -426,7 +426,7 @@ int glx_set_sstep_rows(int lx, int ly, int rows);
 int glx_get_sstep_rows(int lx, int ly, int* rows);
 /* Autotuner for the mixed-precision CG on D^dag D of the lattice: times
  * `reps` iterations in the one-iteration form and in s-step form at 8, 16,
- * 32 and 64 rows per block, installs the fastest in the table above and
+ * 32, 64 and 128 rows per block, installs the fastest in the table above and
  * returns it (0 = one-iteration form) with its time per iteration in ms.
  * Needs FP32 links.  No reference counterpart. */
 int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iteration);

@@ -147,7 +147,7 @@ def test_sstep_autotune(glx, orc):
     lat = glx.Lattice(links, L, L)
     try:
         res = glx.autotune_sstep(lat, reps=10)
-        assert res["rows"] in (0, 8, 16, 32, 64) and res["ms_per_iteration"] > 0
+        assert res["rows"] in (0, 8, 16, 32, 64, 128) and res["ms_per_iteration"] > 0
         assert glx.get_sstep_rows(L, L) == res["rows"]
         x, info = _solve(glx, lat, 0.05, bc, max_iter=20000, eps=1e-10)
         assert info.success
