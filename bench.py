@@ -453,6 +453,9 @@ def main():
     ap.add_argument("--no-eo", action="store_true", help="skip the even/odd D x = b solves")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multishift workload")
     ap.add_argument("--no-cstencil", action="store_true", help="skip the general stencil_2d timing")
+    ap.add_argument("--sstep-rows", type=int, default=0,
+                    help="rows per block of the mixed solver's s-step kernel (0: autotune; a profiler changes the "
+                         "timings the autotuner compares, so profiled runs pin the shape of the unprofiled one)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
                          "for scaling rocprofv3 byte counters)")
@@ -479,7 +482,11 @@ def main():
     lat = glx.Lattice(links, L, L, f32=True, f16=True)
     glx.set_tuning(group=16, use_graph=False)
     tuned = glx.autotune_normal(lat, reps=40)
-    tuned_ss = glx.autotune_sstep(lat, reps=40)
+    if args.sstep_rows > 0:
+        glx.set_sstep_rows(L, L, args.sstep_rows)
+        tuned_ss = {"rows": args.sstep_rows}
+    else:
+        tuned_ss = glx.autotune_sstep(lat, reps=40)
     d_b = glx.DeviceArray(b)
     d_x = glx.DeviceArray(np.zeros_like(b))
     d_x64 = glx.DeviceArray(np.zeros_like(b))
@@ -596,7 +603,8 @@ def main():
         "speedup_vs_fp64": round(ms64 / msmx, 3),
         "speedup_basis": "plain FP64 GPU CG in the reference's order of operations (glx_dev_solve)",
         "speedup_vs_fp64_fused": round(msfu / msmx, 3),
-        "mixed_form": (f"s-step, two CG iterations per launch, {tuned_ss['rows']} rows per block (autotuned)"
+        "mixed_form": (f"s-step, two CG iterations per launch, {tuned_ss['rows']} rows per block "
+                       f"({'set' if args.sstep_rows > 0 else 'autotuned'})"
                        if tuned_ss["rows"] > 0 else "one CG iteration per launch (autotuned)"),
         "mixed_single_pass_ms": round(msm1, 3),
         "mixed_single_pass_iters": infom1.iter,
