@@ -1,19 +1,20 @@
 // glx_sstep.h -- two CG iterations per launch: s-step CG (s = 2, the
 // communication-avoiding CG of Carson and Demmel with a monomial basis) on the
 // staggered normal operator A = D^dag D = m^2 - K^2 (operators.cpp:444-453),
-// complex FP32 vectors.  The mixed-precision solver runs it between FP64
-// reliable updates.
+// and on its even/odd Schur form m^2 - K_eo K_oe over compact even sites
+// (operators.cpp:543-616), complex FP32 vectors.  The mixed-precision solver
+// runs it between FP64 reliable updates.
 //
 // Launch `ko` holds the basis of the previous launch in slot ko & 1,
 //   Y = [P0, P1, P2, R0, R1] = [p, A p, A^2 p, r, A r],
 // and writes the next basis into slot (ko + 1) & 1:
-//   prelude  every block sums the previous launch's 16 partial sums (in block
+//   prelude  every block sums the previous launch's 12 partial sums (in block
 //            order, so all blocks agree) into the Gram matrix G = Y^H Y, and
 //            performs two CG iterations on coordinate 5-vectors in FP64:
 //              p' = e0, r' = e3, x' = 0, delta = G33
 //              twice: w' = B p' (B shifts e0 -> e1 -> e2, e3 -> e4)
-//                     alpha = delta / p'^H G w'; x' += alpha p'; r' -= alpha w'
-//                     delta' = r'^H G r'; beta = delta' / delta; p' = r' + beta p'
+//                     alpha = delta / p'^T G w'; x' += alpha p'; r' -= alpha w'
+//                     delta' = r'^T G r'; beta = delta' / delta; p' = r' + beta p'
 //            A Hermitian gives <A^a u, A^b v> = <u, A^(a+b) v>, so G comes
 //            from 5 + 4 + 3 sums (mu_d over p, nu_d over p and r, rho_d over
 //            r); the CG scalars of a Hermitian A are real, so the coordinates
