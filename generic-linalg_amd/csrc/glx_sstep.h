@@ -72,19 +72,6 @@ struct SsGram {
     return v[5 + d];
   }
 };
-// a^T Re(G) b
-__device__ __forceinline__ double ss_form(const SsGram& g, const double (&a)[5], const double (&b)[5]) {
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i < 5; i++) {
-    double t = 0.0;
-#pragma unroll
-    for (int j = 0; j < 5; j++) t += g.at(i, j) * b[j];
-    s += a[i] * t;
-  }
-  return s;
-}
-
 // What the prelude decided for this launch.
 struct SsPlan {
   double cp[5], cr[5], cx[5];
@@ -96,70 +83,98 @@ struct SsPlan {
 
 // The two iterations on coordinates (v = the sums of the basis in slot
 // ko & 1), from iteration k0 with maxrr2 = max |r|^2 since the last reliable
-// update.
+// update.  Written out with the coordinates' known zeros (the iterations of
+// the generic form p' = e0, r' = e3, w' = B p', alpha = delta / p'^T G w',
+// r' -= alpha w', delta' = r'^T G r', p' = r' + beta p'):
+//   iteration 1  p' = e0, w' = e1: alpha0 = rho0 / mu1, r' = e3 - alpha0 e1,
+//                delta1 = rho0 - 2 alpha0 Re nu1 + alpha0^2 mu2
+//   iteration 2  p' = beta0 e0 - alpha0 e1 + e3, w' = beta0 e1 - alpha0 e2 + e4,
+//                r' = -(alpha0 + alpha1 beta0) e1 + alpha1 alpha0 e2 + e3 - alpha1 e4,
+//                x' = (alpha0 + alpha1 beta0) e0 - alpha1 alpha0 e1 + alpha1 e3
 template <class St>
 __device__ __forceinline__ void ss_iterate(const double* v, const St* s, int k0, double maxrr2, SsPlan& pl) {
   const SsGram g{v};
-  double pp[5], rr[5], xx[5];
-#pragma unroll
-  for (int i = 0; i < 5; i++) {
-    pp[i] = i == 0 ? 1.0 : 0.0;
-    rr[i] = i == 3 ? 1.0 : 0.0;
-    xx[i] = 0.0;
-  }
-  double dl = v[9];
   const double tol = s->eps * s->bsqrt;
   const double dd = s->delta * s->delta;
+  auto stops = [&](double dn, int k) {
+    return dn < dd * maxrr2 || dn < tol * tol || dn == 0.0 || k == s->max_iter - 1;
+  };
+  auto stop_with = [&](const double (&cp)[5], const double (&cr)[5], const double (&cx)[5], int k, double dl) {
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      pl.cp[i] = cp[i];
+      pl.cr[i] = cr[i];
+      pl.cx[i] = cx[i];
+    }
+    pl.stencil = false;
+    pl.done = 2;
+    pl.pending = 0;
+    pl.k = k;
+    pl.rsq = dl;
+    pl.maxrr = maxrr2;
+  };
   pl.stencil = true;
   pl.done = 0;
   pl.pending = 1;
-  pl.steps = 0;
-#pragma unroll
-  for (int j = 0; j < 2; j++) {
-    const double w[5] = {0.0, pp[0], pp[1], 0.0, pp[3]};
-    const double pw = ss_form(g, pp, w);
-    const double a = pw != 0.0 ? dl / pw : 0.0;
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-      xx[i] += a * pp[i];
-      rr[i] -= a * w[i];
-    }
-    double dn = ss_form(g, rr, rr);
-    if (!(dn > 0.0)) dn = 0.0;
-    const int k = k0 + j;
-    pl.hist[j] = dn;
-    pl.steps = j + 1;
-    if (dn > maxrr2) maxrr2 = dn;
-    if (dn < dd * maxrr2 || dn < tol * tol || dn == 0.0 || k == s->max_iter - 1) {
-      // stop after iteration k: x and this iteration's direction are formed,
-      // the reliable update continues from them
-#pragma unroll
-      for (int i = 0; i < 5; i++) {
-        pl.cp[i] = pp[i];
-        pl.cr[i] = rr[i];
-        pl.cx[i] = xx[i];
-      }
-      pl.stencil = false;
-      pl.done = 2;
-      pl.pending = 0;
-      pl.k = k;
-      pl.rsq = dl;
-      pl.maxrr = maxrr2;
-      return;
-    }
-    const double beta = dn / dl;
-#pragma unroll
-    for (int i = 0; i < 5; i++) pp[i] = rr[i] + beta * pp[i];
-    dl = dn;
+  // iteration k0
+  const double dl0 = g.at(3, 3);
+  const double pw0 = g.at(0, 1);
+  const double a0 = pw0 != 0.0 ? dl0 / pw0 : 0.0;
+  double dn1 = dl0 - 2.0 * a0 * g.at(1, 3) + a0 * a0 * g.at(1, 1);
+  if (!(dn1 > 0.0)) dn1 = 0.0;
+  pl.hist[0] = dn1;
+  pl.steps = 1;
+  if (dn1 > maxrr2) maxrr2 = dn1;
+  if (stops(dn1, k0)) {
+    const double cp[5] = {1.0, 0.0, 0.0, 0.0, 0.0}, cr[5] = {0.0, -a0, 0.0, 1.0, 0.0}, cx[5] = {a0, 0.0, 0.0, 0.0, 0.0};
+    stop_with(cp, cr, cx, k0, dl0);
+    return;
   }
+  // iteration k0 + 1
+  const double b0 = dn1 / dl0;
+  const double P[3] = {b0, -a0, 1.0};  // p' on e0, e1, e3
+  const double W[3] = {b0, -a0, 1.0};  // w' on e1, e2, e4
+  constexpr int PI[3] = {0, 1, 3}, WI[3] = {1, 2, 4};
+  double pw1 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; j++) t += g.at(PI[i], WI[j]) * W[j];
+    pw1 += P[i] * t;
+  }
+  const double a1 = pw1 != 0.0 ? dn1 / pw1 : 0.0;
+  const double r1 = -a0 - a1 * b0, r2 = a1 * a0, r4 = -a1;  // r' on e1, e2, e3 (= 1), e4
+  const double R[4] = {r1, r2, 1.0, r4};
+  constexpr int RI[4] = {1, 2, 3, 4};
+  double dn2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) t += g.at(RI[i], RI[j]) * R[j];
+    dn2 += R[i] * t;
+  }
+  if (!(dn2 > 0.0)) dn2 = 0.0;
+  pl.hist[1] = dn2;
+  pl.steps = 2;
+  if (dn2 > maxrr2) maxrr2 = dn2;
+  const double cr[5] = {0.0, r1, r2, 1.0, r4};
+  const double cx[5] = {a0 + a1 * b0, -a1 * a0, 0.0, a1, 0.0};
+  const double p1[5] = {b0, -a0, 0.0, 1.0, 0.0};
+  if (stops(dn2, k0 + 1)) {
+    stop_with(p1, cr, cx, k0 + 1, dn1);
+    return;
+  }
+  const double b1 = dn2 / dn1;
 #pragma unroll
   for (int i = 0; i < 5; i++) {
-    pl.cp[i] = pp[i];
-    pl.cr[i] = rr[i];
-    pl.cx[i] = xx[i];
+    pl.cp[i] = cr[i] + b1 * p1[i];
+    pl.cr[i] = cr[i];
+    pl.cx[i] = cx[i];
   }
   pl.k = k0 + 2;
-  pl.rsq = dl;
+  pl.rsq = dn2;
   pl.maxrr = maxrr2;
 }
 
