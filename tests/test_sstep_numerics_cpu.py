@@ -162,3 +162,56 @@ def test_sstep_max_iter_and_restart(orc):
     x, k, rel = sstep_cg(a64, a32, b, 1e-10, 0.6, 5000)
     assert rel > 10
     assert np.linalg.norm(b - a64(x)) / np.linalg.norm(b) <= 1e-10
+
+
+def _two_iterations_generic(M, dd, tol2, maxrr, max_iter, k0):
+    B = np.zeros((5, 5))
+    B[1, 0] = B[2, 1] = B[4, 3] = 1.0
+    pp, rr, xx, dl = np.eye(5)[0], np.eye(5)[3], np.zeros(5), M[3, 3]
+    for j in range(2):
+        w = B @ pp
+        pw = pp @ M @ w
+        a = dl / pw if pw != 0 else 0.0
+        xx, rr = xx + a * pp, rr - a * w
+        dn = max(rr @ M @ rr, 0.0)
+        maxrr = max(maxrr, dn)
+        if dn < dd * maxrr or dn < tol2 or dn == 0 or k0 + j == max_iter - 1:
+            return "stop", j, pp, rr, xx, dl
+        pp, dl = rr + (dn / dl) * pp, dn
+    return "go", 2, pp, rr, xx, dl
+
+
+def _two_iterations_written_out(M, dd, tol2, maxrr, max_iter, k0):
+    """The device prelude's form: the coordinates' zeros written out."""
+    a0 = M[3, 3] / M[0, 1] if M[0, 1] != 0 else 0.0
+    dn1 = max(M[3, 3] - 2 * a0 * M[1, 3] + a0 * a0 * M[1, 1], 0.0)
+    maxrr = max(maxrr, dn1)
+    if dn1 < dd * maxrr or dn1 < tol2 or dn1 == 0 or k0 == max_iter - 1:
+        return "stop", 0, np.eye(5)[0], np.array([0, -a0, 0, 1.0, 0]), np.array([a0, 0, 0, 0, 0.0]), M[3, 3]
+    b0 = dn1 / M[3, 3]
+    P, W = {0: b0, 1: -a0, 3: 1.0}, {1: b0, 2: -a0, 4: 1.0}
+    pw1 = sum(P[i] * W[j] * M[i, j] for i in P for j in W)
+    a1 = dn1 / pw1 if pw1 != 0 else 0.0
+    R = np.array([0, -a0 - a1 * b0, a1 * a0, 1.0, -a1])
+    X = np.array([a0 + a1 * b0, -a1 * a0, 0, a1, 0])
+    dn2 = max(R @ M @ R, 0.0)
+    maxrr = max(maxrr, dn2)
+    p1 = np.array([b0, -a0, 0, 1.0, 0])
+    if dn2 < dd * maxrr or dn2 < tol2 or dn2 == 0 or k0 + 1 == max_iter - 1:
+        return "stop", 1, p1, R, X, dn1
+    return "go", 2, R + (dn2 / dn1) * p1, R, X, dn2
+
+
+def test_written_out_iterations_match_generic():
+    """The device prelude writes the two coordinate iterations out with the
+    known zeros of p', r', w'; on random Gram matrices (and every stop
+    position) it equals the generic 5 x 5 form."""
+    rng = np.random.default_rng(1)
+    for t in range(500):
+        Y = rng.normal(size=(5, 40))
+        args = (Y @ Y.T, 0.01 * rng.random(), 0.0, 0.0, 5 if t % 3 else 2, 0 if t % 5 else 1)
+        g, h = _two_iterations_generic(*args), _two_iterations_written_out(*args)
+        assert g[:2] == h[:2]
+        for a, b in zip(g[2:5], h[2:5]):
+            assert np.allclose(a, b, rtol=1e-9, atol=1e-12)
+        assert abs(g[5] - h[5]) <= 1e-9 * abs(g[5])
