@@ -336,6 +336,17 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
   const int rows = min(g.sr, ly - y0);  // a multiple of 4
   const int nsteps = rows / 4 + 2;
   const int xm = x0 + CPL * l;
+  // Odd bands march upwards: then the two strips either side of a band
+  // boundary stage their shared rows (one's own rows, the other's halo) at
+  // the same time, at the start or at the end of both marches, and the
+  // second read hits L2.  March row yr is lattice row y0 + rrow(yr); the
+  // rings are kept in march rows, so a stage's lattice neighbours y +- 1 are
+  // the ring slots slp / slm.
+  const bool up = ((tt / g.ntx) & 1) != 0;
+  auto rrow = [&](int yr) { return up ? rows - 1 - yr : yr; };
+  auto slm = [&](int yr) { return up ? (yr + 9) & 7 : (yr + 7) & 7; };  // row y - 1
+  auto slp = [&](int yr) { return up ? (yr + 7) & 7 : (yr + 9) & 7; };  // row y + 1
+  auto par = [&](int yr) { return rrow(yr) & 1; };                      // EO: parity (y0 even)
   const int so = ko & 1, sn = so ^ 1;
   const c64* Y[5] = {bf.v(so, 0), bf.v(so, 1), bf.v(so, 2), bf.v(so, 3), bf.v(so, 4)};
   c64* N[5] = {bf.v(sn, 0), bf.v(sn, 1), bf.v(sn, 2), bf.v(sn, 3), bf.v(sn, 4)};
@@ -390,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
   typename LK::Raw lh;
   auto fetch = [&](int k) {
     const int yr = 4 * k - 4 + w;
-    const int j = wrap_coord(y0 + yr, ly) * lx + xm;
+    const int j = wrap_coord(y0 + rrow(yr), ly) * lx + xm;
     const bool own = yr >= 0 && yr < rows;
     ldc(Y[0], j, ry[0]);
     ldc(Y[3], j, ry[3]);
@@ -400,10 +411,10 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
       ldc(Y[4], j, ry[4]);
       if (own) ldc(bf.x, j, rx);
     }
-    if constexpr (EO) lm.fetch(op.links, wrap_coord(y0 + yr, ly) * lxf + 2 * xm);
+    if constexpr (EO) lm.fetch(op.links, wrap_coord(y0 + rrow(yr), ly) * lxf + 2 * xm);
     else lm.fetch(op.links, j);
     if (hp) {
-      const int jh = wrap_coord(y0 + 4 * k - 4 + hp_row, ly) * lx + xhp;
+      const int jh = wrap_coord(y0 + rrow(4 * k - 4 + hp_row), ly) * lx + xhp;
       hy[0] = Y[0][jh];
       hy[3] = Y[3][jh];
       if (full) {
@@ -412,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
         hy[4] = Y[4][jh];
       }
     }
-    if (hl) lh = op.links.fetch(wrap_coord(y0 + 4 * k - 4 + hl_row, ly) * lxf + xhl);
+    if (hl) lh = op.links.fetch(wrap_coord(y0 + rrow(4 * k - 4 + hl_row), ly) * lxf + xhl);
   };
   if (active) fetch(0);
 
@@ -488,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
       {
         // S0: rows 4k-4 .. 4k-1
         const int yr = 4 * k - 4 + w;
-        const int j = wrap_coord(y0 + yr, ly) * lx + xm;
+        const int j = wrap_coord(y0 + rrow(yr), ly) * lx + xm;
         const bool own = yr >= 0 && yr < rows;
         const int sl = (yr + 8) & 7;
         c64 pv2[CPL], rv2[CPL], xv2[CPL];
@@ -554,10 +565,10 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
         // S1: qp = K p, qr = K r on rows 4k-5 .. 4k-2 (needed: -3 .. rows+2)
         const int y = 4 * k - 5 + w;
         if (y >= -3 && y <= rows + 2) {
-          const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
+          const int s0 = (y + 8) & 7, sm1 = slm(y), sp1 = slp(y);
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
-            if constexpr (EO) ss_q1hop_eo<T, C>(sP, sR, sLx, sLy, sQp, sQr, s0, sm1, sp1, CPL * l + c + 3, y & 1);
+            if constexpr (EO) ss_q1hop_eo<T, C>(sP, sR, sLx, sLy, sQp, sQr, s0, sm1, sp1, CPL * l + c + 3, par(y));
             else ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, s0, sm1, sp1, CPL * l + c + 3);
           }
         }
@@ -566,8 +577,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
           if (yh >= -3 && yh <= rows + 2) {
             const int qc = hc < 3 ? hc : NXB + hc;
             if constexpr (EO)
-              ss_q1hop_eo<T, C>(sP, sR, sLx, sLy, sQp, sQr, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, qc, yh & 1);
-            else ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, qc);
+              ss_q1hop_eo<T, C>(sP, sR, sLx, sLy, sQp, sQr, (yh + 8) & 7, slm(yh), slp(yh), qc, par(yh));
+            else ss_q1hop<T, C>(sP, sR, sLx, sLy, sQp, sQr, (yh + 8) & 7, slm(yh), slp(yh), qc);
           }
         }
       }
@@ -576,16 +587,16 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
         // S2: P1 = m^2 p - K qp on rows -2 .. rows+1
         const int y = 4 * k - 6 + w;
         if (y >= -2 && y <= rows + 1) {
-          const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
+          const int s0 = (y + 8) & 7, sm1 = slm(y), sp1 = slp(y);
           T v1[CPL];
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
             const int c1 = CPL * l + c + 2;
-            if constexpr (EO) v1[c] = ss_p1hop_eo<T, C>(sP, sLx, sLy, sQp, s0, sm1, sp1, c1, y & 1, m2);
+            if constexpr (EO) v1[c] = ss_p1hop_eo<T, C>(sP, sLx, sLy, sQp, s0, sm1, sp1, c1, par(y), m2);
             else v1[c] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, s0, sm1, sp1, c1, m2);
             sP1[s0][c1] = v1[c];
           }
-          if (y >= 0 && y < rows) stc(N[1], (y0 + y) * lx + xm, v1);
+          if (y >= 0 && y < rows) stc(N[1], (y0 + rrow(y)) * lx + xm, v1);
         }
         if (w == 1 && l < 16) {
           const int yh = 4 * k - 6 + (l >> 2), hc = l & 3;
@@ -593,8 +604,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
             const int c1 = hc < 2 ? hc : NXB + hc;
             const int h0 = (yh + 8) & 7;
             if constexpr (EO)
-              sP1[h0][c1] = ss_p1hop_eo<T, C>(sP, sLx, sLy, sQp, h0, (yh + 7) & 7, (yh + 9) & 7, c1, yh & 1, m2);
-            else sP1[h0][c1] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, h0, (yh + 7) & 7, (yh + 9) & 7, c1, m2);
+              sP1[h0][c1] = ss_p1hop_eo<T, C>(sP, sLx, sLy, sQp, h0, slm(yh), slp(yh), c1, par(yh), m2);
+            else sP1[h0][c1] = ss_p1hop<T, C>(sP, sLx, sLy, sQp, h0, slm(yh), slp(yh), c1, m2);
           }
         }
       }
@@ -603,10 +614,10 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
         // S3: q1 = K P1 on rows -1 .. rows
         const int y = 4 * k - 7 + w;
         if (y >= -1 && y <= rows) {
-          const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
+          const int s0 = (y + 8) & 7, sm1 = slm(y), sp1 = slp(y);
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
-            if constexpr (EO) ss_q2hop_eo<T, C>(sP1, sLx, sLy, sQ1, s0, sm1, sp1, CPL * l + c + 1, y & 1);
+            if constexpr (EO) ss_q2hop_eo<T, C>(sP1, sLx, sLy, sQ1, s0, sm1, sp1, CPL * l + c + 1, par(y));
             else ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, s0, sm1, sp1, CPL * l + c + 1);
           }
         }
@@ -615,8 +626,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
           if (yh >= -1 && yh <= rows) {
             const int c2 = (l & 1) == 0 ? 0 : NXB + 1;
             if constexpr (EO)
-              ss_q2hop_eo<T, C>(sP1, sLx, sLy, sQ1, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, c2, yh & 1);
-            else ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, (yh + 8) & 7, (yh + 7) & 7, (yh + 9) & 7, c2);
+              ss_q2hop_eo<T, C>(sP1, sLx, sLy, sQ1, (yh + 8) & 7, slm(yh), slp(yh), c2, par(yh));
+            else ss_q2hop<T, C>(sP1, sLx, sLy, sQ1, (yh + 8) & 7, slm(yh), slp(yh), c2);
           }
         }
       }
@@ -626,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
         // ring still holds rows y-1 .. y+1), and the sums
         const int y = 4 * k - 8 + w;
         if (y >= 0 && y < rows) {
-          const int s0 = (y + 8) & 7, sm1 = (y + 7) & 7, sp1 = (y + 9) & 7;
+          const int s0 = (y + 8) & 7, sm1 = slm(y), sp1 = slp(y);
           T v2[CPL], v4[CPL];
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
@@ -636,13 +647,15 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
             // column b + 4; EO (even site, x = 2 (x0 + b) + e): q1 columns
             // b + e, b + e + 1 (y: b + 1), qr b + e + 2, b + e + 3 (y: b + 3),
             // link column 2b + e + 7
-            const int e = EO ? (y & 1) : 0;
+            const int e = EO ? par(y) : 0;
             const int q1m = EO ? b + e : b, q1p = EO ? b + e + 1 : b + 2;
             const int qrm = EO ? b + e + 2 : b + 2, qrp = EO ? b + e + 3 : b + 4;
             const int lc = EO ? 2 * b + e + 7 : b + 4;
             const bool eta_neg = EO ? e != 0 : (b & 1) != 0;
             const C ux = sLx[s0][lc], uxm = sLx[s0][lc - 1], uy = sLy[s0][lc];
-            const C uym = w == 0 ? sKeep[b] : sLy[sm1][lc];
+            // row y - 1 of wave 0's row in a downward march shares a slot with
+            // the newest staged row (saved in sKeep); upwards it is newer
+            const C uym = (w == 0 && !up) ? sKeep[b] : sLy[sm1][lc];
             const T kq = khop_h<T, C>(sQ1[s0][q1p], sQ1[s0][q1m], sQ1[sp1][b + 1], sQ1[sm1][b + 1], ux, uxm, uy, uym,
                                       eta_neg);
             const T kr = khop_h<T, C>(sQr[s0][qrp], sQr[s0][qrm], sQr[sp1][b + 3], sQr[sm1][b + 3], ux, uxm, uy, uym,
@@ -666,7 +679,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
             part[10] = ss_acc(part[10], rr, r1);
             part[11] = ss_acc(part[11], r1, r1);
           }
-          const int j = (y0 + y) * lx + xm;
+          const int j = (y0 + rrow(y)) * lx + xm;
           stc(N[2], j, v2);
           stc(N[4], j, v4);
         }
