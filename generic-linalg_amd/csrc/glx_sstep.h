@@ -365,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
     else p[j] = v[0];
   };
 
-  // the previous launch's sums: issued first (loads complete in order)
+  // the previous launch's sums
   constexpr int kPreU = 2;
   double pv[kPreU][kSsNR];
   const double* PS = ws.partials + (q ^ 1) * kSsNR * nblk;
@@ -425,9 +425,12 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
     }
     if (hl) lh = op.links.fetch(wrap_coord(y0 + rrow(4 * k - 4 + hl_row), ly) * lxf + xhl);
   };
-  if (active) fetch(0);
 
-  // prelude: the coefficients of this launch
+  // prelude: the coefficients of this launch.  The first rows' loads are
+  // issued after the block sum of the partials: the wait for the partials is
+  // a wait for every load issued before it (the loads sit under branches, so
+  // the count of younger loads is not known), and the coefficient algebra
+  // then runs while the first rows are in flight.
   __shared__ double sm[kSsNR][kWaves];
   SsPlan pl;
   if (full) {
@@ -444,8 +447,10 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
       for (int k = 0; k < kSsNR; k++) acc[k] += PS[k * nblk + b0];
     }
     block_sum_dpp<kSsNR>(acc, sm);
+    if (active) fetch(0);
     ss_iterate(acc, rd, k0, rd->maxrr, pl);
   } else {
+    if (active) fetch(0);
     // restart from P0 (the last direction) and R0 (the true residual)
 #pragma unroll
     for (int i = 0; i < 5; i++) {
