@@ -1336,8 +1336,9 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
 
 int glx_set_normal_rows(int lx, int ly, int esize, int rows) {
   return guarded([&] {
-    if (rows < 0 || (rows > 0 && (rows < 8 || rows % 4 != 0))) throw ArgError("rows must be 0 or a multiple of 4 >= 8");
     if (lx < 0 || ly < 0 || (lx == 0) != (ly == 0)) throw ArgError("bad lattice size");
+    if (!(rows == -1 && lx == 0) && (rows < 0 || (rows > 0 && (rows < 8 || rows % 4 != 0))))
+      throw ArgError("rows must be 0 or a multiple of 4 >= 8 (-1 for the default: by block count)");
     if (lx > 0 && esize != 8 && esize != 16) throw ArgError("esize must be 8 (complex FP32) or 16 (complex FP64)");
     set_normal_rows(lx, ly, esize, rows);
   });

@@ -135,9 +135,19 @@ constexpr bool kind_valid_for(int k) {
 // Launch shape of the row-marching normal-operator kernel: rows per block
 // for a lattice size and vector element size in bytes (set by the autotuner
 // or by hand; 0 = use the tiled kernel), with a default for sizes not in the
-// table.
-int normal_rows_for(int lx, int ly, int esize);
+// table: a fixed value once set, else (-1) the fewest of 16/32/64/128 rows
+// that keep the grid within 512 blocks of 64 lanes x (1 or 2) sites, else
+// 128.  `cols` is the vectors' row length in sites when it differs from lx
+// (the even/odd Schur operator's compact rows).
+int normal_rows_for(int lx, int ly, int esize, int cols = 0);
 void set_normal_rows(int lx, int ly, int esize, int rows);  // lx = 0: set the default
+
+// Rows per block of the even/odd Schur kernel (compact rows of lx / 2 sites;
+// "tiled" does not exist for this operator).
+inline int eo_rows_for(int lx, int ly, int esize) {
+  const int sr = normal_rows_for(lx, ly, esize, lx / 2);
+  return sr > 0 ? sr : 16;
+}
 
 // Whether a launch goes to a row-marching kernel: D^dag D (glx_normal.h;
 // otherwise the tiled kernel) or the compact even/odd Schur operator
@@ -149,17 +159,12 @@ bool normal_path(const DevOperator& o, const In& in, const T* out) {
     return false;
   } else {
     if (o.kind != OP_STAG_NORMAL && o.kind != OP_STAG_EO_SCHUR) return false;
-    int sr = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
-    if (o.kind == OP_STAG_EO_SCHUR) {
-      if (sr <= 0) sr = 16;  // "tiled" does not exist for this operator
-      return eo_fits(o.lx, o.ly, sr, normal_cpl<T>()) && normal_pairs_aligned<T>(in, out);
-    }
+    if (o.kind == OP_STAG_EO_SCHUR)
+      return eo_fits(o.lx, o.ly, eo_rows_for(o.lx, o.ly, int(sizeof(T))), normal_cpl<T>()) &&
+             normal_pairs_aligned<T>(in, out);
+    const int sr = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
     return sr > 0 && normal_fits(o.lx, o.ly, sr, normal_cpl<T>()) && normal_pairs_aligned<T>(in, out);
   }
-}
-inline int eo_rows_for(int lx, int ly, int esize) {
-  const int sr = normal_rows_for(lx, ly, esize);
-  return sr > 0 ? sr : 16;
 }
 
 // Launch the tiled stencil kernel for a runtime operator kind.

@@ -28,7 +28,7 @@ namespace glx {
 namespace {
 struct NormalTable {
   std::mutex mu;
-  int def = 16;
+  int def = -1;  // -1: by block count
   std::map<std::tuple<int, int, int>, int> rows;
 };
 NormalTable& normal_table() {
@@ -37,11 +37,22 @@ NormalTable& normal_table() {
 }
 }  // namespace
 
-int normal_rows_for(int lx, int ly, int esize) {
-  NormalTable& t = normal_table();
-  std::lock_guard<std::mutex> g(t.mu);
-  auto it = t.rows.find(std::make_tuple(lx, ly, esize));
-  return it == t.rows.end() ? t.def : it->second;
+int normal_rows_for(int lx, int ly, int esize, int cols) {
+  {
+    NormalTable& t = normal_table();
+    std::lock_guard<std::mutex> g(t.mu);
+    auto it = t.rows.find(std::make_tuple(lx, ly, esize));
+    if (it != t.rows.end()) return it->second;
+    if (t.def >= 0 || lx <= 0) return t.def;
+  }
+  // By block count: the self-finalising one-iteration kernels reduce every
+  // block's partial sums in every block, a cost that grows with the square of
+  // the block count (at 4096^2 FP64, 16 rows per block take 2x the time of
+  // 128); two blocks per CU fill the chip.
+  const int strips = std::max(1, (cols > 0 ? cols : lx) / (64 * (esize == 16 ? 1 : 2)));
+  for (int r : {16, 32, 64})
+    if (strips * ((ly + r - 1) / r) <= 512) return r;
+  return 128;
 }
 void set_normal_rows(int lx, int ly, int esize, int rows) {
   NormalTable& t = normal_table();

@@ -610,7 +610,8 @@ def get_sstep():
 def set_normal_rows(lx: int, ly: int, esize: int, rows: int):
     """Rows per block of the row-marching D^dag D kernel for an lx x ly
     lattice and element size (16 complex FP64, 8 complex FP32); 0 = tiled
-    kernel; lx = ly = 0 sets the default."""
+    kernel; lx = ly = 0 sets the default (-1, the initial one: the fewest of
+    16/32/64/128 rows that keep the grid within 512 blocks, else 128)."""
     _check(lib().glx_set_normal_rows(lx, ly, esize, rows), "glx_set_normal_rows")
 
 

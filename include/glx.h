@@ -434,7 +434,12 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
  * counterpart; operators.cpp:444-453 is the operator it computes): rows per
  * block of the row-marching kernel for an lx x ly lattice and vector element
  * size esize (16 = complex FP64, 8 = complex FP32); 0 selects the tiled
- * kernel.  lx = ly = 0 sets the default for sizes not in the table (16).
+ * kernel.  lx = ly = 0 sets the default for sizes not in the table; the
+ * initial default, -1, takes the fewest of 16/32/64/128 rows that keep the
+ * grid within 512 blocks (64 lanes of 1 FP64 or 2 FP32 sites per block row;
+ * compact rows of lx / 2 sites for the even/odd Schur operator), else 128.
+ * glx_get_normal_rows returns what a solve of that size would use (for
+ * lx = ly = 0, the default itself).
  * The row-marching kernel needs lx % 64 == 0 (FP64) or lx % 128 == 0 (FP32,
  * two sites per lane), ly % 4 == 0, ly >= 8; other sizes always use the
  * tiled kernel. */

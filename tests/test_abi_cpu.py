@@ -121,6 +121,30 @@ def test_python_binding_imports():
     assert set(glx.OP_KINDS) >= {"staggered", "staggered_normal", "laplace_real", "m2mdeodoe"}
 
 
+def test_normal_rows_default_without_device():
+    """Rows per block of the row-marching D^dag D kernels for sizes not in the
+    table: the fewest of 16/32/64/128 that keep the grid within 512 blocks
+    (64 lanes of 1 FP64 or 2 FP32 sites), else 128 -- the shapes the kernel
+    timings favour at 1024^2, 2048^2 and 4096^2; an explicit default
+    overrides it and -1 restores it (no device work)."""
+    lib = C.CDLL(LIB)
+    lib.glx_last_error.restype = C.c_char_p
+    r = C.c_int()
+    assert lib.glx_get_normal_rows(0, 0, 16, C.byref(r)) == 0 and r.value == -1
+    want = {(256, 256, 16): 16, (512, 512, 16): 16, (1024, 1024, 16): 32, (1024, 1024, 8): 16,
+            (2048, 2048, 16): 128, (2048, 2048, 8): 64, (4096, 4096, 16): 128, (4096, 4096, 8): 128,
+            (1024, 256, 16): 16, (256, 4096, 16): 32}
+    for (lx, ly, es), rows in want.items():
+        # 512 blocks: lx / (64 cpl) strips x ceil(ly / rows) bands
+        assert (lx // (64 * (1 if es == 16 else 2))) * -(-ly // rows) <= 512 or rows == 128
+        assert lib.glx_get_normal_rows(lx, ly, es, C.byref(r)) == 0 and r.value == rows, (lx, ly, es, r.value)
+    assert lib.glx_set_normal_rows(0, 0, 0, 16) == 0
+    assert lib.glx_get_normal_rows(2048, 2048, 16, C.byref(r)) == 0 and r.value == 16
+    assert lib.glx_set_normal_rows(2048, 2048, 16, -1) == 1 and b"rows must be" in lib.glx_last_error()
+    assert lib.glx_set_normal_rows(0, 0, 0, -1) == 0
+    assert lib.glx_get_normal_rows(2048, 2048, 16, C.byref(r)) == 0 and r.value == 128
+
+
 def test_sstep_setting_without_device():
     """glx_set_sstep validates its arguments and glx_get_sstep reads back the
     process default (no device work)."""
