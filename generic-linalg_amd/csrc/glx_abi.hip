@@ -1362,7 +1362,7 @@ int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, dou
     std::vector<c128> ones(n, c128{1.0, 0.0});
     DevBuf b(size_t(n) * sizeof(c128)), x(size_t(n) * sizeof(c128));
     GLX_HIP(hipMemcpy(b.get(), ones.data(), size_t(n) * sizeof(c128), hipMemcpyHostToDevice));
-    const int cands[] = {0, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+    const int cands[] = {0, 8, 12, 16, 24, 32, 48, 64, 96, 128, 256};  // 256: ly >= 2048 only
     SolveOpts so;
     so.max_iter = reps;
     so.eps = 0.0;
@@ -1386,6 +1386,7 @@ int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, dou
       int best = 0;
       double best_ms = 1e300;
       for (int c : cands) {
+        if (c == 256 && o.ly < 2048) continue;
         if (c > 0 && !normal_fits(o.lx, o.ly, c, fp32 ? normal_cpl<c64>() : normal_cpl<c128>())) continue;
         set_normal_rows(o.lx, o.ly, esize, c);
         time_one(fp32);  // warm-up
@@ -1432,7 +1433,8 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
     int best = 0;
     double best_ms = 1e300;
     try {
-      for (int c : {0, 8, 16, 32, 64, 128}) {
+      for (int c : {0, 8, 16, 32, 64, 128, 256}) {
+        if (c == 256 && o.ly < 2048) break;
         if (c > 0 && (o.lx % 128 != 0 || o.ly < 8 || o.ly % 4 != 0)) break;
         set_sstep_rows(o.lx, o.ly, c);
         auto time_one = [&]() {

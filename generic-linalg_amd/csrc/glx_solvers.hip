@@ -48,11 +48,11 @@ int normal_rows_for(int lx, int ly, int esize, int cols) {
   // By block count: the self-finalising one-iteration kernels reduce every
   // block's partial sums in every block, a cost that grows with the square of
   // the block count (at 4096^2 FP64, 16 rows per block take 2x the time of
-  // 128); two blocks per CU fill the chip.
+  // 128; FP32 256 rows 8% less than 128); two blocks per CU fill the chip.
   const int strips = std::max(1, (cols > 0 ? cols : lx) / (64 * (esize == 16 ? 1 : 2)));
-  for (int r : {16, 32, 64})
+  for (int r : {16, 32, 64, 128})
     if (strips * ((ly + r - 1) / r) <= 512) return r;
-  return 128;
+  return 256;
 }
 void set_normal_rows(int lx, int ly, int esize, int rows) {
   NormalTable& t = normal_table();
@@ -1940,14 +1940,15 @@ void set_sstep_rows(int lx, int ly, int rows) {
   else t.rows[std::make_pair(lx, ly)] = rows;
 }
 // Without a table entry: the fewest rows per block (8, 16, 32, 64, 128) that
-// keep the grid within 512 blocks, two per CU, else 128 (measured: 256^2 and
-// 512^2 best at 8, 768^2 and 1024^2 at 16, 2048^2 at 64, 4096^2 at 128; the
-// strips are 128 lattice columns wide for both operators).
+// keep the grid within 512 blocks, two per CU, else 256 (measured: 256^2 and
+// 512^2 best at 8, 768^2 and 1024^2 at 16, 2048^2 at 64, 4096^2 at 256
+// (3% less per launch than 128); the strips are 128 lattice columns wide for
+// both operators).
 int sstep_rows_auto(int lx, int ly) {
   const long strips = lx / (kNX * 2);
-  for (int r = 8; r <= 64; r *= 2)
+  for (int r = 8; r <= 128; r *= 2)
     if (strips * ((ly + r - 1) / r) <= 512) return r;
-  return 128;
+  return 256;
 }
 int sstep_rows_for(int lx, int ly) {
   const int forced = g_sstep_rows.load();

@@ -595,7 +595,8 @@ def get_sstep_rows(lx: int, ly: int) -> int:
 
 def autotune_sstep(lat, reps: int = 40) -> dict:
     """Time the mixed CG on this lattice in one-iteration form and in s-step
-    form at 8 / 16 / 32 / 64 / 128 rows per block; install and return the fastest."""
+    form at 8 / 16 / 32 / 64 / 128 rows per block (and 256 where ly >= 2048);
+    install and return the fastest."""
     r, ms = C.c_int(0), C.c_double(0.0)
     _check(lib().glx_autotune_sstep(lat.h, reps, C.byref(r), C.byref(ms)), "glx_autotune_sstep")
     return {"rows": r.value, "ms_per_iteration": ms.value}
@@ -611,7 +612,7 @@ def set_normal_rows(lx: int, ly: int, esize: int, rows: int):
     """Rows per block of the row-marching D^dag D kernel for an lx x ly
     lattice and element size (16 complex FP64, 8 complex FP32); 0 = tiled
     kernel; lx = ly = 0 sets the default (-1, the initial one: the fewest of
-    16/32/64/128 rows that keep the grid within 512 blocks, else 128)."""
+    16/32/64/128 rows that keep the grid within 512 blocks, else 256)."""
     _check(lib().glx_set_normal_rows(lx, ly, esize, rows), "glx_set_normal_rows")
 
 

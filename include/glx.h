@@ -415,8 +415,8 @@ int glx_set_tuning(int group, int use_graph);
  * coordinates) where lx % 128 == 0, ly % 4 == 0 and ly >= 8; s = 1 one
  * iteration per launch.  rows: rows per block of the s-step kernel for every
  * lattice size (a multiple of 4, >= 8), or 0 (default): per size, from the
- * table below, else the fewest of 8, 16, 32, 64 that keep the launch within
- * 512 blocks of 128 lattice columns (128 beyond). */
+ * table below, else the fewest of 8, 16, 32, 64, 128 that keep the launch
+ * within 512 blocks of 128 lattice columns (256 beyond). */
 int glx_set_sstep(int s, int rows);
 int glx_get_sstep(int* s, int* rows);
 /* The per-size table: rows per block of the s-step kernel for an lx x ly
@@ -426,7 +426,8 @@ int glx_set_sstep_rows(int lx, int ly, int rows);
 int glx_get_sstep_rows(int lx, int ly, int* rows);
 /* Autotuner for the mixed-precision CG on D^dag D of the lattice: times
  * `reps` iterations in the one-iteration form and in s-step form at 8, 16,
- * 32, 64 and 128 rows per block, installs the fastest in the table above and
+ * 32, 64 and 128 rows per block (and 256 where ly >= 2048), installs the
+ * fastest in the table above and
  * returns it (0 = one-iteration form) with its time per iteration in ms.
  * Needs FP32 links.  No reference counterpart. */
 int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iteration);
@@ -437,7 +438,7 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
  * kernel.  lx = ly = 0 sets the default for sizes not in the table; the
  * initial default, -1, takes the fewest of 16/32/64/128 rows that keep the
  * grid within 512 blocks (64 lanes of 1 FP64 or 2 FP32 sites per block row;
- * compact rows of lx / 2 sites for the even/odd Schur operator), else 128.
+ * compact rows of lx / 2 sites for the even/odd Schur operator), else 256.
  * glx_get_normal_rows returns what a solve of that size would use (for
  * lx = ly = 0, the default itself).
  * The row-marching kernel needs lx % 64 == 0 (FP64) or lx % 128 == 0 (FP32,
@@ -445,7 +446,8 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
  * tiled kernel. */
 int glx_set_normal_rows(int lx, int ly, int esize, int rows);
 int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
-/* Autotuner: for every candidate launch shape (tiled; 8..128 rows per block)
+/* Autotuner: for every candidate launch shape (tiled; 8..128 rows per block,
+ * and 256 where ly >= 2048)
  * times `reps` iterations of the single-pass FP64 CG (glx_dev_cg_fused) and,
  * if the lattice has FP32 links, of the mixed-precision CG in its
  * one-iteration form (glx_set_sstep(1, ...)) on D^dag D of the lattice, records the fastest shape per precision in the table above and

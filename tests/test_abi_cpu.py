@@ -124,7 +124,7 @@ def test_python_binding_imports():
 def test_normal_rows_default_without_device():
     """Rows per block of the row-marching D^dag D kernels for sizes not in the
     table: the fewest of 16/32/64/128 that keep the grid within 512 blocks
-    (64 lanes of 1 FP64 or 2 FP32 sites), else 128 -- the shapes the kernel
+    (64 lanes of 1 FP64 or 2 FP32 sites), else 256 -- the shapes the kernel
     timings favour at 1024^2, 2048^2 and 4096^2; an explicit default
     overrides it and -1 restores it (no device work)."""
     lib = C.CDLL(LIB)
@@ -132,11 +132,11 @@ def test_normal_rows_default_without_device():
     r = C.c_int()
     assert lib.glx_get_normal_rows(0, 0, 16, C.byref(r)) == 0 and r.value == -1
     want = {(256, 256, 16): 16, (512, 512, 16): 16, (1024, 1024, 16): 32, (1024, 1024, 8): 16,
-            (2048, 2048, 16): 128, (2048, 2048, 8): 64, (4096, 4096, 16): 128, (4096, 4096, 8): 128,
+            (2048, 2048, 16): 128, (2048, 2048, 8): 64, (4096, 4096, 16): 256, (4096, 4096, 8): 256,
             (1024, 256, 16): 16, (256, 4096, 16): 32}
     for (lx, ly, es), rows in want.items():
         # 512 blocks: lx / (64 cpl) strips x ceil(ly / rows) bands
-        assert (lx // (64 * (1 if es == 16 else 2))) * -(-ly // rows) <= 512 or rows == 128
+        assert (lx // (64 * (1 if es == 16 else 2))) * -(-ly // rows) <= 512 or rows == 256
         assert lib.glx_get_normal_rows(lx, ly, es, C.byref(r)) == 0 and r.value == rows, (lx, ly, es, r.value)
     assert lib.glx_set_normal_rows(0, 0, 0, 16) == 0
     assert lib.glx_get_normal_rows(2048, 2048, 16, C.byref(r)) == 0 and r.value == 16
@@ -161,7 +161,7 @@ def test_sstep_setting_without_device():
     # per size: the automatic choice by block count, the table, removal
     r = C.c_int()
     for (lx, ly), want in {(256, 256): 8, (512, 512): 8, (768, 768): 16, (1024, 1024): 16, (2048, 2048): 64,
-                           (4096, 4096): 128}.items():
+                           (4096, 4096): 256}.items():
         assert lib.glx_get_sstep_rows(lx, ly, C.byref(r)) == 0 and r.value == want, (lx, r.value)
     assert lib.glx_set_sstep_rows(1024, 1024, 0) == 0
     assert lib.glx_get_sstep_rows(1024, 1024, C.byref(r)) == 0 and r.value == 0
