@@ -1,4 +1,4 @@
-// generic_gcr.h -- GCR and GCR(n) (generic_gcr.h:39-44 of the reference),
+// generic_gcr.h -- GCR and GCR(n) (generic_gcr.h:17-22 of the reference),
 // solved on the GPU by libglx.so.
 #ifndef GLX_COMPAT_GCR
 #define GLX_COMPAT_GCR

@@ -15,6 +15,7 @@
 
 #include "../../include/glx.h"
 #include "../compat/generic_inverters.h"
+#include "../compat/generic_inverters_precond.h"
 #include "../compat/operators.h"
 #include "../compat/u1_utils.h"
 #include "../compat/coarse_stencil.h"
@@ -299,6 +300,33 @@ inversion_info minv_unpreconditioned(complex<double>* lhs, complex<double>* rhs,
                                      minv_inverter_params& params, cmv mv, void* extra,
                                      inversion_verbose_struct* verb) {
   return dispatch(lhs, rhs, size, type, params, mv, extra, verb);
+}
+
+// minv_preconditioned (generic_inverters_precond.h:31-63): the interface is
+// kept so that drivers including it compile; the preconditioned solvers are
+// not part of this library.
+namespace {
+template <class T, class MV, class PMV>
+inversion_info precond_dispatch(minv_inverter_precond type, MV, PMV) {
+  if (type == MINV_PRE_INVALID) return inversion_info();
+  throw std::runtime_error("minv_preconditioned: preconditioned solver " + std::to_string(int(type)) +
+                           " is not part of this library (minv_unpreconditioned carries CG, GCR and BiCGStab)");
+}
+}  // namespace
+
+inversion_info minv_preconditioned(double*, double*, int, minv_inverter_precond type, minv_inverter_precond_params&,
+                                   void (*mv)(double*, double*, void*), void*,
+                                   void (*pmv)(double*, double*, int, void*, inversion_verbose_struct*), void*,
+                                   inversion_verbose_struct*) {
+  return precond_dispatch<double>(type, mv, pmv);
+}
+inversion_info minv_preconditioned(complex<double>*, complex<double>*, int, minv_inverter_precond type,
+                                   minv_inverter_precond_params&,
+                                   void (*mv)(complex<double>*, complex<double>*, void*), void*,
+                                   void (*pmv)(complex<double>*, complex<double>*, int, void*,
+                                               inversion_verbose_struct*),
+                                   void*, inversion_verbose_struct*) {
+  return precond_dispatch<complex<double>>(type, mv, pmv);
 }
 
 // ---------------------------------------------------- U(1) gauge utilities --
