@@ -12,11 +12,15 @@ The launch shape of the D^dag D kernel is autotuned for the lattice first
 A "step" is one complete mixed-precision solve from x = 0.  `value` is its
 time to solution in ms (lower is better).  The JSON line also carries the
 FP64 solves' times (the reference-order CG and the single-pass CG), the
-speed-ups, iteration counts, the mixed solution's relative error against the
-FP64 one, the roofline of the mixed solve's dominant kernel (its fused
-iteration stencil) measured live with HIP events, the plain D^dag D stencil
-at this size and at 4096^2 (beyond the 256 MB last-level cache), and the
-reference CPU path timed on the host on a bounded sample of the same solve.
+speed-ups, iteration counts, the solutions' relative errors against the FP64
+GPU solve and against the reference CPU solve of the same system, the
+roofline of the mixed solve's dominant kernel (the s-step launch) measured
+live with HIP events, the plain D and D^dag D stencils at this size, 2x this
+size and 4096^2 (beyond the 256 MB last-level cache) as fractions of the HBM
+peak and of the streaming ceiling measured in the same run, the multishift
+batch with its per-phase roofline, and the reference CPU path timed on the
+host in the same run (the complete headline solve on one core; the batch on
+all host cores).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 1024]
 """
@@ -44,41 +48,38 @@ def make_inputs(L: int, sigma: float, seed: int):
     return links.astype(np.complex128), b.astype(np.complex128)
 
 
-def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0, grad=None, batch=None, small=None):
-    """Reference CPU CG (oracle/_ref, built from the reference sources; the
-    oracle restatement where that build is absent), one thread, on a bounded
-    sample: a fixed number of iterations of the same 1024^2 solve, scaled to
-    the iteration count the FP64 solve needs.  Also, one thread each: the
-    512^2 gradient (reference CG solve + D chi, the oracle's force; the force
-    has no reference counterpart) and one multishift solve of the batched
-    workload, projected to the whole batch by site-iterations."""
+def cpu_baseline(links, b, L, mass, gpu_x, grad=None, batch=None, small=None):
+    """Reference CPU path (oracle/_ref, the reference's own sources built in
+    this tree; the oracle restatement where that build is absent), timed on
+    the host in the same run:
+      * the headline solve itself, complete (FP64 CG to 1e-10 on one core,
+        about 20 s): its time is `value`, and its solution gives the GPU
+        solutions' relative errors against the FP64 CPU reference;
+      * one core each: the 512^2 gradient (reference CG solve + D chi, the
+        oracle's force) and the small workloads;
+      * the whole multishift batch, its 192 independent CG-M solves spread
+        over the host's cores (one solve per thread, longest first; the
+        reference solvers are reentrant), each compared with its GPU
+        solution."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc  # noqa: E402  (CPU baseline leg only)
 
     impl = "ref" if orc.ref_available() else "orc"
-    # calibrate: time a few iterations, then size the sample to the budget
-    n_cal = 5
     t0 = time.perf_counter()
-    orc.solve("cg", "staggered_normal", b, None, links=links, mass=mass, lx=L, ly=L, max_iter=n_cal, eps=1e-10,
-              impl=impl)
-    t_cal = time.perf_counter() - t0
-    per_it = t_cal / (n_cal + 3)  # n iterations + 3 extra operator applications
-    n_it = max(5, min(gpu_fp64_iters, int(budget_s / per_it) - 3))
-    t0 = time.perf_counter()
-    _, info = orc.solve("cg", "staggered_normal", b, None, links=links, mass=mass, lx=L, ly=L, max_iter=n_it,
-                        eps=1e-10, impl=impl)
+    xc, info = orc.solve("cg", "staggered_normal", b, None, links=links, mass=mass, lx=L, ly=L, max_iter=100000,
+                         eps=1e-10, impl=impl)
     dt = time.perf_counter() - t0
-    per_it = dt / (info.iter + 3)
-    projected_ms = per_it * (gpu_fp64_iters + 3) * 1e3
+    nx = float(np.linalg.norm(xc))
     out = {
-        "value": round(projected_ms, 1),
-        "unit": "ms (FP64 CG time to 1e-10, projected)",
+        "value": round(dt * 1e3, 1),
+        "unit": "ms (FP64 CG time to 1e-10)",
         "cores": 1,
         "kind": "reference" if impl == "ref" else "port",
-        "sample": f"{info.iter} CG iterations of the same {L}x{L} solve on one host core "
-                  f"({dt:.2f} s, {per_it * 1e3:.1f} ms/iteration), scaled to the {gpu_fp64_iters} iterations "
-                  f"the FP64 solve takes",
+        "sample": f"the complete {L}x{L} FP64 CG solve ({info.iter} iterations, success {info.success}) on one host "
+                  f"core, {dt * 1e3 / (info.iter + 3):.1f} ms per iteration",
         "cpu_model": cpu_model(),
+        "iters": info.iter,
+        "relerr_vs_cpu": {k: float(np.linalg.norm(v - xc) / nx) for k, v in gpu_x.items()},
     }
     if grad is not None:
         Lg, mg = grad["L"], grad["mass"]
@@ -98,31 +99,59 @@ def cpu_baseline(links, b, L, mass, gpu_fp64_iters, budget_s=20.0, grad=None, ba
         gpu, inputs = small
         L1, b1 = inputs["C1"]
         L2, lk2, b2 = inputs["C2"]
-        for name, solver, op, mass, bb, lk, Ls in (("C1_laplace_cg", "cg", "laplace_real", 0.01, b1, None, L1),
-                                                   ("C2_normal_cg", "cg", "staggered_normal", 0.05, b2, lk2, L2),
-                                                   ("C2_bicgstab_D", "bicgstab", "staggered", 0.05, b2, lk2, L2)):
+        for name, solver, op, mass_, bb, lk, Ls in (("C1_laplace_cg", "cg", "laplace_real", 0.01, b1, None, L1),
+                                                    ("C2_normal_cg", "cg", "staggered_normal", 0.05, b2, lk2, L2),
+                                                    ("C2_bicgstab_D", "bicgstab", "staggered", 0.05, b2, lk2, L2)):
             t0 = time.perf_counter()
-            xc, ci = orc.solve(solver, op, bb, None, links=lk, mass=mass, lx=Ls, ly=Ls, max_iter=100000,
+            xs, ci = orc.solve(solver, op, bb, None, links=lk, mass=mass_, lx=Ls, ly=Ls, max_iter=100000,
                                eps=1e-10, impl=impl)
             dtc = time.perf_counter() - t0
             xg = gpu[name]["x"]
             out[name] = {"ms": round(dtc * 1e3, 2), "iters": ci.iter,
-                         "x_relerr_gpu_vs_cpu": float(np.linalg.norm(xg - xc) / np.linalg.norm(xc))}
+                         "x_relerr_gpu_vs_cpu": float(np.linalg.norm(xg - xs) / np.linalg.norm(xs))}
     if batch is not None:
-        Lb, mb = 128, 0.05
-        rng = np.random.default_rng(8)
-        lk = np.exp(1j * rng.normal(0.0, 0.4, 2 * Lb * Lb))
-        bb = rng.normal(size=Lb * Lb) + 1j * rng.normal(size=Lb * Lb)
-        shifts = 10.0 ** (-3.0 + 0.5 * np.arange(12))
-        t0 = time.perf_counter()
-        _, ci, _, _ = orc.cg_m("staggered_normal", bb, shifts, links=lk, mass=mb, lx=Lb, ly=Lb,
-                               resid_freq_check=10, max_iter=100000, eps=1e-10, impl=impl)
-        dtb = time.perf_counter() - t0
-        per_site_it = dtb / (Lb * Lb * ci.iter)
-        out["batch_s"] = round(per_site_it * batch["site_iters"], 1)
-        out["batch_sample"] = (f"one {Lb}x{Lb} 12-shift CG-M on one core ({impl}, {ci.iter} iterations, {dtb:.2f} s), "
-                               f"projected to the batch's {batch['site_iters']:.3g} site-iterations")
+        out.update(cpu_batch(orc, impl, batch))
     return out
+
+
+def host_threads():
+    """Worker threads for the host leg: the host's cores, at most 16 (one
+    GPU's share of the box)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_batch(orc, impl, batch):
+    """The multishift batch on the host: every system's CG-M (reference code)
+    on its own thread, largest lattices first, all host threads busy; each
+    result is compared with the GPU solution of the same system (per-shift
+    relative error, iteration count)."""
+    import concurrent.futures as cf
+
+    systems, shifts = batch["host"], batch["shifts"]
+    order = sorted(range(len(systems)), key=lambda k: -systems[k]["L"])
+    nthr = host_threads()
+
+    def one(k):
+        s = systems[k]
+        X, io, _, _ = orc.cg_m("staggered_normal", s["b"], shifts, links=s["links"], mass=s["mass"], lx=s["L"],
+                               ly=s["L"], resid_freq_check=10, max_iter=100000, eps=1e-10, impl=impl)
+        err = 0.0
+        for q in range(len(shifts)):
+            xg = s["x"][q].download()
+            err = max(err, float(np.linalg.norm(xg - X[q]) / np.linalg.norm(X[q])))
+        return k, io.iter, err
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(nthr) as ex:
+        res = list(ex.map(one, order))
+    dt = time.perf_counter() - t0
+    same = sum(1 for k, it, _ in res if it == systems[k]["gpu_iter"])
+    near = sum(1 for k, it, _ in res if abs(it - systems[k]["gpu_iter"]) <= 10)
+    return {"batch_s": round(dt, 2), "batch_cores": nthr,
+            "batch_sample": f"all {len(systems)} CG-M solves of the batch ({impl}), one per thread on {nthr} host "
+                            f"threads, longest first",
+            "batch_relerr_gpu_vs_cpu_max": max(e for _, _, e in res),
+            "batch_iters_equal": same, "batch_iters_within_one_check": near}
 
 
 def cpu_model():
@@ -309,30 +338,37 @@ def gradient_timing(glx, L=512, mass=0.05, reps=3):
     return out
 
 
-def batch_multishift(glx, n_lat=48, rhs=4, seq=True):
+def batch_multishift(glx, n_lat=48, rhs=4, seq=True, keep_host=True):
     """48 staggered lattices, L uniform on {64, ..., 512}, m ~ U[0.01, 0.1],
     theta ~ N(0, 0.4^2), 4 complex Gaussian right-hand sides each, 12 shifts
     sigma_k = 10^(-3 + 0.5 k), tol 1e-10, resid_freq_check 10: the 192
     multishift solves in one batched call (launch shapes autotuned per L),
-    and the same solves one call each for comparison."""
+    with the algorithmic bytes of each phase of an iteration counted over the
+    shifts active in it; then the same call again with every phase's launches
+    between HIP events (no graph), for the batch's roofline; and the same
+    solves one call each for comparison.  Returns (figures, host copies of
+    the inputs with the GPU solutions for the CPU leg)."""
     rng = np.random.default_rng(4)
     Ls = rng.choice([64, 128, 192, 256, 384, 512], size=n_lat)
     masses = rng.uniform(0.01, 0.1, size=n_lat)
     shifts = 10.0 ** (-3.0 + 0.5 * np.arange(12))
-    tuned, systems = {}, []
+    tuned, systems, host = {}, [], []
     for i, L in enumerate(Ls):
         L = int(L)
-        lat = glx.Lattice(np.exp(1j * rng.normal(0.0, 0.4, 2 * L * L)), L, L, f32=False, f16=False)
+        links = np.exp(1j * rng.normal(0.0, 0.4, 2 * L * L))
+        lat = glx.Lattice(links, L, L, f32=False, f16=False)
         if L not in tuned:
             tuned[L] = glx.autotune_normal(lat, reps=10)["rows64"]
         for _ in range(rhs):
             b = rng.normal(size=L * L) + 1j * rng.normal(size=L * L)
             xs = [glx.DeviceArray(nbytes=L * L * 16, shape=(L * L,)) for _ in shifts]
             systems.append((lat, float(masses[i]), glx.DeviceArray(b), xs))
+            if keep_host:
+                host.append({"L": L, "mass": float(masses[i]), "links": links, "b": b, "x": xs})
     glx.dev_cg_m_batched(systems[:8], shifts, max_iter=20)  # warm-up (code paths, allocations)
     glx.synchronize()
     t0 = time.perf_counter()
-    infos = glx.dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=100000, eps=1e-10)
+    infos, st = glx.dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=100000, eps=1e-10, stats=True)
     glx.synchronize()
     t_batch = time.perf_counter() - t0
     site_iters = float(sum(lat.lx * lat.ly * inf.iter for (lat, _, _, _), inf in zip(systems, infos)))
@@ -342,6 +378,29 @@ def batch_multishift(glx, n_lat=48, rhs=4, seq=True):
            "iters_max": max(i.iter for i in infos), "iters_sum": sum(i.iter for i in infos),
            "site_iters": site_iters,
            "rows_by_L": {str(k): v for k, v in sorted(tuned.items())}}
+    for h, inf in zip(host, infos):
+        h["gpu_iter"] = inf.iter
+    # the same solve with each phase between HIP events: per-phase roofline
+    _, tp = glx.dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=100000, eps=1e-10,
+                                 time_phases=True)
+    total = st["bytes_stencil"] + st["bytes_r_pass"] + st["bytes_shift_pass"]
+    phases = {}
+    for name, key in (("stencil", "stencil"), ("r_pass", "r_pass"), ("shift_pass", "shift_pass")):
+        by, ms = st["bytes_" + key], tp["ms_" + key]
+        gbs = by / (ms * 1e-3) / 1e9 if ms > 0 else None
+        phases[name] = {"GB": round(by / 1e9, 2), "ms": round(ms, 2), "GBs": None if gbs is None else round(gbs, 1),
+                        "frac": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4)}
+    sp = phases["shift_pass"]
+    out["roofline"] = {
+        "bound": "hbm", "kernel": "k_point_batch<PtCgmXP> (the shift pass: x_s, p_s updates of every active shift "
+                                  "and p, one launch over all unconverged systems)",
+        "achieved": sp["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sp["frac"],
+        "bytes_model": "per site and iteration: D^dag D 64 B; r pass 48 B; shift pass 48 B + 48 B per x_s update "
+                       "+ 16 B per p_s update (shifts active in that iteration)",
+        "shift_updates_x": st["shift_updates_x"], "shift_updates_p": st["shift_updates_p"],
+        "phases": phases,
+        "whole_batch_GBs": round(total / t_batch / 1e9, 1),
+        "whole_batch_frac": round(total / t_batch / 1e9 / HBM_PEAK_GBS, 4)}
     if seq:
         t0 = time.perf_counter()
         for lat, m, db, xs in systems:
@@ -350,6 +409,22 @@ def batch_multishift(glx, n_lat=48, rhs=4, seq=True):
         glx.synchronize()
         out["one_call_each_s"] = round(time.perf_counter() - t0, 3)
     del systems
+    return out, ({"host": host, "shifts": shifts} if keep_host else None)
+
+
+def stream_ceiling(glx, footprints=((64 << 20), (1 << 30)), reps=20):
+    """What the device delivers to plain streaming kernels (read only, write
+    only, copy, and three bytes read per byte written -- the stencils' ratio)
+    over a footprint the size of the 1024^2 working set (64 MiB, inside the
+    256 MB last-level cache) and one beyond it (1 GiB), HIP-event timed."""
+    out = {}
+    for fp in footprints:
+        key = f"{fp >> 20}MiB"
+        out[key] = {}
+        for mode in ("read", "write", "copy", "read3_write1"):
+            ms, by = glx.time_stream(mode, fp, reps)
+            gbs = by / (ms * 1e-3) / 1e9
+            out[key][mode] = {"GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     glx.release_cache()
     return out
 
@@ -438,6 +513,25 @@ def traffic_from_profile(kernel: str, links: str):
     return vals.get("t")
 
 
+def kernel_only(glx, args):
+    """One CG iteration kernel alone on the headline lattice, `reps` launches
+    between HIP events on the kernel's stream (no solve, no autotuning): the
+    program a counter run profiles."""
+    L = args.L
+    links, _ = make_inputs(L, args.sigma, seed=1337)
+    lat = glx.Lattice(links, L, L, f32=True, f16=True)
+    lp = glx.LINKS_F16 if args.links == "f16" else glx.LINKS_F32
+    if args.sstep_rows > 0:
+        glx.set_sstep_rows(L, L, args.sstep_rows)
+    prec = {"sscg": 2 + lp, "fast32": lp, "fast64": 0}[args.kernel_only]
+    ms, bps = glx.time_cg_kernel(lat, args.mass, prec=prec, reps=args.reps)
+    gbs = bps * L * L / (ms * 1e-3) / 1e9
+    print(json.dumps({"kernel": args.kernel_only, "L": L, "links": args.links, "reps": args.reps,
+                      "sstep_rows": glx.get_sstep_rows(L, L), "us_per_launch": round(ms * 1e3, 3),
+                      "bytes_per_site": bps, "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -459,6 +553,11 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
                          "for scaling rocprofv3 byte counters)")
+    ap.add_argument("--kernel-only", choices=["sscg", "fast32", "fast64"], default=None,
+                    help="only time one CG iteration kernel at --L (HIP events, --reps launches) and print its "
+                         "line: the s-step launch, the FP32 one-iteration kernel or the FP64 single-pass kernel "
+                         "(for counter runs of that kernel alone)")
+    ap.add_argument("--reps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the solver path has no multi-GPU code; "
@@ -466,6 +565,9 @@ def main():
         return 2
 
     import glx
+
+    if args.kernel_only:
+        return kernel_only(glx, args)
 
     if args.copy_mib > 0:
         n = args.copy_mib << 16  # complex128 elements (16 bytes)
@@ -566,9 +668,20 @@ def main():
     if not args.no_big:
         big = stencil_beyond_mall(glx)
 
+    ceiling = stream_ceiling(glx)
+    c_small = ceiling["64MiB"]["read3_write1"]["GBs"]
+    c_big = ceiling["1024MiB"]["read3_write1"]["GBs"]
+    for key in ("at_L", "at_2L"):
+        for v in stencil[key].values():
+            if isinstance(v, dict) and "GBs" in v:
+                v["frac_of_ceiling"] = round(v["GBs"] / c_small, 4)
+    if big is not None:
+        big["frac_of_ceiling"] = round(big["fp32_GBs"] / c_big, 4)
+        big["fp64_frac_of_ceiling"] = round(big["fp64_GBs"] / c_big, 4)
+
     eo = None if args.no_eo else eo_solves(glx, lat, b, mass)
     grad = None if args.no_grad else gradient_timing(glx)
-    batch = None if args.no_batch else batch_multishift(glx)
+    batch, batch_host = (None, None) if args.no_batch else batch_multishift(glx, keep_host=not args.no_cpu)
     cst = None if args.no_cstencil else general_stencil(glx)
     small = small_configs(glx)
     gauge = None if args.no_cstencil else gauge_generation(glx)
@@ -576,9 +689,13 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(links, b, L, mass, info64.iter, grad=grad, batch=batch, small=small)
+            cpu = cpu_baseline(links, b, L, mass, {"fp64": x64, "fp64_fused": xfu, "mixed": xmx}, grad=grad,
+                               batch=batch_host, small=small)
         except Exception as e:  # the baseline is reported, never required
             cpu = {"value": None, "unit": "ms", "cores": 1, "kind": "port", "sample": f"failed: {e}"}
+    del batch_host
+    glx.release_cache()
+    rel_cpu = (cpu or {}).get("relerr_vs_cpu", {})
 
     out = {
         "metric": "CG time-to-1e-10 & stencil HBM GB/s, 2D U(1) staggered: FP64 vs mixed",
@@ -616,8 +733,12 @@ def main():
         "mixed_true_relres": float(np.sqrt(infomx.resSq) / np.linalg.norm(b)),
         "fp64_true_relres": float(np.sqrt(info64.resSq) / np.linalg.norm(b)),
         "mixed_relerr_vs_fp64": err_vs_fp64,
+        "mixed_relerr_vs_cpu": rel_cpu.get("mixed"),
+        "fp64_relerr_vs_cpu": rel_cpu.get("fp64"),
+        "fp64_fused_relerr_vs_cpu": rel_cpu.get("fp64_fused"),
         "normal_rows": {"fp64": tuned["rows64"], "fp32": tuned["rows32"]},
         "stencil": dict(stencil, beyond_mall=big),
+        "stream_ceiling": ceiling,
         "cg_kernel_fp64": {"ms_per_launch": it64_ms, "bytes_per_site": it64_bps, "GBs": round(it64_gbs, 1)},
         "cg_kernel_fp32_single_pass": {"ms_per_launch": it1_ms, "bytes_per_site": it1_bps,
                                        "GBs": round(it1_bps * V / (it1_ms * 1e-3) / 1e9, 1)},

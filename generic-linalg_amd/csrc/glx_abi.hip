@@ -20,6 +20,7 @@
 #include <cstring>
 #include <iostream>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <unordered_map>
 #include <vector>
@@ -901,6 +902,36 @@ int dispatch_unpreconditioned(F plain, FR restarted, double* lhs, double* rhs, i
 }
 }  // namespace
 
+namespace {
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_stream(const float4* __restrict__ a, const float4* __restrict__ b,
+                                                    const float4* __restrict__ c, float4* __restrict__ o, long n,
+                                                    float4* __restrict__ sink) {
+  const long stride = long(gridDim.x) * kBlock;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (long i = long(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+    if constexpr (MODE == 0) {
+      const float4 v = a[i];
+      acc.x += v.x;
+      acc.y += v.y;
+      acc.z += v.z;
+      acc.w += v.w;
+    } else if constexpr (MODE == 1) {
+      o[i] = make_float4(1.f, 2.f, 3.f, float(i));
+    } else if constexpr (MODE == 2) {
+      o[i] = a[i];
+    } else {
+      const float4 x = a[i], y = b[i], z = c[i];
+      o[i] = make_float4(x.x + y.x + z.x, x.y + y.y + z.y, x.z + y.z + z.z, x.w + y.w + z.w);
+    }
+  }
+  // keeps the reads of the read-only pass alive; never true for finite data
+  if constexpr (MODE == 0) {
+    if (acc.x == -1.2345e30f) sink[threadIdx.x] = acc;
+  }
+}
+}  // namespace
+
 // ====================================================== exported C ABI ====
 extern "C" {
 
@@ -1128,6 +1159,13 @@ int glx_dev_cg_m(glx_lattice lat, int kind, double mass, int nc, double** x_dev,
 int glx_dev_cg_m_batched(int n_sys, const glx_batch_system* sys, int n_shift, const double* shifts,
                          int resid_freq_check, int max_iter, double eps, int worst_first, int use_graph,
                          glx_inversion_info* infos) {
+  return glx_dev_cg_m_batched_ex(n_sys, sys, n_shift, shifts, resid_freq_check, max_iter, eps, worst_first, use_graph,
+                                 0, infos, nullptr);
+}
+
+int glx_dev_cg_m_batched_ex(int n_sys, const glx_batch_system* sys, int n_shift, const double* shifts,
+                            int resid_freq_check, int max_iter, double eps, int worst_first, int use_graph,
+                            int time_phases, glx_inversion_info* infos, glx_batch_stats* stats) {
   return guarded([&] {
     if (n_sys < 0 || (n_sys > 0 && !sys) || !shifts) throw ArgError("sys and shifts must not be NULL");
     std::vector<BatchSystem> bs(n_sys);
@@ -1144,7 +1182,20 @@ int glx_dev_cg_m_batched(int n_sys, const glx_batch_system* sys, int n_shift, co
     so.eps = eps;
     so.group = g_group;
     so.use_graph = use_graph != 0;
-    std::vector<SolveResult> r = dev_cg_m_batched(bs, n_shift, shifts, resid_freq_check, worst_first != 0, so);
+    BatchStats bst;
+    std::vector<SolveResult> r = dev_cg_m_batched(bs, n_shift, shifts, resid_freq_check, worst_first != 0, so,
+                                                  stats ? &bst : nullptr, time_phases != 0);
+    if (stats) {
+      stats->bytes_stencil = bst.bytes_stencil;
+      stats->bytes_r_pass = bst.bytes_r;
+      stats->bytes_shift_pass = bst.bytes_shift;
+      stats->ms_stencil = bst.ms_stencil;
+      stats->ms_r_pass = bst.ms_r;
+      stats->ms_shift_pass = bst.ms_shift;
+      stats->iterations = bst.iterations;
+      stats->shift_updates_x = bst.upd_x;
+      stats->shift_updates_p = bst.upd_p;
+    }
     if (infos) {
       for (int s = 0; s < n_sys; s++) {
         fill_info(&infos[s], r[s], "CG-M");
@@ -1299,6 +1350,53 @@ int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int 
   });
 }
 
+// ------------------------------------------------------------ stream probe --
+// Plain streaming kernels: what the device delivers to a kernel that only
+// reads, only writes, copies, or reads three bytes for each byte it writes
+// (the stencils' own ratio), with 16-byte accesses per lane in a grid-stride
+// loop.  The ceiling the stencils' fractions of peak are read against.
+
+int glx_dev_time_stream(int mode, size_t bytes, int reps, double* ms_per_pass, double* bytes_per_pass) {
+  return guarded([&] {
+    if (mode < 0 || mode > 3) throw ArgError("mode must be 0 (read), 1 (write), 2 (copy) or 3 (3 reads : 1 write)");
+    if (reps <= 0 || !ms_per_pass) throw ArgError("bad arguments");
+    const int arrays = mode == 3 ? 4 : (mode == 2 ? 2 : 1);
+    const long n = long(bytes / (16 * size_t(arrays)));
+    if (n < 1) throw ArgError("footprint too small");
+    DevBuf buf(size_t(n) * 16 * arrays), sink(kBlock * 16);
+    float4* v = buf.as<float4>();
+    GLX_HIP(hipMemset(buf.get(), 0, size_t(n) * 16 * arrays));
+    const float4* a = v;
+    const float4* b = v + (arrays > 2 ? n : 0);
+    const float4* c = v + (arrays > 2 ? 2 * n : 0);
+    float4* o = v + (arrays - 1) * n;
+    const int nb = int(std::min<long>((n + kBlock - 1) / kBlock, 8192));
+    auto launch = [&]() {
+      switch (mode) {
+        case 0: hipLaunchKernelGGL(k_stream<0>, dim3(nb), dim3(kBlock), 0, 0, a, b, c, o, n, sink.as<float4>()); break;
+        case 1: hipLaunchKernelGGL(k_stream<1>, dim3(nb), dim3(kBlock), 0, 0, a, b, c, o, n, sink.as<float4>()); break;
+        case 2: hipLaunchKernelGGL(k_stream<2>, dim3(nb), dim3(kBlock), 0, 0, a, b, c, o, n, sink.as<float4>()); break;
+        default: hipLaunchKernelGGL(k_stream<3>, dim3(nb), dim3(kBlock), 0, 0, a, b, c, o, n, sink.as<float4>());
+      }
+      GLX_CHECK_LAUNCH();
+    };
+    launch();  // warm-up
+    hipEvent_t e0, e1;
+    GLX_HIP(hipEventCreate(&e0));
+    GLX_HIP(hipEventCreate(&e1));
+    GLX_HIP(hipEventRecord(e0, 0));
+    for (int k = 0; k < reps; k++) launch();
+    GLX_HIP(hipEventRecord(e1, 0));
+    GLX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GLX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_pass = double(ms) / reps;
+    if (bytes_per_pass) *bytes_per_pass = double(n) * 16.0 * arrays;
+  });
+}
+
 int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int link_prec, void* out_dev,
                        const void* in_dev, int reps, double* ms_per_apply) {
   return guarded([&] {
@@ -1337,8 +1435,9 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
 int glx_set_normal_rows(int lx, int ly, int esize, int rows) {
   return guarded([&] {
     if (lx < 0 || ly < 0 || (lx == 0) != (ly == 0)) throw ArgError("bad lattice size");
-    if (!(rows == -1 && lx == 0) && (rows < 0 || (rows > 0 && (rows < 8 || rows % 4 != 0))))
-      throw ArgError("rows must be 0 or a multiple of 4 >= 8 (-1 for the default: by block count)");
+    if (rows < -1 || (rows > 0 && (rows < 8 || rows % 4 != 0)))
+      throw ArgError("rows must be 0 or a multiple of 4 >= 8 (-1: the default by block count, or remove the "
+                     "size's entry)");
     if (lx > 0 && esize != 8 && esize != 16) throw ArgError("esize must be 8 (complex FP32) or 16 (complex FP64)");
     set_normal_rows(lx, ly, esize, rows);
   });
@@ -1385,16 +1484,22 @@ int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, dou
       const int esize = fp32 ? 8 : 16;
       int best = 0;
       double best_ms = 1e300;
-      for (int c : cands) {
-        if (c == 256 && o.ly < 2048) continue;
-        if (c > 0 && !normal_fits(o.lx, o.ly, c, fp32 ? normal_cpl<c64>() : normal_cpl<c128>())) continue;
-        set_normal_rows(o.lx, o.ly, esize, c);
-        time_one(fp32);  // warm-up
-        const double t = std::min(time_one(fp32), time_one(fp32));
-        if (t < best_ms) {
-          best_ms = t;
-          best = c;
+      const int saved = normal_rows_entry(o.lx, o.ly, esize);  // -1: none (erased again on error)
+      try {
+        for (int c : cands) {
+          if (c == 256 && o.ly < 2048) continue;
+          if (c > 0 && !normal_fits(o.lx, o.ly, c, fp32 ? normal_cpl<c64>() : normal_cpl<c128>())) continue;
+          set_normal_rows(o.lx, o.ly, esize, c);
+          time_one(fp32);  // warm-up
+          const double t = std::min(time_one(fp32), time_one(fp32));
+          if (t < best_ms) {
+            best_ms = t;
+            best = c;
+          }
         }
+      } catch (...) {
+        set_normal_rows(o.lx, o.ly, esize, saved);
+        throw;
       }
       set_normal_rows(o.lx, o.ly, esize, best);
       if (fp32) {
@@ -1429,6 +1534,7 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
     mo.link_prec = LINKS_F32;
     mo.sstep = 2;
     const int forced = sstep_rows();
+    const int saved = sstep_rows_entry(o.lx, o.ly);  // -1: none (erased again on error)
     set_sstep(sstep_default(), 0);  // the table decides while timing
     int best = 0;
     double best_ms = 1e300;
@@ -1454,6 +1560,7 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
       }
     } catch (...) {
       set_sstep(sstep_default(), forced);
+      set_sstep_rows(o.lx, o.ly, saved);
       throw;
     }
     set_sstep(sstep_default(), forced);

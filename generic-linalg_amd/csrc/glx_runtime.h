@@ -137,10 +137,13 @@ constexpr bool kind_valid_for(int k) {
 // or by hand; 0 = use the tiled kernel), with a default for sizes not in the
 // table: a fixed value once set, else (-1) the fewest of 16/32/64/128 rows
 // that keep the grid within 512 blocks of 64 lanes x (1 or 2) sites, else
-// 128.  `cols` is the vectors' row length in sites when it differs from lx
+// 256.  `cols` is the vectors' row length in sites when it differs from lx
 // (the even/odd Schur operator's compact rows).
 int normal_rows_for(int lx, int ly, int esize, int cols = 0);
-void set_normal_rows(int lx, int ly, int esize, int rows);  // lx = 0: set the default
+// lx = 0: set the default; otherwise rows < 0 removes the size's entry
+void set_normal_rows(int lx, int ly, int esize, int rows);
+// the size's table entry, or -1 when it has none
+int normal_rows_entry(int lx, int ly, int esize);
 
 // Rows per block of the even/odd Schur kernel (compact rows of lx / 2 sites;
 // "tiled" does not exist for this operator).

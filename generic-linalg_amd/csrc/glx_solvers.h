@@ -67,8 +67,21 @@ struct BatchSystem {
   const void* b;
   void* const* xs;
 };
+// Work and time of a batched solve, per phase of the iteration: the D^dag D
+// launches (64 B per site: p, Ap, 32 B of links), the residual pass (48 B:
+// r read and written, Ap read) and the shift pass (per site 48 B for r and p
+// plus 48 per x_s update and 16 per p_s update).  The ms_* fields are the
+// HIP-event time of each phase's launches when time_phases was asked for
+// (direct launches, no graph), else 0.
+struct BatchStats {
+  double bytes_stencil = 0, bytes_r = 0, bytes_shift = 0;
+  double ms_stencil = 0, ms_r = 0, ms_shift = 0;
+  long long iterations = 0, upd_x = 0, upd_p = 0;
+  int launches_shift = 0;
+};
 std::vector<SolveResult> dev_cg_m_batched(const std::vector<BatchSystem>& sys, int n_shift, const double* shifts,
-                                          int resid_freq_check, bool worst_first, const SolveOpts& o);
+                                          int resid_freq_check, bool worst_first, const SolveOpts& o,
+                                          BatchStats* stats = nullptr, bool time_phases = false);
 
 // Mixed-precision CG with reliable updates (no reference counterpart):
 // FP32 Krylov vectors and FP32 or FP16 links, FP64 true residual and
@@ -92,6 +105,7 @@ int sstep_rows();
 // one-iteration form at that size (rows < 0 removes the entry).
 void set_sstep_rows(int lx, int ly, int rows);
 int sstep_rows_for(int lx, int ly);
+int sstep_rows_entry(int lx, int ly);  // the size's table entry, or -1 when it has none
 int sstep_rows_auto(int lx, int ly);
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m);
 

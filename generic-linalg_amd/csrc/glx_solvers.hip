@@ -58,7 +58,14 @@ void set_normal_rows(int lx, int ly, int esize, int rows) {
   NormalTable& t = normal_table();
   std::lock_guard<std::mutex> g(t.mu);
   if (lx == 0) t.def = rows;
+  else if (rows < 0) t.rows.erase(std::make_tuple(lx, ly, esize));
   else t.rows[std::make_tuple(lx, ly, esize)] = rows;
+}
+int normal_rows_entry(int lx, int ly, int esize) {
+  NormalTable& t = normal_table();
+  std::lock_guard<std::mutex> g(t.mu);
+  auto it = t.rows.find(std::make_tuple(lx, ly, esize));
+  return it != t.rows.end() ? it->second : -1;
 }
 
 DevicePool& DevicePool::get() {
@@ -113,7 +120,8 @@ struct SolveState {
   int max_iter, resid_freq, worst_first, n_shift;
   double eps, bsqrt, rsq, rsqNew;
   double beta_r, resSq, maxrr, delta;  // maxrr: single-pass CG: max of |r|^2 since the last reliable update
-  int reliable, pending, ko, pad3;  // pending: self-finalising single-pass CG; ko: s-step CG launches
+  int reliable, pending, ko;  // pending: self-finalising single-pass CG; ko: s-step CG launches
+  int n_xupd;  // multishift: shifts active before this iteration's check (their x still takes its update)
   c128 alpha, alpha_prev, beta, beta_prev, rho, omega, pap;
   double* hist;
   double* resvec;
@@ -125,6 +133,10 @@ struct SolveState {
   double shifts[kMaxShift];
   int mapping[kMaxShift];
   int vec[kMaxShift];
+  // multishift: sums over the iterations of the shifts whose x_s (n_xupd)
+  // and p_s (n_rem after the check) were updated -- the pass's algorithmic
+  // bytes are 16 (3 upd_x + upd_p) per site
+  long long upd_x, upd_p;
 };
 
 struct Mirror {
@@ -239,7 +251,10 @@ Mirror run_loop(Work& w, int max_iter, int chunk, Enq enq, Pause pause) {
         m = pin.m[cur];
         break;
       }
-      if (next <= m.kfinal) throw HipError("solver state did not advance after a pause");
+      if (next <= m.kfinal) {
+        (void)hipStreamSynchronize(w.st);
+        throw HipError("solver state did not advance after a pause");
+      }
       enqueue_chunk(cur);
       continue;
     }
@@ -249,7 +264,7 @@ Mirror run_loop(Work& w, int max_iter, int chunk, Enq enq, Pause pause) {
       GLX_HIP(hipMemcpy(&m, w.ds, sizeof(Mirror), hipMemcpyDeviceToHost));
       if (m.done == 2) {
         next = pause(m);
-        if (next >= 0 && next <= m.kfinal) throw HipError("solver state did not advance after a pause");
+        if (next >= 0 && next <= m.kfinal) throw HipError("solver state did not advance after a pause");  // drained
         if (next >= 0 && next < max_iter) {
           enqueue_chunk(cur);
           continue;
@@ -1018,27 +1033,18 @@ struct FinCgmBeta {
     }
   }
 };
-// x_s -= beta_s p_s for active shifts; r += beta Ap; ||r||^2
+// r += beta Ap; ||r||^2 (generic_cg_m.cpp:423-426).  The shifts' x_s -= beta_s p_s
+// (:414-420) of the same iteration is deferred to the p_s pass below, which
+// reads each p_s once for both updates.
 template <class T>
-struct PtCgmX {
+struct PtCgmR {
   static constexpr int NR = 1;
-  T* const* xtab;
-  T* const* ptab;
   T* r;
   const T* Ap;
   const SolveState* s;
-  int n_rem;
   typename Scal<T>::type be;
-  __device__ void init() {
-    n_rem = s->n_rem;
-    be = as_scal<T>(s->beta);
-  }
+  __device__ void init() { be = as_scal<T>(s->beta); }
   __device__ void operator()(int i, double* part) {
-    for (int n = 0; n < n_rem; n++) {
-      const int v = s->vec[n];
-      T* xv = xtab[v];
-      xv[i] = xv[i] - scal(as_scal<T>(s->beta_s[n]), ptab[v][i]);
-    }
     const T ri = r[i] + scal(be, Ap[i]);
     r[i] = ri;
     part[0] += norm2(ri);
@@ -1059,6 +1065,10 @@ struct FinCgmR {
     const int k = s->k;
     if (s->hist) s->hist[k] = sqrt(rn) / s->bsqrt;
     const double tol = s->eps * s->bsqrt;
+    // shifts removed below keep their slots at [n_rem, n_xupd) with their
+    // beta_s, for the deferred x_s update of this iteration
+    s->n_xupd = s->n_rem;
+    s->upd_x += s->n_rem;
     if (k % s->resid_freq == 0) {
       for (int n = 0; n < s->n_rem; n++) {
         double zabs;
@@ -1086,6 +1096,7 @@ struct FinCgmR {
       s->done = 1;
       return;
     }
+    s->upd_p += s->n_rem;
     const double al = rn / s->rsq;
     s->alpha = {al, 0.0};
     s->rsq = rn;
@@ -1099,27 +1110,56 @@ struct FinCgmR {
     s->k = k + 1;
   }
 };
-// p_s = zeta r + alpha_s p_s for active shifts; p = r + alpha p
+// Per shift that was active before this iteration's check: x_s -= beta_s p_s
+// with the old p_s (generic_cg_m.cpp:414-420), then, if it is still active,
+// p_s = zeta_s r + alpha_s p_s (:508-514); p = r + alpha p (:516-518).  Each
+// p_s is read once: 64 bytes per shift and site instead of 96 for separate x
+// and p passes.  The arithmetic of every element is the reference's.
 template <class T>
-struct PtCgmP {
+struct PtCgmXP {
   static constexpr int NR = 0;
+  T* const* xtab;
   T* const* ptab;
   T* p;
   const T* r;
   const SolveState* s;
-  int n_rem;
+  int n_rem, n_x;
   typename Scal<T>::type al;
   __device__ void init() {
     n_rem = s->n_rem;
+    n_x = s->n_xupd;
     al = as_scal<T>(s->alpha);
   }
   __device__ void operator()(int i, double*) {
     const T ri = r[i];
-    for (int n = 0; n < n_rem; n++) {
-      T* pv = ptab[s->vec[n]];
-      pv[i] = scal(as_scal<T>(s->zeta[n]), ri) + scal(as_scal<T>(s->alpha_s[n]), pv[i]);
+    for (int n = 0; n < n_x; n++) {
+      const int v = s->vec[n];
+      T* xv = xtab[v];
+      T* pv = ptab[v];
+      const T pold = pv[i];
+      xv[i] = xv[i] - scal(as_scal<T>(s->beta_s[n]), pold);
+      if (n < n_rem) pv[i] = scal(as_scal<T>(s->zeta[n]), ri) + scal(as_scal<T>(s->alpha_s[n]), pold);
     }
     p[i] = ri + scal(al, p[i]);
+  }
+};
+// The x_s update of the iteration the solve stopped on (its p pass does not
+// run): every shift active before that iteration's check.  Launched once,
+// ungated, after the loop; n_xupd is 0 when no iteration ran.
+template <class T>
+struct PtCgmXLast {
+  static constexpr int NR = 0;
+  T* const* xtab;
+  T* const* ptab;
+  const SolveState* s;
+  int n_x;
+  __device__ void init() { n_x = s->n_xupd; }
+  __device__ void operator()(int i, double*) {
+    for (int n = 0; n < n_x; n++) {
+      const int v = s->vec[n];
+      T* xv = xtab[v];
+      xv[i] = xv[i] - scal(as_scal<T>(s->beta_s[n]), ptab[v][i]);
+    }
   }
 };
 // ||(A + sigma) x - b||^2 for one shift
@@ -1188,13 +1228,14 @@ SolveResult cg_m_impl(const DevOperator& o, T* const* xs, const T* b, int n_shif
   launch_point(w.st, n, PtNormB<T>{b}, FinCgmInit{w.ds}, w.ws(), kNoGate);
   run_fused<T>(w, o, LINKS_F64, InPlain<T>{p}, p, Ap, EpiDotInOut<T>{}, FinCgmBeta<T>{w.ds});
   auto enq = [&](int) {
-    launch_point(w.st, n, PtCgmX<T>{xtab, ptab, r, Ap, w.ds}, FinCgmR<T>{w.ds}, w.ws(), w.done());
-    launch_point(w.st, n, PtCgmP<T>{ptab, p, r, w.ds}, FinNone{}, w.ws(), w.done());
+    launch_point(w.st, n, PtCgmR<T>{r, Ap, w.ds}, FinCgmR<T>{w.ds}, w.ws(), w.done());
+    launch_point(w.st, n, PtCgmXP<T>{xtab, ptab, p, r, w.ds}, FinNone{}, w.ws(), w.done());
     run_fused<T>(w, o, LINKS_F64, InPlain<T>{p}, p, Ap, EpiDotInOut<T>{}, FinCgmBeta<T>{w.ds});
   };
   // hipMemcpyAsync of pageable host vectors above must finish before they die
   GLX_HIP(hipStreamSynchronize(w.st));
   Mirror m = run_loop(w, opt.max_iter, opt.chunk, enq, [](const Mirror&) { return -1; });
+  launch_point(w.st, n, PtCgmXLast<T>{xtab, ptab, w.ds}, FinNone{}, w.ws(), kNoGate);
   const int kf = opt.max_iter > 0 ? m.kfinal : 0;
   for (int k = 0; k < n_shift; k++) {
     apply_typed<T>(w.st, o, LINKS_F64, Ap, xs[k], w.ws(), kNoGate);
@@ -1676,11 +1717,31 @@ struct PtFastFinal {
 
 // A non-default (blocking) stream owned by one solve: graph capture needs it,
 // and it still orders after work the caller queued on the null stream.
+// A solve that gives up while groups of iterations are still queued waits
+// for its stream before it throws: the unwinding hands the solve's buffers
+// back to the device pool, and no queued kernel may still write them then
+// (also on a caller's stream, which OwnedStream does not synchronise).
+[[noreturn]] static void fail_drained(hipStream_t st, hipGraphExec_t exec, const char* msg) {
+  (void)hipStreamSynchronize(st);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  throw HipError(msg);
+}
+
+// The stream of a solve: the caller's (`want`), else one created for the
+// solve and destroyed after it.
 struct OwnedStream {
   hipStream_t st = nullptr;
-  OwnedStream() { GLX_HIP(hipStreamCreate(&st)); }
+  bool owned = false;
+  explicit OwnedStream(hipStream_t want = nullptr) {
+    if (want) {
+      st = want;
+    } else {
+      GLX_HIP(hipStreamCreate(&st));
+      owned = true;
+    }
+  }
   ~OwnedStream() {
-    if (st) {
+    if (owned) {
       (void)hipStreamSynchronize(st);
       (void)hipStreamDestroy(st);
     }
@@ -1747,8 +1808,8 @@ template <class TD, class TS, bool MIXED>
 SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpts& opt, const MixedOpts& mo) {
   if (!o.tiled()) throw ArgError("single-pass CG needs a device stencil operator");
   const int n = o.size();
-  OwnedStream own;
-  hipStream_t st = opt.stream ? opt.stream : own.st;
+  OwnedStream own(opt.stream);
+  hipStream_t st = own.st;
   Work w(st, stencil_blocks(o.lx, o.ly), opt.max_iter, opt.want_hist);
   SolveState h;
   std::memset(&h, 0, sizeof(h));
@@ -1756,9 +1817,9 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
   h.eps = opt.eps;
   h.delta = mo.delta;
   h.hist = w.hist.as<double>();
-  PinnedSlots& pin = pinned();
-  __atomic_store_n(pin.hdone, 0, __ATOMIC_RELEASE);
-  h.hdone = pin.hdone_dev;
+  PinnedSlots* pin = &pinned();
+  __atomic_store_n(pin->hdone, 0, __ATOMIC_RELEASE);
+  h.hdone = pin->hdone_dev;
   w.upload(h);
   std::vector<DevBuf> keep;
   auto vec = [&](size_t bytes) {
@@ -1845,17 +1906,14 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
     auto submit = [&](int slot) {
       launch();
       groups++;
-      GLX_HIP(hipEventRecord(pin.ev_nf[slot], st));
+      GLX_HIP(hipEventRecord(pin->ev_nf[slot], st));
     };
     submit(cur);
     for (;;) {
       submit(cur ^ 1);
-      GLX_HIP(hipEventSynchronize(pin.ev_nf[cur]));
-      if (__atomic_load_n(pin.hdone, __ATOMIC_ACQUIRE) == 1) break;
-      if (groups > max_groups) {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        throw HipError("single-pass CG made no progress");
-      }
+      GLX_HIP(hipEventSynchronize(pin->ev_nf[cur]));
+      if (__atomic_load_n(pin->hdone, __ATOMIC_ACQUIRE) == 1) break;
+      if (groups > max_groups) fail_drained(st, exec, "single-pass CG made no progress");
       cur ^= 1;
     }
   } else if (opt.max_iter > 0) {
@@ -1863,18 +1921,15 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
     auto submit = [&](int slot) {
       launch();
       groups++;
-      GLX_HIP(hipMemcpyAsync(&pin.m[slot], w.ds, sizeof(Mirror), hipMemcpyDeviceToHost, st));
-      GLX_HIP(hipEventRecord(pin.ev[slot], st));
+      GLX_HIP(hipMemcpyAsync(&pin->m[slot], w.ds, sizeof(Mirror), hipMemcpyDeviceToHost, st));
+      GLX_HIP(hipEventRecord(pin->ev[slot], st));
     };
     submit(cur);
     for (;;) {
       submit(cur ^ 1);
-      GLX_HIP(hipEventSynchronize(pin.ev[cur]));
-      if (pin.m[cur].done == 1) break;
-      if (groups > max_groups) {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        throw HipError("single-pass CG made no progress");
-      }
+      GLX_HIP(hipEventSynchronize(pin->ev[cur]));
+      if (pin->m[cur].done == 1) break;
+      if (groups > max_groups) fail_drained(st, exec, "single-pass CG made no progress");
       cur ^= 1;
     }
   }
@@ -1950,6 +2005,12 @@ int sstep_rows_auto(int lx, int ly) {
     if (strips * ((ly + r - 1) / r) <= 512) return r;
   return 256;
 }
+int sstep_rows_entry(int lx, int ly) {
+  SsTable& t = ss_table();
+  std::lock_guard<std::mutex> g(t.mu);
+  auto it = t.rows.find(std::make_pair(lx, ly));
+  return it != t.rows.end() ? it->second : -1;
+}
 int sstep_rows_for(int lx, int ly) {
   const int forced = g_sstep_rows.load();
   if (forced > 0) return forced;
@@ -1967,6 +2028,7 @@ constexpr int kSsCPL = 2, kSsCPLEo = 1;
 
 static bool sscg_fits(const DevOperator& o) {
   if (!o.tiled() || o.ly < 8 || o.ly % 4 != 0) return false;
+  if (long(o.lx) * o.ly >= kSsMaxSites) return false;  // its 32-bit offsets; the one-iteration form takes these
   if (o.kind == OP_STAG_NORMAL) return o.lx >= kNX * kSsCPL && o.lx % (kNX * kSsCPL) == 0;
   if (o.kind == OP_STAG_EO_SCHUR) return o.lx % 2 == 0 && o.lx / 2 >= kNX * kSsCPLEo && (o.lx / 2) % (kNX * kSsCPLEo) == 0;
   return false;
@@ -2064,8 +2126,8 @@ struct SsSpace {
 SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveOpts& opt, const MixedOpts& mo,
                       int rows) {
   const int n = o.size();
-  OwnedStream own;
-  hipStream_t st = opt.stream ? opt.stream : own.st;
+  OwnedStream own(opt.stream);
+  hipStream_t st = own.st;
   const EoGeom g = ss_geom(o, rows);
   // two buffers of the sums per block
   Work w(st, std::max(2 * g.g.nblk, stencil_blocks(o.lx, o.ly)), opt.max_iter, opt.want_hist);
@@ -2075,9 +2137,9 @@ SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveO
   h.eps = opt.eps;
   h.delta = mo.delta;
   h.hist = w.hist.as<double>();
-  PinnedSlots& pin = pinned();
-  __atomic_store_n(pin.hdone, 0, __ATOMIC_RELEASE);
-  h.hdone = pin.hdone_dev;
+  PinnedSlots* pin = &pinned();
+  __atomic_store_n(pin->hdone, 0, __ATOMIC_RELEASE);
+  h.hdone = pin->hdone_dev;
   w.upload(h);
   SsSpace sp(n, st);
   DevBuf t64buf(size_t(n) * sizeof(c128));
@@ -2127,17 +2189,14 @@ SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveO
     auto submit = [&](int slot) {
       launch();
       groups++;
-      GLX_HIP(hipEventRecord(pin.ev_nf[slot], st));
+      GLX_HIP(hipEventRecord(pin->ev_nf[slot], st));
     };
     submit(cur);
     for (;;) {
       submit(cur ^ 1);
-      GLX_HIP(hipEventSynchronize(pin.ev_nf[cur]));
-      if (__atomic_load_n(pin.hdone, __ATOMIC_ACQUIRE) == 1) break;
-      if (groups > max_groups) {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        throw HipError("s-step CG made no progress");
-      }
+      GLX_HIP(hipEventSynchronize(pin->ev_nf[cur]));
+      if (__atomic_load_n(pin->hdone, __ATOMIC_ACQUIRE) == 1) break;
+      if (groups > max_groups) fail_drained(st, exec, "s-step CG made no progress");
       cur ^= 1;
     }
   }
@@ -2334,6 +2393,238 @@ struct PtCgmStart {
   }
 };
 
+// ------------------------------------ deferred shift updates (batched) ----
+// In multishift CG the shifted vectors p_s and x_s never feed back into the
+// iteration: beta, zeta_s, beta_s, alpha, alpha_s and the convergence test
+// come from r, p and Ap alone (generic_cg_m.cpp:396-505).  The batched solver
+// therefore defers their updates over a window of at most kWin iterations
+// k0 .. k: with r^(j) the residual of iteration j,
+//   p_s^(k+1) = cp p_s^(k0) + sum_j dp_j r^(k0+1+j)
+//   x_s^(k+1) = x_s^(k0) + ax p_s^(k0) + sum_j ex_j r^(k0+1+j)
+// where the coefficients follow the reference's steps, x_s -= beta_s p_s
+// (:414-420) and p_s = zeta_s r + alpha_s p_s (:508-514), as FP64 scalars on
+// the finaliser.  A window ends at every convergence check (so a shift that
+// converges has its x_s completed there) and after kWin iterations; the next
+// residual pass applies it site by site, reading each x_s and p_s once per
+// window instead of every iteration.  The iteration itself (and so every
+// iteration count and convergence decision) is the reference's; only the
+// rounding of x_s differs.  The residuals of a window live in a ring of
+// kWin + 1 vectors, r^(j) in slot j mod (kWin + 1).
+constexpr int kWin = 10;
+constexpr int kRing = kWin + 1;
+struct CgmWin {
+  int wlen;   // residuals in the window (terms of the p_s coefficients)
+  int flush;  // the window ended at the last finaliser: the next residual pass applies it
+  long long upd_r, upd_x, upd_p;  // applied: residual terms, x_s and p_s updates (site-independent counts)
+  c128 cp[kMaxShift], ax[kMaxShift];
+  c128 dp[kMaxShift][kWin], ex[kMaxShift][kWin];
+};
+
+// The window's terms for one site: r^(k0+1+j), j < wlen, from the ring.
+template <class T>
+__device__ __forceinline__ void cgm_window_terms(const T* ring, long n, int kcur, int wlen, int i, T (&rj)[kWin]) {
+#pragma unroll
+  for (int j = 0; j < kWin; j++) {
+    if (j < wlen) {
+      const int slot = (kcur - wlen + 1 + j + 2 * kRing) % kRing;
+      rj[j] = ring[slot * n + i];
+    }
+  }
+}
+
+// First, if the last finaliser ended a window: x_s += ax p_s + sum ex_j r_j
+// for the shifts active before that check, p_s = cp p_s + sum dp_j r_j for
+// those still active.  Then r^(k+1) = r^(k) + beta Ap (generic_cg_m.cpp:423-426)
+// into the ring, and |r^(k+1)|^2.
+template <class T>
+struct PtCgmRing {
+  static constexpr int NR = 1;
+  T* ring;
+  long n;
+  const T* Ap;
+  T* const* xtab;
+  T* const* ptab;
+  const SolveState* s;
+  const CgmWin* w;
+  typename Scal<T>::type be;
+  int kcur, flush, wlen, nx, np;
+  __device__ void init() {
+    be = as_scal<T>(s->beta);
+    kcur = s->k;
+    flush = w->flush;
+    wlen = w->wlen;
+    nx = s->n_xupd;
+    np = s->n_rem;
+  }
+  __device__ void operator()(int i, double* part) {
+    const T ri = ring[(kcur % kRing) * n + i];
+    if (flush) {
+      T rj[kWin];
+      cgm_window_terms(ring, n, kcur, wlen, i, rj);
+      for (int q = 0; q < nx; q++) {
+        const int v = s->vec[q];
+        T* xv = xtab[v];
+        T* pv = ptab[v];
+        const T ps = pv[i];
+        T xa = xv[i] + w->ax[q] * ps;
+        for (int j = 0; j < wlen; j++) xa = xa + w->ex[q][j] * rj[j];
+        xv[i] = xa;
+        if (q < np) {
+          T pa = w->cp[q] * ps;
+          for (int j = 0; j < wlen; j++) pa = pa + w->dp[q][j] * rj[j];
+          pv[i] = pa;
+        }
+      }
+    }
+    const T rn = ri + scal(be, Ap[i]);
+    ring[((kcur + 1) % kRing) * n + i] = rn;
+    part[0] += norm2(rn);
+  }
+};
+// After the loop: the x_s part of the open window (the solve stopped before
+// its p_s update), for the shifts active before the last check.
+template <class T>
+struct PtCgmFlushX {
+  static constexpr int NR = 0;
+  const T* ring;
+  long n;
+  T* const* xtab;
+  T* const* ptab;
+  const SolveState* s;
+  const CgmWin* w;
+  int kcur, wlen, nx;
+  __device__ void init() {
+    kcur = s->k;
+    wlen = w->wlen;
+    nx = s->n_xupd;
+  }
+  __device__ void operator()(int i, double*) {
+    T rj[kWin];
+    cgm_window_terms(ring, n, kcur, wlen, i, rj);
+    for (int q = 0; q < nx; q++) {
+      const int v = s->vec[q];
+      T xa = xtab[v][i] + w->ax[q] * ptab[v][i];
+      for (int j = 0; j < wlen; j++) xa = xa + w->ex[q][j] * rj[j];
+      xtab[v][i] = xa;
+    }
+  }
+};
+// FinCgmR with the window's coefficients: the x_s step of this iteration,
+// the check (permuting the coefficients with the shifts), alpha, alpha_s, the
+// p_s step, and whether the window ends here.
+struct FinCgmRWin {
+  SolveState* s;
+  CgmWin* w;
+  __device__ static void swp(c128& a, c128& b) {
+    const c128 t = a;
+    a = b;
+    b = t;
+  }
+  __device__ void operator()(const double* v) const {
+    const double rn = v[0];
+    const int k = s->k;
+    if (s->hist) s->hist[k] = sqrt(rn) / s->bsqrt;
+    const double tol = s->eps * s->bsqrt;
+    if (w->flush) {  // the previous window was applied by the residual pass that precedes this finaliser
+      w->flush = 0;
+      w->wlen = 0;
+      for (int q = 0; q < s->n_rem; q++) {
+        w->cp[q] = {1.0, 0.0};
+        w->ax[q] = {0.0, 0.0};
+        for (int j = 0; j < kWin; j++) w->dp[q][j] = w->ex[q][j] = {0.0, 0.0};
+      }
+    }
+    const int wl = w->wlen;
+    s->n_xupd = s->n_rem;
+    // x_s -= beta_s p_s (generic_cg_m.cpp:414-420) on the coefficients
+    for (int q = 0; q < s->n_rem; q++) {
+      const c128 b = s->beta_s[q];
+      w->ax[q] = w->ax[q] - b * w->cp[q];
+      for (int j = 0; j < wl; j++) w->ex[q][j] = w->ex[q][j] - b * w->dp[q][j];
+    }
+    if (k % s->resid_freq == 0) {
+      for (int n = 0; n < s->n_rem; n++) {
+        const double zabs = hypot(s->zeta[n].re, s->zeta[n].im);
+        if (zabs * sqrt(rn) < tol) {
+          s->n_rem--;
+          const int e = s->n_rem;
+          if (e != n) {
+            int t = s->mapping[e]; s->mapping[e] = s->mapping[n]; s->mapping[n] = t;
+            t = s->vec[e]; s->vec[e] = s->vec[n]; s->vec[n] = t;
+            swp(s->alpha_s[e], s->alpha_s[n]);
+            swp(s->beta_s[e], s->beta_s[n]);
+            swp(s->zeta[e], s->zeta[n]);
+            swp(s->zeta_p[e], s->zeta_p[n]);
+            const double d = s->shifts[e]; s->shifts[e] = s->shifts[n]; s->shifts[n] = d;
+            swp(w->cp[e], w->cp[n]);
+            swp(w->ax[e], w->ax[n]);
+            for (int j = 0; j < kWin; j++) {
+              swp(w->dp[e][j], w->dp[n][j]);
+              swp(w->ex[e][j], w->ex[n][j]);
+            }
+            n--;
+          }
+        }
+      }
+    }
+    const double z0 = hypot(s->zeta[0].re, s->zeta[0].im);
+    if ((s->worst_first && z0 * sqrt(rn) < tol) || s->n_rem == 0 || k == s->max_iter - 1) {
+      s->kfinal = k;
+      s->done = 1;
+      return;
+    }
+    const double al = rn / s->rsq;
+    s->alpha = {al, 0.0};
+    s->rsq = rn;
+    for (int n = 0; n < s->n_rem; n++)
+      s->alpha_s[n] = cdiv((s->alpha * s->zeta[n]) * s->beta_s[n], s->zeta_p[n] * s->beta);
+    // p_s = zeta_s r^(k+1) + alpha_s p_s (generic_cg_m.cpp:508-514) on the coefficients
+    for (int q = 0; q < s->n_rem; q++) {
+      const c128 a = s->alpha_s[q];
+      w->cp[q] = a * w->cp[q];
+      for (int j = 0; j < wl; j++) w->dp[q][j] = a * w->dp[q][j];
+      w->dp[q][wl] = s->zeta[q];
+    }
+    w->wlen = wl + 1;
+    if (k % s->resid_freq == 0 || w->wlen == kWin) {
+      w->flush = 1;
+      w->upd_r += w->wlen;
+      w->upd_x += s->n_xupd;
+      w->upd_p += s->n_rem;
+    }
+    s->k = k + 1;
+  }
+};
+// Prologue of the batched D^dag D launch: p^(k+1) = r^(k+1) + alpha p^(k)
+// (generic_cg_m.cpp:516-518, the arithmetic of PtCgmP) formed while staging,
+// stored on the own sites into the other p buffer.
+template <class T>
+struct InCgmP {
+  static constexpr bool kAux = false;
+  const T* ring;
+  long n;
+  const T* pold;
+  T* pnew;
+  const SolveState* s;
+  const T* r;
+  typename Scal<T>::type al;
+  __device__ void init() {
+    r = ring + (s->k % kRing) * n;
+    al = as_scal<T>(s->alpha);
+  }
+  __device__ T* target(T* out) const { return out; }
+  struct Raw {
+    T r, po;
+  };
+  __device__ Raw fetch(int j, bool) const { return Raw{r[j], pold[j]}; }
+  __device__ T finish(const Raw& w, int j, bool own, T&) const {
+    const T pn = w.r + scal(al, w.po);
+    if (own) pnew[j] = pn;
+    return pn;
+  }
+  __device__ T load(int j, bool own, T& aux) const { return finish(fetch(j, own), j, own, aux); }
+};
+
 // Device copies of one launch kind's items and its block table.
 struct BatchLaunch {
   DevBuf items, sys_of_blk, off;
@@ -2361,7 +2652,8 @@ static BatchLaunch make_batch_launch(const std::vector<Item>& items, const std::
 
 template <class T>
 std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, int n_shift, const double* shifts,
-                                         int resid_freq, bool worst_first, const SolveOpts& opt) {
+                                         int resid_freq, bool worst_first, const SolveOpts& opt, BatchStats* stats,
+                                         bool time_phases) {
   static_assert(std::is_same<T, c128>::value, "the batched multishift solver is complex FP64");
   if (sys.empty()) return {};
   if (n_shift < 1 || n_shift > kMaxShift) throw ArgError("n_shift must be in [1, 64]");
@@ -2379,14 +2671,17 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
     if (!sys[s].b || !sys[s].xs) throw ArgError("batched system " + std::to_string(s) + ": b and x are required");
     nsite[s] = o.size();
   }
-  OwnedStream own;
-  hipStream_t st = opt.stream ? opt.stream : own.st;
-  // one allocation for every system's r, p, Ap and p_s
+  OwnedStream own(opt.stream);
+  hipStream_t st = own.st;
+  // one allocation for every system's residual ring (kRing vectors), p in two
+  // buffers (the D^dag D launch forms the next p while reading the last),
+  // Ap and the n_shift p_s
+  constexpr int kLead = kRing + 3;
   size_t total = 0;
   std::vector<size_t> vbase(ns);
   for (int s = 0; s < ns; s++) {
     vbase[s] = total;
-    total += size_t(3 + n_shift) * nsite[s];
+    total += size_t(kLead + n_shift) * nsite[s];
   }
   DevBuf vecs(total * sizeof(T));
   DevBuf states(size_t(ns) * sizeof(SolveState));
@@ -2407,7 +2702,7 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
     max_nb = std::max(max_nb, std::max(nb_pt[s], nb_st[s]));
     for (int k = 0; k < n_shift; k++) {
       htab[(size_t(s) * 2 + 0) * n_shift + k] = static_cast<T*>(sys[s].xs[k]);
-      htab[(size_t(s) * 2 + 1) * n_shift + k] = V + vbase[s] + size_t(3 + k) * nsite[s];
+      htab[(size_t(s) * 2 + 1) * n_shift + k] = V + vbase[s] + size_t(kLead + k) * nsite[s];
     }
   }
   // partial sums: each system owns 2 * max_nb doubles (NR <= 2 everywhere)
@@ -2436,62 +2731,117 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
     h.resvec = resv.as<double>() + size_t(s) * n_shift;
   }
   GLX_HIP(hipMemcpy(ds, hs.data(), size_t(ns) * sizeof(SolveState), hipMemcpyHostToDevice));
+  // the deferred-update windows: empty, cp = 1
+  DevBuf winB(size_t(ns) * sizeof(CgmWin));
+  CgmWin* dwin = winB.as<CgmWin>();
+  {
+    std::vector<CgmWin> hw(ns);
+    for (CgmWin& w : hw) {
+      std::memset(&w, 0, sizeof(w));
+      for (int k = 0; k < kMaxShift; k++) w.cp[k] = {1.0, 0.0};
+    }
+    GLX_HIP(hipMemcpy(dwin, hw.data(), size_t(ns) * sizeof(CgmWin), hipMemcpyHostToDevice));
+  }
 
   auto ws_of = [&](int s) { return RedWs{partB.as<double>() + size_t(s) * 2 * max_nb}; };
   auto gate_of = [&](int s) { return Gate{&ds[s].done, 0}; };
-  auto rvec = [&](int s) { return V + vbase[s]; };
-  auto pvec = [&](int s) { return V + vbase[s] + nsite[s]; };
-  auto apvec = [&](int s) { return V + vbase[s] + 2 * size_t(nsite[s]); };
+  auto ring = [&](int s) { return V + vbase[s]; };
+  auto pbuf = [&](int s, int c) { return V + vbase[s] + size_t(kRing + c) * nsite[s]; };
+  auto apvec = [&](int s) { return V + vbase[s] + size_t(kRing + 2) * nsite[s]; };
   T* const* xtab_of_base = tab;
   auto xtab = [&](int s) { return xtab_of_base + (size_t(s) * 2 + 0) * n_shift; };
   auto ptab = [&](int s) { return xtab_of_base + (size_t(s) * 2 + 1) * n_shift; };
 
+  using InP = InCgmP<T>;
+  using NrmP = NrmItem<T, InP, EpiDotInOut<T>>;
   std::vector<PtItem<PtCgmStart<T>>> i_start(ns);
   std::vector<FinItem<FinCgmInit>> f_start(ns);
-  std::vector<NrmItem<T, InPlain<T>, EpiDotInOut<T>>> i_st(ns);
+  std::vector<NrmItem<T, InPlain<T>, EpiDotInOut<T>>> i_st0(ns);
+  std::vector<NrmP> i_st[2] = {std::vector<NrmP>(ns), std::vector<NrmP>(ns)};
   std::vector<FinItem<FinCgmBeta<T>>> f_st(ns);
-  std::vector<PtItem<PtCgmX<T>>> i_x(ns);
-  std::vector<FinItem<FinCgmR<T>>> f_x(ns);
-  std::vector<PtItem<PtCgmP<T>>> i_p(ns);
+  std::vector<PtItem<PtCgmRing<T>>> i_x(ns);
+  std::vector<FinItem<FinCgmRWin>> f_x(ns);
+  std::vector<PtItem<PtCgmFlushX<T>>> i_last(ns);
   for (int s = 0; s < ns; s++) {
     const Gate none = kNoGate;
-    i_start[s] = {PtCgmStart<T>{static_cast<const T*>(sys[s].b), rvec(s), pvec(s), ptab(s), xtab(s), n_shift},
+    const long n = nsite[s];
+    const OpDev<LinksF64> od = make_opdev(sys[s].op, LinksF64{sys[s].op.u64});
+    i_start[s] = {PtCgmStart<T>{static_cast<const T*>(sys[s].b), ring(s), pbuf(s, 0), ptab(s), xtab(s), n_shift},
                   nsite[s], nb_pt[s], ws_of(s), none};
     f_start[s] = {FinCgmInit{ds + s}, nb_pt[s], ws_of(s), none};
-    i_st[s] = {geo[s], make_opdev(sys[s].op, LinksF64{sys[s].op.u64}), InPlain<T>{pvec(s)}, apvec(s),
-               EpiDotInOut<T>{}, ws_of(s), gate_of(s)};
+    i_st0[s] = {geo[s], od, InPlain<T>{pbuf(s, 0)}, apvec(s), EpiDotInOut<T>{}, ws_of(s), gate_of(s)};
+    // iteration k forms p^(k+1) from p^(k): buffer k & 1 -> (k + 1) & 1
+    for (int c = 0; c < 2; c++)
+      i_st[c][s] = {geo[s], od, InP{ring(s), n, pbuf(s, c), pbuf(s, c ^ 1), ds + s, nullptr, {}}, apvec(s),
+                    EpiDotInOut<T>{}, ws_of(s), gate_of(s)};
     f_st[s] = {FinCgmBeta<T>{ds + s}, nb_st[s], ws_of(s), gate_of(s)};
-    i_x[s] = {PtCgmX<T>{xtab(s), ptab(s), rvec(s), apvec(s), ds + s}, nsite[s], nb_pt[s], ws_of(s), gate_of(s)};
-    f_x[s] = {FinCgmR<T>{ds + s}, nb_pt[s], ws_of(s), gate_of(s)};
-    i_p[s] = {PtCgmP<T>{ptab(s), pvec(s), rvec(s), ds + s}, nsite[s], nb_pt[s], ws_of(s), gate_of(s)};
+    i_x[s] = {PtCgmRing<T>{ring(s), n, apvec(s), xtab(s), ptab(s), ds + s, dwin + s, {}, 0, 0, 0, 0, 0}, nsite[s],
+              nb_pt[s], ws_of(s), gate_of(s)};
+    f_x[s] = {FinCgmRWin{ds + s, dwin + s}, nb_pt[s], ws_of(s), gate_of(s)};
+    i_last[s] = {PtCgmFlushX<T>{ring(s), n, xtab(s), ptab(s), ds + s, dwin + s, 0, 0, 0}, nsite[s], nb_pt[s],
+                 ws_of(s), kNoGate};
   }
   BatchLaunch L_start = make_batch_launch(i_start, nb_pt);
   BatchLaunch F_start = make_batch_launch(f_start, std::vector<int>(ns, 1));
-  BatchLaunch L_st = make_batch_launch(i_st, nb_st);
+  BatchLaunch L_st0 = make_batch_launch(i_st0, nb_st);
+  BatchLaunch L_st[2] = {make_batch_launch(i_st[0], nb_st), make_batch_launch(i_st[1], nb_st)};
   BatchLaunch F_st = make_batch_launch(f_st, std::vector<int>(ns, 1));
   BatchLaunch L_x = make_batch_launch(i_x, nb_pt);
   BatchLaunch F_x = make_batch_launch(f_x, std::vector<int>(ns, 1));
-  BatchLaunch L_p = make_batch_launch(i_p, nb_pt);
+  BatchLaunch L_last = make_batch_launch(i_last, nb_pt);
 
-  auto launch_stencil_phase = [&]() {
-    hipLaunchKernelGGL((k_normal_batch<T, InPlain<T>, EpiDotInOut<T>>), dim3(L_st.nblk_total), dim3(kBlock), 0, st,
-                       L_st.items.as<NrmItem<T, InPlain<T>, EpiDotInOut<T>>>(), L_st.map());
-    GLX_CHECK_LAUNCH();
+  // Optional HIP-event timing of each phase: per slot of the two-deep
+  // submission pipeline, one pair of events around every phase launch of a
+  // group; a slot's events are read after its group is waited for.  Groups
+  // hold an even number of iterations: every group starts on p buffer 0.
+  const int group = 2 * ((std::max(1, opt.group) + 1) / 2);
+  const bool timing = time_phases && stats;
+  std::vector<hipEvent_t> tev;
+  if (timing) {
+    tev.resize(size_t(2) * group * 6);
+    for (auto& e : tev) GLX_HIP(hipEventCreate(&e));
+  }
+  int t_slot = 0, t_iter = 0;
+  auto tmark = [&](int phase, int edge) {
+    if (timing) GLX_HIP(hipEventRecord(tev[((size_t(t_slot) * group + t_iter) * 3 + phase) * 2 + edge], st));
+  };
+  double t_ms[3] = {0, 0, 0};
+  auto tcollect = [&](int slot, int n_it) {
+    if (!timing) return;
+    for (int it = 0; it < n_it; it++)
+      for (int ph = 0; ph < 2; ph++) {  // D^dag D, the residual pass (with the windows it applies)
+        float ms = 0.f;
+        const size_t e = ((size_t(slot) * group + it) * 3 + ph) * 2;
+        GLX_HIP(hipEventElapsedTime(&ms, tev[e], tev[e + 1]));
+        t_ms[ph] += ms;
+      }
+  };
+
+  auto launch_beta = [&]() {
     hipLaunchKernelGGL((k_finalize_batch<EpiDotInOut<T>::NR, FinCgmBeta<T>>), dim3(ns), dim3(kBlock), 0, st,
                        F_st.items.as<FinItem<FinCgmBeta<T>>>());
     GLX_CHECK_LAUNCH();
   };
+  // iteration at group position g: the residual pass (with the window that
+  // ended at the last iteration, if any), its finaliser, then D^dag D on
+  // p^(k+1) formed in the launch's prologue, and beta / zeta
   auto enq_iter = [&]() {
-    hipLaunchKernelGGL((k_point_batch<PtCgmX<T>>), dim3(L_x.nblk_total), dim3(kBlock), 0, st,
-                       L_x.items.as<PtItem<PtCgmX<T>>>(), L_x.map());
+    const int c = t_iter & 1;
+    tmark(1, 0);
+    hipLaunchKernelGGL((k_point_batch<PtCgmRing<T>>), dim3(L_x.nblk_total), dim3(kBlock), 0, st,
+                       L_x.items.as<PtItem<PtCgmRing<T>>>(), L_x.map());
     GLX_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_finalize_batch<1, FinCgmR<T>>), dim3(ns), dim3(kBlock), 0, st,
-                       F_x.items.as<FinItem<FinCgmR<T>>>());
+    tmark(1, 1);
+    hipLaunchKernelGGL((k_finalize_batch<1, FinCgmRWin>), dim3(ns), dim3(kBlock), 0, st,
+                       F_x.items.as<FinItem<FinCgmRWin>>());
     GLX_CHECK_LAUNCH();
-    hipLaunchKernelGGL((k_point_batch<PtCgmP<T>>), dim3(L_p.nblk_total), dim3(kBlock), 0, st,
-                       L_p.items.as<PtItem<PtCgmP<T>>>(), L_p.map());
+    tmark(0, 0);
+    hipLaunchKernelGGL((k_normal_batch<T, InP, EpiDotInOut<T>>), dim3(L_st[c].nblk_total), dim3(kBlock), 0, st,
+                       L_st[c].items.as<NrmP>(), L_st[c].map());
     GLX_CHECK_LAUNCH();
-    launch_stencil_phase();
+    tmark(0, 1);
+    launch_beta();
+    t_iter++;
   };
   // start: r = p = p_s = b, x_s = 0, |b|; then Ap, beta, zeta (the first
   // stencil of generic_cg_m.cpp:100-122 precedes the loop)
@@ -2501,11 +2851,13 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
   hipLaunchKernelGGL((k_finalize_batch<1, FinCgmInit>), dim3(ns), dim3(kBlock), 0, st,
                      F_start.items.as<FinItem<FinCgmInit>>());
   GLX_CHECK_LAUNCH();
-  launch_stencil_phase();
+  hipLaunchKernelGGL((k_normal_batch<T, InPlain<T>, EpiDotInOut<T>>), dim3(L_st0.nblk_total), dim3(kBlock), 0, st,
+                     L_st0.items.as<NrmItem<T, InPlain<T>, EpiDotInOut<T>>>(), L_st0.map());
+  GLX_CHECK_LAUNCH();
+  launch_beta();
 
-  const int group = std::max(1, opt.group);
   hipGraphExec_t exec = nullptr;
-  if (opt.use_graph) {
+  if (opt.use_graph && !timing) {
     hipGraph_t graph = nullptr;
     GLX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     for (int g = 0; g < group; g++) enq_iter();
@@ -2520,13 +2872,16 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
   GLX_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
   GLX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   auto cleanup = [&]() {
-    if (exec) (void)hipGraphExecDestroy(exec);
     (void)hipStreamSynchronize(st);
+    if (exec) (void)hipGraphExecDestroy(exec);
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
     (void)hipHostFree(pin);
+    for (auto& e : tev) (void)hipEventDestroy(e);
   };
   auto submit = [&](int slot) {
+    t_slot = slot;
+    t_iter = 0;
     if (exec) GLX_HIP(hipGraphLaunch(exec, st));
     else
       for (int g = 0; g < group; g++) enq_iter();
@@ -2545,10 +2900,15 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
         submit(cur ^ 1);
         groups++;
         GLX_HIP(hipEventSynchronize(ev[cur]));
+        tcollect(cur, group);
         bool all = true;
         for (int s = 0; s < ns && all; s++) all = pin[size_t(cur) * ns + s].done == 1;
-        if (all) break;
-        if (groups > max_groups) throw HipError("batched multishift CG made no progress");
+        if (all) {
+          GLX_HIP(hipEventSynchronize(ev[cur ^ 1]));  // the group queued ahead (its kernels return at once)
+          tcollect(cur ^ 1, group);
+          break;
+        }
+        if (groups > max_groups) throw HipError("batched multishift CG made no progress");  // cleanup() drains
         cur ^= 1;
       }
     }
@@ -2557,6 +2917,10 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
     throw;
   }
   cleanup();
+  // the x_s part of each system's open window
+  hipLaunchKernelGGL((k_point_batch<PtCgmFlushX<T>>), dim3(L_last.nblk_total), dim3(kBlock), 0, st,
+                     L_last.items.as<PtItem<PtCgmFlushX<T>>>(), L_last.map());
+  GLX_CHECK_LAUNCH();
 
   // per system and shift: |(A + sigma_k) x_k - b|^2 (generic_cg_m.cpp:271-282)
   {
@@ -2602,12 +2966,36 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
     r.bsqrt = hs[s].bsqrt;
     r.resSqmrhs.assign(rv.begin() + size_t(s) * n_shift, rv.begin() + size_t(s + 1) * n_shift);
   }
+  if (stats) {
+    BatchStats& b = *stats;
+    b = BatchStats{};
+    std::vector<CgmWin> hw(ns);
+    GLX_HIP(hipMemcpy(hw.data(), dwin, size_t(ns) * sizeof(CgmWin), hipMemcpyDeviceToHost));
+    for (int s = 0; s < ns; s++) {
+      const double V = double(nsite[s]);
+      const long long it = out[s].iter;
+      // windows applied in the loop, and the open one after it (x_s only)
+      const long long ur = hw[s].upd_r + hw[s].wlen, ux = hw[s].upd_x + hs[s].n_xupd, up = hw[s].upd_p;
+      b.iterations += it;
+      b.upd_x += ux;
+      b.upd_p += up;
+      // the first D^dag D reads p (64 B per site); the later ones also form
+      // p from r and the last p (96 B)
+      b.bytes_stencil += V * (64.0 + 96.0 * double(it - 1));
+      b.bytes_r += V * 48.0 * double(it);
+      b.bytes_shift += V * (16.0 * double(ur) + 48.0 * double(ux) + 16.0 * double(up));
+    }
+    b.ms_stencil = t_ms[0];
+    b.ms_r = t_ms[1];
+    b.ms_shift = t_ms[2];
+  }
   return out;
 }
 
 std::vector<SolveResult> dev_cg_m_batched(const std::vector<BatchSystem>& sys, int n_shift, const double* shifts,
-                                          int resid_freq_check, bool worst_first, const SolveOpts& o) {
-  return cg_m_batch_impl<c128>(sys, n_shift, shifts, resid_freq_check, worst_first, o);
+                                          int resid_freq_check, bool worst_first, const SolveOpts& o,
+                                          BatchStats* stats, bool time_phases) {
+  return cg_m_batch_impl<c128>(sys, n_shift, shifts, resid_freq_check, worst_first, o, stats, time_phases);
 }
 
 // ----------------------------------------------- even/odd (compacted) ---

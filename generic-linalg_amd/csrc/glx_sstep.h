@@ -47,6 +47,8 @@
 
 #include "glx_eo.h"
 
+#include <type_traits>
+
 namespace glx {
 
 constexpr int kSsNR = 12;  // partial sums per block
@@ -213,6 +215,21 @@ __device__ __forceinline__ c64 ss_comb(const float (&c)[5], const c64 (&y)[5], b
   if (full) s = s + c[1] * y[1] + c[2] * y[2] + c[4] * y[4];
   return s;
 }
+// The stream's three combinations with the coefficients' fixed structure
+// (every plan that runs the stencil has c_r = e3 + r1 e1 + r2 e2 + r4 e4 and
+// c_x on e0, e1, e3 only; a restart has c_r = e3, c_x = 0 and zero weights
+// on the vectors it did not write, which are finite leftovers):
+//   p = sum_j cp_j y_j,  r = y3 + cr1 y1 + cr2 y2 + cr4 y4,
+//   dx = cx0 y0 + cx1 y1 + cx3 y3
+__device__ __forceinline__ c64 ss_comb_p(const float (&c)[5], const c64 (&y)[5]) {
+  return c[0] * y[0] + c[3] * y[3] + c[1] * y[1] + c[2] * y[2] + c[4] * y[4];
+}
+__device__ __forceinline__ c64 ss_comb_r(const float (&c)[5], const c64 (&y)[5]) {
+  return y[3] + c[1] * y[1] + c[2] * y[2] + c[4] * y[4];
+}
+__device__ __forceinline__ c64 ss_comb_x(const float (&c)[5], const c64 (&y)[5]) {
+  return c[0] * y[0] + c[3] * y[3] + c[1] * y[1];
+}
 
 // The stencil steps of the march on the LDS rings (row slots s0 = y,
 // sm = y - 1, sp = y + 1; column indices as in the kernel's arrays).
@@ -285,6 +302,41 @@ __device__ __forceinline__ void ss_q2hop_eo(const T (*sP1)[W1], const C (*sLx)[W
                              sLx[s0][li - 1], sLy[s0][li], sLy[sm][li], e == 0);
 }
 
+// Accesses at a 32-bit byte offset from a wave-uniform base: the compiler
+// addresses them as scalar base + vector offset instead of forming 64-bit
+// addresses per lane.  The host launches the kernel only where every array it
+// touches stays below 4 GiB (kSsMaxSites), so the offsets never wrap.
+constexpr long kSsMaxSites = 1L << 28;  // links: 16 bytes per site (FP32)
+template <class T>
+__device__ __forceinline__ const T& ss_at(const void* base, unsigned off) {
+  return *reinterpret_cast<const T*>(static_cast<const char*>(base) + off);
+}
+template <class T>
+__device__ __forceinline__ T& ss_at(void* base, unsigned off) {
+  return *reinterpret_cast<T*>(static_cast<char*>(base) + off);
+}
+// A lattice row within 4 of the strip's own rows, -4 <= y < ly + 4 with
+// ly >= 8: one conditional add or subtract wraps it (wrap_coord's general
+// remainder is never needed here).
+__device__ __forceinline__ int ss_wrap_row(int y, int ly) { return y < 0 ? y + ly : (y >= ly ? y - ly : y); }
+// The links of sites s .. s + N - 1 (N = 1 or 2, s even when N = 2).
+template <int N>
+__device__ __forceinline__ void ss_links(const LinksF32& lk, unsigned s, LinksF32::Raw (&r)[N]) {
+#pragma unroll
+  for (int c = 0; c < N; c++) r[c] = ss_at<LinksF32::Raw>(lk.u, (s + c) * 16u);
+}
+template <int N>
+__device__ __forceinline__ void ss_links(const LinksF16& lk, unsigned s, LinksF16::Raw (&r)[N]) {
+  if constexpr (N == 2) {
+    const Pair2<HalfLink> h = ss_at<Pair2<HalfLink>>(lk.u, s * 8u);
+    const Pair2<float> c = ss_at<Pair2<float>>(lk.scale, s * 4u);
+    r[0] = LinksF16::Raw{h.a, c.a};
+    r[1] = LinksF16::Raw{h.b, c.b};
+  } else {
+    r[0] = LinksF16::Raw{ss_at<HalfLink>(lk.u, s * 8u), ss_at<float>(lk.scale, s * 4u)};
+  }
+}
+
 // EO = false: D^dag D on full-lattice vectors (eg.lx = eg.g.lx); EO = true:
 // m^2 - K_eo K_oe on compact even-site vectors (eg.g over lx / 2 columns,
 // eg.lx the full width for the links).
@@ -299,17 +351,25 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
   // links: columns x0-4 .. x0+NXB+2 (last slot unused); EO: full columns
   // 2 x0 - 7 .. 2 x0 + 2 NXB + 5 (last slot unused)
   constexpr int WL = EO ? 2 * NXB + 14 : NXB + 8;
-  constexpr int WQ = NXB + 6;     // qp, qr: x0-3 .. x0+NXB+2
-  constexpr int W1 = NXB + 4;     // P1: x0-2 .. x0+NXB+1
-  constexpr int W2 = NXB + 2;     // q1: x0-1 .. x0+NXB
-  __shared__ T sP[8][WP];
-  __shared__ T sR[8][WP];
-  __shared__ C sLx[8][WL];
-  __shared__ C sLy[8][WL];
-  __shared__ T sQp[8][WQ];
-  __shared__ T sQr[8][WQ];
-  __shared__ T sP1[8][W1];
-  __shared__ T sQ1[8][W2];
+  // qp, qr: x0-3 .. x0+NXB+2; P1: x0-2 .. x0+NXB+1; q1: x0-1 .. x0+NXB.  Their
+  // rows are padded to the width of p's, so that the rings of all vectors
+  // share one row stride: one vector address per ring slot then serves every
+  // ring (the rings' own offsets become immediates of the LDS instructions)
+  constexpr int WQ = WP, W1 = WP, W2 = WP;
+  static_assert(std::is_same<T, C>::value, "vector and link rings share one array");
+  // every ring in one array (rings of the vectors first, then the links'):
+  // one address per ring slot and lane, the rings at immediate offsets
+  constexpr int kVR = 6, kLR = 2;
+  constexpr int kRingT = kVR * 8 * WP + kLR * 8 * WL;
+  __shared__ T sRing[kRingT];
+  T (*const sP)[WP] = reinterpret_cast<T (*)[WP]>(sRing + 0 * 8 * WP);
+  T (*const sR)[WP] = reinterpret_cast<T (*)[WP]>(sRing + 1 * 8 * WP);
+  T (*const sQp)[WQ] = reinterpret_cast<T (*)[WQ]>(sRing + 2 * 8 * WP);
+  T (*const sQr)[WQ] = reinterpret_cast<T (*)[WQ]>(sRing + 3 * 8 * WP);
+  T (*const sP1)[W1] = reinterpret_cast<T (*)[W1]>(sRing + 4 * 8 * WP);
+  T (*const sQ1)[W2] = reinterpret_cast<T (*)[W2]>(sRing + 5 * 8 * WP);
+  C (*const sLx)[WL] = reinterpret_cast<C (*)[WL]>(sRing + kVR * 8 * WP);
+  C (*const sLy)[WL] = reinterpret_cast<C (*)[WL]>(sRing + kVR * 8 * WP + 8 * WL);
   __shared__ C sKeep[NXB];
 
   // a finished solve (or a pending reliable update): carry the state over
@@ -325,7 +385,9 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
   const int k0 = rd->k, ko = rd->ko;
   const int nblk = gridDim.x;
 
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  // the wave index is uniform across the wave: in a scalar register, every
+  // row, ring slot and row-range test below is scalar work
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   const int tile = xcd_tile(blockIdx.x, g.nblk);
   const bool active = tile < g.ntiles;
   const int tt = active ? tile : 0;  // grid padding only joins the sums
@@ -353,16 +415,16 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
   // the CPL sites of a lane as one access (16 bytes for CPL = 2)
   auto ldc = [](const c64* p, int j, c64 (&o)[CPL]) {
     if constexpr (CPL == 2) {
-      const Pair2<c64> t = ld2(p, j);
+      const Pair2<c64> t = ss_at<Pair2<c64>>(p, unsigned(j) * 8u);
       o[0] = t.a;
       o[1] = t.b;
     } else {
-      o[0] = p[j];
+      o[0] = ss_at<c64>(p, unsigned(j) * 8u);
     }
   };
   auto stc = [](c64* p, int j, const c64 (&v)[CPL]) {
-    if constexpr (CPL == 2) st2(p, j, v[0], v[1]);
-    else p[j] = v[0];
+    if constexpr (CPL == 2) ss_at<Pair2<c64>>(p, unsigned(j) * 8u) = Pair2<c64>{v[0], v[1]};
+    else ss_at<c64>(p, unsigned(j) * 8u) = v[0];
   };
 
   // the previous launch's sums
@@ -396,34 +458,31 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
   const int xhl = EO ? wrap_coord(2 * x0 - 7 + hl_c, lxf) : wrap_coord(x0 - 4 + hl_c, lx);
 
   c64 ry[5][CPL], rx[CPL];
-  LinkRegs<LK, EO ? 2 * CPL : CPL> lm;
+  typename LK::Raw lmr[EO ? 2 * CPL : CPL];
   c64 hy[5];
   typename LK::Raw lh;
+  // all five basis vectors and x are read in every launch: a restart (not
+  // `full`) weights the three it did not write by zero (ss_comb_*), which
+  // keeps one code path for both
   auto fetch = [&](int k) {
     const int yr = 4 * k - 4 + w;
-    const int j = wrap_coord(y0 + rrow(yr), ly) * lx + xm;
+    const int j = ss_wrap_row(y0 + rrow(yr), ly) * lx + xm;
     const bool own = yr >= 0 && yr < rows;
-    ldc(Y[0], j, ry[0]);
-    ldc(Y[3], j, ry[3]);
-    if (full) {
-      ldc(Y[1], j, ry[1]);
-      ldc(Y[2], j, ry[2]);
-      ldc(Y[4], j, ry[4]);
-      if (own) ldc(bf.x, j, rx);
-    }
-    if constexpr (EO) lm.fetch(op.links, wrap_coord(y0 + rrow(yr), ly) * lxf + 2 * xm);
-    else lm.fetch(op.links, j);
+#pragma unroll
+    for (int i = 0; i < 5; i++) ldc(Y[i], j, ry[i]);
+    if (own) ldc(bf.x, j, rx);
+    if constexpr (EO) ss_links(op.links, unsigned(ss_wrap_row(y0 + rrow(yr), ly) * lxf + 2 * xm), lmr);
+    else ss_links(op.links, unsigned(j), lmr);
     if (hp) {
-      const int jh = wrap_coord(y0 + rrow(4 * k - 4 + hp_row), ly) * lx + xhp;
-      hy[0] = Y[0][jh];
-      hy[3] = Y[3][jh];
-      if (full) {
-        hy[1] = Y[1][jh];
-        hy[2] = Y[2][jh];
-        hy[4] = Y[4][jh];
-      }
+      const int jh = ss_wrap_row(y0 + rrow(4 * k - 4 + hp_row), ly) * lx + xhp;
+#pragma unroll
+      for (int i = 0; i < 5; i++) hy[i] = ss_at<c64>(Y[i], unsigned(jh) * 8u);
     }
-    if (hl) lh = op.links.fetch(wrap_coord(y0 + rrow(4 * k - 4 + hl_row), ly) * lxf + xhl);
+    if (hl) {
+      typename LK::Raw t[1];
+      ss_links(op.links, unsigned(ss_wrap_row(y0 + rrow(4 * k - 4 + hl_row), ly) * lxf + xhl), t);
+      lh = t[0];
+    }
   };
 
   // prelude: the coefficients of this launch.  The first rows' loads are
@@ -504,23 +563,27 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
       {
         // S0: rows 4k-4 .. 4k-1
         const int yr = 4 * k - 4 + w;
-        const int j = wrap_coord(y0 + rrow(yr), ly) * lx + xm;
+        const int j = ss_wrap_row(y0 + rrow(yr), ly) * lx + xm;
         const bool own = yr >= 0 && yr < rows;
         const int sl = (yr + 8) & 7;
         c64 pv2[CPL], rv2[CPL], xv2[CPL];
 #pragma unroll
         for (int c = 0; c < CPL; c++) {
           const c64 yv[5] = {ry[0][c], ry[1][c], ry[2][c], ry[3][c], ry[4][c]};
-          pv2[c] = ss_comb(cp, yv, full);
-          rv2[c] = ss_comb(cr, yv, full);
-          if (full) xv2[c] = rx[c] + ss_comb(cx, yv, true);
+          pv2[c] = ss_comb_p(cp, yv);
+          rv2[c] = ss_comb_r(cr, yv);
           sP[sl][CPL * l + c + 4] = pv2[c];
           sR[sl][CPL * l + c + 4] = rv2[c];
         }
         if (own) {
+#pragma unroll
+          for (int c = 0; c < CPL; c++) {
+            const c64 yv[5] = {ry[0][c], ry[1][c], ry[2][c], ry[3][c], ry[4][c]};
+            xv2[c] = rx[c] + ss_comb_x(cx, yv);
+          }
           stc(N[0], j, pv2);
           stc(N[3], j, rv2);
-          if (full) stc(bf.x, j, xv2);
+          stc(bf.x, j, xv2);
         }
         // wave 3 stages row 4k-1 into the slot of row 4k-9, whose U_y the
         // first S4 row of this step still needs (for EO: at the full columns
@@ -529,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
 #pragma unroll
           for (int t = 0; t < 2 * CPL; t++) {
             C ux, uy;
-            LK::finish(lm.r[t], ux, uy);
+            LK::finish(lmr[t], ux, uy);
             ux = 0.5f * ux;
             uy = 0.5f * uy;
             const int li = 2 * CPL * l + t + 7;
@@ -541,7 +604,7 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
 #pragma unroll
           for (int c = 0; c < CPL; c++) {
             C ux, uy;
-            LK::finish(lm.r[c], ux, uy);
+            LK::finish(lmr[c], ux, uy);
             ux = 0.5f * ux;
             uy = 0.5f * uy;
             if (w == 3) sKeep[CPL * l + c] = sLy[sl][CPL * l + c + 4];
@@ -551,8 +614,8 @@ __global__ __launch_bounds__(kBlock) void k_sscg(EoGeom eg, OpDev<LK> op, SsBufs
         }
         if (hp) {
           const int hs = (4 * k - 4 + hp_row + 8) & 7;
-          sP[hs][hp_c] = ss_comb(cp, hy, full);
-          sR[hs][hp_c] = ss_comb(cr, hy, full);
+          sP[hs][hp_c] = ss_comb_p(cp, hy);
+          sR[hs][hp_c] = ss_comb_r(cr, hy);
         }
         if (hl) {
           const int hs = (4 * k - 4 + hl_row + 8) & 7;

@@ -19,7 +19,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
-LIB_PATH = os.path.join(PKG, "libglx.so")
+# GLX_LIBRARY names another build of the library (e.g. to compare two
+# versions of a kernel in one run); the default is the in-tree build.
+LIB_PATH = os.environ.get("GLX_LIBRARY") or os.path.join(PKG, "libglx.so")
 
 OP_KINDS = {
     "laplace_real": 0,
@@ -138,6 +140,10 @@ def lib():
                 f.restype = i
                 f.argtypes = [vp, vp, i, i, d, i, vp, vp, VP, IP]
         for t in ("r", "c"):
+            f = getattr(L, f"glx_minv_unpreconditioned_{t}")
+            f.restype = i
+            f.argtypes = [vp, vp, i, i, vp, vp, vp, VP, IP]
+        for t in ("r", "c"):
             f = getattr(L, f"glx_minv_vector_cg_m_{t}")
             f.restype = i
             f.argtypes = [vp, vp, i, i, i, i, d, vp, vp, vp, i, VP, IP]
@@ -215,6 +221,9 @@ def lib():
         L.glx_dev_staggered_grad.argtypes = [vp, d, vp, i, i, d, vp, vp, C.POINTER(d), C.POINTER(d), C.POINTER(d), IP]
         L.glx_dev_cg_m_batched.restype = i
         L.glx_dev_cg_m_batched.argtypes = [i, vp, i, vp, i, i, d, i, i, vp]
+        if hasattr(L, "glx_dev_cg_m_batched_ex"):
+            L.glx_dev_cg_m_batched_ex.restype = i
+            L.glx_dev_cg_m_batched_ex.argtypes = [i, vp, i, vp, i, i, d, i, i, i, vp, vp]
         L.glx_dev_solve_staggered_eo.restype = i
         L.glx_dev_solve_staggered_eo.argtypes = [vp, d, vp, vp, i, i, d, C.POINTER(d), IP]
         L.glx_dev_eo_prepare.restype = i
@@ -229,6 +238,9 @@ def lib():
         L.glx_autotune_normal.argtypes = [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(d), C.POINTER(d)]
         L.glx_dev_apply.restype = i
         L.glx_dev_apply.argtypes = [vp, i, d, i, i, i, vp, vp]
+        if hasattr(L, "glx_dev_time_stream"):
+            L.glx_dev_time_stream.restype = i
+            L.glx_dev_time_stream.argtypes = [i, C.c_size_t, i, C.POINTER(d), C.POINTER(d)]
         L.glx_dev_time_apply.restype = i
         L.glx_dev_time_apply.argtypes = [vp, i, d, i, i, vp, vp, i, C.POINTER(C.c_double)]
         L.glx_dev_alloc.restype = i
@@ -368,6 +380,39 @@ def solve(solver: str, op: str, b, x0=None, *, links=None, mass=0.0, lx=0, ly=0,
         rc = getattr(L, f"glx_minv_vector_{solver}_{t}")(
             x.ctypes.data, b.ctypes.data, b.size, max_iter, eps, cb, ep, C.byref(tx.v), C.byref(ci))
     _check(rc, f"minv_vector_{solver}")
+    return x, _info(ci, tx.text())
+
+
+# minv_inverter (generic_inverters.h:70-81)
+MINV = {"cg": 0, "cr": 1, "gcr": 2, "bicgstab": 3, "bicgstab_l": 4, "gmres": 5, "sor": 6, "minres": 7,
+        "invalid": -1}
+
+
+class InverterParams(C.Structure):
+    """minv_inverter_params (generic_inverters.h:86-105), field by field."""
+    _fields_ = [("tol", C.c_double), ("max_iters", C.c_int), ("restart", C.c_int), ("restart_freq", C.c_int),
+                ("sor_omega", C.c_double), ("minres_omega", C.c_double), ("bicgstabl_l", C.c_int)]
+
+
+def minv_unpreconditioned(solver: str, op: str, b, x0=None, *, links=None, mass=0.0, lx=0, ly=0, nc=1, tol=1e-10,
+                          max_iters=10000, restart=False, restart_freq=0, verb_level=0, prefix="", fmt="%g",
+                          callback=None, extra=None):
+    """The reference's dispatcher minv_unpreconditioned
+    (generic_inverter.cpp:18-188) with host vectors; `callback` / `extra`
+    as in solve()."""
+    real = op == "laplace_real"
+    dt = np.float64 if real else np.complex128
+    b = np.ascontiguousarray(b, dtype=dt)
+    x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=dt, copy=True)
+    ex = None if callback is not None else _Extra(op, links, mass, lx, ly, nc)
+    cb = callback if callback is not None else op_callback(op)
+    ep = extra if callback is not None else ex.ptr
+    prm = InverterParams(tol, max_iters, int(restart), restart_freq, 1.0, 1.0, 2)
+    tx = _Text(verb_level, prefix, fmt)
+    ci = InversionInfo()
+    f = lib().glx_minv_unpreconditioned_r if real else lib().glx_minv_unpreconditioned_c
+    _check(f(x.ctypes.data, b.ctypes.data, b.size, MINV[solver], C.byref(prm), cb, ep, C.byref(tx.v), C.byref(ci)),
+           "minv_unpreconditioned")
     return x, _info(ci, tx.text())
 
 
@@ -693,11 +738,22 @@ class BatchSystem(C.Structure):
     _fields_ = [("lat", C.c_void_p), ("mass", C.c_double), ("b_dev", C.c_void_p), ("x_dev", C.c_void_p)]
 
 
+class BatchStats(C.Structure):
+    _fields_ = [("bytes_stencil", C.c_double), ("bytes_r_pass", C.c_double), ("bytes_shift_pass", C.c_double),
+                ("ms_stencil", C.c_double), ("ms_r_pass", C.c_double), ("ms_shift_pass", C.c_double),
+                ("iterations", C.c_longlong), ("shift_updates_x", C.c_longlong), ("shift_updates_p", C.c_longlong)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 def dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=10000, eps=1e-10, worst_first=False,
-                     use_graph=True) -> list:
+                     use_graph=True, stats=False, time_phases=False):
     """Batched multishift CG on the staggered normal operator: systems is a
     list of (lattice, mass, b: DeviceArray, xs: list of DeviceArray), all with
-    the same shifts.  Returns one Info per system (resSqmrhs per shift)."""
+    the same shifts.  Returns one Info per system (resSqmrhs per shift); with
+    stats, (infos, dict of the per-phase algorithmic bytes and, with
+    time_phases, HIP-event times) from glx_dev_cg_m_batched_ex."""
     sh = np.ascontiguousarray(shifts, dtype=np.float64)
     n = sh.size
     ns = len(systems)
@@ -714,6 +770,12 @@ def dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=10000, eps=1
     for s in range(ns):
         infos[s].resSqmrhs = res[s].ctypes.data
         infos[s].n_rhs = n
+    if stats or time_phases:
+        st = BatchStats()
+        _check(lib().glx_dev_cg_m_batched_ex(ns, C.cast(arr, C.c_void_p), n, sh.ctypes.data, resid_freq_check,
+                                             max_iter, eps, int(worst_first), int(use_graph), int(time_phases),
+                                             C.cast(infos, C.c_void_p), C.byref(st)), "glx_dev_cg_m_batched_ex")
+        return [_info(infos[s], "", res[s]) for s in range(ns)], st.as_dict()
     _check(lib().glx_dev_cg_m_batched(ns, C.cast(arr, C.c_void_p), n, sh.ctypes.data, resid_freq_check, max_iter,
                                       eps, int(worst_first), int(use_graph), C.cast(infos, C.c_void_p)),
            "glx_dev_cg_m_batched")
@@ -731,6 +793,18 @@ def time_apply(lat, op: str, mass: float, out: DeviceArray, inp: DeviceArray, re
     _check(lib().glx_dev_time_apply(lat.h, OP_KINDS[op], mass, int(fp32), link_prec, out.ptr, inp.ptr, reps,
                                     C.byref(ms)), "glx_dev_time_apply")
     return ms.value
+
+
+STREAM_MODES = {"read": 0, "write": 1, "copy": 2, "read3_write1": 3}
+
+
+def time_stream(mode: str, nbytes: int, reps: int = 20):
+    """(ms per pass, bytes per pass) of a plain streaming kernel over a
+    footprint of nbytes (glx_dev_time_stream)."""
+    ms, by = C.c_double(0.0), C.c_double(0.0)
+    _check(lib().glx_dev_time_stream(STREAM_MODES[mode], nbytes, reps, C.byref(ms), C.byref(by)),
+           "glx_dev_time_stream")
+    return ms.value, by.value
 
 
 def time_cg_kernel(lat, mass: float, prec: int = 0, reps: int = 100):

@@ -238,7 +238,7 @@ int glx_unregister_operator(glx_matvec callback);
 
 /* ------------------------------------------------------------------------
  * Solvers, host pointers (generic_cg.h:18-23, generic_cg_m.h:16-17,
- * generic_bicgstab.h:17-21, generic_gcr.h:39-44).  _r: real double vectors;
+ * generic_bicgstab.h:17-21, generic_gcr.h:17-22).  _r: real double vectors;
  * _c: complex vectors as interleaved doubles.  verbosity may be NULL.
  * ---------------------------------------------------------------------- */
 int glx_minv_vector_cg_r(double* phi, double* phi0, int size, int max_iter, double eps, glx_matvec matrix_vector,
@@ -363,6 +363,25 @@ typedef struct {
 int glx_dev_cg_m_batched(int n_sys, const glx_batch_system* sys, int n_shift, const double* shifts,
                          int resid_freq_check, int max_iter, double eps, int worst_first, int use_graph,
                          glx_inversion_info* infos);
+/* The same solve, also reporting its work per phase of an iteration as
+ * algorithmic HBM bytes: the D^dag D launches (64 B per site: p and Ap,
+ * 32 B of links), the residual pass r += beta Ap (48 B per site) and the
+ * shift pass (x_s -= beta_s p_s with the old p_s, then p_s = zeta_s r +
+ * alpha_s p_s, p = r + alpha p: 48 B per site plus 48 per x_s and 16 per p_s
+ * update, counted over the shifts active in each iteration).  With
+ * time_phases the launches are issued directly (no graph) between HIP events
+ * and ms_* is each phase's total time; else ms_* = 0.  No reference
+ * counterpart. */
+typedef struct {
+  double bytes_stencil, bytes_r_pass, bytes_shift_pass;
+  double ms_stencil, ms_r_pass, ms_shift_pass;
+  long long iterations;     /* summed over the systems */
+  long long shift_updates_x; /* x_s updates summed over systems and iterations */
+  long long shift_updates_p; /* p_s updates */
+} glx_batch_stats;
+int glx_dev_cg_m_batched_ex(int n_sys, const glx_batch_system* sys, int n_shift, const double* shifts,
+                            int resid_freq_check, int max_iter, double eps, int worst_first, int use_graph,
+                            int time_phases, glx_inversion_info* infos, glx_batch_stats* stats);
 /* Mixed-precision CG (no reference counterpart): FP32 Krylov vectors with
  * FP32 (link_prec = 1) or FP16 + scale (link_prec = 2) links; FP64 true
  * residual r = b - A x and FP64 solution accumulation at every reliable
@@ -439,6 +458,7 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
  * initial default, -1, takes the fewest of 16/32/64/128 rows that keep the
  * grid within 512 blocks (64 lanes of 1 FP64 or 2 FP32 sites per block row;
  * compact rows of lx / 2 sites for the even/odd Schur operator), else 256.
+ * rows = -1 with a size removes that size's entry.
  * glx_get_normal_rows returns what a solve of that size would use (for
  * lx = ly = 0, the default itself).
  * The row-marching kernel needs lx % 64 == 0 (FP64) or lx % 128 == 0 (FP32,
@@ -462,6 +482,15 @@ int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int 
  * events on the stream the kernels run on. */
 int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int link_prec, void* out_dev,
                        const void* in_dev, int reps, double* ms_per_apply);
+
+/* The streaming ceiling the stencils are read against (no reference
+ * counterpart): average ms per pass of a plain kernel over a footprint of
+ * `bytes` (16-byte accesses per lane, HIP events around `reps` passes on the
+ * null stream): mode 0 reads it, 1 writes it, 2 copies half of it into the
+ * other half, 3 reads three quarters and writes the fourth (the stencils'
+ * 3 : 1 ratio of bytes read to bytes written).  bytes_per_pass (may be NULL)
+ * receives the bytes one pass moves. */
+int glx_dev_time_stream(int mode, size_t bytes, int reps, double* ms_per_pass, double* bytes_per_pass);
 
 /* Live timing (HIP events around `reps` back-to-back launches on one stream)
  * of the single-pass CG's fused iteration kernel on D^dag D of the lattice:

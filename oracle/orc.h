@@ -87,6 +87,10 @@ double orc_norm2sq_r(const double* a, int n);
 double orc_norm2sq_c(const orc_c* a, int n);
 double orc_diffnorm2sq_r(const double* a, const double* b, int n);
 double orc_diffnorm2sq_c(const orc_c* a, const orc_c* b, int n);
+/* long double accumulation, rounded once (no reference counterpart) */
+orc_c orc_dot_cx(const orc_c* a, const orc_c* b, int n);
+double orc_norm2sq_cx(const orc_c* a, int n);
+double orc_diffnorm2sq_cx(const orc_c* a, const orc_c* b, int n);
 
 /* ---- operators: the callback contract void op(T* lhs, T* rhs, void* extra) -- */
 typedef void (*orc_mv_r)(double* lhs, double* rhs, void* extra);

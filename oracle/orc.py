@@ -123,16 +123,18 @@ def orc_lib():
             build()
         lib = C.CDLL(path)
         vp, i, d = C.c_void_p, C.c_int, C.c_double
-        for nm in ("orc_cg_r", "orc_cg_c", "orc_bicgstab_r", "orc_bicgstab_c", "orc_gcr_r", "orc_gcr_c"):
+        for nm in ("orc_cg_r", "orc_cg_c", "orc_bicgstab_r", "orc_bicgstab_c", "orc_gcr_r", "orc_gcr_c",
+                   "orc_cg_cx", "orc_bicgstab_cx", "orc_gcr_cx"):
             f = getattr(lib, nm)
             f.restype = OrcInfo
             f.argtypes = [vp, vp, i, i, d, vp, vp, C.POINTER(OrcVerb), vp]
         for nm in ("orc_cg_restart_r", "orc_cg_restart_c", "orc_bicgstab_restart_r", "orc_bicgstab_restart_c",
-                   "orc_gcr_restart_r", "orc_gcr_restart_c"):
+                   "orc_gcr_restart_r", "orc_gcr_restart_c", "orc_cg_restart_cx", "orc_bicgstab_restart_cx",
+                   "orc_gcr_restart_cx"):
             f = getattr(lib, nm)
             f.restype = OrcInfo
             f.argtypes = [vp, vp, i, i, d, i, vp, vp, C.POINTER(OrcVerb)]
-        for nm in ("orc_cg_m_r", "orc_cg_m_c"):
+        for nm in ("orc_cg_m_r", "orc_cg_m_c", "orc_cg_m_cx"):
             f = getattr(lib, nm)
             f.restype = OrcInfo
             f.argtypes = [vp, vp, i, i, i, i, d, vp, vp, vp, i, C.POINTER(OrcVerb), vp]
@@ -148,6 +150,12 @@ def orc_lib():
         lib.orc_diffnorm2sq_c.argtypes = [vp, vp, i]
         lib.orc_dot_c.restype = C2
         lib.orc_dot_c.argtypes = [vp, vp, i]
+        lib.orc_dot_cx.restype = C2
+        lib.orc_dot_cx.argtypes = [vp, vp, i]
+        lib.orc_norm2sq_cx.restype = d
+        lib.orc_norm2sq_cx.argtypes = [vp, i]
+        lib.orc_diffnorm2sq_cx.restype = d
+        lib.orc_diffnorm2sq_cx.argtypes = [vp, vp, i]
         lib.orc_mt_seed.argtypes = [C.POINTER(OrcMT), C.c_uint32]
         lib.orc_mt_next.restype = C.c_uint32
         lib.orc_mt_next.argtypes = [C.POINTER(OrcMT)]
@@ -359,8 +367,13 @@ def eoprec_reconstruct(lhs_e, rhs_o, links, mass, lx, ly, stencil=False, impl="o
 # --------------------------------------------------------------------------
 # Solvers.
 def solve(solver: str, op: str, b, x0=None, *, links=None, mass=0.0, lx=0, ly=0, nc=1, max_iter=10000,
-          eps=1e-10, restart_freq=0, verb_level=0, prefix="", fmt="%g", impl="orc", hist=False):
-    """Run CG / BiCGStab / GCR (optionally restarted).  Returns (x, SolveInfo[, hist])."""
+          eps=1e-10, restart_freq=0, verb_level=0, prefix="", fmt="%g", impl="orc", hist=False,
+          reductions="serial"):
+    """Run CG / BiCGStab / GCR (optionally restarted).  Returns (x, SolveInfo[, hist]).
+
+    reductions="extended" (complex operators, impl "orc"): the same solver with
+    its dot products and norms accumulated in long double and rounded once
+    (orc_dot_cx), instead of the reference's serial double sums."""
     real = op == "laplace_real"
     dt = np.float64 if real else np.complex128
     b = np.ascontiguousarray(b, dtype=dt)
@@ -386,6 +399,10 @@ def solve(solver: str, op: str, b, x0=None, *, links=None, mass=0.0, lx=0, ly=0,
     ctx = OpContext(op, links, mass, lx, ly, nc)
     verb = OrcVerb(verb_level, prefix.encode(), fmt.encode(), C.cast(buf, C.c_void_p), cap, 0)
     sfx = "r" if real else "c"
+    if reductions == "extended":
+        if real:
+            raise ValueError("extended reductions exist for the complex solvers only")
+        sfx = "cx"
     h = np.zeros(max(max_iter, 1)) if hist else None
     if restart_freq > 0:
         f = getattr(lib, f"orc_{solver}_restart_{sfx}")
@@ -402,8 +419,9 @@ def solve(solver: str, op: str, b, x0=None, *, links=None, mass=0.0, lx=0, ly=0,
 
 
 def cg_m(op: str, b, shifts, *, links=None, mass=0.0, lx=0, ly=0, nc=1, resid_freq_check=10, max_iter=10000,
-         eps=1e-10, worst_first=False, verb_level=0, prefix="", fmt="%g", impl="orc"):
-    """Multishift CG.  Returns (X[n_shift, size], SolveInfo, resSqmrhs, shifts_after)."""
+         eps=1e-10, worst_first=False, verb_level=0, prefix="", fmt="%g", impl="orc", reductions="serial"):
+    """Multishift CG.  Returns (X[n_shift, size], SolveInfo, resSqmrhs, shifts_after).
+    reductions="extended": long double reductions, as in solve()."""
     real = op == "laplace_real"
     dt = np.float64 if real else np.complex128
     b = np.ascontiguousarray(b, dtype=dt)
@@ -428,7 +446,9 @@ def cg_m(op: str, b, shifts, *, links=None, mass=0.0, lx=0, ly=0, nc=1, resid_fr
     ctx = OpContext(op, links, mass, lx, ly, nc)
     verb = OrcVerb(verb_level, prefix.encode(), fmt.encode(), C.cast(buf, C.c_void_p), cap, 0)
     ptrs = (C.c_void_p * n)(*[X[k].ctypes.data for k in range(n)])
-    f = lib.orc_cg_m_r if real else lib.orc_cg_m_c
+    if reductions == "extended" and real:
+        raise ValueError("extended reductions exist for the complex solvers only")
+    f = lib.orc_cg_m_r if real else (lib.orc_cg_m_cx if reductions == "extended" else lib.orc_cg_m_c)
     r = f(ptrs, b.ctypes.data, n, size, resid_freq_check, max_iter, eps, sh.ctypes.data, ctx.fn, ctx.extra,
           int(worst_first), C.byref(verb), res.ctypes.data)
     text = C.string_at(C.cast(buf, C.c_void_p), verb.len).decode()
@@ -438,6 +458,8 @@ def cg_m(op: str, b, shifts, *, links=None, mass=0.0, lx=0, ly=0, nc=1, resid_fr
 # --------------------------------------------------------------------------
 # BLAS-1 and random inputs.
 def blas(which: str, a, b=None, impl="orc"):
+    """generic_vector.h's dot / norm2sq / diffnorm2sq (serial sums); impl
+    "orcx" is the long double variant (complex only)."""
     k = {"dot": 0, "norm2sq": 1, "diffnorm2sq": 2}[which]
     cplx = np.iscomplexobj(a)
     dt = np.complex128 if cplx else np.float64
@@ -449,6 +471,15 @@ def blas(which: str, a, b=None, impl="orc"):
         ref_lib().ref_blas(k, int(cplx), a.ctypes.data, b.ctypes.data, n, out.ctypes.data)
         return complex(out[0], out[1]) if (cplx and k == 0) else out[0]
     lib = orc_lib()
+    if impl == "orcx":
+        if not cplx:
+            raise ValueError("the long double reductions are complex only")
+        if k == 0:
+            v = lib.orc_dot_cx(a.ctypes.data, b.ctypes.data, n)
+            return complex(v.re, v.im)
+        if k == 1:
+            return lib.orc_norm2sq_cx(a.ctypes.data, n)
+        return lib.orc_diffnorm2sq_cx(a.ctypes.data, b.ctypes.data, n)
     if k == 0:
         if cplx:
             v = lib.orc_dot_c(a.ctypes.data, b.ctypes.data, n)

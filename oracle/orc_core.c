@@ -118,6 +118,59 @@ double orc_diffnorm2sq_c(const orc_c* a, const orc_c* b, int n)
   return s;
 }
 
+/* The same three reductions accumulated in long double with Neumaier's
+ * compensation (x86 80-bit: a product of two doubles carries 64 significant
+ * bits; the compensated sum keeps the error of millions of terms far below
+ * double's last place), rounded to double once: the correctly rounded
+ * reductions an FP64 solver approaches when its summation order is not the
+ * reference's serial one.  Not in the reference; the "extended" reduction
+ * variant of the complex solvers (orc_solvers.c). */
+typedef struct {
+  long double s, c;
+} orc_acc;
+static void acc_add(orc_acc* a, long double t)
+{
+  const long double u = a->s + t;
+  if (fabsl(a->s) >= fabsl(t)) a->c += (a->s - u) + t;
+  else a->c += (t - u) + a->s;
+  a->s = u;
+}
+static double acc_val(const orc_acc* a) { return (double)(a->s + a->c); }
+
+orc_c orc_dot_cx(const orc_c* a, const orc_c* b, int n)
+{
+  orc_acc re = {0.0L, 0.0L}, im = {0.0L, 0.0L};
+  for (int i = 0; i < n; i++) {
+    const long double ar = creal(a[i]), ai = cimag(a[i]), br = creal(b[i]), bi = cimag(b[i]);
+    acc_add(&re, ar * br);
+    acc_add(&re, ai * bi);
+    acc_add(&im, ar * bi);
+    acc_add(&im, -(ai * br));
+  }
+  return CMPLX(acc_val(&re), acc_val(&im));
+}
+double orc_norm2sq_cx(const orc_c* a, int n)
+{
+  orc_acc s = {0.0L, 0.0L};
+  for (int i = 0; i < n; i++) {
+    const long double ar = creal(a[i]), ai = cimag(a[i]);
+    acc_add(&s, ar * ar);
+    acc_add(&s, ai * ai);
+  }
+  return acc_val(&s);
+}
+double orc_diffnorm2sq_cx(const orc_c* a, const orc_c* b, int n)
+{
+  orc_acc s = {0.0L, 0.0L};
+  for (int i = 0; i < n; i++) {
+    const orc_c d = orc_sub(a[i], b[i]);
+    const long double dr = creal(d), di = cimag(d);
+    acc_add(&s, dr * dr);
+    acc_add(&s, di * di);
+  }
+  return acc_val(&s);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Lattice (lattice/lattice.h:52-139) for nd = 2: x fastest, colour fastest of all. */
 int orc_coord_to_index(const int* dims, int nc, const int* coord, int color)

@@ -70,3 +70,21 @@
 #define CONVABS(z) cabs(z)
 #define ABSV(z) cabs(z)
 #include "orc_solvers_tmpl.h"
+#undef SFX
+#undef DOT
+#undef NRM
+#undef DIFF
+
+/* ---- double _Complex, reductions in long double ----
+ * The complex solvers again with the dot products and norms accumulated in
+ * long double and rounded once (orc_dot_cx ...): every other operation, and
+ * its order, is the reference's.  This is the FP64 solver with correctly
+ * rounded reductions, the common target of implementations whose summation
+ * order differs from the reference's serial one (a GPU's tree reductions):
+ * at the 256^2 and 1024^2 staggered solves the serial sums move the
+ * reference's own solution by ~5e-12 from it.  Not in the reference. */
+#define SFX cx
+#define DOT orc_dot_cx
+#define NRM orc_norm2sq_cx
+#define DIFF orc_diffnorm2sq_cx
+#include "orc_solvers_tmpl.h"

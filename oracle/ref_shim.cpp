@@ -126,14 +126,22 @@ cmv complex_op(int opid) {
 }
 
 // Redirects std::cout into a string for the duration of a call.
+// Redirects std::cout into `out` for the duration of a call.  Inactive when
+// the solve prints nothing (verbosity NONE): then the call touches no global
+// state and calls from several threads may run at once.
 struct Capture {
   std::ostringstream ss;
-  std::streambuf* old;
+  std::streambuf* old = nullptr;
   std::ios_base::fmtflags flags;
   std::streamsize precision;
   char* out;
   int cap;
-  Capture(char* o, int c, int sci, int prec) : out(o), cap(c) {
+  bool active;
+  Capture(char* o, int c, int sci, int prec, bool on = true) : out(o), cap(c), active(on) {
+    if (!active) {
+      if (out && cap > 0) out[0] = 0;
+      return;
+    }
     old = std::cout.rdbuf(ss.rdbuf());
     flags = std::cout.flags();
     precision = std::cout.precision();
@@ -141,6 +149,7 @@ struct Capture {
     if (prec > 0) std::cout << std::setprecision(prec);
   }
   ~Capture() {
+    if (!active) return;
     std::cout.rdbuf(old);
     std::cout.flags(flags);
     std::cout.precision(precision);
@@ -302,7 +311,7 @@ int ref_solve(int solver, int opid, double* phi, double* phi0, int size, int max
   verb.precond_verb_prefix = "";
   inversion_info inf;
   {
-    Capture cap(out, outcap, sci, prec);
+    Capture cap(out, outcap, sci, prec, verb_level != 0);
     if (opid == OP_LAPLACE_REAL) {
       void (*f)(double*, double*, void*) = square_laplace;
       if (solver == 0)
@@ -348,7 +357,7 @@ int ref_cg_m(int opid, double* phi, double* phi0, int n_shift, int size, int res
   verb.precond_verb_prefix = "";
   inversion_info inf;
   {
-    Capture cap(out, outcap, sci, prec);
+    Capture cap(out, outcap, sci, prec, verb_level != 0);
     if (opid == OP_LAPLACE_REAL) {
       double** ph = new double*[n_shift];
       for (int n = 0; n < n_shift; n++) ph[n] = phi + (size_t)n * size;

@@ -33,3 +33,23 @@ def orc():
     import orc as o
     o.orc_lib()
     return o
+
+
+@pytest.fixture
+def rows_reset(glx, monkeypatch):
+    """Launch-shape table entries a test sets through glx.set_normal_rows are
+    removed afterwards and the default goes back to -1 (by block count), so
+    the tests after it run the library's own launch shapes."""
+    real = glx.set_normal_rows
+    touched = set()
+
+    def recording(lx, ly, esize, rows):
+        if lx:
+            touched.add((lx, ly, esize))
+        real(lx, ly, esize, rows)
+
+    monkeypatch.setattr(glx, "set_normal_rows", recording)
+    yield touched
+    for lx, ly, es in touched:
+        real(lx, ly, es, -1)
+    real(0, 0, 0, -1)

@@ -140,8 +140,13 @@ def test_normal_rows_default_without_device():
         assert lib.glx_get_normal_rows(lx, ly, es, C.byref(r)) == 0 and r.value == rows, (lx, ly, es, r.value)
     assert lib.glx_set_normal_rows(0, 0, 0, 16) == 0
     assert lib.glx_get_normal_rows(2048, 2048, 16, C.byref(r)) == 0 and r.value == 16
-    assert lib.glx_set_normal_rows(2048, 2048, 16, -1) == 1 and b"rows must be" in lib.glx_last_error()
+    assert lib.glx_set_normal_rows(2048, 2048, 16, -2) == 1 and b"rows must be" in lib.glx_last_error()
     assert lib.glx_set_normal_rows(0, 0, 0, -1) == 0
+    assert lib.glx_get_normal_rows(2048, 2048, 16, C.byref(r)) == 0 and r.value == 128
+    # a size's entry overrides the default until -1 removes it
+    assert lib.glx_set_normal_rows(2048, 2048, 16, 8) == 0
+    assert lib.glx_get_normal_rows(2048, 2048, 16, C.byref(r)) == 0 and r.value == 8
+    assert lib.glx_set_normal_rows(2048, 2048, 16, -1) == 0
     assert lib.glx_get_normal_rows(2048, 2048, 16, C.byref(r)) == 0 and r.value == 128
 
 

@@ -30,12 +30,6 @@ def expand(ve, lx, ly):
     return out
 
 
-@pytest.fixture
-def rows_reset(glx):
-    yield
-    glx.set_normal_rows(0, 0, 0, 16)
-
-
 @pytest.mark.parametrize("lx,ly", [(128, 8), (128, 12), (256, 72), (384, 40)])
 @pytest.mark.parametrize("rows", [8, 16, 64])
 def test_schur_operator_fp64(glx, orc, rows_reset, lx, ly, rows):
