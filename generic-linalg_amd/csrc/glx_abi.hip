@@ -1235,6 +1235,86 @@ int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, cons
   });
 }
 
+// ----------------------------------------------- allocation-free mixed CG --
+// The mixed-precision CG above, with every device buffer of the solve (state,
+// reduction partials, residual history, the FP32 Krylov basis or vectors, the
+// FP64 reliable-update vector) carved out of `scratch`, one per-GPU array of
+// kScratchBytes held for the life of the process.  The solve runs on the
+// calling thread's per-thread default stream and creates no stream, event,
+// graph or pinned host memory; it polls its state synchronously after each
+// group of iterations.  Calls on one GPU share its array: calls that overlap
+// on one GPU (other threads) race.  Sizes: at most kNoAllocSites lattice
+// sites (the 104 MiB of the s-step form at 1024 x 1024 fit) and
+// kNoAllocMaxIter iterations (the history buffer).
+namespace {
+constexpr size_t kScratchBytes = size_t(112) << 20;
+constexpr long kNoAllocSites = 1L << 20;
+constexpr int kNoAllocMaxIter = 1 << 18;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+struct ScratchAddr {
+  std::once_flag once;
+  void* ptr = nullptr;
+  hipError_t err = hipSuccess;
+};
+ScratchAddr g_scratch_addr[kNoAllocMaxDevices];
+
+// The device address of this device's `scratch`, looked up once per device
+// with that device current.  Host code uses only this address: the array's
+// own name is a host-side stand-in.
+void* scratch_on_current_device() {
+  int dev = 0;
+  GLX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) throw ArgError("device index beyond the allocation-free table");
+  ScratchAddr& a = g_scratch_addr[dev];
+  std::call_once(a.once, [&] { a.err = hipGetSymbolAddress(&a.ptr, HIP_SYMBOL(scratch)); });
+  if (a.err != hipSuccess || !a.ptr)
+    throw HipError(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(a.err));
+  return a.ptr;
+}
+}  // namespace
+
+int glx_dev_cg_mixed_noalloc(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev,
+                             int max_iter, double eps, double delta, int link_prec, glx_verbose* verb,
+                             glx_inversion_info* info) {
+  return guarded([&] {
+    if (!x_dev || !b_dev) throw ArgError("x_dev and b_dev must not be NULL");
+    if (!(delta > 0.0 && delta < 1.0)) throw ArgError("delta must be in (0, 1)");
+    DevOperator o = lattice_op(lat, kind, mass, 1);
+    if (o.lx <= 0) throw ArgError("operator without links needs a lattice for its dimensions");
+    if (op_links(kind)) {
+      if (link_prec == LINKS_F16 && !o.u16) throw ArgError("lattice was created without GLX_LINKS_F16");
+      if (link_prec != LINKS_F16 && !o.u32) throw ArgError("lattice was created without GLX_LINKS_F32");
+    }
+    if (long(o.size()) > kNoAllocSites)
+      throw ArgError("glx_dev_cg_mixed_noalloc: " + std::to_string(o.size()) + " sites exceed its maximum of " +
+                     std::to_string(kNoAllocSites) + " (1024 x 1024); use glx_dev_cg_mixed");
+    if (max_iter > kNoAllocMaxIter)
+      throw ArgError("glx_dev_cg_mixed_noalloc: max_iter beyond its maximum of " + std::to_string(kNoAllocMaxIter));
+    void* base = scratch_on_current_device();
+    Printer pr(verb);
+    SolveOpts so;
+    so.max_iter = max_iter;
+    so.eps = eps;
+    so.want_hist = pr.level() == GLX_VERB_DETAIL;
+    so.group = g_group;
+    so.use_graph = false;
+    so.stream = hipStreamPerThread;
+    MixedOpts mo;
+    mo.link_prec = link_prec == LINKS_F16 ? LINKS_F16 : LINKS_F32;
+    mo.delta = delta;
+    SolveResult r;
+    {
+      NoAllocScope scope(base, kScratchBytes, hipStreamPerThread);
+      r = dev_cg_mixed(o, x_dev, b_dev, so, mo);
+    }
+    for (int k = 0; k < int(r.relres_hist.size()); k++) pr.resid("CG-mixed", k + 1, 2 + k, r.relres_hist[k]);
+    pr.summary("CG-mixed", r.success, r.iter, r.ops_count, std::sqrt(r.resSq) / r.bsqrt);
+    fill_info(info, r, "CG-mixed");
+  });
+}
+
 int glx_dev_staggered_grad(glx_lattice lat, double mass, const double* phi_dev, int precision, int max_iter,
                            double eps, double* grad_dev, double* chi_dev, double* dS_dm, double* S, double* S_robust,
                            glx_inversion_info* info) {

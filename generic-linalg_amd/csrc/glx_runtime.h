@@ -31,24 +31,58 @@ class DevicePool {
   std::map<void*, size_t> live_;
 };
 
-// RAII device buffer from the pool.
+// ------------------------------------------------ allocation-free mode --
+// While a thread holds a NoAllocScope, the device buffers of its solves are
+// carved out of the scope's region (a caller-owned array, bump allocation,
+// all of it reused by the next scope), its solves run on the scope's stream,
+// and they create no stream, event, graph or pinned host memory.  Running
+// out of the region is an error, never a fallback to the pool.
+class NoAllocScope {
+ public:
+  NoAllocScope(void* base, size_t cap, hipStream_t stream);
+  ~NoAllocScope();
+  NoAllocScope(const NoAllocScope&) = delete;
+  NoAllocScope& operator=(const NoAllocScope&) = delete;
+  void* take(size_t bytes);
+  hipStream_t stream() const { return stream_; }
+  size_t used() const { return used_; }
+
+ private:
+  char* base_;
+  size_t cap_, used_ = 0;
+  hipStream_t stream_;
+  NoAllocScope* prev_;
+};
+// the calling thread's scope, or nullptr
+NoAllocScope* noalloc_scope();
+
+// RAII device buffer from the pool (or from the thread's NoAllocScope).
 class DevBuf {
  public:
   DevBuf() = default;
-  explicit DevBuf(size_t bytes) : bytes_(bytes) { p_ = bytes ? DevicePool::get().alloc(bytes) : nullptr; }
+  explicit DevBuf(size_t bytes) : bytes_(bytes) {
+    if (!bytes) return;
+    if (NoAllocScope* s = noalloc_scope()) {
+      p_ = s->take(bytes);
+      scoped_ = true;
+    } else {
+      p_ = DevicePool::get().alloc(bytes);
+    }
+  }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) { o.p_ = nullptr; }
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_), scoped_(o.scoped_) { o.p_ = nullptr; }
   DevBuf& operator=(DevBuf&& o) noexcept {
     reset();
     p_ = o.p_;
     bytes_ = o.bytes_;
+    scoped_ = o.scoped_;
     o.p_ = nullptr;
     return *this;
   }
   ~DevBuf() { reset(); }
   void reset() {
-    if (p_) DevicePool::get().release(p_);
+    if (p_ && !scoped_) DevicePool::get().release(p_);
     p_ = nullptr;
   }
   template <class T>
@@ -59,6 +93,7 @@ class DevBuf {
  private:
   void* p_ = nullptr;
   size_t bytes_ = 0;
+  bool scoped_ = false;  // carved out of a NoAllocScope: nothing to release
 };
 
 // Upper bound on blocks of any reducing launch (pointwise kernels use at most

@@ -68,6 +68,24 @@ int normal_rows_entry(int lx, int ly, int esize) {
   return it != t.rows.end() ? it->second : -1;
 }
 
+namespace {
+thread_local NoAllocScope* t_noalloc = nullptr;
+}
+NoAllocScope::NoAllocScope(void* base, size_t cap, hipStream_t stream)
+    : base_(static_cast<char*>(base)), cap_(cap), stream_(stream), prev_(t_noalloc) {
+  t_noalloc = this;
+}
+NoAllocScope::~NoAllocScope() { t_noalloc = prev_; }
+void* NoAllocScope::take(size_t bytes) {
+  const size_t at = (used_ + 255) & ~size_t(255);
+  if (at > cap_ || bytes > cap_ - at)
+    throw ArgError("the solve needs more than the " + std::to_string(cap_ >> 20) +
+                   " MiB of scratch memory the allocation-free entry point has");
+  used_ = at + bytes;
+  return base_ + at;
+}
+NoAllocScope* noalloc_scope() { return t_noalloc; }
+
 DevicePool& DevicePool::get() {
   static DevicePool* pool = new DevicePool;  // outlives every static user
   return *pool;
@@ -168,9 +186,15 @@ struct PinnedSlots {
     GLX_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
   }
 };
-static PinnedSlots& pinned() {
+static PinnedSlots& pinned_slots() {
   static thread_local PinnedSlots* s = new PinnedSlots;
   return *s;
+}
+// the pinned slots of the calling thread; an allocation-free solve never
+// reaches them (they are allocated on a thread's first use)
+static PinnedSlots& pinned() {
+  if (noalloc_scope()) throw std::logic_error("pinned host slots requested in allocation-free mode");
+  return pinned_slots();
 }
 
 // Per-solve workspace: device state + reduction space.
@@ -1727,14 +1751,16 @@ struct PtFastFinal {
   throw HipError(msg);
 }
 
-// The stream of a solve: the caller's (`want`), else one created for the
-// solve and destroyed after it.
+// The stream of a solve: the caller's (`want`), else the allocation-free
+// scope's, else one created for the solve and destroyed after it.
 struct OwnedStream {
   hipStream_t st = nullptr;
   bool owned = false;
   explicit OwnedStream(hipStream_t want = nullptr) {
     if (want) {
       st = want;
+    } else if (NoAllocScope* s = noalloc_scope()) {
+      st = s->stream();
     } else {
       GLX_HIP(hipStreamCreate(&st));
       owned = true;
@@ -1817,9 +1843,11 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
   h.eps = opt.eps;
   h.delta = mo.delta;
   h.hist = w.hist.as<double>();
-  PinnedSlots* pin = &pinned();
-  __atomic_store_n(pin->hdone, 0, __ATOMIC_RELEASE);
-  h.hdone = pin->hdone_dev;
+  // allocation-free mode: no pinned slots; the host reads the state after
+  // each group instead (synchronously)
+  PinnedSlots* pin = noalloc_scope() ? nullptr : &pinned();
+  if (pin) __atomic_store_n(pin->hdone, 0, __ATOMIC_RELEASE);
+  h.hdone = pin ? pin->hdone_dev : nullptr;
   w.upload(h);
   std::vector<DevBuf> keep;
   auto vec = [&](size_t bytes) {
@@ -1901,7 +1929,18 @@ SolveResult fastcg_impl(const DevOperator& o, TD* x, const TD* b, const SolveOpt
   // copies the state back after each group.
   const long max_groups = 4L * (opt.max_iter / group + 2) + 16;
   long groups = 0;
-  if (opt.max_iter > 0 && sf) {
+  if (opt.max_iter > 0 && !pin) {
+    // every group ends on the home copy of the state
+    for (;;) {
+      launch();
+      groups++;
+      Mirror mm;
+      GLX_HIP(hipMemcpyAsync(&mm, w.ds, sizeof(Mirror), hipMemcpyDeviceToHost, st));
+      GLX_HIP(hipStreamSynchronize(st));
+      if (mm.done == 1) break;
+      if (groups > max_groups) fail_drained(st, exec, "single-pass CG made no progress");
+    }
+  } else if (opt.max_iter > 0 && sf) {
     int cur = 0;
     auto submit = [&](int slot) {
       launch();
@@ -2137,9 +2176,9 @@ SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveO
   h.eps = opt.eps;
   h.delta = mo.delta;
   h.hist = w.hist.as<double>();
-  PinnedSlots* pin = &pinned();
-  __atomic_store_n(pin->hdone, 0, __ATOMIC_RELEASE);
-  h.hdone = pin->hdone_dev;
+  PinnedSlots* pin = noalloc_scope() ? nullptr : &pinned();  // as in fastcg_impl
+  if (pin) __atomic_store_n(pin->hdone, 0, __ATOMIC_RELEASE);
+  h.hdone = pin ? pin->hdone_dev : nullptr;
   w.upload(h);
   SsSpace sp(n, st);
   DevBuf t64buf(size_t(n) * sizeof(c128));
@@ -2184,7 +2223,17 @@ SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveO
   // flag into host memory
   const long max_groups = 4L * (opt.max_iter / group + 2) + 16;
   long groups = 0;
-  if (opt.max_iter > 0) {
+  if (opt.max_iter > 0 && !pin) {
+    for (;;) {
+      launch();
+      groups++;
+      Mirror mm;
+      GLX_HIP(hipMemcpyAsync(&mm, w.ds, sizeof(Mirror), hipMemcpyDeviceToHost, st));
+      GLX_HIP(hipStreamSynchronize(st));
+      if (mm.done == 1) break;
+      if (groups > max_groups) fail_drained(st, exec, "s-step CG made no progress");
+    }
+  } else if (opt.max_iter > 0) {
     int cur = 0;
     auto submit = [&](int slot) {
       launch();

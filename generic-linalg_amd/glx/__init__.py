@@ -201,6 +201,9 @@ def lib():
         L.glx_dev_cg_m.argtypes = [vp, i, d, i, vp, vp, i, vp, i, i, d, i, VP, IP]
         L.glx_dev_cg_mixed.restype = i
         L.glx_dev_cg_mixed.argtypes = [vp, i, d, vp, vp, i, d, d, i, VP, IP]
+        if hasattr(L, "glx_dev_cg_mixed_noalloc"):  # absent from builds that predate it
+            L.glx_dev_cg_mixed_noalloc.restype = i
+            L.glx_dev_cg_mixed_noalloc.argtypes = [vp, i, d, vp, vp, i, d, d, i, VP, IP]
         L.glx_dev_cg_fused.restype = i
         L.glx_dev_cg_fused.argtypes = [vp, i, d, vp, vp, i, d, VP, IP]
         L.glx_set_tuning.restype = i
@@ -578,6 +581,17 @@ def dev_cg_mixed(lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_
     ci = InversionInfo()
     _check(lib().glx_dev_cg_mixed(lat.h, OP_KINDS[op], mass, x.ptr, b.ptr, max_iter, eps, delta, link_prec,
                                   C.byref(tx.v), C.byref(ci)), "glx_dev_cg_mixed")
+    return _info(ci, tx.text())
+
+
+def dev_cg_mixed_noalloc(lat, op: str, mass: float, x: DeviceArray, b: DeviceArray, max_iter=100000, eps=1e-10,
+                         delta=0.1, link_prec=LINKS_F32, verb_level=0) -> Info:
+    """dev_cg_mixed without device allocation (glx_dev_cg_mixed_noalloc):
+    the library's fixed scratch array, at most 2^20 sites."""
+    tx = _Text(verb_level, "", "%g")
+    ci = InversionInfo()
+    _check(lib().glx_dev_cg_mixed_noalloc(lat.h, OP_KINDS[op], mass, x.ptr, b.ptr, max_iter, eps, delta, link_prec,
+                                          C.byref(tx.v), C.byref(ci)), "glx_dev_cg_mixed_noalloc")
     return _info(ci, tx.text())
 
 

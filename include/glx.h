@@ -390,6 +390,17 @@ int glx_dev_cg_m_batched_ex(int n_sys, const glx_batch_system* sys, int n_shift,
  * residual is below eps. */
 int glx_dev_cg_mixed(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev, int max_iter,
                      double eps, double delta, int link_prec, glx_verbose* verbosity, glx_inversion_info* info);
+/* glx_dev_cg_mixed without any allocation (no reference counterpart: for
+ * callers that cannot allocate -- serving loops, harnesses that forbid
+ * hipMalloc): the same solve, bit for bit, with every device buffer taken
+ * from a fixed 112 MiB array per GPU that the library holds for the life of
+ * the process, on the calling thread's per-thread default stream, creating
+ * no stream, event, graph or pinned host memory.  At most 2^20 lattice sites
+ * (1024 x 1024) and 2^18 iterations; larger sizes return GLX_ERR_ARG.  Calls
+ * on one GPU share its array: calls that overlap on one GPU race. */
+int glx_dev_cg_mixed_noalloc(glx_lattice lat, int kind, double mass, double* x_dev, const double* b_dev,
+                             int max_iter, double eps, double delta, int link_prec, glx_verbose* verbosity,
+                             glx_inversion_info* info);
 /* FP64 CG in single-pass form (no reference counterpart as such): the same
  * algorithm as GLX_SOLVER_CG with one fused stencil launch per iteration
  * (|r_{k+1}|^2 from the recurrence), so the rounding -- not the method --
