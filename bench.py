@@ -384,19 +384,26 @@ def batch_multishift(glx, n_lat=48, rhs=4, seq=True, keep_host=True):
     _, tp = glx.dev_cg_m_batched(systems, shifts, resid_freq_check=10, max_iter=100000, eps=1e-10,
                                  time_phases=True)
     total = st["bytes_stencil"] + st["bytes_r_pass"] + st["bytes_shift_pass"]
+    # two launches per iteration: D^dag D (forming p in its prologue) and the
+    # residual pass, which also applies each window's x_s, p_s updates
     phases = {}
-    for name, key in (("stencil", "stencil"), ("r_pass", "r_pass"), ("shift_pass", "shift_pass")):
-        by, ms = st["bytes_" + key], tp["ms_" + key]
+    for name, by, ms in (("stencil", st["bytes_stencil"], tp["ms_stencil"]),
+                         ("r_pass", st["bytes_r_pass"] + st["bytes_shift_pass"], tp["ms_r_pass"])):
         gbs = by / (ms * 1e-3) / 1e9 if ms > 0 else None
         phases[name] = {"GB": round(by / 1e9, 2), "ms": round(ms, 2), "GBs": None if gbs is None else round(gbs, 1),
                         "frac": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4)}
-    sp = phases["shift_pass"]
+    phases["r_pass"]["of_which_shift_updates_GB"] = round(st["bytes_shift_pass"] / 1e9, 2)
+    dom = max(phases, key=lambda k: phases[k]["ms"])
+    kernels = {"stencil": "k_normal_batch<InCgmP, EpiDotInOut> (D^dag D with p = r + alpha p in its prologue and "
+                          "<p, Ap> in its epilogue, one launch over all unconverged systems)",
+               "r_pass": "k_point_batch<PtCgmWindow> + k_point_batch<PtCgmRing> (at each window's end the x_s, "
+                         "p_s updates of every active shift; then r += beta Ap into the residual ring and |r|^2)"}
     out["roofline"] = {
-        "bound": "hbm", "kernel": "k_point_batch<PtCgmXP> (the shift pass: x_s, p_s updates of every active shift "
-                                  "and p, one launch over all unconverged systems)",
-        "achieved": sp["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sp["frac"],
-        "bytes_model": "per site and iteration: D^dag D 64 B; r pass 48 B; shift pass 48 B + 48 B per x_s update "
-                       "+ 16 B per p_s update (shifts active in that iteration)",
+        "bound": "hbm", "kernel": kernels[dom], "phase": dom,
+        "achieved": phases[dom]["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": phases[dom]["frac"],
+        "bytes_model": "per site and iteration: D^dag D 96 B (64 B in the first); residual pass 48 B; at each "
+                       "window's end (every 10 iterations and at every convergence check) 16 B per window term "
+                       "+ 48 B per x_s update + 16 B per p_s update, over the shifts active in the window",
         "shift_updates_x": st["shift_updates_x"], "shift_updates_p": st["shift_updates_p"],
         "phases": phases,
         "whole_batch_GBs": round(total / t_batch / 1e9, 1),
@@ -516,7 +523,12 @@ def traffic_from_profile(kernel: str, links: str):
 def kernel_only(glx, args):
     """One CG iteration kernel alone on the headline lattice, `reps` launches
     between HIP events on the kernel's stream (no solve, no autotuning): the
-    program a counter run profiles."""
+    program a counter run profiles.  'batch': the batched multishift call
+    alone (with its phase timing), for profiles of its kernels."""
+    if args.kernel_only == "batch":
+        out, _ = batch_multishift(glx, seq=False, keep_host=False)
+        print(json.dumps(out))
+        return 0
     L = args.L
     links, _ = make_inputs(L, args.sigma, seed=1337)
     lat = glx.Lattice(links, L, L, f32=True, f16=True)
@@ -553,10 +565,11 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
                          "for scaling rocprofv3 byte counters)")
-    ap.add_argument("--kernel-only", choices=["sscg", "fast32", "fast64"], default=None,
+    ap.add_argument("--kernel-only", choices=["sscg", "fast32", "fast64", "batch"], default=None,
                     help="only time one CG iteration kernel at --L (HIP events, --reps launches) and print its "
                          "line: the s-step launch, the FP32 one-iteration kernel or the FP64 single-pass kernel "
-                         "(for counter runs of that kernel alone)")
+                         "(for counter runs of that kernel alone); 'batch': only the batched multishift call and "
+                         "its phase timing, without the one-call-each comparison")
     ap.add_argument("--reps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -566,9 +579,6 @@ def main():
 
     import glx
 
-    if args.kernel_only:
-        return kernel_only(glx, args)
-
     if args.copy_mib > 0:
         n = args.copy_mib << 16  # complex128 elements (16 bytes)
         src = glx.DeviceArray(np.ones(n, np.complex128))
@@ -577,6 +587,9 @@ def main():
             glx.copy_device(dst, src)
         glx.synchronize()
         del src, dst
+
+    if args.kernel_only:
+        return kernel_only(glx, args)
 
     L, mass = args.L, args.mass
     V = L * L

@@ -68,9 +68,12 @@ struct BatchSystem {
   void* const* xs;
 };
 // Work and time of a batched solve, per phase of the iteration: the D^dag D
-// launches (64 B per site: p, Ap, 32 B of links), the residual pass (48 B:
-// r read and written, Ap read) and the shift pass (per site 48 B for r and p
-// plus 48 per x_s update and 16 per p_s update).  The ms_* fields are the
+// launches (96 B per site: r, the last p and 32 B of links read, p and Ap
+// written; 64 B in the first iteration) and the residual pass (48 B: r read
+// and written, Ap read).  bytes_shift counts the windows' x_s and p_s
+// updates, which the residual pass's phase applies in a launch of its own
+// (16 B per window term, 48 per x_s and 16 per p_s update); ms_shift stays
+// 0.  The ms_* fields are the
 // HIP-event time of each phase's launches when time_phases was asked for
 // (direct launches, no graph), else 0.
 struct BatchStats {

@@ -2481,51 +2481,125 @@ __device__ __forceinline__ void cgm_window_terms(const T* ring, long n, int kcur
   }
 }
 
-// First, if the last finaliser ended a window: x_s += ax p_s + sum ex_j r_j
-// for the shifts active before that check, p_s = cp p_s + sum dp_j r_j for
-// those still active.  Then r^(k+1) = r^(k) + beta Ap (generic_cg_m.cpp:423-426)
-// into the ring, and |r^(k+1)|^2.
+// If the last finaliser ended a window: x_s += ax p_s + sum ex_j r_j for the
+// shifts active before that check, p_s = cp p_s + sum dp_j r_j for those
+// still active (a launch that finds no ended window returns at once).  The
+// window's scalars and the shifts' vectors are staged in LDS once per block:
+// read there, every lane of a wave reads one address (a broadcast), where
+// loads from the state in global memory would be one vector load per lane
+// and coefficient.
+template <class T>
+struct PtCgmWindow {
+  static constexpr int NR = 0;
+  static constexpr int kU = 4;              // shifts whose loads are in flight together
+  static constexpr int kC = 2 * kWin + 2;   // per shift: ax, cp, ex[kWin], dp[kWin]
+  const T* ring;
+  long n;
+  T* const* xtab;
+  T* const* ptab;
+  const SolveState* s;
+  const CgmWin* w;
+  int kcur, flush, wlen, nx, np;
+  // the block's LDS tables (used through these functions, so that every
+  // access is known to be an LDS access)
+  __device__ static c128* lds_c() {
+    __shared__ c128 a[kMaxShift * kC];
+    return a;
+  }
+  __device__ static T** lds_x() {
+    __shared__ T* a[kMaxShift];
+    return a;
+  }
+  __device__ static T** lds_p() {
+    __shared__ T* a[kMaxShift];
+    return a;
+  }
+  __device__ void init() {
+    // (wave-uniform: the branches on them are scalar)
+    flush = __builtin_amdgcn_readfirstlane(w->flush);
+    if (!flush) return;
+    kcur = __builtin_amdgcn_readfirstlane(s->k);
+    wlen = __builtin_amdgcn_readfirstlane(w->wlen);
+    nx = __builtin_amdgcn_readfirstlane(s->n_xupd);
+    np = __builtin_amdgcn_readfirstlane(s->n_rem);
+    c128* lc = lds_c();
+    for (int t = threadIdx.x; t < nx * kC; t += blockDim.x) {
+      const int q = t / kC, c = t % kC;
+      lc[t] = c == 0 ? w->ax[q] : (c == 1 ? w->cp[q] : (c < 2 + kWin ? w->ex[q][c - 2] : w->dp[q][c - 2 - kWin]));
+    }
+    T** lx = lds_x();
+    T** lp = lds_p();
+    for (int q = threadIdx.x; q < nx; q += blockDim.x) {
+      const int v = s->vec[q];
+      lx[q] = xtab[v];
+      lp[q] = ptab[v];
+    }
+    __syncthreads();
+  }
+  // the window of shift slot q at one site (the loops run over the fixed
+  // window length: a term index known at compile time keeps rj in registers)
+  __device__ __forceinline__ void apply(int q, T ps, T xo, const T (&rj)[kWin], T* xp, T* pp) const {
+    const c128* c = lds_c() + q * kC;
+    T xa = xo + c[0] * ps;
+#pragma unroll
+    for (int j = 0; j < kWin; j++)
+      if (j < wlen) xa = xa + c[2 + j] * rj[j];
+    *xp = xa;
+    if (q < np) {
+      T pa = c[1] * ps;
+#pragma unroll
+      for (int j = 0; j < kWin; j++)
+        if (j < wlen) pa = pa + c[2 + kWin + j] * rj[j];
+      *pp = pa;
+    }
+  }
+  __device__ void operator()(int i, double*) {
+    if (!flush) return;
+    T rj[kWin];
+    cgm_window_terms(ring, n, kcur, wlen, i, rj);
+    T* const* sx = lds_x();
+    T* const* sp = lds_p();
+    // kU shifts at a time: one shift after another would wait for each
+    // shift's loads in turn
+    int q = 0;
+    for (; q + kU <= nx; q += kU) {
+      T* xv[kU];
+      T* pv[kU];
+      T ps[kU], xo[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        xv[u] = sx[q + u] + i;
+        pv[u] = sp[q + u] + i;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        ps[u] = *pv[u];
+        xo[u] = *xv[u];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) apply(q + u, ps[u], xo[u], rj, xv[u], pv[u]);
+    }
+    for (; q < nx; q++) apply(q, sp[q][i], sx[q][i], rj, sx[q] + i, sp[q] + i);
+  }
+};
+// r^(k+1) = r^(k) + beta Ap (generic_cg_m.cpp:423-426) into the residual
+// ring, and |r^(k+1)|^2 (after PtCgmWindow, which reads the window's slots;
+// slot k + 1 is not among them).
 template <class T>
 struct PtCgmRing {
   static constexpr int NR = 1;
   T* ring;
   long n;
   const T* Ap;
-  T* const* xtab;
-  T* const* ptab;
   const SolveState* s;
-  const CgmWin* w;
   typename Scal<T>::type be;
-  int kcur, flush, wlen, nx, np;
+  int kcur;
   __device__ void init() {
     be = as_scal<T>(s->beta);
     kcur = s->k;
-    flush = w->flush;
-    wlen = w->wlen;
-    nx = s->n_xupd;
-    np = s->n_rem;
   }
   __device__ void operator()(int i, double* part) {
-    const T ri = ring[(kcur % kRing) * n + i];
-    if (flush) {
-      T rj[kWin];
-      cgm_window_terms(ring, n, kcur, wlen, i, rj);
-      for (int q = 0; q < nx; q++) {
-        const int v = s->vec[q];
-        T* xv = xtab[v];
-        T* pv = ptab[v];
-        const T ps = pv[i];
-        T xa = xv[i] + w->ax[q] * ps;
-        for (int j = 0; j < wlen; j++) xa = xa + w->ex[q][j] * rj[j];
-        xv[i] = xa;
-        if (q < np) {
-          T pa = w->cp[q] * ps;
-          for (int j = 0; j < wlen; j++) pa = pa + w->dp[q][j] * rj[j];
-          pv[i] = pa;
-        }
-      }
-    }
-    const T rn = ri + scal(be, Ap[i]);
+    const T rn = ring[(kcur % kRing) * n + i] + scal(be, Ap[i]);
     ring[((kcur + 1) % kRing) * n + i] = rn;
     part[0] += norm2(rn);
   }
@@ -2553,7 +2627,9 @@ struct PtCgmFlushX {
     for (int q = 0; q < nx; q++) {
       const int v = s->vec[q];
       T xa = xtab[v][i] + w->ax[q] * ptab[v][i];
-      for (int j = 0; j < wlen; j++) xa = xa + w->ex[q][j] * rj[j];
+#pragma unroll
+      for (int j = 0; j < kWin; j++)
+        if (j < wlen) xa = xa + w->ex[q][j] * rj[j];
       xtab[v][i] = xa;
     }
   }
@@ -2574,7 +2650,7 @@ struct FinCgmRWin {
     const int k = s->k;
     if (s->hist) s->hist[k] = sqrt(rn) / s->bsqrt;
     const double tol = s->eps * s->bsqrt;
-    if (w->flush) {  // the previous window was applied by the residual pass that precedes this finaliser
+    if (w->flush) {  // the previous window was applied by this iteration's window launch
       w->flush = 0;
       w->wlen = 0;
       for (int q = 0; q < s->n_rem; q++) {
@@ -2741,12 +2817,18 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
   T** tab = tabs.as<T*>();
   std::vector<T*> htab(size_t(ns) * 2 * n_shift);
   std::vector<NGeom> geo(ns);
-  std::vector<int> nb_pt(ns), nb_st(ns);
+  std::vector<int> nb_pt(ns), nb_st(ns), nb_w(ns), nb_r(ns);
   int max_nb = 0;
   for (int s = 0; s < ns; s++) {
     const DevOperator& o = sys[s].op;
     geo[s] = make_ngeom(o.lx, o.ly, rows[s], normal_cpl<T>());
     nb_pt[s] = point_blocks(nsite[s]);
+    // the residual pass: 4 sites per thread, the window launch: 16 (a block
+    // of a converged system, or of a window launch with no ended window,
+    // costs a few dependent loads before it returns: fewer blocks make such
+    // launches cheap)
+    nb_r[s] = std::max(1, (nb_pt[s] + 3) / 4);
+    nb_w[s] = std::max(1, (nb_pt[s] + 15) / 16);
     nb_st[s] = geo[s].nblk;
     max_nb = std::max(max_nb, std::max(nb_pt[s], nb_st[s]));
     for (int k = 0; k < n_shift; k++) {
@@ -2808,6 +2890,7 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
   std::vector<NrmItem<T, InPlain<T>, EpiDotInOut<T>>> i_st0(ns);
   std::vector<NrmP> i_st[2] = {std::vector<NrmP>(ns), std::vector<NrmP>(ns)};
   std::vector<FinItem<FinCgmBeta<T>>> f_st(ns);
+  std::vector<PtItem<PtCgmWindow<T>>> i_w(ns);
   std::vector<PtItem<PtCgmRing<T>>> i_x(ns);
   std::vector<FinItem<FinCgmRWin>> f_x(ns);
   std::vector<PtItem<PtCgmFlushX<T>>> i_last(ns);
@@ -2824,9 +2907,10 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
       i_st[c][s] = {geo[s], od, InP{ring(s), n, pbuf(s, c), pbuf(s, c ^ 1), ds + s, nullptr, {}}, apvec(s),
                     EpiDotInOut<T>{}, ws_of(s), gate_of(s)};
     f_st[s] = {FinCgmBeta<T>{ds + s}, nb_st[s], ws_of(s), gate_of(s)};
-    i_x[s] = {PtCgmRing<T>{ring(s), n, apvec(s), xtab(s), ptab(s), ds + s, dwin + s, {}, 0, 0, 0, 0, 0}, nsite[s],
-              nb_pt[s], ws_of(s), gate_of(s)};
-    f_x[s] = {FinCgmRWin{ds + s, dwin + s}, nb_pt[s], ws_of(s), gate_of(s)};
+    i_w[s] = {PtCgmWindow<T>{ring(s), n, xtab(s), ptab(s), ds + s, dwin + s, 0, 0, 0, 0, 0}, nsite[s], nb_w[s],
+              ws_of(s), gate_of(s)};
+    i_x[s] = {PtCgmRing<T>{ring(s), n, apvec(s), ds + s, {}, 0}, nsite[s], nb_r[s], ws_of(s), gate_of(s)};
+    f_x[s] = {FinCgmRWin{ds + s, dwin + s}, nb_r[s], ws_of(s), gate_of(s)};
     i_last[s] = {PtCgmFlushX<T>{ring(s), n, xtab(s), ptab(s), ds + s, dwin + s, 0, 0, 0}, nsite[s], nb_pt[s],
                  ws_of(s), kNoGate};
   }
@@ -2835,7 +2919,8 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
   BatchLaunch L_st0 = make_batch_launch(i_st0, nb_st);
   BatchLaunch L_st[2] = {make_batch_launch(i_st[0], nb_st), make_batch_launch(i_st[1], nb_st)};
   BatchLaunch F_st = make_batch_launch(f_st, std::vector<int>(ns, 1));
-  BatchLaunch L_x = make_batch_launch(i_x, nb_pt);
+  BatchLaunch L_w = make_batch_launch(i_w, nb_w);
+  BatchLaunch L_x = make_batch_launch(i_x, nb_r);
   BatchLaunch F_x = make_batch_launch(f_x, std::vector<int>(ns, 1));
   BatchLaunch L_last = make_batch_launch(i_last, nb_pt);
 
@@ -2871,12 +2956,16 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
                        F_st.items.as<FinItem<FinCgmBeta<T>>>());
     GLX_CHECK_LAUNCH();
   };
-  // iteration at group position g: the residual pass (with the window that
-  // ended at the last iteration, if any), its finaliser, then D^dag D on
-  // p^(k+1) formed in the launch's prologue, and beta / zeta
+  // iteration at group position g: the window that ended at the last
+  // iteration (a launch that returns at once where none did), the residual
+  // pass and its finaliser, then D^dag D on p^(k+1) formed in the launch's
+  // prologue, and beta / zeta
   auto enq_iter = [&]() {
     const int c = t_iter & 1;
     tmark(1, 0);
+    hipLaunchKernelGGL((k_point_batch<PtCgmWindow<T>>), dim3(L_w.nblk_total), dim3(kBlock), 0, st,
+                       L_w.items.as<PtItem<PtCgmWindow<T>>>(), L_w.map());
+    GLX_CHECK_LAUNCH();
     hipLaunchKernelGGL((k_point_batch<PtCgmRing<T>>), dim3(L_x.nblk_total), dim3(kBlock), 0, st,
                        L_x.items.as<PtItem<PtCgmRing<T>>>(), L_x.map());
     GLX_CHECK_LAUNCH();

@@ -364,14 +364,17 @@ int glx_dev_cg_m_batched(int n_sys, const glx_batch_system* sys, int n_shift, co
                          int resid_freq_check, int max_iter, double eps, int worst_first, int use_graph,
                          glx_inversion_info* infos);
 /* The same solve, also reporting its work per phase of an iteration as
- * algorithmic HBM bytes: the D^dag D launches (64 B per site: p and Ap,
- * 32 B of links), the residual pass r += beta Ap (48 B per site) and the
- * shift pass (x_s -= beta_s p_s with the old p_s, then p_s = zeta_s r +
- * alpha_s p_s, p = r + alpha p: 48 B per site plus 48 per x_s and 16 per p_s
- * update, counted over the shifts active in each iteration).  With
- * time_phases the launches are issued directly (no graph) between HIP events
- * and ms_* is each phase's total time; else ms_* = 0.  No reference
- * counterpart. */
+ * algorithmic HBM bytes.  An iteration has two phases: the D^dag D launch,
+ * which forms p = r + alpha p in its prologue (96 B per site: r, the last p
+ * and 32 B of links read, p and Ap written; 64 B in the first iteration),
+ * and the residual pass r += beta Ap (48 B per site).  The shifts' x_s and
+ * p_s are updated window by window (up to 10 iterations at a time) in the
+ * residual pass's phase, by a launch of its own before r += beta Ap:
+ * bytes_shift_pass counts that work (16 B per site per window term read, 48
+ * per x_s update, 16 per p_s update), and its time is part of ms_r_pass
+ * (ms_shift_pass stays 0).  With time_phases the launches are issued
+ * directly (no graph) between HIP events and ms_* is each phase's total
+ * time; else ms_* = 0.  No reference counterpart. */
 typedef struct {
   double bytes_stencil, bytes_r_pass, bytes_shift_pass;
   double ms_stencil, ms_r_pass, ms_shift_pass;

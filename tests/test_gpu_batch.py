@@ -73,7 +73,11 @@ def test_batched_multishift_matches_oracle(glx, orc, use_graph):
 
 
 def test_batched_equals_single_solves(glx, orc):
-    """The batch gives each system what a solve of its own gives."""
+    """The batch gives each system what a solve of its own gives: the same
+    iterations and ops_count, and solutions equal to the rounding of their
+    different reduction orders (block counts) and of the batch's windowed
+    shift updates, which move x_k by ~1e-12 here (the bar of the oracle
+    comparisons, 1e-11)."""
     shapes = [(64, 64), (128, 128)]
     systems, ref = make_batch(glx, orc, shapes, rhs_per_lattice=1, seed=9)
     infos = glx.dev_cg_m_batched(systems, SHIFTS, max_iter=4000)
@@ -82,7 +86,7 @@ def test_batched_equals_single_solves(glx, orc):
         one = glx.dev_cg_m(lat, "staggered_normal", mass, ys, db, SHIFTS, max_iter=4000)
         assert (info.iter, info.ops_count) == (one.iter, one.ops_count)
         for k in range(SHIFTS.size):
-            assert relerr(xs[k].download(), ys[k].download()) < 1e-12
+            assert relerr(xs[k].download(), ys[k].download()) < 1e-11
 
 
 def test_batched_rejects_unsupported_lattice(glx, orc):

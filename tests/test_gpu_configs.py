@@ -14,21 +14,27 @@ the serial sums alone move the reference's solution by 5e-12 (256^2) and
 
   staggered 256^2, m = 0.05, theta ~ N(0, 0.4^2) (beta = 6.25):
       CG on D^dag D through the host-pointer C ABI (the drop-in): the
-      oracle's iteration count, ops_count and success, x within 1e-12;
+      oracle's iteration count, ops_count and success, x within 1e-12 of
+      the correctly reduced oracle's;
       BiCGStab on D: inside the oracle's own envelope under 1-ulp
       perturbations of b (generic_bicgstab.cpp:205-341 is not stable under
       rounding: the spread is measured here, not assumed).
   staggered 1024^2, m = 0.02, theta ~ N(0, 0.5^2) (beta = 4):
       FP64 CG (reference order) with the oracle's iterations and x within
-      1e-12; single-pass FP64 CG; the mixed-precision CG in both forms (two
-      iterations per launch and one) with FP32 and FP16 links, each with its
-      true residual recomputed by the oracle's FP64 operator and its error
-      against the oracle's FP64 solution printed and bounded.
+      1e-12 of the correctly reduced oracle's; single-pass FP64 CG; the
+      mixed-precision CG in both forms (two iterations per launch and one)
+      with FP32 and FP16 links, each with its true residual recomputed by
+      the oracle's FP64 operator and its error against the oracle's FP64
+      solution printed and bounded.
   multishift batch: one lattice for each L in {64, 128, 192, 256, 384, 512},
       m ~ U[0.01, 0.1], theta ~ N(0, 0.4^2), 4 right-hand sides each, the 12
       shifts sigma_k = 10^(-3 + 0.5 k), resid_freq_check 10, through one
       batched call; each of the 24 systems against its own oracle CG-M
-      (generic_cg_m.cpp:312-600).
+      (generic_cg_m.cpp:312-600): its iterations and ops_count, every x_k
+      within 1e-11 of the correctly reduced oracle's, every final |r_k|^2
+      within 10% of the reference's beyond the two oracles' spread; a
+      system whose limiting shift crosses the tolerance within rounding at
+      a check may stop one check apart (at most two of the 24).
 
 The oracle runs in threads (its C calls release the GIL), so the 1024^2 CG
 (~20 s on one core) and the 24 multishift solves fit the per-test limit.
@@ -244,8 +250,8 @@ def test_c4_batch_every_size_matches_oracle(glx, orc):
     orc_runs, ext_runs = runs[0::2], runs[1::2]
     tol = 1e-10
     n_exact = 0
-    for (lat, mass, db, xs), (links, b, m, L), info, (X, io, ro, so), (Xe, _, _, _) in zip(systems, ref, infos,
-                                                                                           orc_runs, ext_runs):
+    for (lat, mass, db, xs), (links, b, m, L), info, (X, io, ro, so), (Xe, ie, re, _) in zip(systems, ref, infos,
+                                                                                             orc_runs, ext_runs):
         assert np.array_equal(so, SHIFTS)
         assert info.resSq == 0.0 and info.name == "CG-M" and info.success == io.success
         bn2 = float(np.vdot(b, b).real)
@@ -253,17 +259,33 @@ def test_c4_batch_every_size_matches_oracle(glx, orc):
         errs = [relerr(xg[k], X[k]) for k in range(SHIFTS.size)]
         errs_e = [relerr(xg[k], Xe[k]) for k in range(SHIFTS.size)]
         d_se = [relerr(X[k], Xe[k]) for k in range(SHIFTS.size)]
-        print(f"L={L} m={m:.4f}: iter {info.iter} (oracle {io.iter}), max_k |x_k - x_ref,k|/|x_ref,k| = "
-              f"{max(errs):.2e}, vs extended {max(errs_e):.2e} (reference vs extended {max(d_se):.2e})")
+        kap = [(m * m + s + 4.0) / (m * m + s) for s in SHIFTS]
+        print(f"L={L} m={m:.4f}: iter {info.iter} (oracle {io.iter}, extended {ie.iter}), max_k |x_k - x_ref,k|"
+              f"/|x_ref,k| = {max(errs):.2e}, vs extended {max(errs_e):.2e} (reference vs extended {max(d_se):.2e})")
         if info.iter == io.iter:
             n_exact += 1
             assert info.ops_count == io.ops_count
             for k in range(SHIFTS.size):
-                # within 1e-11 of the correctly reduced oracle, and no farther
-                # from the reference than it is
-                assert errs_e[k] < 1e-11, (L, k, errs_e[k])
-                assert errs[k] < d_se[k] + 1e-11, (L, k, errs[k], d_se[k])
-                assert abs(info.resSqmrhs[k] - ro[k]) <= 0.1 * ro[k] + 1e-24 * bn2, (k, info.resSqmrhs[k], ro[k])
+                if ie.iter == io.iter:
+                    # within 1e-11 of the correctly reduced oracle, and no
+                    # farther from the reference than it is
+                    assert errs_e[k] < 1e-11, (L, k, errs_e[k])
+                    assert errs[k] < d_se[k] + 1e-11, (L, k, errs[k], d_se[k])
+                    # the final |r_k|^2 of a converged shift moves with the
+                    # rounding of the sums (the oracle's serial and correctly
+                    # rounded sums give values up to 1.8x apart on these
+                    # systems' shifts at L = 64 and 128): within 10% of the
+                    # reference's beyond that spread
+                    spread = abs(re[k] - ro[k])
+                    assert abs(info.resSqmrhs[k] - ro[k]) <= spread + 0.1 * ro[k] + 1e-24 * bn2, \
+                        (k, info.resSqmrhs[k], ro[k], re[k])
+                else:
+                    # the correctly reduced oracle stopped at another check
+                    # (its limiting shift crossed the tolerance within
+                    # rounding): the reference's own stop, bounded as below
+                    assert abs(ie.iter - io.iter) == 10, (L, ie.iter, io.iter)
+                    assert errs[k] < 2 * tol * kap[k], (L, k)
+                    assert np.sqrt(info.resSqmrhs[k] / bn2) <= tol * 1.01
         else:
             # CG-M tests convergence every resid_freq_check iterations: a
             # limiting shift whose residual reaches the tolerance within
@@ -274,13 +296,15 @@ def test_c4_batch_every_size_matches_oracle(glx, orc):
             first = info.resSqmrhs if info.iter < io.iter else ro
             assert np.sqrt(max(first) / bn2) >= 0.95 * tol, (L, np.sqrt(max(first) / bn2))
             for k in range(SHIFTS.size):
-                kap = (m * m + SHIFTS[k] + 4.0) / (m * m + SHIFTS[k])
-                assert errs[k] < 2 * tol * kap, (L, k)
+                assert errs[k] < 2 * tol * kap[k], (L, k)
                 assert np.sqrt(info.resSqmrhs[k] / bn2) <= tol * 1.01 and np.sqrt(ro[k] / bn2) <= tol * 1.01
     assert n_exact >= len(systems) - 2, n_exact
-    # the reported work: every system's iterations, and shift updates
-    # between the shifts' lower bound (all converge at the first check) and
-    # all 12 active in every iteration
+    # the reported work: every system's iterations; x_s updates (one per
+    # active shift and window) at most 12 per iteration, p_s updates at most
+    # as many; D^dag D 64 B per site in the first iteration, 96 B (p formed
+    # from r and the last p) in every later one
     assert st["iterations"] == sum(i.iter for i in infos)
     assert st["shift_updates_x"] <= 12 * st["iterations"] and st["shift_updates_p"] <= st["shift_updates_x"]
-    assert st["bytes_stencil"] == sum(64.0 * lat.lx * lat.ly * i.iter for (lat, _, _, _), i in zip(systems, infos))
+    assert st["bytes_stencil"] == sum(lat.lx * lat.ly * (64.0 + 96.0 * (i.iter - 1))
+                                      for (lat, _, _, _), i in zip(systems, infos))
+    assert st["bytes_r_pass"] == sum(48.0 * lat.lx * lat.ly * i.iter for (lat, _, _, _), i in zip(systems, infos))
