@@ -539,7 +539,8 @@ def kernel_only(glx, args):
     ms, bps = glx.time_cg_kernel(lat, args.mass, prec=prec, reps=args.reps)
     gbs = bps * L * L / (ms * 1e-3) / 1e9
     print(json.dumps({"kernel": args.kernel_only, "L": L, "links": args.links, "reps": args.reps,
-                      "sstep_rows": glx.get_sstep_rows(L, L), "us_per_launch": round(ms * 1e3, 3),
+                      "sstep_rows": glx.get_sstep_rows(L, L), "sstep_form": glx.get_sstep_form(),
+                      "band_rows": args.band_rows, "us_per_launch": round(ms * 1e3, 3),
                       "bytes_per_site": bps, "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}))
     return 0
 
@@ -571,6 +572,10 @@ def main():
                          "(for counter runs of that kernel alone); 'batch': only the batched multishift call and "
                          "its phase timing, without the one-call-each comparison")
     ap.add_argument("--reps", type=int, default=100)
+    ap.add_argument("--sstep-form", type=int, choices=[0, 1], default=None,
+                    help="s-step launch form on D^dag D: 0 blocks of four waves through LDS rings, 1 wave strips in "
+                         "registers (default: the library's)")
+    ap.add_argument("--band-rows", type=int, default=0, help="rows per wave of the wave-strip form (0: by size)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: the solver path has no multi-GPU code; "
@@ -587,6 +592,9 @@ def main():
             glx.copy_device(dst, src)
         glx.synchronize()
         del src, dst
+
+    if args.sstep_form is not None:
+        glx.set_sstep_form(args.sstep_form, args.band_rows)
 
     if args.kernel_only:
         return kernel_only(glx, args)

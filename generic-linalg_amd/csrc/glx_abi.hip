@@ -1392,6 +1392,15 @@ int glx_set_sstep_rows(int lx, int ly, int rows) {
     set_sstep_rows(lx, ly, rows < 0 ? -1 : rows);
   });
 }
+int glx_set_sstep_form(int form, int band_rows) {
+  return guarded([&] { set_sstep_form(form, band_rows); });
+}
+int glx_get_sstep_form(int* form) {
+  return guarded([&] {
+    if (!form) throw ArgError("form is NULL");
+    *form = sstep_form();
+  });
+}
 int glx_get_sstep_rows(int lx, int ly, int* rows) {
   return guarded([&] {
     if (!rows) throw ArgError("rows is NULL");

@@ -110,6 +110,12 @@ void set_sstep_rows(int lx, int ly, int rows);
 int sstep_rows_for(int lx, int ly);
 int sstep_rows_entry(int lx, int ly);  // the size's table entry, or -1 when it has none
 int sstep_rows_auto(int lx, int ly);
+// The form of the s-step launch on the full-lattice D^dag D: 0 blocks of four
+// waves marching a strip together through LDS rings, 1 every wave marching a
+// strip of its own in registers (band_rows rows per wave, 0: by size).
+void set_sstep_form(int form, int band_rows);
+int sstep_form();
+int sw_rows_for(int lx, int ly);
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m);
 
 // FP64 CG in the single-pass form the mixed solver uses (one fused stencil

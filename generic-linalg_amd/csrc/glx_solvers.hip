@@ -14,6 +14,7 @@
 // by the next kernel; the host reads the 16-byte control word once per chunk.
 #include "glx_solvers.h"
 #include "glx_sstep.h"
+#include "glx_sswave.h"
 
 #include <tuple>
 
@@ -2027,6 +2028,32 @@ void set_sstep(int s, int rows) {
 }
 int sstep_default() { return g_sstep.load(); }
 int sstep_rows() { return g_sstep_rows.load(); }
+namespace {
+std::atomic<int> g_sstep_form{0};
+std::atomic<int> g_sw_rows{0};  // 0: by lattice size
+}  // namespace
+void set_sstep_form(int form, int band_rows) {
+  if (form != 0 && form != 1) throw ArgError("s-step form must be 0 (blocks of four waves) or 1 (wave strips)");
+  if (band_rows < 0 || band_rows > kSwMaxRows)
+    throw ArgError("s-step band rows must be in [0, " + std::to_string(kSwMaxRows) + "]");
+  g_sstep_form.store(form);
+  g_sw_rows.store(band_rows);
+}
+int sstep_form() { return g_sstep_form.load(); }
+
+// Rows per wave of the wave-strip form: the fewest of the form 6 k - 8 (a
+// wave marches rows + 8 rows in rounds of six) that keep the waves within one
+// per SIMD (1024), at least 10 and at most 52.
+int sw_rows_for(int lx, int ly) {
+  const int forced = g_sw_rows.load();
+  if (forced > 0) return forced;
+  const int nstrip = (lx + kSwOwn - 1) / kSwOwn;
+  const int bands = std::max(1, 1024 / nstrip);
+  const int need = (ly + bands - 1) / bands;
+  int rows = 10;
+  while (rows < need && rows + 6 <= kSwMaxRows) rows += 6;
+  return rows;
+}
 void set_sstep_rows(int lx, int ly, int rows) {
   SsTable& t = ss_table();
   std::lock_guard<std::mutex> g(t.mu);
@@ -2080,6 +2107,22 @@ static EoGeom ss_geom(const DevOperator& o, int rows) {
   e.lx = o.lx;
   return e;
 }
+// The s-step launch of an operator: blocks of four waves marching a strip
+// together (k_sscg, `rows` rows per block), or, for the full-lattice D^dag D
+// in the wave-strip form, every wave its own strip (k_sscg_w).
+struct SsGeo {
+  bool wave = false;
+  EoGeom e;
+  SwGeom s;
+  int nblk() const { return wave ? s.nblk : e.g.nblk; }
+};
+static SsGeo ss_geo(const DevOperator& o, int rows) {
+  SsGeo g;
+  g.wave = o.kind == OP_STAG_NORMAL && sstep_form() == 1;
+  if (g.wave) g.s = make_swgeom(o.lx, o.ly, sw_rows_for(o.lx, o.ly));
+  else g.e = ss_geom(o, rows);
+  return g;
+}
 
 // setup: R0 of slot 0 = b - A x in FP32, with |b|^2 and |r|^2
 struct PtSsInit {
@@ -2129,10 +2172,14 @@ struct EpiSsTrue {
 
 // One s-step launch with the link storage of `link_prec`.
 template <class LK>
-static void launch_sscg_lk(hipStream_t st, const DevOperator& o, const LK& lk, const EoGeom& eg, SsBufs bf,
+static void launch_sscg_lk(hipStream_t st, const DevOperator& o, const LK& lk, const SsGeo& sg, SsBufs bf,
                            const SolveState* rd, SolveState* wr, int q, RedWs ws) {
   const OpDev<LK> od = make_opdev(o, lk);
-  if (o.kind == OP_STAG_EO_SCHUR)
+  const EoGeom& eg = sg.e;
+  if (sg.wave)
+    hipLaunchKernelGGL((k_sscg_w<LK, SolveState>), dim3(sg.s.nblk), dim3(kBlock), 0, st, sg.s, od, bf, rd, wr, q,
+                       ws);
+  else if (o.kind == OP_STAG_EO_SCHUR)
     hipLaunchKernelGGL((k_sscg<kSsCPLEo, true, LK, SolveState>), dim3(eg.g.nblk), dim3(kBlock), 0, st, eg, od, bf,
                        rd, wr, q, ws);
   else
@@ -2140,7 +2187,7 @@ static void launch_sscg_lk(hipStream_t st, const DevOperator& o, const LK& lk, c
                        rd, wr, q, ws);
   GLX_CHECK_LAUNCH();
 }
-static void launch_sscg(hipStream_t st, const DevOperator& o, int link_prec, const EoGeom& eg, SsBufs bf,
+static void launch_sscg(hipStream_t st, const DevOperator& o, int link_prec, const SsGeo& eg, SsBufs bf,
                         const SolveState* rd, SolveState* wr, int q, RedWs ws) {
   if (link_prec == LINKS_F16) {
     if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");
@@ -2167,9 +2214,9 @@ SolveResult sscg_impl(const DevOperator& o, c128* x, const c128* b, const SolveO
   const int n = o.size();
   OwnedStream own(opt.stream);
   hipStream_t st = own.st;
-  const EoGeom g = ss_geom(o, rows);
+  const SsGeo g = ss_geo(o, rows);
   // two buffers of the sums per block
-  Work w(st, std::max(2 * g.g.nblk, stencil_blocks(o.lx, o.ly)), opt.max_iter, opt.want_hist);
+  Work w(st, std::max(2 * g.nblk(), stencil_blocks(o.lx, o.ly)), opt.max_iter, opt.want_hist);
   SolveState h;
   std::memset(&h, 0, sizeof(h));
   h.max_iter = opt.max_iter;
@@ -2277,8 +2324,8 @@ double time_sscg_kernel_impl(const DevOperator& o, int link_prec, int reps, int 
   const int n = o.size();
   OwnedStream own;
   hipStream_t st = own.st;
-  const EoGeom g = ss_geom(o, rows);
-  Work w(st, std::max(2 * g.g.nblk, stencil_blocks(o.lx, o.ly)), 0, false);
+  const SsGeo g = ss_geo(o, rows);
+  Work w(st, std::max(2 * g.nblk(), stencil_blocks(o.lx, o.ly)), 0, false);
   SsSpace sp(n, st);
   GLX_HIP(hipMemsetAsync(sp.bf.v(0, 3), 0x3c, size_t(n) * sizeof(c64), st));
   DevBuf shadow_buf(sizeof(SolveState));

@@ -216,6 +216,11 @@ def lib():
         L.glx_set_sstep_rows.argtypes = [i, i, i]
         L.glx_get_sstep_rows.restype = i
         L.glx_get_sstep_rows.argtypes = [i, i, C.POINTER(i)]
+        if hasattr(L, "glx_set_sstep_form"):  # absent from builds that predate it
+            L.glx_set_sstep_form.restype = i
+            L.glx_set_sstep_form.argtypes = [i, i]
+            L.glx_get_sstep_form.restype = i
+            L.glx_get_sstep_form.argtypes = [C.POINTER(i)]
         L.glx_autotune_sstep.restype = i
         L.glx_autotune_sstep.argtypes = [vp, i, C.POINTER(i), C.POINTER(d)]
         L.glx_dev_time_cg_kernel.restype = i
@@ -644,6 +649,20 @@ def set_sstep(s: int = 2, rows: int = 0):
 def set_sstep_rows(lx: int, ly: int, rows: int):
     """Per-size entry: s-step rows per block, 0 = one-iteration form, -1 = remove."""
     _check(lib().glx_set_sstep_rows(lx, ly, rows), "glx_set_sstep_rows")
+
+
+def set_sstep_form(form: int = 0, band_rows: int = 0):
+    """The s-step launch on the full-lattice D^dag D: form 0 (default) blocks
+    of four waves marching a strip together through LDS rings, form 1 every
+    wave marching a strip of its own in registers (band_rows rows per wave,
+    0: by lattice size)."""
+    _check(lib().glx_set_sstep_form(form, band_rows), "glx_set_sstep_form")
+
+
+def get_sstep_form() -> int:
+    f = C.c_int(0)
+    _check(lib().glx_get_sstep_form(C.byref(f)), "glx_get_sstep_form")
+    return f.value
 
 
 def get_sstep_rows(lx: int, ly: int) -> int:

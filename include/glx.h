@@ -457,6 +457,14 @@ int glx_get_sstep(int* s, int* rows);
  * glx_get_sstep_rows returns what a solve of that size would use. */
 int glx_set_sstep_rows(int lx, int ly, int rows);
 int glx_get_sstep_rows(int lx, int ly, int* rows);
+/* The form of the s-step launch on the full-lattice D^dag D (no reference
+ * counterpart; the results agree to the rounding of the Gram sums): form 0
+ * (default) blocks of four waves march a strip of 128 columns together
+ * through LDS rings; form 1 every wave marches a strip of 56 columns of its
+ * own in registers, band_rows rows per wave (0: the fewest, at least 8, that
+ * keep the launch within 1024 waves). */
+int glx_set_sstep_form(int form, int band_rows);
+int glx_get_sstep_form(int* form);
 /* Autotuner for the mixed-precision CG on D^dag D of the lattice: times
  * `reps` iterations in the one-iteration form and in s-step form at 8, 16,
  * 32, 64 and 128 rows per block (and 256 where ly >= 2048), installs the
