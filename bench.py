@@ -539,7 +539,7 @@ def kernel_only(glx, args):
     ms, bps = glx.time_cg_kernel(lat, args.mass, prec=prec, reps=args.reps)
     gbs = bps * L * L / (ms * 1e-3) / 1e9
     print(json.dumps({"kernel": args.kernel_only, "L": L, "links": args.links, "reps": args.reps,
-                      "sstep_rows": glx.get_sstep_rows(L, L), "sstep_form": glx.get_sstep_form(),
+                      "sstep_rows": glx.get_sstep_rows(L, L), "sstep_form": glx.get_sstep_form_for(L, L),
                       "band_rows": args.band_rows, "us_per_launch": round(ms * 1e3, 3),
                       "bytes_per_site": bps, "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}))
     return 0
@@ -679,6 +679,7 @@ def main():
     # A r and twelve FP64 partial sums), HIP events around back-to-back
     # launches on its own stream; the one-iteration kernel beside it
     it_ms, it_bps = glx.time_cg_kernel(lat, mass, prec=2 + link_prec, reps=200)
+    ss_form = glx.get_sstep_form_for(L, L)  # the form the timed launch (and the solves) took
     it1_ms, it1_bps = glx.time_cg_kernel(lat, mass, prec=link_prec, reps=200)
     it64_ms, it64_bps = glx.time_cg_kernel(lat, mass, prec=0, reps=200)
     it_gbs = it_bps * V / (it_ms * 1e-3) / 1e9
@@ -741,8 +742,10 @@ def main():
         "speedup_vs_fp64": round(ms64 / msmx, 3),
         "speedup_basis": "plain FP64 GPU CG in the reference's order of operations (glx_dev_solve)",
         "speedup_vs_fp64_fused": round(msfu / msmx, 3),
-        "mixed_form": (f"s-step, two CG iterations per launch, {tuned_ss['rows']} rows per block "
-                       f"({'set' if args.sstep_rows > 0 else 'autotuned'})"
+        "mixed_form": (("s-step, two CG iterations per launch, "
+                        + ("wave strips (form 1)" if glx.get_sstep_form_for(L, L) == 1
+                           else f"{tuned_ss['rows']} rows per block (form 0)")
+                        + f" ({'set' if args.sstep_rows > 0 else 'autotuned'})")
                        if tuned_ss["rows"] > 0 else "one CG iteration per launch (autotuned)"),
         "mixed_single_pass_ms": round(msm1, 3),
         "mixed_single_pass_iters": infom1.iter,
@@ -765,13 +768,16 @@ def main():
                                        "GBs": round(it1_bps * V / (it1_ms * 1e-3) / 1e9, 1)},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_sscg<2, Links{args.links.upper()}> (s-step mixed CG launch = two CG iterations: "
-                      f"p, r, x from the previous basis + A p, A^2 p, A r + 12 partial sums)",
+            "kernel": (f"k_sscg_w<Links{args.links.upper()}> (s-step mixed CG launch = two CG iterations, wave-strip "
+                       f"form: p, r, x from the previous basis + A p, A^2 p, A r + 12 partial sums)"
+                       if ss_form == 1 else
+                       f"k_sscg<2, Links{args.links.upper()}> (s-step mixed CG launch = two CG iterations: "
+                       f"p, r, x from the previous basis + A p, A^2 p, A r + 12 partial sums)"),
             "achieved": round(it_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(it_gbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile("k_sscg", args.links),
+            "traffic": traffic_from_profile("k_sscg_w" if ss_form == 1 else "k_sscg", args.links),
             "bytes_per_launch": it_bps * V,
             "bytes_per_site": it_bps,
             "ms_per_launch": it_ms,

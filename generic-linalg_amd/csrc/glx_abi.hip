@@ -1401,6 +1401,19 @@ int glx_get_sstep_form(int* form) {
     *form = sstep_form();
   });
 }
+int glx_set_sstep_form_entry(int lx, int ly, int form) {
+  return guarded([&] {
+    if (lx <= 0 || ly <= 0) throw ArgError("bad lattice size");
+    set_sstep_form_entry(lx, ly, form);
+  });
+}
+int glx_get_sstep_form_for(int lx, int ly, int* form) {
+  return guarded([&] {
+    if (!form) throw ArgError("form is NULL");
+    if (lx <= 0 || ly <= 0) throw ArgError("bad lattice size");
+    *form = sstep_form_for(lx, ly);
+  });
+}
 int glx_get_sstep_rows(int lx, int ly, int* rows) {
   return guarded([&] {
     if (!rows) throw ArgError("rows is NULL");
@@ -1624,14 +1637,28 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
     mo.sstep = 2;
     const int forced = sstep_rows();
     const int saved = sstep_rows_entry(o.lx, o.ly);  // -1: none (erased again on error)
-    set_sstep(sstep_default(), 0);  // the table decides while timing
-    int best = 0;
+    const int forced_form = sstep_form(), saved_form = sstep_form_entry(o.lx, o.ly);
+    set_sstep(sstep_default(), 0);  // the tables decide while timing
+    set_sstep_form(-1, 0);
+    int best = 0, best_form = 0;
     double best_ms = 1e300;
+    auto restore = [&]() {
+      set_sstep(sstep_default(), forced);
+      set_sstep_form(forced_form, 0);
+    };
     try {
+      // candidates: (rows, form); the wave-strip form with the rows of the
+      // size's default block shape (rows only route the solve to s-step form)
+      std::vector<std::pair<int, int>> cand;
       for (int c : {0, 8, 16, 32, 64, 128, 256}) {
         if (c == 256 && o.ly < 2048) break;
         if (c > 0 && (o.lx % 128 != 0 || o.ly < 8 || o.ly % 4 != 0)) break;
+        cand.push_back({c, 0});
+      }
+      if (cand.size() > 1) cand.push_back({sstep_rows_auto(o.lx, o.ly), 1});
+      for (auto [c, f] : cand) {
         set_sstep_rows(o.lx, o.ly, c);
+        set_sstep_form_entry(o.lx, o.ly, f);
         auto time_one = [&]() {
           GLX_HIP(hipMemset(x.get(), 0, size_t(n) * sizeof(c128)));
           GLX_HIP(hipDeviceSynchronize());
@@ -1645,15 +1672,18 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
         if (t < best_ms) {
           best_ms = t;
           best = c;
+          best_form = f;
         }
       }
     } catch (...) {
-      set_sstep(sstep_default(), forced);
+      restore();
       set_sstep_rows(o.lx, o.ly, saved);
+      set_sstep_form_entry(o.lx, o.ly, saved_form);
       throw;
     }
-    set_sstep(sstep_default(), forced);
+    restore();
     set_sstep_rows(o.lx, o.ly, best);
+    set_sstep_form_entry(o.lx, o.ly, best_form);
     if (rows) *rows = best;
     if (ms_per_iteration) *ms_per_iteration = best_ms;
   });

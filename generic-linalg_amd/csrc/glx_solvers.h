@@ -112,9 +112,14 @@ int sstep_rows_entry(int lx, int ly);  // the size's table entry, or -1 when it 
 int sstep_rows_auto(int lx, int ly);
 // The form of the s-step launch on the full-lattice D^dag D: 0 blocks of four
 // waves marching a strip together through LDS rings, 1 every wave marching a
-// strip of its own in registers (band_rows rows per wave, 0: by size).
+// strip of its own in registers (band_rows rows per wave, 0: by size); -1
+// (default): per lattice size, from the table (set_sstep_form_entry, the
+// autotuner), else by size (sstep_form_for).
 void set_sstep_form(int form, int band_rows);
 int sstep_form();
+void set_sstep_form_entry(int lx, int ly, int form);  // -1 removes the entry
+int sstep_form_entry(int lx, int ly);                 // -1: none
+int sstep_form_for(int lx, int ly);
 int sw_rows_for(int lx, int ly);
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m);
 

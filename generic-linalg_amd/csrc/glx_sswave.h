@@ -190,47 +190,41 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
     ss_links<1>(op.links, j, d.lk);
   };
 
-  // the previous launch's sums
   constexpr int kPreU = 2;
   double pv[kPreU][kSsNR];
   const double* PS = ws.partials + (q ^ 1) * kSsNR * nblk;
-  if (full) {
+  // the previous launch's sums and the first rows: every load issued before
+  // any is waited for, and none under a branch (clamped indices, masked
+  // values), so that the wait for the sums leaves the rows in flight
 #pragma unroll
-    for (int u = 0; u < kPreU; u++) {
-      const int bb = threadIdx.x + u * kBlock;
+  for (int u = 0; u < kPreU; u++) {
+    const int bb = min(int(threadIdx.x) + u * kBlock, nblk - 1);
 #pragma unroll
-      for (int k = 0; k < kSsNR; k++) pv[u][k] = bb < nblk ? PS[k * nblk + bb] : 0.0;
-    }
+    for (int k = 0; k < kSsNR; k++) pv[u][k] = PS[k * nblk + bb];
   }
+  __builtin_amdgcn_sched_barrier(0);  // (keeps the sums' loads ahead of the rows')
   SwRow<LK> ld[kSwAhead];
+#pragma unroll
+  for (int a = 0; a < kSwAhead; a++) load_row(ld[a], a);  // (waves without a task load task 0's rows)
   __shared__ double sm[kSsNR][kWaves];
   SsPlan pl;
-  // prelude as in k_sscg: the first rows' loads are issued after the block
-  // sum of the partials, the coefficient algebra runs while they are in flight
   if (full) {
     double acc[kSsNR];
 #pragma unroll
-    for (int k = 0; k < kSsNR; k++) acc[k] = pv[0][k];
+    for (int k = 0; k < kSsNR; k++) acc[k] = int(threadIdx.x) < nblk ? pv[0][k] : 0.0;
 #pragma unroll
     for (int u = 1; u < kPreU; u++) {
+      const bool in = int(threadIdx.x) + u * kBlock < nblk;
 #pragma unroll
-      for (int k = 0; k < kSsNR; k++) acc[k] += pv[u][k];
+      for (int k = 0; k < kSsNR; k++) acc[k] += in ? pv[u][k] : 0.0;
     }
     for (int b0 = threadIdx.x + kBlock * kPreU; b0 < nblk; b0 += kBlock) {
 #pragma unroll
       for (int k = 0; k < kSsNR; k++) acc[k] += PS[k * nblk + b0];
     }
     block_sum_dpp<kSsNR>(acc, sm);
-    if (active) {
-#pragma unroll
-      for (int a = 0; a < kSwAhead; a++) load_row(ld[a], a);
-    }
     ss_iterate(acc, rd, k0, rd->maxrr, pl);
   } else {
-    if (active) {
-#pragma unroll
-      for (int a = 0; a < kSwAhead; a++) load_row(ld[a], a);
-    }
 #pragma unroll
     for (int i = 0; i < 5; i++) {
       pl.cp[i] = i == 3 ? 1.0 : (i == 0 ? rd->beta_r : 0.0);

@@ -221,6 +221,10 @@ def lib():
             L.glx_set_sstep_form.argtypes = [i, i]
             L.glx_get_sstep_form.restype = i
             L.glx_get_sstep_form.argtypes = [C.POINTER(i)]
+            L.glx_set_sstep_form_entry.restype = i
+            L.glx_set_sstep_form_entry.argtypes = [i, i, i]
+            L.glx_get_sstep_form_for.restype = i
+            L.glx_get_sstep_form_for.argtypes = [i, i, C.POINTER(i)]
         L.glx_autotune_sstep.restype = i
         L.glx_autotune_sstep.argtypes = [vp, i, C.POINTER(i), C.POINTER(d)]
         L.glx_dev_time_cg_kernel.restype = i
@@ -651,17 +655,28 @@ def set_sstep_rows(lx: int, ly: int, rows: int):
     _check(lib().glx_set_sstep_rows(lx, ly, rows), "glx_set_sstep_rows")
 
 
-def set_sstep_form(form: int = 0, band_rows: int = 0):
-    """The s-step launch on the full-lattice D^dag D: form 0 (default) blocks
-    of four waves marching a strip together through LDS rings, form 1 every
-    wave marching a strip of its own in registers (band_rows rows per wave,
-    0: by lattice size)."""
+def set_sstep_form(form: int = -1, band_rows: int = 0):
+    """The s-step launch on the full-lattice D^dag D: form 0 blocks of four
+    waves marching a strip together through LDS rings, form 1 every wave
+    marching a strip of its own in registers (band_rows rows per wave, 0: by
+    lattice size); -1 (default): per lattice size."""
     _check(lib().glx_set_sstep_form(form, band_rows), "glx_set_sstep_form")
 
 
 def get_sstep_form() -> int:
     f = C.c_int(0)
     _check(lib().glx_get_sstep_form(C.byref(f)), "glx_get_sstep_form")
+    return f.value
+
+
+def set_sstep_form_entry(lx: int, ly: int, form: int):
+    """Per-size entry of the s-step form (0 or 1), -1 = remove."""
+    _check(lib().glx_set_sstep_form_entry(lx, ly, form), "glx_set_sstep_form_entry")
+
+
+def get_sstep_form_for(lx: int, ly: int) -> int:
+    f = C.c_int(0)
+    _check(lib().glx_get_sstep_form_for(lx, ly, C.byref(f)), "glx_get_sstep_form_for")
     return f.value
 
 
@@ -672,12 +687,14 @@ def get_sstep_rows(lx: int, ly: int) -> int:
 
 
 def autotune_sstep(lat, reps: int = 40) -> dict:
-    """Time the mixed CG on this lattice in one-iteration form and in s-step
-    form at 8 / 16 / 32 / 64 / 128 rows per block (and 256 where ly >= 2048);
-    install and return the fastest."""
+    """Time the mixed CG on this lattice in one-iteration form, in s-step form
+    0 at 8 / 16 / 32 / 64 / 128 rows per block (and 256 where ly >= 2048) and
+    in s-step form 1 (wave strips); install and return the fastest (rows 0:
+    the one-iteration form; form: the s-step launch's form)."""
     r, ms = C.c_int(0), C.c_double(0.0)
     _check(lib().glx_autotune_sstep(lat.h, reps, C.byref(r), C.byref(ms)), "glx_autotune_sstep")
-    return {"rows": r.value, "ms_per_iteration": ms.value}
+    form = get_sstep_form_for(lat.lx, lat.ly) if hasattr(lib(), "glx_get_sstep_form_for") else 0
+    return {"rows": r.value, "form": form, "ms_per_iteration": ms.value}
 
 
 def get_sstep():

@@ -458,19 +458,26 @@ int glx_get_sstep(int* s, int* rows);
 int glx_set_sstep_rows(int lx, int ly, int rows);
 int glx_get_sstep_rows(int lx, int ly, int* rows);
 /* The form of the s-step launch on the full-lattice D^dag D (no reference
- * counterpart; the results agree to the rounding of the Gram sums): form 0
- * (default) blocks of four waves march a strip of 128 columns together
+ * counterpart; the results agree to the rounding of the hops and the Gram
+ * sums): form 0 blocks of four waves march a strip of 128 columns together
  * through LDS rings; form 1 every wave marches a strip of 56 columns of its
- * own in registers, band_rows rows per wave (0: the fewest, at least 8, that
- * keep the launch within 1024 waves). */
+ * own in registers, band_rows rows per wave (0: the fewest of 10, 16, ...,
+ * 52 that keep the launch within 1024 waves; at most 56).  form -1 (default)
+ * chooses per lattice size: the size's entry (glx_set_sstep_form_entry, set
+ * by glx_autotune_sstep), else form 1 from 512^2 sites up to lattices that
+ * fit 1024 waves at bands of 52 rows, form 0 otherwise.
+ * glx_get_sstep_form_for returns what a solve of that size would use. */
 int glx_set_sstep_form(int form, int band_rows);
 int glx_get_sstep_form(int* form);
+int glx_set_sstep_form_entry(int lx, int ly, int form);
+int glx_get_sstep_form_for(int lx, int ly, int* form);
 /* Autotuner for the mixed-precision CG on D^dag D of the lattice: times
- * `reps` iterations in the one-iteration form and in s-step form at 8, 16,
- * 32, 64 and 128 rows per block (and 256 where ly >= 2048), installs the
- * fastest in the table above and
- * returns it (0 = one-iteration form) with its time per iteration in ms.
- * Needs FP32 links.  No reference counterpart. */
+ * `reps` iterations in the one-iteration form, in s-step form 0 at 8, 16,
+ * 32, 64 and 128 rows per block (and 256 where ly >= 2048) and in s-step
+ * form 1, installs the fastest in the tables above (rows per block, and the
+ * form for the size) and returns the rows (0 = one-iteration form) with the
+ * time per iteration in ms; glx_get_sstep_form_for gives the form.  Needs
+ * FP32 links.  No reference counterpart. */
 int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iteration);
 /* Launch shape of the fused normal operator D^dag D (no reference
  * counterpart; operators.cpp:444-453 is the operator it computes): rows per

@@ -176,3 +176,27 @@ def test_sstep_setting_without_device():
     assert lib.glx_set_sstep(2, 24) == 0
     assert lib.glx_get_sstep_rows(256, 256, C.byref(r)) == 0 and r.value == 24
     assert lib.glx_set_sstep(2, 0) == 0
+
+
+def test_sstep_form_setting_without_device():
+    """The s-step form: per size by default (wave strips from 512^2 sites up
+    to lattices within 1024 waves of 52-row bands), a per-size table entry,
+    and a process-wide choice; invalid values are refused (no device work)."""
+    lib = C.CDLL(LIB)
+    lib.glx_last_error.restype = C.c_char_p
+    f = C.c_int()
+    assert lib.glx_get_sstep_form(C.byref(f)) == 0 and f.value == -1
+    for (lx, ly), want in {(256, 256): 0, (512, 512): 1, (768, 768): 1, (1024, 1024): 1, (1536, 1536): 1,
+                           (2048, 2048): 0, (4096, 4096): 0, (1024, 64): 0, (2048, 512): 1}.items():
+        assert lib.glx_get_sstep_form_for(lx, ly, C.byref(f)) == 0 and f.value == want, (lx, ly, f.value)
+    assert lib.glx_set_sstep_form_entry(1024, 1024, 0) == 0
+    assert lib.glx_get_sstep_form_for(1024, 1024, C.byref(f)) == 0 and f.value == 0
+    assert lib.glx_set_sstep_form_entry(1024, 1024, 2) == 1 and b"entry" in lib.glx_last_error()
+    assert lib.glx_set_sstep_form_entry(1024, 1024, -1) == 0
+    assert lib.glx_get_sstep_form_for(1024, 1024, C.byref(f)) == 0 and f.value == 1
+    assert lib.glx_set_sstep_form(0, 0) == 0
+    assert lib.glx_get_sstep_form_for(1024, 1024, C.byref(f)) == 0 and f.value == 0
+    assert lib.glx_set_sstep_form(1, 57) == 1 and b"band rows" in lib.glx_last_error()
+    assert lib.glx_set_sstep_form(2, 0) == 1 and b"form must be" in lib.glx_last_error()
+    assert lib.glx_set_sstep_form(-1, 0) == 0
+    assert lib.glx_get_sstep_form(C.byref(f)) == 0 and f.value == -1

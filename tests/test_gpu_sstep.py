@@ -147,14 +147,15 @@ def test_sstep_autotune(glx, orc):
     lat = glx.Lattice(links, L, L)
     try:
         res = glx.autotune_sstep(lat, reps=10)
-        assert res["rows"] in (0, 8, 16, 32, 64, 128) and res["ms_per_iteration"] > 0
-        assert glx.get_sstep_rows(L, L) == res["rows"]
+        assert res["rows"] in (0, 8, 16, 32, 64, 128) and res["ms_per_iteration"] > 0 and res["form"] in (0, 1)
+        assert glx.get_sstep_rows(L, L) == res["rows"] and glx.get_sstep_form_for(L, L) == res["form"]
         x, info = _solve(glx, lat, 0.05, bc, max_iter=20000, eps=1e-10)
         assert info.success
         _check_solution(orc, links, bc, x, info, L, L, 0.05, 1e-10)
     finally:
         glx.set_sstep_rows(L, L, -1)
-    assert glx.get_sstep_rows(L, L) == 8
+        glx.set_sstep_form_entry(L, L, -1)
+    assert glx.get_sstep_rows(L, L) == 8 and glx.get_sstep_form_for(L, L) == 0
 
 
 @pytest.mark.parametrize("lx,ly,link", [(256, 64, "mixed"), (128, 8, "mixed"), (384, 40, "mixed_f16"), (512, 72, "mixed")])

@@ -160,4 +160,4 @@ def test_wave_form_kernel_timing(glx, orc):
     finally:
         glx.set_sstep_form()
     assert ms > 0 and by == 112
-    assert glx.get_sstep_form() == 0
+    assert glx.get_sstep_form() == -1
