@@ -2448,6 +2448,10 @@ SolveResult dev_cg_fused(const DevOperator& op, void* x, const void* b, const So
 struct BatchMap {
   const int* sys_of_blk;  // system of each block of the launch
   const int* off;         // first block of each system
+  const int* lb_of_blk;   // or, where set, each block's index within its system
+  __device__ __forceinline__ int local(int s) const {
+    return lb_of_blk ? lb_of_blk[blockIdx.x] : int(blockIdx.x) - off[s];
+  }
 };
 
 template <class F>
@@ -2460,7 +2464,7 @@ struct PtItem {
 template <class F>
 __global__ __launch_bounds__(kBlock) void k_point_batch(const PtItem<F>* items, BatchMap bm) {
   const int s = bm.sys_of_blk[blockIdx.x];
-  const int lb = blockIdx.x - bm.off[s];
+  const int lb = bm.local(s);
   const PtItem<F> it = items[s];
   if (lb >= it.nblk || gate_closed(it.gate)) return;
   F f = it.f;
@@ -2495,7 +2499,7 @@ template <class T, class In, class Epi>
 __global__ __launch_bounds__(kBlock) void k_normal_batch(const NrmItem<T, In, Epi>* items, BatchMap bm) {
   static_assert(!has_selffin<In>::value, "a batched launch has no self-finalising form");
   const int s = bm.sys_of_blk[blockIdx.x];
-  const int lb = blockIdx.x - bm.off[s];
+  const int lb = bm.local(s);
   const NGeom g = items[s].g;
   if (lb >= g.nblk || gate_closed(items[s].gate)) return;
   const OpDev<LinksF64> op = items[s].op;
@@ -2837,9 +2841,9 @@ struct InCgmP {
 
 // Device copies of one launch kind's items and its block table.
 struct BatchLaunch {
-  DevBuf items, sys_of_blk, off;
+  DevBuf items, sys_of_blk, off, lb_of_blk;
   int nblk_total = 0;
-  BatchMap map() const { return BatchMap{sys_of_blk.as<int>(), off.as<int>()}; }
+  BatchMap map() const { return BatchMap{sys_of_blk.as<int>(), off.as<int>(), lb_of_blk.as<int>()}; }
 };
 template <class Item>
 static BatchLaunch make_batch_launch(const std::vector<Item>& items, const std::vector<int>& nblk) {
@@ -2860,6 +2864,50 @@ static BatchLaunch make_batch_launch(const std::vector<Item>& items, const std::
   return b;
 }
 
+// The same launch with the systems that share one lattice's links (`group`:
+// equal keys, equal block counts) interleaved: block B of a group of n
+// systems with 8 per blocks each is system r = (B / 8) % n at index
+// 8 (B / 8n) + B % 8 within it.  The hardware deals block ids to the 8 XCDs
+// in turn, so the n blocks of one tile (the tile xcd_tile gives that index)
+// run on one XCD one after another, and the tile's links come from memory
+// once and from that XCD's L2 for the other right-hand sides.
+template <class Item, class Key>
+static BatchLaunch make_batch_launch_shared(const std::vector<Item>& items, const std::vector<int>& nblk,
+                                            const std::vector<Key>& group) {
+  BatchLaunch b;
+  const int ns = int(items.size());
+  std::vector<int> sob, lbb, off(ns, 0);
+  std::vector<bool> done(ns, false);
+  for (int s0 = 0; s0 < ns; s0++) {
+    if (done[s0]) continue;
+    std::vector<int> g;
+    for (int s = s0; s < ns; s++)
+      if (!done[s] && group[s] == group[s0] && nblk[s] == nblk[s0]) {
+        g.push_back(s);
+        done[s] = true;
+      }
+    const int n = int(g.size()), per = ((nblk[s0] + 7) / 8);
+    for (int j = 0; j < per; j++)
+      for (int r = 0; r < n; r++)
+        for (int x = 0; x < 8; x++) {
+          sob.push_back(g[r]);
+          lbb.push_back(8 * j + x);
+        }
+  }
+  b.nblk_total = int(sob.size());
+  b.items = DevBuf(items.size() * sizeof(Item));
+  b.sys_of_blk = DevBuf(sob.size() * sizeof(int));
+  b.off = DevBuf(off.size() * sizeof(int));
+  b.lb_of_blk = DevBuf(lbb.size() * sizeof(int));
+  GLX_HIP(hipMemcpy(b.items.get(), items.data(), items.size() * sizeof(Item), hipMemcpyHostToDevice));
+  GLX_HIP(hipMemcpy(b.sys_of_blk.get(), sob.data(), sob.size() * sizeof(int), hipMemcpyHostToDevice));
+  GLX_HIP(hipMemcpy(b.off.get(), off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+  GLX_HIP(hipMemcpy(b.lb_of_blk.get(), lbb.data(), lbb.size() * sizeof(int), hipMemcpyHostToDevice));
+  return b;
+}
+
+constexpr int kBatchRows = 32;
+
 template <class T>
 std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, int n_shift, const double* shifts,
                                          int resid_freq, bool worst_first, const SolveOpts& opt, BatchStats* stats,
@@ -2873,8 +2921,13 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
   for (int s = 0; s < ns; s++) {
     const DevOperator& o = sys[s].op;
     if (o.kind != OP_STAG_NORMAL || !o.u64) throw ArgError("batched systems must be staggered normal operators");
-    rows[s] = normal_rows_for(o.lx, o.ly, int(sizeof(T)));
-    if (rows[s] <= 0 || !normal_fits(o.lx, o.ly, rows[s], normal_cpl<T>())) rows[s] = 16;
+    // rows per block of D^dag D: the batch's grid holds every system's
+    // blocks, so it takes long strips (little halo) whatever shape a lone
+    // lattice of the size would take (measured on the 48-lattice batch, rows
+    // for all sizes: 8 128 ms, 16 109, 24 107, 32 105, 48 106, 64 109 ms of
+    // D^dag D)
+    rows[s] = kBatchRows;
+    if (!normal_fits(o.lx, o.ly, rows[s], normal_cpl<T>())) rows[s] = 16;
     if (!normal_fits(o.lx, o.ly, rows[s], normal_cpl<T>()))
       throw ArgError("batched system " + std::to_string(s) + ": lattice " + std::to_string(o.lx) + "x" +
                      std::to_string(o.ly) + " needs lx % 64 == 0, ly % 4 == 0, ly >= 8");
@@ -3001,8 +3054,12 @@ std::vector<SolveResult> cg_m_batch_impl(const std::vector<BatchSystem>& sys, in
   }
   BatchLaunch L_start = make_batch_launch(i_start, nb_pt);
   BatchLaunch F_start = make_batch_launch(f_start, std::vector<int>(ns, 1));
-  BatchLaunch L_st0 = make_batch_launch(i_st0, nb_st);
-  BatchLaunch L_st[2] = {make_batch_launch(i_st[0], nb_st), make_batch_launch(i_st[1], nb_st)};
+  // D^dag D: the right-hand sides of one lattice share its links
+  std::vector<const void*> link_key(ns);
+  for (int s = 0; s < ns; s++) link_key[s] = sys[s].op.u64;
+  BatchLaunch L_st0 = make_batch_launch_shared(i_st0, nb_st, link_key);
+  BatchLaunch L_st[2] = {make_batch_launch_shared(i_st[0], nb_st, link_key),
+                         make_batch_launch_shared(i_st[1], nb_st, link_key)};
   BatchLaunch F_st = make_batch_launch(f_st, std::vector<int>(ns, 1));
   BatchLaunch L_w = make_batch_launch(i_w, nb_w);
   BatchLaunch L_x = make_batch_launch(i_x, nb_r);

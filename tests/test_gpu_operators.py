@@ -71,8 +71,8 @@ def test_stencil_coefficients(glx, orc, op):
 
 
 def test_large_lattice_normal_operator(glx, orc):
-    """1024^2 fused D^dag D against the oracle on a row band (size-independent)."""
-    lx = ly = 256
+    """The fused D^dag D at the 1024^2 workload size against the oracle."""
+    lx = ly = 1024
     links, bc, _ = lattice_inputs(orc, lx, ly, beta=4.0, seed=21)
     got = glx.apply("staggered_normal", bc, links, 0.02, lx, ly)
     want = orc.apply("staggered_normal", bc, links, 0.02, lx, ly)
