@@ -63,14 +63,10 @@ inline SwGeom make_swgeom(int lx, int ly, int rows) {
   return g;
 }
 
-// The value lane l + 1 (next) or l - 1 (prev) holds; the wave's end lanes read 0.
-__device__ __forceinline__ float sw_next1(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
-}
+// The value lane l - 1 holds; lane 0 reads 0.
 __device__ __forceinline__ float sw_prev1(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
 }
-__device__ __forceinline__ c64 sw_next(c64 v) { return c64{sw_next1(v.re), sw_next1(v.im)}; }
 __device__ __forceinline__ c64 sw_prev(c64 v) { return c64{sw_prev1(v.re), sw_prev1(v.im)}; }
 // a float every lane of the wave holds (into a scalar register)
 __device__ __forceinline__ float sw_uniform(float v) {
@@ -81,42 +77,66 @@ __device__ __forceinline__ float sw_uniform(float v) {
 // y-links already multiplied by the site's staggered phase (+-1):
 //   K f = ux f(x+1) - conj(uxm) f(x-1) + uy f(y+1) - conj(uym) f(y-1)
 // as two chains of fused multiply-adds (the hop of khop_h, rounded
-// differently).  sw_mhop gives m2 v - K f.
-__device__ __forceinline__ c64 sw_hop(c64 fxp, c64 fxm, c64 fyp, c64 fym, c64 ux, c64 uxm, c64 uy, c64 uym) {
-  float re = ux.re * fxp.re;
-  re = fmaf(-ux.im, fxp.im, re);
-  re = fmaf(-uxm.re, fxm.re, re);
-  re = fmaf(-uxm.im, fxm.im, re);
+// differently); sw_mhop gives m2 v - K f.  The x-neighbours are read from
+// lanes l +- 1 inside the multiply-adds (DPP operands of v_mul / v_fmac: the
+// compiler fuses a DPP move into v_mul and v_add but not into v_fmac), in
+// inline asm that begins with `s_nop 1`: a DPP operand must be written by
+// VALU at least two wait states earlier, and the compiler's hazard checks do
+// not look inside asm.  The wave's end lanes read 0 (bound_ctrl).
+#define GLX_DPP_NEXT "wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:0"
+#define GLX_DPP_PREV "wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:0"
+// ux f(x+1) - conj(uxm) f(x-1)
+__device__ __forceinline__ c64 sw_hop_x(c64 f, c64 ux, c64 uxm) {
+  float re, im;
+  asm("s_nop 1\n\t"
+      "v_mul_f32_dpp %0, %2, %4 " GLX_DPP_NEXT "\n\t"    // re  = ux.re f+.re
+      "v_mul_f32_dpp %1, %3, %4 " GLX_DPP_NEXT "\n\t"    // im  = ux.re f+.im
+      "v_fmac_f32_dpp %0, %3, -%5 " GLX_DPP_NEXT "\n\t"  // re -= ux.im f+.im
+      "v_fmac_f32_dpp %1, %2, %5 " GLX_DPP_NEXT "\n\t"   // im += ux.im f+.re
+      "v_fmac_f32_dpp %0, %2, -%6 " GLX_DPP_PREV "\n\t"  // re -= uxm.re f-.re
+      "v_fmac_f32_dpp %1, %3, -%6 " GLX_DPP_PREV "\n\t"  // im -= uxm.re f-.im
+      "v_fmac_f32_dpp %0, %3, -%7 " GLX_DPP_PREV "\n\t"  // re -= uxm.im f-.im
+      "v_fmac_f32_dpp %1, %2, %7 " GLX_DPP_PREV          // im += uxm.im f-.re
+      : "=&v"(re), "=&v"(im)
+      : "v"(f.re), "v"(f.im), "v"(ux.re), "v"(ux.im), "v"(uxm.re), "v"(uxm.im));
+  return c64{re, im};
+}
+// acc - (ux f(x+1) - conj(uxm) f(x-1))
+__device__ __forceinline__ c64 sw_mhop_x(c64 acc, c64 f, c64 ux, c64 uxm) {
+  float re = acc.re, im = acc.im;
+  asm("s_nop 1\n\t"
+      "v_fmac_f32_dpp %0, %2, -%4 " GLX_DPP_NEXT "\n\t"  // re -= ux.re f+.re
+      "v_fmac_f32_dpp %1, %3, -%4 " GLX_DPP_NEXT "\n\t"  // im -= ux.re f+.im
+      "v_fmac_f32_dpp %0, %3, %5 " GLX_DPP_NEXT "\n\t"   // re += ux.im f+.im
+      "v_fmac_f32_dpp %1, %2, -%5 " GLX_DPP_NEXT "\n\t"  // im -= ux.im f+.re
+      "v_fmac_f32_dpp %0, %2, %6 " GLX_DPP_PREV "\n\t"   // re += uxm.re f-.re
+      "v_fmac_f32_dpp %1, %3, %6 " GLX_DPP_PREV "\n\t"   // im += uxm.re f-.im
+      "v_fmac_f32_dpp %0, %3, %7 " GLX_DPP_PREV "\n\t"   // re += uxm.im f-.im
+      "v_fmac_f32_dpp %1, %2, -%7 " GLX_DPP_PREV         // im -= uxm.im f-.re
+      : "+v"(re), "+v"(im)
+      : "v"(f.re), "v"(f.im), "v"(ux.re), "v"(ux.im), "v"(uxm.re), "v"(uxm.im));
+  return c64{re, im};
+}
+__device__ __forceinline__ c64 sw_hop(c64 f, c64 fyp, c64 fym, c64 ux, c64 uxm, c64 uy, c64 uym) {
+  const c64 x = sw_hop_x(f, ux, uxm);
+  float re = x.re, im = x.im;
   re = fmaf(uy.re, fyp.re, re);
   re = fmaf(-uy.im, fyp.im, re);
   re = fmaf(-uym.re, fym.re, re);
   re = fmaf(-uym.im, fym.im, re);
-  float im = ux.re * fxp.im;
-  im = fmaf(ux.im, fxp.re, im);
-  im = fmaf(-uxm.re, fxm.im, im);
-  im = fmaf(uxm.im, fxm.re, im);
   im = fmaf(uy.re, fyp.im, im);
   im = fmaf(uy.im, fyp.re, im);
   im = fmaf(-uym.re, fym.im, im);
   im = fmaf(uym.im, fym.re, im);
   return c64{re, im};
 }
-__device__ __forceinline__ c64 sw_mhop(float m2, c64 v, c64 fxp, c64 fxm, c64 fyp, c64 fym, c64 ux, c64 uxm, c64 uy,
-                                       c64 uym) {
-  float re = m2 * v.re;
-  re = fmaf(-ux.re, fxp.re, re);
-  re = fmaf(ux.im, fxp.im, re);
-  re = fmaf(uxm.re, fxm.re, re);
-  re = fmaf(uxm.im, fxm.im, re);
+__device__ __forceinline__ c64 sw_mhop(float m2, c64 v, c64 f, c64 fyp, c64 fym, c64 ux, c64 uxm, c64 uy, c64 uym) {
+  const c64 x = sw_mhop_x(c64{m2 * v.re, m2 * v.im}, f, ux, uxm);
+  float re = x.re, im = x.im;
   re = fmaf(-uy.re, fyp.re, re);
   re = fmaf(uy.im, fyp.im, re);
   re = fmaf(uym.re, fym.re, re);
   re = fmaf(uym.im, fym.im, re);
-  float im = m2 * v.im;
-  im = fmaf(-ux.re, fxp.im, im);
-  im = fmaf(-ux.im, fxp.re, im);
-  im = fmaf(uxm.re, fxm.im, im);
-  im = fmaf(-uxm.im, fxm.re, im);
   im = fmaf(-uy.re, fyp.im, im);
   im = fmaf(-uy.im, fyp.re, im);
   im = fmaf(uym.re, fym.im, im);
@@ -303,18 +323,18 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
         load_row(cur, t + kSwAhead);
       }
       // S1: row t - 1
-      qp[S1] = sw_hop(sw_next(p[S1]), sw_prev(p[S1]), p[S0], p[S2], ux[S1], uxm[S1], uy[S1], uy[S2]);
-      qr[S1] = sw_hop(sw_next(r[S1]), sw_prev(r[S1]), r[S0], r[S2], ux[S1], uxm[S1], uy[S1], uy[S2]);
+      qp[S1] = sw_hop(p[S1], p[S0], p[S2], ux[S1], uxm[S1], uy[S1], uy[S2]);
+      qr[S1] = sw_hop(r[S1], r[S0], r[S2], ux[S1], uxm[S1], uy[S1], uy[S2]);
       // S2: row t - 2
-      P1[S2] = sw_mhop(m2, p[S2], sw_next(qp[S2]), sw_prev(qp[S2]), qp[S1], qp[S3], ux[S2], uxm[S2], uy[S2], uy[S3]);
+      P1[S2] = sw_mhop(m2, p[S2], qp[S2], qp[S1], qp[S3], ux[S2], uxm[S2], uy[S2], uy[S3]);
       // S3: row t - 3
-      q1[S3] = sw_hop(sw_next(P1[S3]), sw_prev(P1[S3]), P1[S2], P1[S4], ux[S3], uxm[S3], uy[S3], uy[S4]);
+      q1[S3] = sw_hop(P1[S3], P1[S2], P1[S4], ux[S3], uxm[S3], uy[S3], uy[S4]);
       // S4: row t - 4, the owned rows only
       const bool o4 = owned(t - 4);
       c64 p2, r1;
       if (o4) {
-        p2 = sw_mhop(m2, P1[S4], sw_next(q1[S4]), sw_prev(q1[S4]), q1[S3], q1[S5], ux[S4], uxm[S4], uy[S4], uy[S5]);
-        r1 = sw_mhop(m2, r[S4], sw_next(qr[S4]), sw_prev(qr[S4]), qr[S3], qr[S5], ux[S4], uxm[S4], uy[S4], uy[S5]);
+        p2 = sw_mhop(m2, P1[S4], q1[S4], q1[S3], q1[S5], ux[S4], uxm[S4], uy[S4], uy[S5]);
+        r1 = sw_mhop(m2, r[S4], qr[S4], qr[S3], qr[S5], ux[S4], uxm[S4], uy[S4], uy[S5]);
       }
       // the stores and sums of the lanes that own their column, in one
       // masked region per step
