@@ -39,23 +39,12 @@ the serial sums alone move the reference's solution by 5e-12 (256^2) and
 The oracle runs in threads (its C calls release the GIL), so the 1024^2 CG
 (~20 s on one core) and the 24 multishift solves fit the per-test limit.
 """
-import concurrent.futures as cf
-import os
-
 import numpy as np
 import pytest
 
-from helpers import lattice_inputs, relerr
+from helpers import bicgstab_envelope, check_in_envelope, lattice_inputs, pmap, relerr
 
 pytestmark = pytest.mark.gpu
-
-WORKERS = min(8, os.cpu_count() or 4)
-
-
-def pmap(fn, items):
-    with cf.ThreadPoolExecutor(WORKERS) as ex:
-        return list(ex.map(fn, items))
-
 
 # ------------------------------------------------------------ 256^2, FP64 --
 L2, M2 = 256, 0.05
@@ -92,41 +81,6 @@ def test_c2_cg_normal_matches_oracle(glx, orc, c2):
     assert extended[1].iter == io.iter
     check_fp64_cg("C2 CG on D^dag D", xg, ig, serial, extended)
     assert abs(ig.resSq - io.resSq) <= 0.05 * io.resSq
-
-
-def _perturbed(b, k, frac=0.01):
-    """b with the last bit of the real part of a seeded 1 % of its entries
-    flipped up (k odd) or down (k even)."""
-    rng = np.random.default_rng(1000 + k)
-    idx = rng.choice(b.size, size=max(1, int(b.size * frac)), replace=False)
-    re = b.real.copy()
-    re[idx] = np.nextafter(re[idx], np.inf if k % 2 else -np.inf)
-    return re + 1j * b.imag
-
-
-def bicgstab_envelope(orc, op, b, kw, n=8, restart_freq=0):
-    """The oracle's own spread: BiCGStab on b and on n 1-ulp perturbations
-    of it.  Returns (x of b, info of b, iteration counts, relative distances
-    of the perturbed solutions from the unperturbed one)."""
-    kw = dict(kw, restart_freq=restart_freq)
-    runs = pmap(lambda k: orc.solve("bicgstab", op, b if k < 0 else _perturbed(b, k), None, **kw),
-                list(range(-1, n)))
-    x0, i0 = runs[0]
-    its = np.array([i.iter for _, i in runs])
-    dx = np.array([relerr(x, x0) for x, _ in runs[1:]])
-    return x0, i0, its, dx
-
-
-def check_in_envelope(xg, ig, x0, its, dx, what):
-    """GPU iterations within the oracle runs' mean +- 3 sigma (and never
-    below min - 3 sigma), its distance from the oracle's unperturbed x within
-    twice the largest distance a 1-ulp perturbation causes."""
-    mu, sd = float(its.mean()), float(max(its.std(), 1.0))
-    dist = relerr(xg, x0)
-    print(f"{what}: GPU iter {ig.iter}, oracle iters {sorted(its.tolist())} (mean {mu:.1f}, sd {sd:.1f}); "
-          f"|x_gpu - x_o|/|x_o| = {dist:.2e}, oracle spread {dx.min():.2e} .. {dx.max():.2e}")
-    assert abs(ig.iter - mu) <= 3.0 * sd, (ig.iter, mu, sd)
-    assert dist <= 2.0 * dx.max(), (dist, dx.max())
 
 
 def test_c2_bicgstab_inside_oracle_envelope(glx, orc, c2):

@@ -11,7 +11,7 @@ import json
 import numpy as np
 import pytest
 
-from helpers import lattice_inputs, relerr
+from helpers import ENVELOPE, cg_envelope, lattice_inputs, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -79,16 +79,14 @@ def test_mixed_and_fused_with_normal_kernel(glx, orc, rows_reset, rows):
     glx.set_normal_rows(lx, ly, 8, rows)
     lat = glx.Lattice(links, lx, ly)
     db = glx.DeviceArray(bc)
-    xo, io = orc.solve("cg", "staggered_normal", bc, None, links=links, mass=mass, lx=lx, ly=ly, max_iter=20000,
-                       eps=1e-10)
-    cond = (mass * mass + 2.0) / (mass * mass)
+    xs, err_o, xo, io = cg_envelope(orc, "staggered_normal", bc, links, mass, lx, ly)
     for fn in (glx.dev_cg_fused, glx.dev_cg_mixed):
         dx = glx.DeviceArray(np.zeros_like(bc))
         info = fn(lat, "staggered_normal", mass, dx, db, max_iter=20000, eps=1e-10)
         x = dx.download()
         r = bc - orc.apply("staggered_normal", x, links, mass, lx, ly)
         assert info.success and np.linalg.norm(r) / np.linalg.norm(bc) <= 1.05e-10
-        assert relerr(x, xo) <= 2e-10 * cond
+        assert relerr(x, xs) <= ENVELOPE * err_o, (relerr(x, xs), err_o)
         assert info.iter <= 1.5 * io.iter + 20
 
 

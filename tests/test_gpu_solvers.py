@@ -20,7 +20,8 @@ field except the last digits of residuals.
 import numpy as np
 import pytest
 
-from helpers import lattice_inputs, relerr
+from helpers import (bicgstab_envelope, check_in_envelope, lattice_inputs, oracle_text_envelope, perturbed, pmap,
+                     relerr, text_envelope)
 
 pytestmark = pytest.mark.gpu
 
@@ -38,28 +39,37 @@ def check_info(got, want, name=None, bnorm2=None, eps=1e-10):
         assert got.resSq <= (eps * eps) * bnorm2 * 1.0001
 
 
-def check_text(got, want, rtol=0.25, floor=1e-13, iter_rtol=0.25):
-    """Verbosity lines agree in every field; residual values agree to rtol on
-    summary lines and to iter_rtol on per-iteration lines, except at the FP64
-    rounding floor, below `floor`, where only their smallness is compared.
-    CG's residual 2-norm is not monotone, and on ill-conditioned systems the
-    iterated residuals of two FP64 implementations differ by up to ~10 %
-    mid-solve; a converged true residual (~1e-11) carries the solutions'
-    1e-12-level difference amplified by |A| |x| / |b| (~6 % measured).  The
-    solutions themselves are compared separately, to 1e-11."""
+# The verbosity text's RelRes values may differ from the oracle's by this
+# multiple of the oracle's own spread on the same line (text_envelope: 1-ulp
+# perturbations of b and the long double reductions), or by the printed
+# precision (%g: six significant digits).
+TEXT_ENVELOPE = 3.0
+TEXT_QUANT = 1e-5
+
+
+def check_text(got, want, env, floor=1e-13):
+    """Verbosity lines agree in every field but RelRes exactly; the RelRes
+    values of line i agree to max(TEXT_QUANT, TEXT_ENVELOPE * env[i]), except
+    at the FP64 rounding floor, below `floor`, where only their smallness is
+    compared.  Returns the largest deviation seen as a multiple of its
+    line's allowance."""
     gl, wl = got.splitlines(), want.splitlines()
-    assert len(gl) == len(wl), (len(gl), len(wl), gl[-3:], wl[-3:])
-    for a, b in zip(gl, wl):
+    assert len(gl) == len(wl) == len(env), (len(gl), len(wl), len(env), gl[-3:], wl[-3:])
+    worst = 0.0
+    for a, b, e in zip(gl, wl, env):
         ha, hb = a.split("RelRes")[0], b.split("RelRes")[0]
         assert ha == hb, (a, b)
         va = [float(t) for t in a.split("RelRes")[1].split()]
         vb = [float(t) for t in b.split("RelRes")[1].split()]
         assert len(va) == len(vb)
-        tol = rtol if "Success" in a else iter_rtol
+        tol = max(TEXT_QUANT, TEXT_ENVELOPE * e)
         for x, y in zip(va, vb):
             if max(abs(x), abs(y)) < floor:
                 continue
-            assert abs(x - y) <= tol * abs(y), (a, b)
+            dev = abs(x - y) / abs(y)
+            worst = max(worst, dev / tol)
+            assert dev <= tol, (a, b, dev, e)
+    return worst
 
 
 def oracle_sensitivity(orc, solver, op, b, kw, trials=3, seed=0):
@@ -108,7 +118,9 @@ def test_solver_matches_oracle(glx, orc, solver, op, lx, ly, mass, beta, with_x0
     xg, ig = glx.solve(solver, op, b, x0, **kw)
     check_info(ig, io, io.name, bnorm2=float(np.vdot(b, b).real))
     assert relerr(xg, xo) < xtol(op), relerr(xg, xo)
-    check_text(ig.text, io.text)
+    base, env = oracle_text_envelope(orc, solver, op, b, x0, kw)
+    assert base == io.text
+    print(f"{solver} {op}: text within {check_text(ig.text, io.text, env):.2f} of its allowance")
 
 
 @pytest.mark.parametrize("op,lx,ly,mass,beta,with_x0", BICG_CASES)
@@ -151,18 +163,33 @@ def test_eoprec_solve_matches_direct(glx, orc):
 
 @pytest.mark.parametrize("solver", ["cg", "bicgstab", "gcr"])
 def test_restarted_matches_oracle(glx, orc, solver):
-    if solver == "bicgstab":
-        pytest.skip("restarted BiCGStab trajectories are not rounding-stable; covered by test_bicgstab_restart")
+    """CG(9) and GCR(9): the oracle's iterations, ops and text (to its
+    envelope); BiCGStab(9), whose trajectory is not stable under rounding:
+    iterations and solution inside the oracle's own spread under 1-ulp
+    perturbations of b."""
     lx = ly = 16
     op = "staggered_normal" if solver == "cg" else "staggered"
     links, bc, _ = lattice_inputs(orc, lx, ly, beta=6.25, seed=5)
     kw = dict(links=links, mass=0.05, lx=lx, ly=ly, max_iter=600, eps=1e-10, restart_freq=9, verb_level=3,
               prefix="[R]: ")
-    xo, io = orc.solve(solver, op, bc, None, **kw)
+    if solver == "bicgstab":
+        # at m = 0.05 the oracle's own BiCGStab(9) takes 503 .. 764 iterations
+        # over 1-ulp perturbations of b; m = 0.2 keeps the spread (143 .. 186)
+        # well inside max_iter
+        kw0 = dict(links=links, mass=0.2, lx=lx, ly=ly, max_iter=2000, eps=1e-10)
+        xg, ig = glx.solve(solver, op, bc, None, **dict(kw0, restart_freq=9))
+        x0, i0, its, dx = bicgstab_envelope(orc, op, bc, kw0, restart_freq=9)
+        assert ig.name == i0.name == "BiCGStab(9)" and ig.success and i0.success
+        check_in_envelope(xg, ig, x0, its, dx, "BiCGStab(9) 16^2")
+        r = bc - orc.apply(op, xg, links, 0.2, lx, ly)
+        assert np.linalg.norm(r) <= 1e-10 * np.linalg.norm(bc) * 1.0001
+        return
     xg, ig = glx.solve(solver, op, bc, None, **kw)
+    xo, io = orc.solve(solver, op, bc, None, **kw)
     check_info(ig, io, io.name)
     assert relerr(xg, xo) < 1e-9
-    check_text(ig.text, io.text)
+    _, env = oracle_text_envelope(orc, solver, op, bc, None, kw)
+    print(f"{solver}(9): text within {check_text(ig.text, io.text, env):.2f} of its allowance")
 
 
 @pytest.mark.parametrize("verb", [0, 1, 2, 3])
@@ -173,7 +200,8 @@ def test_verbosity_levels(glx, orc, verb):
               prefix="[V]: ", fmt="%.6e")
     _, io = orc.solve("cg", "staggered_normal", bc, None, **kw)
     _, ig = glx.solve("cg", "staggered_normal", bc, None, **kw)
-    check_text(ig.text, io.text)
+    _, env = oracle_text_envelope(orc, "cg", "staggered_normal", bc, None, kw)
+    check_text(ig.text, io.text, env)
 
 
 def test_max_iter_quirks(glx, orc):
@@ -235,4 +263,8 @@ def test_multishift_matches_oracle(glx, orc, op, mass, wf):
         assert relerr(Xg[k], Xo[k]) < xtol(op), (k, relerr(Xg[k], Xo[k]))
         # per-shift squared residuals: both below tolerance or both above
         assert (ig.resSqmrhs[k] <= 1e-20 * bn2) == (ro[k] <= 1e-20 * bn2) or abs(ig.resSqmrhs[k] - ro[k]) < 0.1 * ro[k]
-    check_text(ig.text, io.text)
+    jobs = list(range(3)) + ([-1] if np.iscomplexobj(b) else [])
+    others = pmap(lambda k: orc.cg_m(op, b if k < 0 else perturbed(b, k), sh,
+                                     **dict(kw, reductions="extended" if k < 0 else "serial"))[1].text, jobs)
+    print(f"CG-M {op}: text within {check_text(ig.text, io.text, text_envelope(io.text, others)):.2f} of its "
+          f"allowance")

@@ -12,7 +12,7 @@ up to a few, with a solution that matches the FP64 oracle's to the accuracy
 the tolerance allows."""
 import numpy as np
 
-from helpers import lattice_inputs, relerr
+from helpers import ENVELOPE, cg_envelope, lattice_inputs, relerr
 
 
 def _ops(orc, links, mass, L):
@@ -143,10 +143,8 @@ def test_sstep_converges_like_cg(orc):
     r = b - a64(xs)
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
     assert abs(ks - kp) <= 3, (ks, kp)
-    xo, io = orc.solve("cg", "staggered_normal", b, None, links=links, mass=mass, lx=L, ly=L, max_iter=5000,
-                       eps=1e-10)
-    cond = (mass * mass + 2.0) / (mass * mass)
-    assert relerr(xs, xo) <= 2e-10 * cond
+    x_star, err_o, xo, io = cg_envelope(orc, "staggered_normal", b, links, mass, L, L)
+    assert relerr(xs, x_star) <= ENVELOPE * err_o, (relerr(xs, x_star), err_o)
     assert abs(ks - io.iter) <= 0.05 * io.iter + 3, (ks, io.iter)
 
 
