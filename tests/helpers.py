@@ -33,10 +33,14 @@ def relerr(a, b):
 
 
 # A solver that stops on the same tolerance as an FP64 CG solve is held to
-# this multiple of that solve's own error (measured: the mixed-precision
-# solves reach 0.8 - 1.1 times it).  The a-priori bound of either error,
-# cond(A) * tol, is 10^2 - 10^4 times larger on these systems.
-ENVELOPE = 2.0
+# this multiple of that solve's own error.  Two solves that stop at the same
+# residual norm can differ severalfold in error (the residual's content in
+# the low modes decides it: the CPU restatement of the s-step algorithm stops
+# at 8.8e-11 with 7.6 times the error of the oracle's CG at 8.2e-11); the GPU
+# solvers follow the FP64 CG trajectory closely and measure 0.8 - 1.1 times
+# it.  The a-priori bound of either error, cond(A) * tol, is 10^2 - 10^4
+# times larger on these systems.
+ENVELOPE = 4.0
 
 
 def cg_envelope(orc, op, b, links, mass, lx, ly, eps=1e-10, max_iter=100000):
