@@ -1754,6 +1754,13 @@ struct PtFastFinal {
 
 // The stream of a solve: the caller's (`want`), else the allocation-free
 // scope's, else one created for the solve and destroyed after it.
+// The stream is taken from the calling thread's streams of finished solves
+// (created on first use, never destroyed): a stream created and destroyed
+// per solve costs a good part of a millisecond.
+static std::vector<hipStream_t>& idle_streams() {
+  static thread_local std::vector<hipStream_t>* v = new std::vector<hipStream_t>;
+  return *v;
+}
 struct OwnedStream {
   hipStream_t st = nullptr;
   bool owned = false;
@@ -1763,14 +1770,20 @@ struct OwnedStream {
     } else if (NoAllocScope* s = noalloc_scope()) {
       st = s->stream();
     } else {
-      GLX_HIP(hipStreamCreate(&st));
+      std::vector<hipStream_t>& idle = idle_streams();
+      if (!idle.empty()) {
+        st = idle.back();
+        idle.pop_back();
+      } else {
+        GLX_HIP(hipStreamCreate(&st));
+      }
       owned = true;
     }
   }
   ~OwnedStream() {
     if (owned) {
       (void)hipStreamSynchronize(st);
-      (void)hipStreamDestroy(st);
+      idle_streams().push_back(st);
     }
   }
 };
