@@ -82,7 +82,9 @@ __device__ __forceinline__ float sw_uniform(float v) {
 // compiler fuses a DPP move into v_mul and v_add but not into v_fmac), in
 // inline asm that begins with `s_nop 1`: a DPP operand must be written by
 // VALU at least two wait states earlier, and the compiler's hazard checks do
-// not look inside asm.  The wave's end lanes read 0 (bound_ctrl).
+// not look inside asm.  The wave's end lanes read 0 (bound_ctrl).  Every hop
+// runs with all 64 lanes active (clang marks inline asm convergent, so it is
+// never moved under a lane-dependent branch).
 #define GLX_DPP_NEXT "wave_shl:1 row_mask:0xf bank_mask:0xf bound_ctrl:0"
 #define GLX_DPP_PREV "wave_shr:1 row_mask:0xf bank_mask:0xf bound_ctrl:0"
 // ux f(x+1) - conj(uxm) f(x-1)
