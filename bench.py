@@ -537,10 +537,13 @@ def kernel_only(glx, args):
     prec = {"sscg": 2 + lp, "fast32": lp, "fast64": 0}[args.kernel_only]
     ms, bps = glx.time_cg_kernel(lat, args.mass, prec=prec, reps=args.reps)
     gbs = bps * L * L / (ms * 1e-3) / 1e9
-    print(json.dumps({"kernel": args.kernel_only, "L": L, "links": args.links, "reps": args.reps,
-                      "sstep_rows": glx.get_sstep_rows(L, L), "sstep_form": glx.get_sstep_form_for(L, L),
-                      "band_rows": args.band_rows, "us_per_launch": round(ms * 1e3, 3),
-                      "bytes_per_site": bps, "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}))
+    out = {"kernel": args.kernel_only, "L": L, "links": args.links, "reps": args.reps,
+           "sstep_rows": glx.get_sstep_rows(L, L), "sstep_form": glx.get_sstep_form_for(L, L),
+           "band_rows": args.band_rows, "us_per_launch": round(ms * 1e3, 3),
+           "bytes_per_site": bps, "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    if args.kernel_only == "sscg" and out["sstep_form"] == 1:
+        out["traffic_only_us_per_launch"] = round(glx.time_sscg_traffic(lat, lp, reps=args.reps) * 1e3, 3)
+    print(json.dumps(out))
     return 0
 
 
@@ -683,6 +686,14 @@ def main():
     it64_ms, it64_bps = glx.time_cg_kernel(lat, mass, prec=0, reps=200)
     it_gbs = it_bps * V / (it_ms * 1e-3) / 1e9
     it64_gbs = it64_bps * V / (it64_ms * 1e-3) / 1e9
+    # the wave-strip launch's loads and stores alone (no arithmetic): the
+    # floor its access pattern sets; frac_of_floor = that floor / the launch
+    pattern = None
+    if ss_form == 1:
+        pat_ms = glx.time_sscg_traffic(lat, link_prec, reps=200)
+        pat_gbs = it_bps * V / (pat_ms * 1e-3) / 1e9
+        pattern = {"ms_per_launch": pat_ms, "GBs": round(pat_gbs, 1), "frac": round(pat_gbs / HBM_PEAK_GBS, 4),
+                   "frac_of_floor": round(pat_ms / it_ms, 4)}
 
     stencil = stencil_timing(glx, lat, b, mass, L)
     big = None
@@ -780,6 +791,7 @@ def main():
             "bytes_per_launch": it_bps * V,
             "bytes_per_site": it_bps,
             "ms_per_launch": it_ms,
+            "pattern_floor": pattern,
         },
         "eo_solve_D": eo,
         "gradient": grad,

@@ -1698,6 +1698,14 @@ int glx_dev_time_cg_kernel(glx_lattice lat, double mass, int prec, int reps, dou
   });
 }
 
+int glx_dev_time_sscg_traffic(glx_lattice lat, int link_prec, int reps, double* ms_per_launch) {
+  return guarded([&] {
+    if (!ms_per_launch) throw ArgError("ms_per_launch is NULL");
+    DevOperator o = lattice_op(lat, OP_STAG_NORMAL, 0.0, 1);
+    *ms_per_launch = time_sscg_traffic_impl(o, link_prec == LINKS_F16 ? LINKS_F16 : LINKS_F32, reps);
+  });
+}
+
 // ------------------------------------------------ general stencil handle --
 struct glx_cstencil_s {
   glx_stencil_2d_nc desc{};  // dimensions, sdir, has_two, shifts (pointers cleared)

@@ -121,6 +121,7 @@ void set_sstep_form_entry(int lx, int ly, int form);  // -1 removes the entry
 int sstep_form_entry(int lx, int ly);                 // -1: none
 int sstep_form_for(int lx, int ly);
 int sw_rows_for(int lx, int ly);
+double time_sscg_traffic_impl(const DevOperator& o, int link_prec, int reps);
 SolveResult dev_cg_mixed(const DevOperator& op, void* x, const void* b, const SolveOpts& o, const MixedOpts& m);
 
 // FP64 CG in the single-pass form the mixed solver uses (one fused stencil

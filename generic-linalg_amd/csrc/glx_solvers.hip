@@ -2404,6 +2404,48 @@ double time_sscg_kernel_impl(const DevOperator& o, int link_prec, int reps, int 
   return double(ms) / reps;
 }
 
+// Live timing of the wave-strip s-step launch's memory traffic alone
+// (k_sscg_w_traffic, the same loads and stores with no arithmetic): `reps`
+// launches between two HIP events over a zeroed basis.  Returns ms per
+// launch.  Only lattices the s-step form takes.
+double time_sscg_traffic_impl(const DevOperator& o, int link_prec, int reps) {
+  if (o.kind != OP_STAG_NORMAL || !sscg_fits(o))
+    throw ArgError("the s-step traffic timing takes D^dag D on lattices the s-step form takes");
+  if (reps < 1) throw ArgError("reps must be >= 1");
+  const int n = o.size();
+  OwnedStream own;
+  hipStream_t st = own.st;
+  const SwGeom g = make_swgeom(o.lx, o.ly, sw_rows_for(o.lx, o.ly));
+  SsSpace sp(n, st);
+  int ko = 0;
+  auto launch = [&]() {
+    if (link_prec == LINKS_F16) {
+      if (!o.u16 || !o.sc16) throw ArgError("operator has no FP16 links on the device");
+      hipLaunchKernelGGL((k_sscg_w_traffic<LinksF16>), dim3(g.nblk), dim3(kBlock), 0, st, g,
+                         make_opdev(o, LinksF16{o.u16, o.sc16}), sp.bf, ko);
+    } else {
+      if (!o.u32) throw ArgError("operator has no FP32 links on the device");
+      hipLaunchKernelGGL((k_sscg_w_traffic<LinksF32>), dim3(g.nblk), dim3(kBlock), 0, st, g,
+                         make_opdev(o, LinksF32{o.u32}), sp.bf, ko);
+    }
+    GLX_CHECK_LAUNCH();
+    ko ^= 1;
+  };
+  launch();
+  hipEvent_t a, b;
+  GLX_HIP(hipEventCreate(&a));
+  GLX_HIP(hipEventCreate(&b));
+  GLX_HIP(hipEventRecord(a, st));
+  for (int k = 0; k < reps; k++) launch();
+  GLX_HIP(hipEventRecord(b, st));
+  GLX_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  GLX_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return double(ms) / reps;
+}
+
 // ============================================================ dispatch ====
 static bool real_op(const DevOperator& o) {
   return o.is_host() ? !o.host_complex : o.kind == OP_LAPLACE_REAL;

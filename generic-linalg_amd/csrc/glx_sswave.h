@@ -408,4 +408,82 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
   }
 }
 
+// The memory traffic of k_sscg_w alone: the same waves, march, loads (three
+// rows ahead) and stores at the same addresses, with the arithmetic replaced
+// by a few adds -- what the one-wave-per-SIMD strip pattern itself streams
+// at, the floor of k_sscg_w's time.  Timing only: it writes sums of its
+// inputs into the next basis slot and x.
+template <class LK>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_sscg_w_traffic(SwGeom g, OpDev<LK> op, SsBufs bf, int ko) {
+  const int nblk = gridDim.x;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int task = xcd_tile(blockIdx.x, nblk) * kWaves + w;
+  if (task >= g.ntask) return;
+  const int strip = task % g.nstrip, band = task / g.nstrip;
+  const int lx = g.lx, ly = g.ly;
+  const int xo = strip * kSwOwn;
+  const int xl = wrap_coord(xo - kSwHalo + l, lx);
+  const bool own_lane = l >= kSwHalo && l < kSwHalo + kSwOwn && xo + l - kSwHalo < lx;
+  const int yb = band * g.rows;
+  const int rows = min(g.rows, ly - yb);
+  const int steps = rows + 2 * kSwHalo;
+  const bool up = (band & 1) != 0;
+  auto row_of = [&](int t) {
+    return ss_wrap_row(up ? yb + rows + kSwHalo - 1 - t : yb - kSwHalo + t, ly);
+  };
+  const int rowtab = row_of(min(l, steps - 1)) * lx;
+  auto rowj = [&](int t) { return __builtin_amdgcn_readlane(rowtab, t); };
+  auto owned = [&](int t) { return t >= kSwHalo && t < rows + kSwHalo; };
+  const int so = ko & 1, sn = so ^ 1;
+  const c64* Y[5] = {bf.v(so, 0), bf.v(so, 1), bf.v(so, 2), bf.v(so, 3), bf.v(so, 4)};
+  c64* N[5] = {bf.v(sn, 0), bf.v(sn, 1), bf.v(sn, 2), bf.v(sn, 3), bf.v(sn, 4)};
+  auto load_row = [&](SwRow<LK>& d, int t) {
+    const unsigned j = unsigned(rowj(min(t, steps - 1)) + xl);
+#pragma unroll
+    for (int i = 0; i < 5; i++) d.y[i] = ss_at<c64>(Y[i], j * 8u);
+    d.x = ss_at<c64>(bf.x, j * 8u);
+    ss_links<1>(op.links, j, d.lk);
+  };
+  SwRow<LK> ld[kSwAhead];
+#pragma unroll
+  for (int a = 0; a < kSwAhead; a++) load_row(ld[a], a);
+  c64 acc{0.0f, 0.0f};
+  auto step = [&](auto kc, int t) __attribute__((always_inline)) {
+    constexpr int K = decltype(kc)::value;
+    SwRow<LK>& cur = ld[K % kSwAhead];
+    typename LK::C u0, u1;
+    LK::finish(cur.lk[0], u0, u1);
+    c64 v[6];
+#pragma unroll
+    for (int i = 0; i < 5; i++) v[i] = cur.y[i] + acc;
+    v[5] = cur.x + u0 + u1;
+    acc = acc + c64{1e-30f * v[0].re, 1e-30f * v[4].im};
+    load_row(cur, t + kSwAhead);
+    if (own_lane) {
+      if (owned(t)) {
+        const unsigned j = unsigned(rowj(t) + xl) * 8u;
+        ss_at<c64>(N[0], j) = v[0];
+        ss_at<c64>(N[3], j) = v[3];
+        ss_at<c64>(bf.x, j) = v[5];
+      }
+      if (owned(t - 2)) ss_at<c64>(N[1], unsigned(rowj(t - 2) + xl) * 8u) = v[1];
+      if (owned(t - 4)) {
+        const unsigned j = unsigned(rowj(t - 4) + xl) * 8u;
+        ss_at<c64>(N[2], j) = v[2];
+        ss_at<c64>(N[4], j) = v[4];
+      }
+    }
+  };
+  using std::integral_constant;
+  for (int t0 = 0; t0 < steps; t0 += kSwWin) {
+    step(integral_constant<int, 0>{}, t0);
+    step(integral_constant<int, 1>{}, t0 + 1);
+    step(integral_constant<int, 2>{}, t0 + 2);
+    step(integral_constant<int, 3>{}, t0 + 3);
+    step(integral_constant<int, 4>{}, t0 + 4);
+    step(integral_constant<int, 5>{}, t0 + 5);
+  }
+}
+
 }  // namespace glx

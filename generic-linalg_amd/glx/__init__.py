@@ -250,6 +250,9 @@ def lib():
         L.glx_autotune_normal.argtypes = [vp, i, C.POINTER(i), C.POINTER(i), C.POINTER(d), C.POINTER(d)]
         L.glx_dev_apply.restype = i
         L.glx_dev_apply.argtypes = [vp, i, d, i, i, i, vp, vp]
+        if hasattr(L, "glx_dev_time_sscg_traffic"):
+            L.glx_dev_time_sscg_traffic.restype = i
+            L.glx_dev_time_sscg_traffic.argtypes = [C.c_void_p, i, i, C.POINTER(d)]
         if hasattr(L, "glx_dev_time_stream"):
             L.glx_dev_time_stream.restype = i
             L.glx_dev_time_stream.argtypes = [i, C.c_size_t, i, C.POINTER(d), C.POINTER(d)]
@@ -868,6 +871,14 @@ def time_cg_kernel(lat, mass: float, prec: int = 0, reps: int = 100):
     _check(lib().glx_dev_time_cg_kernel(lat.h, mass, prec, reps, C.byref(ms), C.byref(by)),
            "glx_dev_time_cg_kernel")
     return ms.value, by.value
+
+
+def time_sscg_traffic(lat, link_prec: int = LINKS_F32, reps: int = 100) -> float:
+    """ms per launch of the wave-strip s-step launch's memory traffic alone
+    (the same loads and stores without the arithmetic; glx_dev_time_sscg_traffic)."""
+    ms = C.c_double(0.0)
+    _check(lib().glx_dev_time_sscg_traffic(lat.h, link_prec, reps, C.byref(ms)), "glx_dev_time_sscg_traffic")
+    return ms.value
 
 
 def copy_device(dst: DeviceArray, src: DeviceArray):

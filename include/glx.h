@@ -529,6 +529,12 @@ int glx_dev_time_stream(int mode, size_t bytes, int reps, double* ms_per_pass, d
  * HBM bytes per site one launch moves.  No reference counterpart. */
 int glx_dev_time_cg_kernel(glx_lattice lat, double mass, int prec, int reps, double* ms_per_launch,
                            double* bytes_per_site);
+/* Live timing of the memory traffic of the wave-strip s-step launch alone:
+ * the same waves, loads and stores as the launch glx_dev_cg_mixed runs on
+ * this lattice in that form, without its arithmetic (link_prec
+ * GLX_LINKS_F32 or GLX_LINKS_F16) -- the floor the access pattern sets for
+ * that launch.  Returns ms per launch.  No reference counterpart. */
+int glx_dev_time_sscg_traffic(glx_lattice lat, int link_prec, int reps, double* ms_per_launch);
 
 /* A general stencil resident in HBM (coefficients transposed for coalesced
  * reads), for device vectors of lx*ly*nc complex entries.  kind is one of

@@ -148,6 +148,26 @@ def test_wave_form_workload_1024(glx, orc):
     assert info.iter <= 1.2 * io.iter
 
 
+def test_traffic_only_timing(glx, orc):
+    """The wave-strip launch's loads and stores alone take less time than
+    the launch itself, for both link precisions; lattices the s-step form
+    does not take are refused."""
+    L = 512
+    links, _, _ = lattice_inputs(orc, L, L, beta=4.0, seed=2)
+    lat = glx.Lattice(links, L, L, f32=True, f16=True)
+    glx.set_sstep_form(1, 0)
+    try:
+        for lp in (glx.LINKS_F32, glx.LINKS_F16):
+            t = glx.time_sscg_traffic(lat, lp, reps=20)
+            ms, _ = glx.time_cg_kernel(lat, 0.05, prec=2 + lp, reps=20)
+            assert 0 < t < ms, (lp, t, ms)
+    finally:
+        glx.set_sstep_form()
+    small = glx.Lattice(links[:2 * 100 * 512], 100, 512)
+    with pytest.raises(glx.GlxError):
+        glx.time_sscg_traffic(small, glx.LINKS_F32, reps=1)
+
+
 def test_wave_form_kernel_timing(glx, orc):
     """The timing entry runs the wave-strip launch when that form is set."""
     L = 256
