@@ -1755,14 +1755,22 @@ struct PtFastFinal {
 // The stream of a solve: the caller's (`want`), else the allocation-free
 // scope's, else one created for the solve and destroyed after it.
 // The stream is taken from the calling thread's streams of finished solves
-// (created on first use, never destroyed): a stream created and destroyed
-// per solve costs a good part of a millisecond.
-static std::vector<hipStream_t>& idle_streams() {
-  static thread_local std::vector<hipStream_t>* v = new std::vector<hipStream_t>;
-  return *v;
+// on the current device (a stream created and destroyed per solve costs a
+// good part of a millisecond); the thread's streams are destroyed when it
+// exits.
+struct IdleStreams {
+  std::vector<std::pair<int, hipStream_t>> v;  // (device, stream)
+  ~IdleStreams() {
+    for (auto& e : v) (void)hipStreamDestroy(e.second);
+  }
+};
+static IdleStreams& idle_streams() {
+  static thread_local IdleStreams s;
+  return s;
 }
 struct OwnedStream {
   hipStream_t st = nullptr;
+  int dev = -1;
   bool owned = false;
   explicit OwnedStream(hipStream_t want = nullptr) {
     if (want) {
@@ -1770,20 +1778,23 @@ struct OwnedStream {
     } else if (NoAllocScope* s = noalloc_scope()) {
       st = s->stream();
     } else {
-      std::vector<hipStream_t>& idle = idle_streams();
-      if (!idle.empty()) {
-        st = idle.back();
-        idle.pop_back();
-      } else {
-        GLX_HIP(hipStreamCreate(&st));
+      GLX_HIP(hipGetDevice(&dev));
+      auto& idle = idle_streams().v;
+      for (size_t i = idle.size(); i-- > 0;) {
+        if (idle[i].first == dev) {
+          st = idle[i].second;
+          idle.erase(idle.begin() + long(i));
+          break;
+        }
       }
+      if (!st) GLX_HIP(hipStreamCreate(&st));
       owned = true;
     }
   }
   ~OwnedStream() {
     if (owned) {
       (void)hipStreamSynchronize(st);
-      idle_streams().push_back(st);
+      idle_streams().v.emplace_back(dev, st);
     }
   }
 };
