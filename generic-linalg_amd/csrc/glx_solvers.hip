@@ -2180,7 +2180,11 @@ struct SsGeo {
 };
 static SsGeo ss_geo(const DevOperator& o, int rows) {
   SsGeo g;
-  g.wave = o.kind == OP_STAG_NORMAL && sstep_form_for(o.lx, o.ly) == 1;
+  g.wave = (o.kind == OP_STAG_NORMAL || o.kind == OP_STAG_EO_SCHUR) && sstep_form_for(o.lx, o.ly) == 1;
+  // the even/odd Schur operator takes the wave strips by size only above
+  // 512^2 sites (mixed even/odd solves, form 0 against 1: 512^2 7.0 against
+  // 7.6 ms, 768^2 10.5 against 9.1, 1024^2 11.2 against 10.5)
+  if (o.kind == OP_STAG_EO_SCHUR && sstep_form() < 0 && long(o.lx) * o.ly <= 512L * 512L) g.wave = false;
   if (g.wave) g.s = make_swgeom(o.lx, o.ly, sw_rows_for(o.lx, o.ly));
   else g.e = ss_geom(o, rows);
   return g;
@@ -2238,7 +2242,10 @@ static void launch_sscg_lk(hipStream_t st, const DevOperator& o, const LK& lk, c
                            const SolveState* rd, SolveState* wr, int q, RedWs ws) {
   const OpDev<LK> od = make_opdev(o, lk);
   const EoGeom& eg = sg.e;
-  if (sg.wave)
+  if (sg.wave && o.kind == OP_STAG_EO_SCHUR)
+    hipLaunchKernelGGL((k_sscg_w<LK, SolveState, true>), dim3(sg.s.nblk), dim3(kBlock), 0, st, sg.s, od, bf, rd, wr,
+                       q, ws);
+  else if (sg.wave)
     hipLaunchKernelGGL((k_sscg_w<LK, SolveState>), dim3(sg.s.nblk), dim3(kBlock), 0, st, sg.s, od, bf, rd, wr, q,
                        ws);
   else if (o.kind == OP_STAG_EO_SCHUR)

@@ -153,7 +153,14 @@ struct SwRow {
   typename LK::Raw lk[1];
 };
 
-template <class LK, class St>
+// EO: the even/odd Schur operator m^2 - D_eo D_oe on compact even-site
+// vectors (y (lx / 2) + (x >> 1)).  The lanes still hold full lattice
+// columns: a lane whose site of the row is even loads and stores the compact
+// entry, one whose site is odd holds zeros in p, r, P1, P2 and R1 (the hops
+// carry even-site values to odd sites and back, so those stay exactly zero
+// and add nothing to the sums).  The basis and x then move half the bytes;
+// the links are the full lattice's.
+template <class LK, class St, bool EO = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
 void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, RedWs ws) {
   static_assert(kSwWin % kSwAhead == 0, "the load slots rotate with the windows");
@@ -194,9 +201,17 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
   // the first site of every march row, lane t holding row t's (steps <= 64;
   // read with readlane, one instruction per use, where the row arithmetic
   // would be several scalar ones)
-  const int rowtab = row_of(min(l, steps - 1)) * lx;
+  // (EO: the compact row offsets, y lx / 2)
+  const int rowtab = row_of(min(l, steps - 1)) * (EO ? lx / 2 : lx);
   auto rowj = [&](int t) { return __builtin_amdgcn_readlane(rowtab, t); };
   auto owned = [&](int t) { return t >= kSwHalo && t < rows + kSwHalo; };
+  // the lane's vector entry in march row t, and its site's links
+  const int xv = EO ? xl >> 1 : xl;
+  auto vj = [&](int t) { return unsigned(rowj(t) + xv); };
+  auto lj = [&](int t) { return EO ? unsigned(2 * rowj(t) + xl) : unsigned(rowj(t) + xl); };
+  // EO: whether the lane's site is even in march rows of even t (ev0) and
+  // of odd t (ev1); rows alternate parity along the march (ly is even)
+  const bool ev0 = ((xl + row_of(0)) & 1) == 0, ev1 = !ev0;
   const int so = ko & 1, sn = so ^ 1;
   const c64* Y[5] = {bf.v(so, 0), bf.v(so, 1), bf.v(so, 2), bf.v(so, 3), bf.v(so, 4)};
   c64* N[5] = {bf.v(sn, 0), bf.v(sn, 1), bf.v(sn, 2), bf.v(sn, 3), bf.v(sn, 4)};
@@ -205,11 +220,12 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
   // would make the compiler wait for all loads in flight where the branches
   // join); past the last row the last row is loaded again
   auto load_row = [&](SwRow<LK>& d, int t) {
-    const unsigned j = unsigned(rowj(min(t, steps - 1)) + xl);
+    const int tt = min(t, steps - 1);
+    const unsigned j = vj(tt);
 #pragma unroll
     for (int i = 0; i < 5; i++) d.y[i] = ss_at<c64>(Y[i], j * 8u);
     d.x = ss_at<c64>(bf.x, j * 8u);
-    ss_links<1>(op.links, j, d.lk);
+    ss_links<1>(op.links, lj(tt), d.lk);
   };
 
   constexpr int kPreU = 2;
@@ -274,7 +290,8 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
     // stopped: x += Y c_x and P0' = Y c_p on the owned sites
     if (active && own_lane) {
       for (int t = kSwHalo; t < rows + kSwHalo; t++) {
-        const unsigned j = unsigned(row_of(t) * lx + xl);
+        if (EO && !((t & 1) ? ev1 : ev0)) continue;
+        const unsigned j = vj(t);
         c64 yv[5];
 #pragma unroll
         for (int i = 0; i < 5; i++) yv[i] = ss_at<c64>(Y[i], j * 8u);
@@ -306,7 +323,11 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
       // S0: row t
       {
         SwRow<LK>& cur = ld[LA];
-        const c64 yv[5] = {cur.y[0], cur.y[1], cur.y[2], cur.y[3], cur.y[4]};
+        // (EO: the lanes whose site is odd take zeros)
+        const c64 z{0.0f, 0.0f};
+        const bool ev = (K & 1) ? ev1 : ev0;
+        const c64 yv[5] = {!EO || ev ? cur.y[0] : z, !EO || ev ? cur.y[1] : z, !EO || ev ? cur.y[2] : z,
+                           !EO || ev ? cur.y[3] : z, !EO || ev ? cur.y[4] : z};
         p[S0] = ss_comb_p(cp, yv);
         r[S0] = ss_comb_r(cr, yv);
         C u0, u1;
@@ -321,7 +342,7 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
         const c64 vd = yh * u1, vu = c64{-yh * u1.re, yh * u1.im};
         uy[S0] = up ? uy[S0] : vd;
         uy[S1] = up ? vu : uy[S1];
-        xn = cur.x + ss_comb_x(cx, yv);
+        xn = (!EO || ev ? cur.x : z) + ss_comb_x(cx, yv);
         load_row(cur, t + kSwAhead);
       }
       // S1: row t - 1
@@ -339,15 +360,16 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
         r1 = sw_mhop(m2, r[S4], qr[S4], qr[S3], qr[S5], ux[S4], uxm[S4], uy[S4], uy[S5]);
       }
       // the stores and sums of the lanes that own their column, in one
-      // masked region per step
-      if (own_lane) {
+      // masked region per step (EO: the rows t, t - 2 and t - 4 have one
+      // parity; only the lanes whose site is even store)
+      if (own_lane && (!EO || ((K & 1) ? ev1 : ev0))) {
         if (owned(t)) {
-          const unsigned j = unsigned(rowj(t) + xl) * 8u;
+          const unsigned j = vj(t) * 8u;
           ss_at<c64>(N[0], j) = p[S0];
           ss_at<c64>(N[3], j) = r[S0];
           ss_at<c64>(bf.x, j) = xn;
         }
-        if (owned(t - 2)) ss_at<c64>(N[1], unsigned(rowj(t - 2) + xl) * 8u) = P1[S2];
+        if (owned(t - 2)) ss_at<c64>(N[1], vj(t - 2) * 8u) = P1[S2];
         if (o4) {
           const c64 pp = p[S4], pP1 = P1[S4], rr = r[S4];
           part[0] = ss_acc(part[0], pp, pp);
@@ -362,7 +384,7 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
           part[9] = ss_acc(part[9], rr, rr);
           part[10] = ss_acc(part[10], rr, r1);
           part[11] = ss_acc(part[11], r1, r1);
-          const unsigned j = unsigned(rowj(t - 4) + xl) * 8u;
+          const unsigned j = vj(t - 4) * 8u;
           ss_at<c64>(N[2], j) = p2;
           ss_at<c64>(N[4], j) = r1;
         }

@@ -466,7 +466,9 @@ int glx_get_sstep_rows(int lx, int ly, int* rows);
  * chooses per lattice size: the size's entry (glx_set_sstep_form_entry, set
  * by glx_autotune_sstep), else form 1 from 512^2 sites up to lattices that
  * fit 1024 waves at bands of 52 rows, form 0 otherwise.
- * glx_get_sstep_form_for returns what a solve of that size would use. */
+ * glx_get_sstep_form_for returns what a solve of that size would use.  The
+ * mixed CG of the even/odd solve (the compact Schur operator) follows the
+ * same choice, except that by size it takes form 1 only above 512^2 sites. */
 int glx_set_sstep_form(int form, int band_rows);
 int glx_get_sstep_form(int* form);
 int glx_set_sstep_form_entry(int lx, int ly, int form);

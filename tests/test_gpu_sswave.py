@@ -20,7 +20,7 @@ case, as for the default form:
 import numpy as np
 import pytest
 
-from helpers import ENVELOPE, cg_envelope, lattice_inputs, relerr
+from helpers import ENVELOPE, cg_envelope, eo_envelope, lattice_inputs, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -180,3 +180,55 @@ def test_wave_form_kernel_timing(glx, orc):
         glx.set_sstep_form()
     assert ms > 0 and by == 112
     assert glx.get_sstep_form() == -1
+
+
+@pytest.mark.parametrize("lx,ly,link", [(256, 64, "mixed"), (128, 8, "mixed"), (384, 40, "mixed_f16"),
+                                        (512, 72, "mixed"), (640, 44, "mixed")])
+def test_wave_form_even_odd(glx, orc, lx, ly, link):
+    """The even/odd solve of D x = b whose mixed CG runs the wave-strip
+    launch on the compact Schur operator: the full residual bound of the
+    pipeline, the solution within ENVELOPE of the FP64 even/odd CG's error,
+    and the iteration count within max(6, 3 %) of the block form's."""
+    m = 0.05
+    links, bc, _ = lattice_inputs(orc, lx, ly, beta=6.25, seed=43 + lx + ly)
+    lat = glx.Lattice(links, lx, ly)
+
+    def solve(form):
+        glx.set_sstep_form(form, 0)
+        try:
+            dx = glx.DeviceArray(np.zeros_like(bc))
+            info, r2 = glx.dev_solve_staggered_eo(lat, m, dx, glx.DeviceArray(bc), link, max_iter=5000, eps=1e-10)
+            return dx.download(), info
+        finally:
+            glx.set_sstep_form()
+
+    x, info = solve(1)
+    assert info.success
+    r = bc - orc.apply("staggered", x, links, m, lx, ly)
+    assert np.linalg.norm(r) <= 1e-10 * (m + 2.0) / m * np.linalg.norm(bc) * 1.01
+    xs, err_o = eo_envelope(orc, bc, links, m, lx, ly)
+    print(f"{lx}x{ly} {link} wave form: {info.iter} iterations, |x - x*|/|x*| {relerr(x, xs):.2e}, FP64 even/odd "
+          f"CG at the same tolerance {err_o:.2e}")
+    assert relerr(x, xs) <= ENVELOPE * err_o, (relerr(x, xs), err_o)
+    _, i0 = solve(0)
+    assert abs(info.iter - i0.iter) <= max(6, 0.03 * i0.iter), (info.iter, i0.iter)
+
+
+def test_wave_form_even_odd_workload_1024(glx, orc):
+    """The 1024 x 1024 even/odd solve (m = 0.02, theta ~ N(0, 0.5^2)) in the
+    wave-strip form (its default at this size) against the oracle's FP64
+    even/odd pipeline."""
+    L, m = 1024, 0.02
+    links, bc, _ = lattice_inputs(orc, L, L, beta=4.0, seed=3003)
+    lat = glx.Lattice(links, L, L)
+    assert glx.get_sstep_form_for(L, L) == 1
+    dx = glx.DeviceArray(np.zeros_like(bc))
+    info, r2 = glx.dev_solve_staggered_eo(lat, m, dx, glx.DeviceArray(bc), "mixed", max_iter=100000, eps=1e-10)
+    x = dx.download()
+    assert info.success
+    r = bc - orc.apply("staggered", x, links, m, L, L)
+    assert np.linalg.norm(r) <= 1e-10 * (m + 2.0) / m * np.linalg.norm(bc) * 1.01
+    xs, err_o = eo_envelope(orc, bc, links, m, L, L)
+    print(f"1024^2 even/odd wave form: {info.iter} iterations, |x - x*|/|x*| {relerr(x, xs):.2e} (FP64 even/odd CG "
+          f"{err_o:.2e})")
+    assert relerr(x, xs) <= ENVELOPE * err_o, (relerr(x, xs), err_o)
