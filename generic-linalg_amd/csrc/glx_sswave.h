@@ -319,7 +319,13 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
     for (int s = 0; s < kSwWin; s++)
       p[s] = r[s] = qp[s] = qr[s] = P1[s] = q1[s] = ux[s] = uxm[s] = uy[s] = c64{0.0f, 0.0f};
 
-    auto step = [&](auto kc, int t, auto first) __attribute__((always_inline)) {
+    // R0: the first round of the march (t = K), apart so that its steps take
+    // the rows the prelude loaded and leave out the stages whose rows no
+    // owned result needs (S1 before t = 2, S2 before t = 4, S3 and S4 in the
+    // whole round: the owned rows start at t = 4, so S4 needs q1 from row 3,
+    // q1 needs P1 from row 2, P1 needs qp from row 1)
+    auto step = [&](auto kc, int t, auto rc) __attribute__((always_inline)) {
+      constexpr bool R0 = decltype(rc)::value;
       constexpr int K = decltype(kc)::value;
       constexpr int S0 = K, S1 = (K + 5) % 6, S2 = (K + 4) % 6, S3 = (K + 3) % 6, S4 = (K + 2) % 6,
                     S5 = (K + 1) % 6;
@@ -350,18 +356,21 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
         uy[S0] = up ? uy[S0] : vd;
         uy[S1] = up ? vu : uy[S1];
         xn = (!EO || ev ? cur.x : z) + ss_comb_x(cx, yv);
-        if constexpr (decltype(first)::value) cur = ld2[LA];
+        if constexpr (R0 && K < kSwAhead) cur = ld2[LA];
         else load_row(cur, t + kSwAhead);
       }
       // S1: row t - 1
-      qp[S1] = sw_hop(p[S1], p[S0], p[S2], ux[S1], uxm[S1], uy[S1], uy[S2]);
-      qr[S1] = sw_hop(r[S1], r[S0], r[S2], ux[S1], uxm[S1], uy[S1], uy[S2]);
+      if constexpr (!R0 || K >= 2) {
+        qp[S1] = sw_hop(p[S1], p[S0], p[S2], ux[S1], uxm[S1], uy[S1], uy[S2]);
+        qr[S1] = sw_hop(r[S1], r[S0], r[S2], ux[S1], uxm[S1], uy[S1], uy[S2]);
+      }
       // S2: row t - 2
-      P1[S2] = sw_mhop(m2, p[S2], qp[S2], qp[S1], qp[S3], ux[S2], uxm[S2], uy[S2], uy[S3]);
+      if constexpr (!R0 || K >= 4)
+        P1[S2] = sw_mhop(m2, p[S2], qp[S2], qp[S1], qp[S3], ux[S2], uxm[S2], uy[S2], uy[S3]);
       // S3: row t - 3
-      q1[S3] = sw_hop(P1[S3], P1[S2], P1[S4], ux[S3], uxm[S3], uy[S3], uy[S4]);
+      if constexpr (!R0) q1[S3] = sw_hop(P1[S3], P1[S2], P1[S4], ux[S3], uxm[S3], uy[S3], uy[S4]);
       // S4: row t - 4, the owned rows only
-      const bool o4 = owned(t - 4);
+      const bool o4 = !R0 && owned(t - 4);
       c64 p2, r1;
       if (o4) {
         p2 = sw_mhop(m2, P1[S4], q1[S4], q1[S3], q1[S5], ux[S4], uxm[S4], uy[S4], uy[S5]);
@@ -377,7 +386,7 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
           ss_at<c64>(N[3], j) = r[S0];
           ss_at<c64>(bf.x, j) = xn;
         }
-        if (owned(t - 2)) ss_at<c64>(N[1], vj(t - 2) * 8u) = P1[S2];
+        if (!R0 && owned(t - 2)) ss_at<c64>(N[1], vj(t - 2) * 8u) = P1[S2];
         if (o4) {
           const c64 pp = p[S4], pP1 = P1[S4], rr = r[S4];
           part[0] = ss_acc(part[0], pp, pp);
@@ -401,16 +410,15 @@ void k_sscg_w(SwGeom g, OpDev<LK> op, SsBufs bf, const St* rd, St* wr, int q, Re
     // whole rounds of six steps: the steps past the last row store nothing
     // (no row they reach is owned) and load the last row again
     using std::integral_constant;
-    // (the first round apart: its steps take the rows loaded in the prelude)
     using std::true_type;
     using std::false_type;
     if (active) {
       step(integral_constant<int, 0>{}, 0, true_type{});
       step(integral_constant<int, 1>{}, 1, true_type{});
       step(integral_constant<int, 2>{}, 2, true_type{});
-      step(integral_constant<int, 3>{}, 3, false_type{});
-      step(integral_constant<int, 4>{}, 4, false_type{});
-      step(integral_constant<int, 5>{}, 5, false_type{});
+      step(integral_constant<int, 3>{}, 3, true_type{});
+      step(integral_constant<int, 4>{}, 4, true_type{});
+      step(integral_constant<int, 5>{}, 5, true_type{});
     }
     for (int t0 = kSwWin; t0 < (active ? steps : 0); t0 += kSwWin) {
       step(integral_constant<int, 0>{}, t0, false_type{});
