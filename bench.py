@@ -458,7 +458,8 @@ def eo_solves(glx, lat, b, mass, reps=3):
 
 
 def stencil_timing(glx, lat, b, mass, L, reps=200):
-    """The operators alone at L^2 and 2 L^2: D (the tiled kernel) and the fused
+    """The operators alone at L^2 and 2 L^2: D (the wave-march kernel on
+    lattices of whole 64-column strips, else the tiled kernel) and the fused
     D^dag D, FP64 and FP32 (and FP16 links), HIP events around `reps`
     back-to-back launches.  A lone D^dag D application takes its own best
     rows per block (the table's entry is tuned for the CG iteration kernel,
@@ -489,6 +490,8 @@ def stencil_timing(glx, lat, b, mass, L, reps=200):
             glx.set_normal_rows(n, n, 16, saved[0])
             glx.set_normal_rows(n, n, 8, saved[1])
             for prec, (ms, rows) in best.items():
+                if tag == "D":  # the wave-march kernel's rows per wave (0: the tiled kernel)
+                    rows = glx.get_dwave_rows(n, n, 16 if prec == "fp64" else 8)
                 by = {"fp64": 64, "fp32": 32, "fp16link": 28}[prec] * V
                 gbs = by / (ms * 1e-3) / 1e9
                 res[f"{tag}_{prec}"] = {"us": round(ms * 1e3, 2), "GBs": round(gbs, 1),
@@ -526,6 +529,16 @@ def kernel_only(glx, args):
     alone (with its phase timing), for profiles of its kernels."""
     if args.kernel_only == "batch":
         out, _ = batch_multishift(glx, seq=False, keep_host=False)
+        print(json.dumps(out))
+        return 0
+    if args.kernel_only == "stencil":
+        links, b = make_inputs(args.L, args.sigma, seed=1337)
+        lat = glx.Lattice(links, args.L, args.L, f32=True, f16=True)
+        out = stencil_timing(glx, lat, b, args.mass, args.L, reps=args.reps)
+        c_small = stream_ceiling(glx)["64MiB"]["read3_write1"]["GBs"]
+        for v in out["at_L"].values():
+            if isinstance(v, dict) and "GBs" in v:
+                v["frac_of_ceiling"] = round(v["GBs"] / c_small, 4)
         print(json.dumps(out))
         return 0
     L = args.L
@@ -568,11 +581,12 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (a plain copy kernel of known size, "
                          "for scaling rocprofv3 byte counters)")
-    ap.add_argument("--kernel-only", choices=["sscg", "fast32", "fast64", "batch"], default=None,
+    ap.add_argument("--kernel-only", choices=["sscg", "fast32", "fast64", "batch", "stencil"], default=None,
                     help="only time one CG iteration kernel at --L (HIP events, --reps launches) and print its "
                          "line: the s-step launch, the FP32 one-iteration kernel or the FP64 single-pass kernel "
                          "(for counter runs of that kernel alone); 'batch': only the batched multishift call and "
-                         "its phase timing, without the one-call-each comparison")
+                         "its phase timing, without the one-call-each comparison; 'stencil': only the operators alone "
+                         "(D and D^dag D at L^2 and 2 L^2) and the streaming ceiling")
     ap.add_argument("--reps", type=int, default=100)
     ap.add_argument("--sstep-form", type=int, choices=[0, 1], default=None,
                     help="s-step launch form on D^dag D: 0 blocks of four waves through LDS rings, 1 wave strips in "

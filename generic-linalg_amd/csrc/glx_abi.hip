@@ -1534,6 +1534,20 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
   });
 }
 
+int glx_set_dwave_rows(int rows) {
+  return guarded([&] {
+    if (rows < -1 || rows > 4096) throw ArgError("rows must be -1 (by lattice size), 0 (tiled) or 1..4096");
+    set_dwave_rows(rows);
+  });
+}
+int glx_get_dwave_rows(int lx, int ly, int esize, int* rows) {
+  return guarded([&] {
+    if (!rows) throw ArgError("rows is NULL");
+    if (lx <= 0 || ly <= 0) throw ArgError("bad lattice size");
+    if (esize != 8 && esize != 16) throw ArgError("esize must be 8 (complex FP32) or 16 (complex FP64)");
+    *rows = dwave_fits(lx, ly) ? dwave_rows_for(lx, ly, esize) : 0;
+  });
+}
 int glx_set_normal_rows(int lx, int ly, int esize, int rows) {
   return guarded([&] {
     if (lx < 0 || ly < 0 || (lx == 0) != (ly == 0)) throw ArgError("bad lattice size");

@@ -1,0 +1,177 @@
+// glx_dwave.h -- the staggered operator D = m - K and D^dag = m + K
+// (operators.cpp:184-239 and :372-441) as a wave-march kernel: registers
+// only, no LDS, no barriers.
+//
+// A wave owns a strip of 64 lattice columns (lane l <-> column x0 + l) and a
+// band of `rows` rows, and marches the band one row per step.  It holds the
+// input rows y - 1, y, y + 1 and the links of rows y and y -/+ 1 in a ring of
+// kDwSlots row records and keeps the loads of kDwAhead further rows in
+// flight.  The x-neighbours come from the adjacent lanes by DPP wave shifts;
+// the two columns beyond the strip (x0 - 1 for lane 0, x0 + 64 for lane 63)
+// and lane 0's U_x(x0 - 1) come from one extra load each per row, in which
+// every other lane reads its own site again (an L1 hit).  Odd bands march
+// upwards, so the halo row at a band boundary is read by both bands near the
+// same time (an L2 hit for the second).
+//
+// Per site: the input element and the two links read, the output written
+// once (2 s_v + 2 s_u bytes); the halo rows, one per band end, mostly hit L2.
+//
+// The site arithmetic is the tiled kernel's (glx_stencil.h), term for term,
+// so both kernels give the same result up to the compiler's contraction.
+//
+// Requirements (checked on the host by dwave_fits): lx a multiple of 64,
+// ly >= 4; complex vectors; the plain operator (no prologue, epilogue or
+// reduction).  Other shapes and the fused solver launches use the tiled
+// kernel.
+#pragma once
+
+#include "glx_stencil.h"
+
+#include <utility>
+
+namespace glx {
+
+constexpr int kDwAhead = 3;             // rows whose loads are in flight beyond the three in use
+constexpr int kDwSlots = kDwAhead + 3;  // row records in the ring (the march unrolls rounds of this many rows)
+constexpr int kDwWaves = 4;             // waves per block
+
+struct DwGeom {
+  int lx, ly;
+  int nstrip;  // lx / 64
+  int rows;    // rows per band
+  int ntask;   // nstrip * bands
+  int nblk;    // launched blocks of kDwWaves waves (multiple of 8)
+};
+
+inline bool dwave_fits(int lx, int ly) { return lx >= 64 && lx % 64 == 0 && ly >= 4; }
+
+inline DwGeom make_dwgeom(int lx, int ly, int rows) {
+  DwGeom g;
+  g.lx = lx;
+  g.ly = ly;
+  g.nstrip = lx / 64;
+  g.rows = rows;
+  g.ntask = g.nstrip * ((ly + rows - 1) / rows);
+  const int blocks = (g.ntask + kDwWaves - 1) / kDwWaves;
+  g.nblk = ((blocks + 7) / 8) * 8;
+  return g;
+}
+
+// DPP wave shifts of a register value, word by word: lane l takes lane
+// l + 1's value (kDwNext, "wave_shl:1") or lane l - 1's (kDwPrev,
+// "wave_shr:1"); the lane with no source (63, resp. 0) keeps `old`.
+constexpr int kDwNext = 0x130;
+constexpr int kDwPrev = 0x138;
+template <int CTRL, class V>
+__device__ __forceinline__ V dw_shift(V old, V src) {
+  static_assert(sizeof(V) % 4 == 0, "whole words only");
+  constexpr int N = int(sizeof(V) / 4);
+  int o[N], s[N], r[N];
+  __builtin_memcpy(o, &old, sizeof(V));
+  __builtin_memcpy(s, &src, sizeof(V));
+#pragma unroll
+  for (int i = 0; i < N; i++) r[i] = __builtin_amdgcn_update_dpp(o[i], s[i], CTRL, 0xf, 0xf, false);
+  V out;
+  __builtin_memcpy(&out, r, sizeof(V));
+  return out;
+}
+
+template <class T, class LK>
+struct DwRow {
+  T v, ve;                   // the site's element; the edge element (lane 0: x0 - 1, lane 63: x0 + 64)
+  typename LK::Raw lk, lke;  // the site's links; lane 0: those of x0 - 1
+};
+
+// f(K, t0 + K, tag) for K = 0 .. N - 1, K a compile-time constant
+template <class F, class Tag, int... K>
+__device__ __forceinline__ void dw_round(F& f, int t0, Tag tag, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}, t0 + K, tag), ...);
+}
+
+// a lattice row index in [-1, ly] brought into [0, ly)
+__device__ __forceinline__ int dw_wrap(int y, int ly) { return y < 0 ? y + ly : (y >= ly ? y - ly : y); }
+
+template <int KIND, bool UP, class T, class LK>
+__device__ __forceinline__ void dwave_march(const DwGeom& g, const OpDev<LK>& op, const T* __restrict__ in,
+                                            T* __restrict__ out, int x, int xe, int y0, int rows, bool eta_neg) {
+  using C = typename LK::C;
+  using R = typename Real<T>::type;
+  const int lx = g.lx, ly = g.ly;
+  // march row m = 0 .. rows + 1 -> lattice row (downwards: y0 - 1 + m)
+  auto row_of = [&](int m) { return dw_wrap(UP ? y0 + rows - m : y0 - 1 + m, ly); };
+  auto load = [&](DwRow<T, LK>& d, int m) {
+    const int y = row_of(min(m, rows + 1));  // (the rows past the band's end load its last row again)
+    const int j = y * lx + x, je = y * lx + xe;
+    d.v = in[j];
+    d.ve = in[je];
+    d.lk = op.links.fetch(j);
+    d.lke = op.links.fetch(je);
+  };
+  DwRow<T, LK> ring[kDwSlots];
+#pragma unroll
+  for (int s = 0; s < kDwSlots - 1; s++) load(ring[s], s);
+  // step t writes march row t + 1 from march rows t, t + 1, t + 2
+  auto step = [&](auto kc, int t, auto tail) __attribute__((always_inline)) {
+    constexpr int K = decltype(kc)::value;
+    constexpr int S0 = K, S1 = (K + 1) % kDwSlots, S2 = (K + 2) % kDwSlots, SN = (K + kDwSlots - 1) % kDwSlots;
+    // (the scheduling fences keep each step's arithmetic between its own
+    // loads and the next step's: the compiler otherwise hoists the whole
+    // round's arithmetic to its top, which waits for every row in flight)
+    __builtin_amdgcn_sched_barrier(0);
+    load(ring[SN], t + kDwSlots - 1);  // (the slot of march row t - 1, done with)
+    const DwRow<T, LK>& a = ring[S0];
+    const DwRow<T, LK>& b = ring[S1];
+    const DwRow<T, LK>& c = ring[S2];
+    C ux, uy, uxe, t0, uym;
+    LK::finish(b.lk, ux, uy);
+    LK::finish(b.lke, uxe, t0);
+    LK::finish(UP ? c.lk : a.lk, t0, uym);  // U_y(y - 1)
+    const T fyp = UP ? a.v : c.v, fym = UP ? c.v : a.v;
+    const T fxp = dw_shift<kDwNext>(b.ve, b.v), fxm = dw_shift<kDwPrev>(b.ve, b.v);
+    const C uxm = dw_shift<kDwPrev>(uxe, ux);
+    T ax = umul(ux, fxp) - ucmul(uxm, fxm);
+    T ay = umul(uy, fyp) - ucmul(uym, fym);
+    if (eta_neg) ay = -ay;
+    const T kp = R(0.5) * (ax + ay);
+    T v;
+    if constexpr (KIND == OP_STAG) v = rscal(op.mass, b.v) - kp;
+    else v = rscal(op.mass, b.v) + kp;
+    if (!decltype(tail)::value || t < rows) out[row_of(t + 1) * lx + x] = v;
+  };
+  // whole rounds, every step storing its row; then the round that ends past
+  // the band's last row, whose steps past it store nothing
+  const int tfull = rows - rows % kDwSlots;
+  for (int t0 = 0; t0 < tfull; t0 += kDwSlots)
+    dw_round(step, t0, std::false_type{}, std::make_integer_sequence<int, kDwSlots>{});
+  if (tfull < rows) dw_round(step, tfull, std::true_type{}, std::make_integer_sequence<int, kDwSlots>{});
+}
+
+// (amdgpu_waves_per_eu: without it the compiler aims at eight waves per SIMD,
+// 64 registers, and moves the ring's loads down to their uses)
+template <int KIND, class T, class LK>
+__global__ __launch_bounds__(64 * kDwWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_dwave(DwGeom g, OpDev<LK> op, const T* __restrict__ in,
+                                                  T* __restrict__ out, Gate gate) {
+  if (gate_closed(gate)) return;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int task = xcd_tile(blockIdx.x, g.nblk) * kDwWaves + w;
+  if (task >= g.ntask) return;
+  const int strip = task % g.nstrip, band = task / g.nstrip;
+  const int x0 = strip * 64, x = x0 + l;
+  const int xe = l == 0 ? (x0 == 0 ? g.lx - 1 : x0 - 1) : (l == 63 ? (x0 + 64 == g.lx ? 0 : x0 + 64) : x);
+  const int y0 = band * g.rows;
+  const int rows = min(g.rows, g.ly - y0);
+  const bool eta_neg = (x & 1) != 0;
+  if (band & 1) dwave_march<KIND, true>(g, op, in, out, x, xe, y0, rows, eta_neg);
+  else dwave_march<KIND, false>(g, op, in, out, x, xe, y0, rows, eta_neg);
+}
+
+template <class T, class LK>
+void launch_dwave(hipStream_t st, int kind, const DwGeom& g, const OpDev<LK>& op, const T* in, T* out, Gate gate) {
+  if (kind == OP_STAG)
+    hipLaunchKernelGGL((k_dwave<OP_STAG, T, LK>), dim3(g.nblk), dim3(64 * kDwWaves), 0, st, g, op, in, out, gate);
+  else
+    hipLaunchKernelGGL((k_dwave<OP_STAG_DAG, T, LK>), dim3(g.nblk), dim3(64 * kDwWaves), 0, st, g, op, in, out, gate);
+  GLX_CHECK_LAUNCH();
+}
+
+}  // namespace glx

@@ -6,6 +6,7 @@
 #pragma once
 
 #include "glx_cstencil.h"
+#include "glx_dwave.h"
 #include "glx_eo.h"
 
 #include <functional>
@@ -180,6 +181,13 @@ void set_normal_rows(int lx, int ly, int esize, int rows);
 // the size's table entry, or -1 when it has none
 int normal_rows_entry(int lx, int ly, int esize);
 
+// Rows per wave of the wave-march D / D^dag kernel (glx_dwave.h) for a
+// lattice size and vector element size in bytes; 0 = use the tiled kernel.
+// set_dwave_rows: -1 by lattice size (the default), 0 the tiled kernel
+// always, > 0 that many rows.
+int dwave_rows_for(int lx, int ly, int esize);
+void set_dwave_rows(int rows);
+
 // Rows per block of the even/odd Schur kernel (compact rows of lx / 2 sites;
 // "tiled" does not exist for this operator).
 inline int eo_rows_for(int lx, int ly, int esize) {
@@ -223,6 +231,17 @@ void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T*
                                                           normal_cpl<T>()),
                                            od, in, out, epi, fin, ws, done, fin_launch);
       return;
+    }
+  }
+  // the plain D and D^dag on lattices of whole 64-column strips: wave-march kernel
+  if constexpr (is_cx<T>::value && std::is_same<In, InPlain<T>>::value && std::is_same<Epi, EpiNone<T>>::value &&
+                std::is_same<Fin, FinNone>::value) {
+    if ((o.kind == OP_STAG || o.kind == OP_STAG_DAG) && dwave_fits(o.lx, o.ly)) {
+      const int rows = dwave_rows_for(o.lx, o.ly, int(sizeof(T)));
+      if (rows > 0) {
+        launch_dwave<T, LK>(st, o.kind, make_dwgeom(o.lx, o.ly, rows), od, in.p, out, done);
+        return;
+      }
     }
   }
   if (o.kind == OP_STAG_EO_SCHUR)

@@ -55,6 +55,23 @@ int normal_rows_for(int lx, int ly, int esize, int cols) {
     if (strips * ((ly + r - 1) / r) <= 512) return r;
   return 256;
 }
+static std::atomic<int> g_dw_rows{-1};
+// By lattice size: 16 rows per wave for complex FP32 and 18 for FP64 (at
+// 1024^2, FP32 5.95 us at 16 rows against 6.3 at 18 and 6.9 at 8; FP64 11.5
+// us at 18 against 12.2 at 16), unless the lattice then fills fewer than 768
+// of the 1024 SIMDs: then the rows that give about one wave per SIMD, at
+// least 4.
+int dwave_rows_for(int lx, int ly, int esize) {
+  const int forced = g_dw_rows.load();
+  if (forced >= 0) return forced;
+  const long nstrip = lx / 64;
+  const int cap = esize == 16 ? 18 : 16;
+  if (nstrip * ((ly + cap - 1) / cap) >= 768) return cap;
+  return std::max(4, int((ly * nstrip + 1023) / 1024));
+}
+void set_dwave_rows(int rows) {
+  g_dw_rows.store(rows < 0 ? -1 : rows);
+}
 void set_normal_rows(int lx, int ly, int esize, int rows) {
   NormalTable& t = normal_table();
   std::lock_guard<std::mutex> g(t.mu);

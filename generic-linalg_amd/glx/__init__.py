@@ -244,6 +244,10 @@ def lib():
         L.glx_dev_eo_reconstruct.argtypes = [vp, d, vp, vp, vp]
         L.glx_set_normal_rows.restype = i
         L.glx_set_normal_rows.argtypes = [i, i, i, i]
+        L.glx_set_dwave_rows.restype = i
+        L.glx_set_dwave_rows.argtypes = [i]
+        L.glx_get_dwave_rows.restype = i
+        L.glx_get_dwave_rows.argtypes = [i, i, i, C.POINTER(i)]
         L.glx_get_normal_rows.restype = i
         L.glx_get_normal_rows.argtypes = [i, i, i, C.POINTER(i)]
         L.glx_autotune_normal.restype = i
@@ -706,6 +710,21 @@ def get_sstep():
     s, rows = C.c_int(), C.c_int()
     _check(lib().glx_get_sstep(C.byref(s), C.byref(rows)), "glx_get_sstep")
     return s.value, rows.value
+
+
+def set_dwave_rows(rows: int = -1):
+    """Rows per wave of the wave-march kernel of the plain D and D^dag
+    (lattices with lx % 64 == 0, ly >= 4): -1 by lattice size (the default),
+    0 the tiled kernel always, > 0 that many."""
+    _check(lib().glx_set_dwave_rows(rows), "glx_set_dwave_rows")
+
+
+def get_dwave_rows(lx: int, ly: int, esize: int = 8) -> int:
+    """Rows per wave an lx x ly apply of D or D^dag with vector element size
+    esize (8 complex FP32, 16 complex FP64) would use (0: tiled kernel)."""
+    r = C.c_int(0)
+    _check(lib().glx_get_dwave_rows(lx, ly, esize, C.byref(r)), "glx_get_dwave_rows")
+    return r.value
 
 
 def set_normal_rows(lx: int, ly: int, esize: int, rows: int):

@@ -497,6 +497,18 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
  * tiled kernel. */
 int glx_set_normal_rows(int lx, int ly, int esize, int rows);
 int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
+/* Launch shape of the plain staggered D and D^dag on complex vectors (no
+ * reference counterpart; operators.cpp:184-239 and :372-441 are the
+ * operators they compute): rows per wave of the wave-march kernel, which
+ * serves lattices with lx % 64 == 0 and ly >= 4 (other sizes, and the
+ * solvers' fused launches, use the tiled kernel).  rows = -1 (the initial
+ * setting): by lattice size (16 rows for complex FP32, 18 for FP64, fewer
+ * where the lattice would fill fewer than 768 SIMDs); 0: the tiled kernel
+ * always; > 0: that many rows.  glx_get_dwave_rows returns what an lx x ly
+ * apply with vector element size esize (8 = complex FP32, 16 = complex FP64)
+ * would use (0: the tiled kernel). */
+int glx_set_dwave_rows(int rows);
+int glx_get_dwave_rows(int lx, int ly, int esize, int* rows);
 /* Autotuner: for every candidate launch shape (tiled; 8..128 rows per block,
  * and 256 where ly >= 2048)
  * times `reps` iterations of the single-pass FP64 CG (glx_dev_cg_fused) and,

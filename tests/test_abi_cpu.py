@@ -150,6 +150,29 @@ def test_normal_rows_default_without_device():
     assert lib.glx_get_normal_rows(2048, 2048, 16, C.byref(r)) == 0 and r.value == 128
 
 
+def test_dwave_rows_setting_without_device():
+    """glx_set_dwave_rows validates its argument; glx_get_dwave_rows gives
+    the measured shapes (16 rows FP32, 18 FP64 where the lattice fills at
+    least 768 SIMDs, else about one wave per SIMD, at least 4 rows) and 0
+    (the tiled kernel) for widths that are not whole 64-column strips (no
+    device work)."""
+    lib = C.CDLL(LIB)
+    lib.glx_last_error.restype = C.c_char_p
+    r = C.c_int()
+    want = {(1024, 1024, 8): 16, (1024, 1024, 16): 18, (2048, 2048, 8): 16, (4096, 4096, 16): 18,
+            (512, 512, 8): 4, (768, 768, 8): 9, (64, 64, 16): 4, (96, 64, 8): 0, (64, 2, 8): 0}
+    for (lx, ly, es), rows in want.items():
+        assert lib.glx_get_dwave_rows(lx, ly, es, C.byref(r)) == 0 and r.value == rows, (lx, ly, es, r.value)
+    assert lib.glx_set_dwave_rows(7) == 0
+    assert lib.glx_get_dwave_rows(1024, 1024, 8, C.byref(r)) == 0 and r.value == 7
+    assert lib.glx_set_dwave_rows(0) == 0
+    assert lib.glx_get_dwave_rows(1024, 1024, 16, C.byref(r)) == 0 and r.value == 0
+    assert lib.glx_set_dwave_rows(-2) == 1 and b"rows must be" in lib.glx_last_error()
+    assert lib.glx_get_dwave_rows(0, 8, 8, C.byref(r)) == 1
+    assert lib.glx_get_dwave_rows(64, 8, 4, C.byref(r)) == 1 and b"esize" in lib.glx_last_error()
+    assert lib.glx_set_dwave_rows(-1) == 0
+
+
 def test_sstep_setting_without_device():
     """glx_set_sstep validates its arguments and glx_get_sstep reads back the
     process default (no device work)."""
