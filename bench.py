@@ -168,8 +168,9 @@ def cpu_model():
 def stencil_beyond_mall(glx, L=4096):
     """FP32 D^dag D (FP32 links) at 4096^2: 128 MiB per vector and 256 MiB of
     links, beyond the 256 MB last-level cache, so the figure is HBM-bound.
-    The launch shape of the operator alone is tuned here over rows per block
-    (the autotuner tunes the CG iteration kernel, which stages more data)."""
+    The wave-march kernel at its own rows against the row-marching kernel
+    at each candidate rows per block (the autotuner tunes the CG iteration
+    kernel, which stages more data); the faster is reported."""
     V = L * L
     rng = np.random.default_rng(5)
     links = np.exp(1j * rng.normal(0.0, 0.5, size=2 * V)).astype(np.complex128)
@@ -178,26 +179,33 @@ def stencil_beyond_mall(glx, L=4096):
     d_in = glx.DeviceArray(x)
     d_out = glx.DeviceArray(np.zeros_like(x))
     best = None
-    for rows in (32, 64, 96, 128):
-        glx.set_normal_rows(L, L, 8, rows)
+    for rows in ("wave", 32, 64, 96, 128):
+        glx.set_ddwave_rows(-1 if rows == "wave" else 0)
+        if rows != "wave":
+            glx.set_normal_rows(L, L, 8, rows)
         ms = glx.time_apply(lat, "staggered_normal", 0.02, d_out, d_in, reps=50, fp32=True,
                             link_prec=glx.LINKS_F32)
         if best is None or ms < best[1]:
             best = (rows, ms)
     rows, ms = best
-    glx.set_normal_rows(L, L, 8, rows)
+    if rows != "wave":
+        glx.set_normal_rows(L, L, 8, rows)
     gbs = 32 * V / (ms * 1e-3) / 1e9
     del d_in, d_out
     # the same operator in FP64 (64 B per site: 1 GiB per launch)
     x64 = x.astype(np.complex128)
     d_in, d_out = glx.DeviceArray(x64), glx.DeviceArray(np.zeros_like(x64))
     best64 = None
-    for r64 in (32, 64, 128):
-        glx.set_normal_rows(L, L, 16, r64)
+    for r64 in ("wave", 32, 64, 128):
+        glx.set_ddwave_rows(-1 if r64 == "wave" else 0)
+        if r64 != "wave":
+            glx.set_normal_rows(L, L, 16, r64)
         m64 = glx.time_apply(lat, "staggered_normal", 0.02, d_out, d_in, reps=30)
         if best64 is None or m64 < best64[1]:
             best64 = (r64, m64)
-    glx.set_normal_rows(L, L, 16, best64[0])
+    if best64[0] != "wave":
+        glx.set_normal_rows(L, L, 16, best64[0])
+    glx.set_ddwave_rows(-1)
     gbs64 = 64 * V / (best64[1] * 1e-3) / 1e9
     del lat, d_in, d_out
     glx.release_cache()
@@ -460,8 +468,9 @@ def eo_solves(glx, lat, b, mass, reps=3):
 def stencil_timing(glx, lat, b, mass, L, reps=200):
     """The operators alone at L^2 and 2 L^2: D (the wave-march kernel on
     lattices of whole 64-column strips, else the tiled kernel) and the fused
-    D^dag D, FP64 and FP32 (and FP16 links), HIP events around `reps`
-    back-to-back launches.  A lone D^dag D application takes its own best
+    D^dag D (the wave-march kernel; DdD_rowmarch: the row-marching kernel),
+    FP64 and FP32 (and FP16 links), HIP events around `reps` back-to-back
+    launches.  A lone row-marching D^dag D application takes its own best
     rows per block (the table's entry is tuned for the CG iteration kernel,
     whose prologue and epilogue change the balance); the table is restored
     afterwards.  Algorithmic bytes per application: V (2 s_v + 2 s_u [+ 4])."""
@@ -473,9 +482,12 @@ def stencil_timing(glx, lat, b, mass, L, reps=200):
         v32 = vb.astype(np.complex64)
         i32, o32 = glx.DeviceArray(v32), glx.DeviceArray(np.zeros_like(v32))
         res = {"L": n}
-        for op, tag in (("staggered", "D"), ("staggered_normal", "DdD")):
+        for op, tag in (("staggered", "D"), ("staggered_normal", "DdD"), ("staggered_normal", "DdD_rowmarch")):
             saved = (glx.get_normal_rows(n, n, 16), glx.get_normal_rows(n, n, 8))
-            cand = (0, 8, 16, 32, 64) if tag == "DdD" else (None,)
+            # DdD: the wave-march kernel at its own rows; DdD_rowmarch: the
+            # row-marching (or tiled, 0) kernel at its best rows per block
+            glx.set_ddwave_rows(0 if tag == "DdD_rowmarch" else -1)
+            cand = (0, 8, 16, 32, 64) if tag == "DdD_rowmarch" else (None,)
             best = {}
             for rows in cand:
                 if rows is not None:
@@ -489,9 +501,12 @@ def stencil_timing(glx, lat, b, mass, L, reps=200):
                         best[prec] = (ms, rows)
             glx.set_normal_rows(n, n, 16, saved[0])
             glx.set_normal_rows(n, n, 8, saved[1])
+            glx.set_ddwave_rows(-1)
             for prec, (ms, rows) in best.items():
-                if tag == "D":  # the wave-march kernel's rows per wave (0: the tiled kernel)
+                if tag == "D":  # the wave-march kernels' rows per wave (0: another kernel)
                     rows = glx.get_dwave_rows(n, n, 16 if prec == "fp64" else 8)
+                elif tag == "DdD":
+                    rows = glx.get_ddwave_rows(n, n, 16 if prec == "fp64" else 8)
                 by = {"fp64": 64, "fp32": 32, "fp16link": 28}[prec] * V
                 gbs = by / (ms * 1e-3) / 1e9
                 res[f"{tag}_{prec}"] = {"us": round(ms * 1e3, 2), "GBs": round(gbs, 1),

@@ -1534,6 +1534,20 @@ int glx_dev_time_apply(glx_lattice lat, int kind, double mass, int fp32, int lin
   });
 }
 
+int glx_set_ddwave_rows(int rows) {
+  return guarded([&] {
+    if (rows < -1 || rows > 4096) throw ArgError("rows must be -1 (by lattice size), 0 (off) or 1..4096");
+    set_ddwave_rows(rows);
+  });
+}
+int glx_get_ddwave_rows(int lx, int ly, int esize, int* rows) {
+  return guarded([&] {
+    if (!rows) throw ArgError("rows is NULL");
+    if (lx <= 0 || ly <= 0) throw ArgError("bad lattice size");
+    if (esize != 8 && esize != 16) throw ArgError("esize must be 8 (complex FP32) or 16 (complex FP64)");
+    *rows = ddwave_fits(lx, ly) ? ddwave_rows_for(lx, ly, esize) : 0;
+  });
+}
 int glx_set_dwave_rows(int rows) {
   return guarded([&] {
     if (rows < -1 || rows > 4096) throw ArgError("rows must be -1 (by lattice size), 0 (tiled) or 1..4096");

@@ -174,4 +174,134 @@ void launch_dwave(hipStream_t st, int kind, const DwGeom& g, const OpDev<LK>& op
   GLX_CHECK_LAUNCH();
 }
 
+// ------------------------------------------------------------ D^dag D --
+// The fused normal operator D^dag D = m^2 - K^2 (operators.cpp:444-453) in
+// the same wave-march form: q = K p is formed one march row behind the loads
+// and m^2 p - K q two rows behind, so a wave needs two halo lanes on each
+// side and owns 60 columns (strips start at x0 - 2, x0 = 60 s); the halo
+// columns are the neighbouring strips' own (L2 hits).  A band of `rows` rows
+// marches rows + 4 lattice rows (two halo rows at each end).
+constexpr int kDdOwn = 60;
+constexpr int kDdSlots = 7;  // rows t - 3 .. t in use, three more in flight
+
+struct DdGeom {
+  int lx, ly;
+  int nstrip;  // ceil(lx / 60)
+  int rows;    // rows per band
+  int ntask;   // nstrip * bands
+  int nblk;    // launched blocks of kDwWaves waves (multiple of 8)
+};
+
+inline bool ddwave_fits(int lx, int ly) { return lx >= 64 && lx % 2 == 0 && ly >= 4; }
+
+inline DdGeom make_ddgeom(int lx, int ly, int rows) {
+  DdGeom g;
+  g.lx = lx;
+  g.ly = ly;
+  g.nstrip = (lx + kDdOwn - 1) / kDdOwn;
+  g.rows = rows;
+  g.ntask = g.nstrip * ((ly + rows - 1) / rows);
+  const int blocks = (g.ntask + kDwWaves - 1) / kDwWaves;
+  g.nblk = ((blocks + 7) / 8) * 8;
+  return g;
+}
+
+template <class T, class LK>
+struct DdRow {
+  T v;
+  typename LK::Raw lk;
+};
+
+// a lattice row index in [-2, ly + 1] brought into [0, ly)
+__device__ __forceinline__ int dd_wrap(int y, int ly) { return y < 0 ? y + ly : (y >= ly ? y - ly : y); }
+
+template <bool UP, class T, class LK>
+__device__ __forceinline__ void ddwave_march(const DdGeom& g, const OpDev<LK>& op, const T* __restrict__ in,
+                                             T* __restrict__ out, int x, bool own, int y0, int rows, bool eta_neg) {
+  using C = typename LK::C;
+  using R = typename Real<T>::type;
+  const int lx = g.lx, ly = g.ly;
+  const int last = rows + 3;  // march rows 0 .. rows + 3
+  // march row m -> lattice row (downwards: y0 - 2 + m)
+  auto row_of = [&](int m) { return dd_wrap(UP ? y0 + rows + 1 - m : y0 - 2 + m, ly); };
+  auto load = [&](DdRow<T, LK>& d, int m) {
+    const int j = row_of(min(m, last)) * lx + x;
+    d.v = in[j];
+    d.lk = op.links.fetch(j);
+  };
+  DdRow<T, LK> ring[kDdSlots];
+  T q[kDdSlots];
+#pragma unroll
+  for (int s = 0; s < kDdSlots - 4; s++) load(ring[s], s);
+  const R m2 = R(op.mass * op.mass);
+  // K f at the lane's site from the rows above and below in march order:
+  // fa, fb, fc = march rows m - 1, m, m + 1; links of row m (ux, uy) and
+  // U_y of lattice row y - 1 (march row m - 1 downwards, m + 1 upwards)
+  auto khop_m = [&](const T& fa, const T& fb, const T& fc, C ux, C uy, C uym) {
+    const T fxp = dw_shift<kDwNext>(fb, fb), fxm = dw_shift<kDwPrev>(fb, fb);
+    const C uxm = dw_shift<kDwPrev>(ux, ux);
+    const T fyp = UP ? fa : fc, fym = UP ? fc : fa;
+    T ax = umul(ux, fxp) - ucmul(uxm, fxm);
+    T ay = umul(uy, fyp) - ucmul(uym, fym);
+    if (eta_neg) ay = -ay;
+    return R(0.5) * (ax + ay);
+  };
+  // step t: loads march row t + 3 into the slot of row t - 4; q of row t - 1
+  // from p rows t - 2 .. t; out of row t - 2 from q rows t - 3 .. t - 1
+  auto step = [&](auto kc, int t, auto phase) __attribute__((always_inline)) {
+    constexpr int K = decltype(kc)::value;
+    constexpr int PH = decltype(phase)::value;  // 0 first round, 1 whole, 2 last
+    constexpr int S0 = K % kDdSlots, S1 = (K + kDdSlots - 1) % kDdSlots, S2 = (K + kDdSlots - 2) % kDdSlots,
+                  S3 = (K + kDdSlots - 3) % kDdSlots, SN = (K + kDdSlots - 4) % kDdSlots;
+    __builtin_amdgcn_sched_barrier(0);
+    load(ring[SN], t + kDdSlots - 4);
+    {
+      C ux, uy, t0, uym;
+      LK::finish(ring[S1].lk, ux, uy);
+      LK::finish(UP ? ring[S0].lk : ring[S2].lk, t0, uym);
+      q[S1] = khop_m(ring[S2].v, ring[S1].v, ring[S0].v, ux, uy, uym);
+    }
+    if (PH != 0 || K >= 4) {
+      C ux, uy, t0, uym;
+      LK::finish(ring[S2].lk, ux, uy);
+      LK::finish(UP ? ring[S1].lk : ring[S3].lk, t0, uym);
+      const T kq = khop_m(q[S3], q[S2], q[S1], ux, uy, uym);
+      const T v = rscal(m2, ring[S2].v) - kq;
+      if ((PH == 1 || t <= last) && own) out[row_of(t - 2) * lx + x] = v;
+    }
+  };
+  using std::integral_constant;
+  dw_round(step, 0, integral_constant<int, 0>{}, std::make_integer_sequence<int, kDdSlots>{});
+  const int tfull = (last + 1) - (last + 1) % kDdSlots;  // steps 0 .. last
+  for (int t0 = kDdSlots; t0 < tfull; t0 += kDdSlots)
+    dw_round(step, t0, integral_constant<int, 1>{}, std::make_integer_sequence<int, kDdSlots>{});
+  if (tfull <= last && tfull >= kDdSlots)
+    dw_round(step, tfull, integral_constant<int, 2>{}, std::make_integer_sequence<int, kDdSlots>{});
+}
+
+template <class T, class LK>
+__global__ __launch_bounds__(64 * kDwWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_ddwave(
+    DdGeom g, OpDev<LK> op, const T* __restrict__ in, T* __restrict__ out, Gate gate) {
+  if (gate_closed(gate)) return;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int task = xcd_tile(blockIdx.x, g.nblk) * kDwWaves + w;
+  if (task >= g.ntask) return;
+  const int strip = task % g.nstrip, band = task / g.nstrip;
+  const int xo = strip * kDdOwn;  // first owned column
+  const int xu = xo - 2 + l;      // this lane's column before wrapping
+  const int x = xu < 0 ? xu + g.lx : (xu >= g.lx ? xu - g.lx : xu);
+  const bool own = l >= 2 && l < 2 + kDdOwn && xu < g.lx;
+  const int y0 = band * g.rows;
+  const int rows = min(g.rows, g.ly - y0);
+  const bool eta_neg = (x & 1) != 0;
+  if (band & 1) ddwave_march<true>(g, op, in, out, x, own, y0, rows, eta_neg);
+  else ddwave_march<false>(g, op, in, out, x, own, y0, rows, eta_neg);
+}
+
+template <class T, class LK>
+void launch_ddwave(hipStream_t st, const DdGeom& g, const OpDev<LK>& op, const T* in, T* out, Gate gate) {
+  hipLaunchKernelGGL((k_ddwave<T, LK>), dim3(g.nblk), dim3(64 * kDwWaves), 0, st, g, op, in, out, gate);
+  GLX_CHECK_LAUNCH();
+}
+
 }  // namespace glx

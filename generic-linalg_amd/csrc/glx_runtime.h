@@ -187,6 +187,8 @@ int normal_rows_entry(int lx, int ly, int esize);
 // always, > 0 that many rows.
 int dwave_rows_for(int lx, int ly, int esize);
 void set_dwave_rows(int rows);
+int ddwave_rows_for(int lx, int ly, int esize);
+void set_ddwave_rows(int rows);
 
 // Rows per block of the even/odd Schur kernel (compact rows of lx / 2 sites;
 // "tiled" does not exist for this operator).
@@ -219,6 +221,17 @@ void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T*
                    Gate done, bool finalize = true) {
   const Geom g = make_geom(o.lx, o.ly);
   const OpDev<LK> od = make_opdev(o, lk);
+  // the plain D^dag D: wave-march kernel where enabled
+  if constexpr (is_cx<T>::value && std::is_same<In, InPlain<T>>::value && std::is_same<Epi, EpiNone<T>>::value &&
+                std::is_same<Fin, FinNone>::value) {
+    if (o.kind == OP_STAG_NORMAL && ddwave_fits(o.lx, o.ly)) {
+      const int rows = ddwave_rows_for(o.lx, o.ly, int(sizeof(T)));
+      if (rows > 0) {
+        launch_ddwave<T, LK>(st, make_ddgeom(o.lx, o.ly, rows), od, in.p, out, done);
+        return;
+      }
+    }
+  }
   if (normal_path<T>(o, in, out)) {
     if constexpr (is_cx<T>::value) {
       const bool fin_launch = has_selffin<In>::value ? false : finalize;

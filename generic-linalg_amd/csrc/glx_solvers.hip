@@ -69,6 +69,22 @@ int dwave_rows_for(int lx, int ly, int esize) {
   if (nstrip * ((ly + cap - 1) / cap) >= 768) return cap;
   return std::max(4, int((ly * nstrip + 1023) / 1024));
 }
+// By lattice size: bands of 7 k - 4 rows march whole rounds of seven steps;
+// 24 rows where that still gives 2048 waves (2048^2: FP32 21.5 us against
+// 22.2-22.8 at 16-20 rows), else 10 where that gives 768 (1024^2: FP32 6.4 us,
+// FP64 12.1, against 7.1-8.1 and 12.8-15.1 at 6-20 rows), else about one wave
+// per SIMD, at least 3 rows (512^2: 5 rows, 4.2 us).
+static std::atomic<int> g_dd_rows{-1};
+int ddwave_rows_for(int lx, int ly, int esize) {
+  (void)esize;
+  const int forced = g_dd_rows.load();
+  if (forced >= 0) return forced;
+  const long nstrip = (lx + kDdOwn - 1) / kDdOwn;
+  if (nstrip * ((ly + 23) / 24) >= 2048) return 24;
+  if (nstrip * ((ly + 9) / 10) >= 768) return 10;
+  return std::max(3, int((ly * nstrip + 1023) / 1024));
+}
+void set_ddwave_rows(int rows) { g_dd_rows.store(rows < 0 ? -1 : rows); }
 void set_dwave_rows(int rows) {
   g_dw_rows.store(rows < 0 ? -1 : rows);
 }

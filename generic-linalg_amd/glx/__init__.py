@@ -246,6 +246,10 @@ def lib():
         L.glx_set_normal_rows.argtypes = [i, i, i, i]
         L.glx_set_dwave_rows.restype = i
         L.glx_set_dwave_rows.argtypes = [i]
+        L.glx_set_ddwave_rows.restype = i
+        L.glx_set_ddwave_rows.argtypes = [i]
+        L.glx_get_ddwave_rows.restype = i
+        L.glx_get_ddwave_rows.argtypes = [i, i, i, C.POINTER(i)]
         L.glx_get_dwave_rows.restype = i
         L.glx_get_dwave_rows.argtypes = [i, i, i, C.POINTER(i)]
         L.glx_get_normal_rows.restype = i
@@ -717,6 +721,20 @@ def set_dwave_rows(rows: int = -1):
     (lattices with lx % 64 == 0, ly >= 4): -1 by lattice size (the default),
     0 the tiled kernel always, > 0 that many."""
     _check(lib().glx_set_dwave_rows(rows), "glx_set_dwave_rows")
+
+
+def set_ddwave_rows(rows: int = -1):
+    """Rows per wave of the wave-march kernel of the plain D^dag D: -1 by
+    lattice size (the default), 0 the row-marching or tiled kernel always,
+    > 0 that many."""
+    _check(lib().glx_set_ddwave_rows(rows), "glx_set_ddwave_rows")
+
+
+def get_ddwave_rows(lx: int, ly: int, esize: int = 8) -> int:
+    """Rows per wave a plain lx x ly D^dag D apply would use (0: another kernel)."""
+    r = C.c_int(0)
+    _check(lib().glx_get_ddwave_rows(lx, ly, esize, C.byref(r)), "glx_get_ddwave_rows")
+    return r.value
 
 
 def get_dwave_rows(lx: int, ly: int, esize: int = 8) -> int:

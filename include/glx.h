@@ -508,6 +508,15 @@ int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
  * apply with vector element size esize (8 = complex FP32, 16 = complex FP64)
  * would use (0: the tiled kernel). */
 int glx_set_dwave_rows(int rows);
+/* The same for the wave-march kernel of the plain fused D^dag D (no
+ * reference counterpart; operators.cpp:444-453 is the operator), which
+ * serves lattices with lx even, lx >= 64 and ly >= 4 (60 owned columns per
+ * wave); the solvers' fused iterations keep the row-marching and tiled
+ * kernels.  rows = -1 (the initial setting): by lattice size (24 rows where
+ * that gives 2048 waves, else 10 where that gives 768, else about one wave
+ * per SIMD); 0: the row-marching or tiled kernel always; > 0: that many. */
+int glx_set_ddwave_rows(int rows);
+int glx_get_ddwave_rows(int lx, int ly, int esize, int* rows);
 int glx_get_dwave_rows(int lx, int ly, int esize, int* rows);
 /* Autotuner: for every candidate launch shape (tiled; 8..128 rows per block,
  * and 256 where ly >= 2048)

@@ -39,7 +39,10 @@ def orc():
 def rows_reset(glx, monkeypatch):
     """Launch-shape table entries a test sets through glx.set_normal_rows are
     removed afterwards and the default goes back to -1 (by block count), so
-    the tests after it run the library's own launch shapes."""
+    the tests after it run the library's own launch shapes.  Meanwhile the
+    plain D^dag D goes to the row-marching or tiled kernel the test selects,
+    not to the wave-march kernel that takes it by default."""
+    glx.set_ddwave_rows(0)
     real = glx.set_normal_rows
     touched = set()
 
@@ -53,3 +56,4 @@ def rows_reset(glx, monkeypatch):
     for lx, ly, es in touched:
         real(lx, ly, es, -1)
     real(0, 0, 0, -1)
+    glx.set_ddwave_rows(-1)

@@ -171,6 +171,15 @@ def test_dwave_rows_setting_without_device():
     assert lib.glx_get_dwave_rows(0, 8, 8, C.byref(r)) == 1
     assert lib.glx_get_dwave_rows(64, 8, 4, C.byref(r)) == 1 and b"esize" in lib.glx_last_error()
     assert lib.glx_set_dwave_rows(-1) == 0
+    # the D^dag D kernel: 60 owned columns per wave, 7 k - 4 rows where the lattice is large enough
+    want = {(2048, 2048, 8): 24, (4096, 4096, 16): 24, (1024, 1024, 8): 10, (1024, 1024, 16): 10,
+            (512, 512, 8): 5, (64, 64, 16): 3, (63, 64, 8): 0, (64, 3, 8): 0}
+    for (lx, ly, es), rows in want.items():
+        assert lib.glx_get_ddwave_rows(lx, ly, es, C.byref(r)) == 0 and r.value == rows, (lx, ly, es, r.value)
+    assert lib.glx_set_ddwave_rows(0) == 0
+    assert lib.glx_get_ddwave_rows(1024, 1024, 8, C.byref(r)) == 0 and r.value == 0
+    assert lib.glx_set_ddwave_rows(-2) == 1 and b"rows must be" in lib.glx_last_error()
+    assert lib.glx_set_ddwave_rows(-1) == 0
 
 
 def test_sstep_setting_without_device():

@@ -93,3 +93,48 @@ def test_dwave_falls_back_to_tiled(glx, orc, dw_reset):
     lat = glx.Lattice(links, lx, ly)
     got = run(glx, lat, "staggered", 0.3, bc, False, glx.LINKS_F32, -1)
     assert relerr(got, orc.apply("staggered", bc, links, 0.3, lx, ly)) < 1e-14
+
+
+DD_SHAPES = [(64, 4), (64, 64), (128, 37), (190, 100), (320, 7), (256, 256)]
+
+
+@pytest.fixture
+def dd_reset(glx):
+    yield
+    glx.set_ddwave_rows(-1)
+
+
+def run_dd(glx, lat, m, v, fp32, link_prec, rows):
+    glx.set_ddwave_rows(rows)
+    din, dout = glx.DeviceArray(v), glx.DeviceArray(np.zeros_like(v))
+    glx.dev_apply(lat, "staggered_normal", m, dout, din, fp32=fp32, link_prec=link_prec)
+    return dout.download()
+
+
+@pytest.mark.parametrize("shape", DD_SHAPES)
+def test_ddwave_fp64_matches_oracle(glx, orc, dd_reset, shape):
+    """The wave-march D^dag D (60 owned columns per wave, strips that wrap
+    the lattice, ragged last strips and bands) against the oracle."""
+    lx, ly = shape
+    links, bc, _ = lattice_inputs(orc, lx, ly, beta=4.0, seed=5 * lx + ly)
+    lat = glx.Lattice(links, lx, ly)
+    m = 0.3
+    want = orc.apply("staggered_normal", bc, links, m, lx, ly)
+    for rows in (-1, 1, 3, 10, 17, ly):
+        got = run_dd(glx, lat, m, bc, False, glx.LINKS_F32, rows)
+        assert glx.get_ddwave_rows(lx, ly, 16) > 0
+        assert relerr(got, want) < 1e-14, (rows, relerr(got, want))
+
+
+@pytest.mark.parametrize("shape", DD_SHAPES)
+@pytest.mark.parametrize("f16", [False, True])
+def test_ddwave_fp32_matches_tiled(glx, orc, dd_reset, shape, f16):
+    lx, ly = shape
+    links, bc, _ = lattice_inputs(orc, lx, ly, beta=4.0, seed=7 * lx + ly)
+    lat = glx.Lattice(links, lx, ly, f32=True, f16=f16)
+    lp = glx.LINKS_F16 if f16 else glx.LINKS_F32
+    v = bc.astype(np.complex64)
+    ref = run_dd(glx, lat, 0.3, v, True, lp, 0)
+    for rows in (-1, 2, 10, 17, ly):
+        got = run_dd(glx, lat, 0.3, v, True, lp, rows)
+        assert relerr(got, ref) < 1e-6, (rows, relerr(got, ref))
