@@ -57,28 +57,32 @@ int normal_rows_for(int lx, int ly, int esize, int cols) {
 }
 static std::atomic<int> g_dw_rows{-1};
 // By lattice size: 16 rows per wave for complex FP32 and 18 for FP64 (at
-// 1024^2, FP32 5.95 us at 16 rows against 6.3 at 18 and 6.9 at 8; FP64 11.5
-// us at 18 against 12.2 at 16), unless the lattice then fills fewer than 768
-// of the 1024 SIMDs: then the rows that give about one wave per SIMD, at
+// 1024^2, FP32 5.2 us at 16 rows against 5.6 at 18 and 5.9 at 8; FP64 11.1
+// us at 18 against 11.5 at 16), FP64 16 where 18 rows give 2048 waves (4096^2:
+// 195 us at 16 against 205 at 18), unless the lattice then fills fewer than
+// 768 of the 1024 SIMDs: then the rows that give about one wave per SIMD, at
 // least 4.
 int dwave_rows_for(int lx, int ly, int esize) {
   const int forced = g_dw_rows.load();
   if (forced >= 0) return forced;
   const long nstrip = lx / 64;
-  const int cap = esize == 16 ? 18 : 16;
+  const int cap = esize == 16 && nstrip * ((ly + 17) / 18) < 2048 ? 18 : 16;
   if (nstrip * ((ly + cap - 1) / cap) >= 768) return cap;
   return std::max(4, int((ly * nstrip + 1023) / 1024));
 }
-// By lattice size: bands of 7 k - 4 rows march whole rounds of seven steps;
-// 24 rows where that still gives 2048 waves (2048^2: FP32 21.5 us against
-// 22.2-22.8 at 16-20 rows), else 10 where that gives 768 (1024^2: FP32 6.4 us,
-// FP64 12.1, against 7.1-8.1 and 12.8-15.1 at 6-20 rows), else about one wave
-// per SIMD, at least 3 rows (512^2: 5 rows, 4.2 us).
+// By lattice size: none (the row-marching kernel) where one application
+// moves more than 256 MiB (4096^2, beyond the last-level cache: FP32 107.9 us
+// against 105.9, FP64 210 against 202); else bands of 7 k - 4 rows, which
+// march whole rounds of seven steps: 24 rows where that still gives 2048 waves
+// (2048^2: FP32 21.5 us against 22.2-22.8 at 16-20 rows), else 10 where that
+// gives 768 (1024^2: FP32 6.4 us, FP64 12.1, against 7.1-8.1 and 12.8-15.1
+// at 6-20 rows), else about one wave per SIMD, at least 3 rows (512^2: 5
+// rows, 4.2 us).
 static std::atomic<int> g_dd_rows{-1};
 int ddwave_rows_for(int lx, int ly, int esize) {
-  (void)esize;
   const int forced = g_dd_rows.load();
   if (forced >= 0) return forced;
+  if (long(lx) * ly * (esize == 16 ? 64 : 32) > (256L << 20)) return 0;
   const long nstrip = (lx + kDdOwn - 1) / kDdOwn;
   if (nstrip * ((ly + 23) / 24) >= 2048) return 24;
   if (nstrip * ((ly + 9) / 10) >= 768) return 10;

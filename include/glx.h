@@ -502,8 +502,9 @@ int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
  * operators they compute): rows per wave of the wave-march kernel, which
  * serves lattices with lx % 64 == 0 and ly >= 4 (other sizes, and the
  * solvers' fused launches, use the tiled kernel).  rows = -1 (the initial
- * setting): by lattice size (16 rows for complex FP32, 18 for FP64, fewer
- * where the lattice would fill fewer than 768 SIMDs); 0: the tiled kernel
+ * setting): by lattice size (16 rows for complex FP32, 18 for FP64 or 16 on
+ * lattices that give 2048 waves, fewer where the lattice would fill fewer
+ * than 768 SIMDs); 0: the tiled kernel
  * always; > 0: that many rows.  glx_get_dwave_rows returns what an lx x ly
  * apply with vector element size esize (8 = complex FP32, 16 = complex FP64)
  * would use (0: the tiled kernel). */
@@ -512,9 +513,9 @@ int glx_set_dwave_rows(int rows);
  * reference counterpart; operators.cpp:444-453 is the operator), which
  * serves lattices with lx even, lx >= 64 and ly >= 4 (60 owned columns per
  * wave); the solvers' fused iterations keep the row-marching and tiled
- * kernels.  rows = -1 (the initial setting): by lattice size (24 rows where
- * that gives 2048 waves, else 10 where that gives 768, else about one wave
- * per SIMD); 0: the row-marching or tiled kernel always; > 0: that many. */
+ * kernels.  rows = -1 (the initial setting): by lattice size (none where
+ * one application moves more than 256 MiB, else 24 rows where that gives
+ * 2048 waves, 10 where that gives 768, else about one wave per SIMD); 0: the row-marching or tiled kernel always; > 0: that many. */
 int glx_set_ddwave_rows(int rows);
 int glx_get_ddwave_rows(int lx, int ly, int esize, int* rows);
 int glx_get_dwave_rows(int lx, int ly, int esize, int* rows);

@@ -152,14 +152,16 @@ def test_normal_rows_default_without_device():
 
 def test_dwave_rows_setting_without_device():
     """glx_set_dwave_rows validates its argument; glx_get_dwave_rows gives
-    the measured shapes (16 rows FP32, 18 FP64 where the lattice fills at
-    least 768 SIMDs, else about one wave per SIMD, at least 4 rows) and 0
-    (the tiled kernel) for widths that are not whole 64-column strips (no
-    device work)."""
+    the measured shapes (16 rows FP32; FP64 18, or 16 on large lattices;
+    else about one wave per SIMD, at least 4 rows) and 0 (the tiled kernel)
+    for widths that are not whole 64-column strips; likewise the D^dag D
+    kernel, which leaves applications above 256 MiB to the row-marching
+    kernel (no device work)."""
     lib = C.CDLL(LIB)
     lib.glx_last_error.restype = C.c_char_p
     r = C.c_int()
-    want = {(1024, 1024, 8): 16, (1024, 1024, 16): 18, (2048, 2048, 8): 16, (4096, 4096, 16): 18,
+    want = {(1024, 1024, 8): 16, (1024, 1024, 16): 18, (2048, 2048, 8): 16, (2048, 2048, 16): 16,
+            (4096, 4096, 16): 16,
             (512, 512, 8): 4, (768, 768, 8): 9, (64, 64, 16): 4, (96, 64, 8): 0, (64, 2, 8): 0}
     for (lx, ly, es), rows in want.items():
         assert lib.glx_get_dwave_rows(lx, ly, es, C.byref(r)) == 0 and r.value == rows, (lx, ly, es, r.value)
@@ -172,7 +174,8 @@ def test_dwave_rows_setting_without_device():
     assert lib.glx_get_dwave_rows(64, 8, 4, C.byref(r)) == 1 and b"esize" in lib.glx_last_error()
     assert lib.glx_set_dwave_rows(-1) == 0
     # the D^dag D kernel: 60 owned columns per wave, 7 k - 4 rows where the lattice is large enough
-    want = {(2048, 2048, 8): 24, (4096, 4096, 16): 24, (1024, 1024, 8): 10, (1024, 1024, 16): 10,
+    want = {(2048, 2048, 8): 24, (2048, 2048, 16): 24, (4096, 4096, 8): 0, (4096, 4096, 16): 0,
+            (1024, 1024, 8): 10, (1024, 1024, 16): 10,
             (512, 512, 8): 5, (64, 64, 16): 3, (63, 64, 8): 0, (64, 3, 8): 0}
     for (lx, ly, es), rows in want.items():
         assert lib.glx_get_ddwave_rows(lx, ly, es, C.byref(r)) == 0 and r.value == rows, (lx, ly, es, r.value)
