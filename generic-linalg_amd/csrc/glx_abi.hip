@@ -1446,6 +1446,7 @@ int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int 
                   const void* in_dev) {
   return guarded([&] {
     if (!out_dev || !in_dev) throw ArgError("out_dev and in_dev must not be NULL");
+    if (out_dev == in_dev) throw ArgError("out_dev and in_dev must be different buffers");
     DevOperator o = lattice_op(lat, kind, mass, nc);
     if (o.lx <= 0) throw ArgError("operator without links needs a lattice for its dimensions");
     dev_apply(o, out_dev, in_dev, link_prec, fp32, nullptr);

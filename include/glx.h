@@ -528,7 +528,9 @@ int glx_get_dwave_rows(int lx, int ly, int esize, int* rows);
  * NULL. */
 int glx_autotune_normal(glx_lattice lat, int reps, int* rows64, int* rows32, double* ms64, double* ms32);
 /* out = A in on device vectors; fp32 selects complex<float> vectors with the
- * FP32 (link_prec 1) or FP16 (link_prec 2) links. */
+ * FP32 (link_prec 1) or FP16 (link_prec 2) links.  out and in must be
+ * different buffers (every kernel reads neighbours of sites another wave
+ * writes); the same pointer is an error. */
 int glx_dev_apply(glx_lattice lat, int kind, double mass, int nc, int fp32, int link_prec, void* out_dev,
                   const void* in_dev);
 /* Average duration in ms of `reps` back-to-back applications, timed with HIP

@@ -63,6 +63,13 @@ def test_argument_errors_without_device():
     g.restype = C.c_int
     g.argtypes = [vp, C.c_int, vp]
     assert g(None, 3, None) == 1
+    # an in-place operator application is refused (every kernel reads sites another wave writes)
+    a = lib.glx_dev_apply
+    a.restype = C.c_int
+    a.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, vp, vp]
+    buf = C.create_string_buffer(64)
+    assert a(None, 3, 0.1, 1, 0, 1, C.addressof(buf), C.addressof(buf)) == 1
+    assert b"different buffers" in lib.glx_last_error()
 
 
 class _Params(C.Structure):
