@@ -1,6 +1,7 @@
-// glx_dwave.h -- the staggered operator D = m - K and D^dag = m + K
-// (operators.cpp:184-239 and :372-441) as a wave-march kernel: registers
-// only, no LDS, no barriers.
+// glx_dwave.h -- the staggered operators D = m - K, D^dag = m + K
+// (operators.cpp:184-239 and :372-441) and the fused D^dag D = m^2 - K^2
+// (operators.cpp:444-453) as wave-march kernels: registers only, no LDS, no
+// barriers.  D and D^dag first; D^dag D (k_ddwave) at the end of the file.
 //
 // A wave owns a strip of 64 lattice columns (lane l <-> column x0 + l) and a
 // band of `rows` rows, and marches the band one row per step.  It holds the
@@ -22,7 +23,8 @@
 // Requirements (checked on the host by dwave_fits): lx a multiple of 64,
 // ly >= 4; complex vectors; the plain operator (no prologue, epilogue or
 // reduction).  Other shapes and the fused solver launches use the tiled
-// kernel.
+// kernel.  In is never out: waves read rows and columns that other waves
+// write.
 #pragma once
 
 #include "glx_stencil.h"
@@ -149,8 +151,8 @@ __device__ __forceinline__ void dwave_march(const DwGeom& g, const OpDev<LK>& op
 // (amdgpu_waves_per_eu: without it the compiler aims at eight waves per SIMD,
 // 64 registers, and moves the ring's loads down to their uses)
 template <int KIND, class T, class LK>
-__global__ __launch_bounds__(64 * kDwWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_dwave(DwGeom g, OpDev<LK> op, const T* __restrict__ in,
-                                                  T* __restrict__ out, Gate gate) {
+__global__ __launch_bounds__(64 * kDwWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_dwave(
+    DwGeom g, OpDev<LK> op, const T* __restrict__ in, T* __restrict__ out, Gate gate) {
   if (gate_closed(gate)) return;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
   const int task = xcd_tile(blockIdx.x, g.nblk) * kDwWaves + w;
