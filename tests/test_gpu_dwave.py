@@ -138,3 +138,20 @@ def test_ddwave_fp32_matches_tiled(glx, orc, dd_reset, shape, f16):
     for rows in (-1, 2, 10, 17, ly):
         got = run_dd(glx, lat, 0.3, v, True, lp, rows)
         assert relerr(got, ref) < 1e-6, (rows, relerr(got, ref))
+
+
+def test_ddwave_workload_1024(glx, orc, dd_reset):
+    """The wave-march D^dag D at its own launch shape on the 1024^2 lattice
+    of the mixed-CG workload: FP64 against the oracle, FP32 against the
+    row-marching kernel."""
+    lx = ly = 1024
+    assert glx.get_ddwave_rows(lx, ly, 8) == 10 and glx.get_ddwave_rows(lx, ly, 16) == 10
+    links, bc, _ = lattice_inputs(orc, lx, ly, beta=4.0, seed=2024)
+    lat = glx.Lattice(links, lx, ly, f32=True)
+    want = orc.apply("staggered_normal", bc, links, 0.02, lx, ly)
+    got = run_dd(glx, lat, 0.02, bc, False, glx.LINKS_F32, -1)
+    assert relerr(got, want) < 1e-14
+    v32 = bc.astype(np.complex64)
+    ref = run_dd(glx, lat, 0.02, v32, True, glx.LINKS_F32, 0)
+    got = run_dd(glx, lat, 0.02, v32, True, glx.LINKS_F32, -1)
+    assert relerr(got, ref) < 1e-6
