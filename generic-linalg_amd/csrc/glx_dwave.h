@@ -29,6 +29,7 @@
 
 #include "glx_stencil.h"
 
+#include <type_traits>
 #include <utility>
 
 namespace glx {
@@ -45,7 +46,11 @@ struct DwGeom {
   int nblk;    // launched blocks of kDwWaves waves (multiple of 8)
 };
 
-inline bool dwave_fits(int lx, int ly) { return lx >= 64 && lx % 64 == 0 && ly >= 4; }
+// (the march addresses sites by 32-bit byte offsets: at most 32 bytes per
+// site, FP64 links, below 2^32)
+inline bool dwave_fits(int lx, int ly) {
+  return lx >= 64 && lx % 64 == 0 && ly >= 4 && long(lx) * ly <= (1L << 27);
+}
 
 inline DwGeom make_dwgeom(int lx, int ly, int rows) {
   DwGeom g;
@@ -101,13 +106,21 @@ __device__ __forceinline__ void dwave_march(const DwGeom& g, const OpDev<LK>& op
   const int lx = g.lx, ly = g.ly;
   // march row m = 0 .. rows + 1 -> lattice row (downwards: y0 - 1 + m)
   auto row_of = [&](int m) { return dw_wrap(UP ? y0 + rows - m : y0 - 1 + m, ly); };
+  // (32-bit byte offsets from the uniform bases: one address instruction per
+  // access instead of a 64-bit multiply-add)
+  auto at = [](const void* base, unsigned off) -> const char* { return static_cast<const char*>(base) + off; };
   auto load = [&](DwRow<T, LK>& d, int m) {
     const int y = row_of(min(m, rows + 1));  // (the rows past the band's end load its last row again)
-    const int j = y * lx + x, je = y * lx + xe;
-    d.v = in[j];
-    d.ve = in[je];
-    d.lk = op.links.fetch(j);
-    d.lke = op.links.fetch(je);
+    const unsigned j = unsigned(y * lx + x), je = unsigned(y * lx + xe);
+    d.v = *reinterpret_cast<const T*>(at(in, j * unsigned(sizeof(T))));
+    d.ve = *reinterpret_cast<const T*>(at(in, je * unsigned(sizeof(T))));
+    if constexpr (std::is_same<LK, LinksF16>::value) {
+      d.lk = op.links.fetch(int(j));
+      d.lke = op.links.fetch(int(je));
+    } else {
+      d.lk = *reinterpret_cast<const typename LK::Raw*>(at(op.links.u, j * unsigned(sizeof(typename LK::Raw))));
+      d.lke = *reinterpret_cast<const typename LK::Raw*>(at(op.links.u, je * unsigned(sizeof(typename LK::Raw))));
+    }
   };
   DwRow<T, LK> ring[kDwSlots];
 #pragma unroll
@@ -138,7 +151,8 @@ __device__ __forceinline__ void dwave_march(const DwGeom& g, const OpDev<LK>& op
     T v;
     if constexpr (KIND == OP_STAG) v = rscal(op.mass, b.v) - kp;
     else v = rscal(op.mass, b.v) + kp;
-    if (!decltype(tail)::value || t < rows) out[row_of(t + 1) * lx + x] = v;
+    if (!decltype(tail)::value || t < rows)
+      *reinterpret_cast<T*>(reinterpret_cast<char*>(out) + unsigned(row_of(t + 1) * lx + x) * unsigned(sizeof(T))) = v;
   };
   // whole rounds, every step storing its row; then the round that ends past
   // the band's last row, whose steps past it store nothing

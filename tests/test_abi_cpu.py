@@ -169,7 +169,8 @@ def test_dwave_rows_setting_without_device():
     r = C.c_int()
     want = {(1024, 1024, 8): 16, (1024, 1024, 16): 18, (2048, 2048, 8): 16, (2048, 2048, 16): 16,
             (4096, 4096, 16): 16,
-            (512, 512, 8): 4, (768, 768, 8): 9, (64, 64, 16): 4, (96, 64, 8): 0, (64, 2, 8): 0}
+            (512, 512, 8): 4, (768, 768, 8): 9, (64, 64, 16): 4, (96, 64, 8): 0, (64, 2, 8): 0,
+            (16384, 16384, 8): 0}
     for (lx, ly, es), rows in want.items():
         assert lib.glx_get_dwave_rows(lx, ly, es, C.byref(r)) == 0 and r.value == rows, (lx, ly, es, r.value)
     assert lib.glx_set_dwave_rows(7) == 0
