@@ -1,7 +1,9 @@
 // glx_dwave.h -- the staggered operators D = m - K, D^dag = m + K
-// (operators.cpp:184-239 and :372-441) and the fused D^dag D = m^2 - K^2
+// (operators.cpp:184-239 and :372-441), gamma5 D (:316-368) and the
+// even/odd pieces D_eo, D_oe (:456-539), and the fused D^dag D = m^2 - K^2
 // (operators.cpp:444-453) as wave-march kernels: registers only, no LDS, no
-// barriers.  D and D^dag first; D^dag D (k_ddwave) at the end of the file.
+// barriers.  The one-hop kinds first; D^dag D (k_ddwave) at the end of the
+// file.
 //
 // A wave owns a strip of 64 lattice columns (lane l <-> column x0 + l) and a
 // band of `rows` rows, and marches the band one row per step.  It holds the
@@ -149,8 +151,21 @@ __device__ __forceinline__ void dwave_march(const DwGeom& g, const OpDev<LK>& op
     if (eta_neg) ay = -ay;
     const T kp = R(0.5) * (ax + ay);
     T v;
-    if constexpr (KIND == OP_STAG) v = rscal(op.mass, b.v) - kp;
-    else v = rscal(op.mass, b.v) + kp;
+    const int y = row_of(t + 1);
+    const bool odd = ((x + y) & 1) != 0;
+    if constexpr (KIND == OP_STAG) {
+      v = rscal(op.mass, b.v) - kp;
+    } else if constexpr (KIND == OP_STAG_DAG) {
+      v = rscal(op.mass, b.v) + kp;
+    } else if constexpr (KIND == OP_STAG_G5) {
+      v = rscal(op.mass, b.v) - kp;
+      if (odd) v = -v;
+    } else if constexpr (KIND == OP_DEO) {
+      v = odd ? zero_of<T>() : -kp;
+    } else {
+      v = odd ? -kp : zero_of<T>();
+    }
+    (void)odd;
     if (!decltype(tail)::value || t < rows)
       *reinterpret_cast<T*>(reinterpret_cast<char*>(out) + unsigned(row_of(t + 1) * lx + x) * unsigned(sizeof(T))) = v;
   };
@@ -183,10 +198,26 @@ __global__ __launch_bounds__(64 * kDwWaves) __attribute__((amdgpu_waves_per_eu(2
 
 template <class T, class LK>
 void launch_dwave(hipStream_t st, int kind, const DwGeom& g, const OpDev<LK>& op, const T* in, T* out, Gate gate) {
-  if (kind == OP_STAG)
-    hipLaunchKernelGGL((k_dwave<OP_STAG, T, LK>), dim3(g.nblk), dim3(64 * kDwWaves), 0, st, g, op, in, out, gate);
-  else
-    hipLaunchKernelGGL((k_dwave<OP_STAG_DAG, T, LK>), dim3(g.nblk), dim3(64 * kDwWaves), 0, st, g, op, in, out, gate);
+  const dim3 grid(g.nblk), block(64 * kDwWaves);
+  switch (kind) {
+    case OP_STAG:
+      hipLaunchKernelGGL((k_dwave<OP_STAG, T, LK>), grid, block, 0, st, g, op, in, out, gate);
+      break;
+    case OP_STAG_DAG:
+      hipLaunchKernelGGL((k_dwave<OP_STAG_DAG, T, LK>), grid, block, 0, st, g, op, in, out, gate);
+      break;
+    case OP_STAG_G5:
+      hipLaunchKernelGGL((k_dwave<OP_STAG_G5, T, LK>), grid, block, 0, st, g, op, in, out, gate);
+      break;
+    case OP_DEO:
+      hipLaunchKernelGGL((k_dwave<OP_DEO, T, LK>), grid, block, 0, st, g, op, in, out, gate);
+      break;
+    case OP_DOE:
+      hipLaunchKernelGGL((k_dwave<OP_DOE, T, LK>), grid, block, 0, st, g, op, in, out, gate);
+      break;
+    default:
+      throw ArgError("the wave-march kernel has no operator kind " + std::to_string(kind));
+  }
   GLX_CHECK_LAUNCH();
 }
 

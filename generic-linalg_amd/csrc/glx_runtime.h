@@ -246,10 +246,13 @@ void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T*
       return;
     }
   }
-  // the plain D and D^dag on lattices of whole 64-column strips: wave-march kernel
+  // the plain one-hop staggered kinds on lattices of whole 64-column strips:
+  // wave-march kernel
   if constexpr (is_cx<T>::value && std::is_same<In, InPlain<T>>::value && std::is_same<Epi, EpiNone<T>>::value &&
                 std::is_same<Fin, FinNone>::value) {
-    if ((o.kind == OP_STAG || o.kind == OP_STAG_DAG) && dwave_fits(o.lx, o.ly)) {
+    if ((o.kind == OP_STAG || o.kind == OP_STAG_DAG || o.kind == OP_STAG_G5 || o.kind == OP_DEO ||
+         o.kind == OP_DOE) &&
+        dwave_fits(o.lx, o.ly)) {
       const int rows = dwave_rows_for(o.lx, o.ly, int(sizeof(T)));
       if (rows > 0) {
         launch_dwave<T, LK>(st, o.kind, make_dwgeom(o.lx, o.ly, rows), od, in.p, out, done);

@@ -717,7 +717,8 @@ def get_sstep():
 
 
 def set_dwave_rows(rows: int = -1):
-    """Rows per wave of the wave-march kernel of the plain D and D^dag
+    """Rows per wave of the wave-march kernel of the plain one-hop staggered
+    kinds (D, D^dag, gamma5 D, D_eo, D_oe)
     (lattices with lx % 64 == 0, ly >= 4): -1 by lattice size (the default),
     0 the tiled kernel always, > 0 that many."""
     _check(lib().glx_set_dwave_rows(rows), "glx_set_dwave_rows")

@@ -497,9 +497,10 @@ int glx_autotune_sstep(glx_lattice lat, int reps, int* rows, double* ms_per_iter
  * tiled kernel. */
 int glx_set_normal_rows(int lx, int ly, int esize, int rows);
 int glx_get_normal_rows(int lx, int ly, int esize, int* rows);
-/* Launch shape of the plain staggered D and D^dag on complex vectors (no
- * reference counterpart; operators.cpp:184-239 and :372-441 are the
- * operators they compute): rows per wave of the wave-march kernel, which
+/* Launch shape of the plain staggered D, D^dag, gamma5 D, D_eo and D_oe on
+ * complex vectors (no reference counterpart; operators.cpp:184-239,
+ * :316-368, :372-441 and :456-539 are the operators they compute): rows per
+ * wave of the wave-march kernel, which
  * serves lattices with lx % 64 == 0 and ly >= 4 (other sizes, and the
  * solvers' fused launches, use the tiled kernel).  rows = -1 (the initial
  * setting): by lattice size (16 rows for complex FP32, 18 for FP64 or 16 on

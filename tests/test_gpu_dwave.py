@@ -18,7 +18,7 @@ from helpers import lattice_inputs, relerr
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(64, 4), (64, 64), (128, 37), (192, 100), (320, 7), (256, 256)]
-OPS = ["staggered", "staggered_dagger"]
+OPS = ["staggered", "staggered_dagger", "staggered_gamma5", "deo", "doe"]
 
 
 @pytest.fixture
