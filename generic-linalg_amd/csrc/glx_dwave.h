@@ -222,8 +222,9 @@ void launch_dwave(hipStream_t st, int kind, const DwGeom& g, const OpDev<LK>& op
 }
 
 // ------------------------------------------------------------ D^dag D --
-// The fused normal operator D^dag D = m^2 - K^2 (operators.cpp:444-453) in
-// the same wave-march form: q = K p is formed one march row behind the loads
+// The fused normal operator D^dag D = m^2 - K^2 (operators.cpp:444-453), and
+// m^2 - D_eo D_oe on full-lattice vectors (operators.cpp:566-588: the same
+// march, odd outputs zero), in the same wave-march form: q = K p is formed one march row behind the loads
 // and m^2 p - K q two rows behind, so a wave needs two halo lanes on each
 // side and owns 60 columns (strips start at x0 - 2, x0 = 60 s); the halo
 // columns are the neighbouring strips' own (L2 hits).  A band of `rows` rows
@@ -262,7 +263,7 @@ struct DdRow {
 // a lattice row index in [-2, ly + 1] brought into [0, ly)
 __device__ __forceinline__ int dd_wrap(int y, int ly) { return y < 0 ? y + ly : (y >= ly ? y - ly : y); }
 
-template <bool UP, class T, class LK>
+template <int KIND, bool UP, class T, class LK>
 __device__ __forceinline__ void ddwave_march(const DdGeom& g, const OpDev<LK>& op, const T* __restrict__ in,
                                              T* __restrict__ out, int x, bool own, int y0, int rows, bool eta_neg) {
   using C = typename LK::C;
@@ -307,13 +308,22 @@ __device__ __forceinline__ void ddwave_march(const DdGeom& g, const OpDev<LK>& o
       LK::finish(ring[S1].lk, ux, uy);
       LK::finish(UP ? ring[S0].lk : ring[S2].lk, t0, uym);
       q[S1] = khop_m(ring[S2].v, ring[S1].v, ring[S0].v, ux, uy, uym);
+      // m^2 - D_eo D_oe: q is D_oe p, zero on the even sites (an even
+      // site's neighbour across the wrap of an odd ly is even)
+      if constexpr (KIND == OP_M2MDEODOE) {
+        if (((x + row_of(t - 1)) & 1) == 0) q[S1] = zero_of<T>();
+      }
     }
     if (PH != 0 || K >= 4) {
       C ux, uy, t0, uym;
       LK::finish(ring[S2].lk, ux, uy);
       LK::finish(UP ? ring[S1].lk : ring[S3].lk, t0, uym);
       const T kq = khop_m(q[S3], q[S2], q[S1], ux, uy, uym);
-      const T v = rscal(m2, ring[S2].v) - kq;
+      T v = rscal(m2, ring[S2].v) - kq;
+      // m^2 - D_eo D_oe: the even sites take m^2 p - K q, the odd ones 0
+      if constexpr (KIND == OP_M2MDEODOE) {
+        if (((x + row_of(t - 2)) & 1) != 0) v = zero_of<T>();
+      }
       if ((PH == 1 || t <= last) && own) out[row_of(t - 2) * lx + x] = v;
     }
   };
@@ -326,7 +336,7 @@ __device__ __forceinline__ void ddwave_march(const DdGeom& g, const OpDev<LK>& o
     dw_round(step, tfull, integral_constant<int, 2>{}, std::make_integer_sequence<int, kDdSlots>{});
 }
 
-template <class T, class LK>
+template <int KIND, class T, class LK>
 __global__ __launch_bounds__(64 * kDwWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_ddwave(
     DdGeom g, OpDev<LK> op, const T* __restrict__ in, T* __restrict__ out, Gate gate) {
   if (gate_closed(gate)) return;
@@ -341,13 +351,21 @@ __global__ __launch_bounds__(64 * kDwWaves) __attribute__((amdgpu_waves_per_eu(2
   const int y0 = band * g.rows;
   const int rows = min(g.rows, g.ly - y0);
   const bool eta_neg = (x & 1) != 0;
-  if (band & 1) ddwave_march<true>(g, op, in, out, x, own, y0, rows, eta_neg);
-  else ddwave_march<false>(g, op, in, out, x, own, y0, rows, eta_neg);
+  if (band & 1) ddwave_march<KIND, true>(g, op, in, out, x, own, y0, rows, eta_neg);
+  else ddwave_march<KIND, false>(g, op, in, out, x, own, y0, rows, eta_neg);
 }
 
 template <class T, class LK>
-void launch_ddwave(hipStream_t st, const DdGeom& g, const OpDev<LK>& op, const T* in, T* out, Gate gate) {
-  hipLaunchKernelGGL((k_ddwave<T, LK>), dim3(g.nblk), dim3(64 * kDwWaves), 0, st, g, op, in, out, gate);
+void launch_ddwave(hipStream_t st, int kind, const DdGeom& g, const OpDev<LK>& op, const T* in, T* out,
+                   Gate gate) {
+  if (kind == OP_STAG_NORMAL)
+    hipLaunchKernelGGL((k_ddwave<OP_STAG_NORMAL, T, LK>), dim3(g.nblk), dim3(64 * kDwWaves), 0, st, g, op, in, out,
+                       gate);
+  else if (kind == OP_M2MDEODOE)
+    hipLaunchKernelGGL((k_ddwave<OP_M2MDEODOE, T, LK>), dim3(g.nblk), dim3(64 * kDwWaves), 0, st, g, op, in, out,
+                       gate);
+  else
+    throw ArgError("the wave-march D^dag D kernel has no operator kind " + std::to_string(kind));
   GLX_CHECK_LAUNCH();
 }
 

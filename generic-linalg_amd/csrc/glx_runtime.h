@@ -224,10 +224,10 @@ void dispatch_kind(hipStream_t st, const DevOperator& o, const LK& lk, In in, T*
   // the plain D^dag D: wave-march kernel where enabled
   if constexpr (is_cx<T>::value && std::is_same<In, InPlain<T>>::value && std::is_same<Epi, EpiNone<T>>::value &&
                 std::is_same<Fin, FinNone>::value) {
-    if (o.kind == OP_STAG_NORMAL && ddwave_fits(o.lx, o.ly)) {
+    if ((o.kind == OP_STAG_NORMAL || o.kind == OP_M2MDEODOE) && ddwave_fits(o.lx, o.ly)) {
       const int rows = ddwave_rows_for(o.lx, o.ly, int(sizeof(T)));
       if (rows > 0) {
-        launch_ddwave<T, LK>(st, make_ddgeom(o.lx, o.ly, rows), od, in.p, out, done);
+        launch_ddwave<T, LK>(st, o.kind, make_ddgeom(o.lx, o.ly, rows), od, in.p, out, done);
         return;
       }
     }

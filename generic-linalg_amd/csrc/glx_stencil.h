@@ -382,6 +382,11 @@ __global__ __launch_bounds__(kBlock) void k_stencil(Geom g, OpDev<LK> op, In in,
         } else {
           const int lc = (qy + 1) * LW + (qx + 1);
           q = hop_k<T, C>(sp, PW, pc, su, LW, lc, (gx & 1) != 0);
+          // m^2 - D_eo D_oe: q is D_oe p, zero on the even sites (an even
+          // site's neighbour across the wrap of an odd ly is even)
+          if constexpr (KIND == OP_M2MDEODOE) {
+            if (((gx + gy) & 1) == 0) q = zero_of<T>();
+          }
         }
         sq[e] = q;
       }

@@ -104,39 +104,45 @@ def dd_reset(glx):
     glx.set_ddwave_rows(-1)
 
 
-def run_dd(glx, lat, m, v, fp32, link_prec, rows):
+def run_dd(glx, lat, m, v, fp32, link_prec, rows, op="staggered_normal"):
     glx.set_ddwave_rows(rows)
     din, dout = glx.DeviceArray(v), glx.DeviceArray(np.zeros_like(v))
-    glx.dev_apply(lat, "staggered_normal", m, dout, din, fp32=fp32, link_prec=link_prec)
+    glx.dev_apply(lat, op, m, dout, din, fp32=fp32, link_prec=link_prec)
     return dout.download()
 
 
+DD_OPS = ["staggered_normal", "m2mdeodoe"]
+
+
 @pytest.mark.parametrize("shape", DD_SHAPES)
-def test_ddwave_fp64_matches_oracle(glx, orc, dd_reset, shape):
-    """The wave-march D^dag D (60 owned columns per wave, strips that wrap
-    the lattice, ragged last strips and bands) against the oracle."""
+@pytest.mark.parametrize("op", DD_OPS)
+def test_ddwave_fp64_matches_oracle(glx, orc, dd_reset, shape, op):
+    """The wave-march D^dag D and m^2 - D_eo D_oe (60 owned columns per
+    wave, strips that wrap the lattice, ragged last strips and bands)
+    against the oracle."""
     lx, ly = shape
     links, bc, _ = lattice_inputs(orc, lx, ly, beta=4.0, seed=5 * lx + ly)
     lat = glx.Lattice(links, lx, ly)
     m = 0.3
-    want = orc.apply("staggered_normal", bc, links, m, lx, ly)
+    want = orc.apply(op, bc, links, m, lx, ly)
     for rows in (-1, 1, 3, 10, 17, ly):
-        got = run_dd(glx, lat, m, bc, False, glx.LINKS_F32, rows)
+        got = run_dd(glx, lat, m, bc, False, glx.LINKS_F32, rows, op)
         assert glx.get_ddwave_rows(lx, ly, 16) > 0
         assert relerr(got, want) < 1e-14, (rows, relerr(got, want))
 
 
 @pytest.mark.parametrize("shape", DD_SHAPES)
 @pytest.mark.parametrize("f16", [False, True])
-def test_ddwave_fp32_matches_tiled(glx, orc, dd_reset, shape, f16):
+@pytest.mark.parametrize("op", DD_OPS)
+def test_ddwave_fp32_matches_tiled(glx, orc, dd_reset, shape, f16, op):
     lx, ly = shape
     links, bc, _ = lattice_inputs(orc, lx, ly, beta=4.0, seed=7 * lx + ly)
     lat = glx.Lattice(links, lx, ly, f32=True, f16=f16)
     lp = glx.LINKS_F16 if f16 else glx.LINKS_F32
     v = bc.astype(np.complex64)
-    ref = run_dd(glx, lat, 0.3, v, True, lp, 0)
+    ref = run_dd(glx, lat, 0.3, v, True, lp, 0, op)
     for rows in (-1, 2, 10, 17, ly):
-        got = run_dd(glx, lat, 0.3, v, True, lp, rows)
+        got = run_dd(glx, lat, 0.3, v, True, lp, rows, op)
         assert relerr(got, ref) < 1e-6, (rows, relerr(got, ref))
 
 

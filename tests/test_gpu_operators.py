@@ -31,6 +31,17 @@ def test_operator_matches_oracle(glx, orc, op, lx, ly):
     assert relerr(got, want) < TOL, (op, lx, ly, relerr(got, want))
 
 
+@pytest.mark.parametrize("lx,ly", [(16, 7), (24, 9), (130, 11)])
+def test_m2mdeodoe_odd_ly(glx, orc, lx, ly):
+    """m^2 - D_eo D_oe where ly is odd: across the periodic wrap an even
+    site's neighbour is even, so the intermediate D_oe p must be zero on
+    the even sites (it is the oracle's D_oe, operators.cpp:499-539)."""
+    links, bc, _ = lattice_inputs(orc, lx, ly, beta=2.0, seed=11 + ly)
+    want = orc.apply("m2mdeodoe", bc, links, 0.37, lx, ly)
+    got = glx.apply("m2mdeodoe", bc, links, 0.37, lx, ly)
+    assert relerr(got, want) < TOL, (lx, ly, relerr(got, want))
+
+
 @pytest.mark.parametrize("lx,ly,nc", [(16, 16, 1), (24, 10, 1), (8, 6, 2), (5, 7, 3)])
 def test_real_laplace_matches_oracle(glx, orc, lx, ly, nc):
     g = orc.Rng(3)
